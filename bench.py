@@ -1,0 +1,241 @@
+"""Benchmark: Mrays/s of the MI355X traversal path on the incoherent-ray workload.
+
+Workload (one "step" = one frame): a 1,310,720-triangle displaced icosphere
+(frequency 256, radius 32, 4-octave value-noise displacement of 0.15 R),
+BVH built on the GPU (binned SAH, 8 bins, leaf size <= 2), 1920x1080 pinhole
+primary rays in 8x8-tile order, nearest-hit trace, one cosine-weighted
+diffuse bounce per hit (PCG32-seeded, origin offset 1e-4 along the normal),
+nearest-hit trace of the bounce rays.  Inputs are resident in HBM before the
+timed region; every step regenerates its rays on the GPU.
+
+value = (primary + bounce rays) / frame time.  Primary and bounce Mrays/s are
+reported separately from their own kernel times.  The roofline object prices
+the bounce trace (the dominant kernel) at the algorithmic bytes per ray
+28 + 20 + 32 + 64 P + 36 T (ray, hit, root node, P child-pair fetches, T
+triangle tests; P and T from the instrumented trace of the same rays) against
+8 TB/s.  The CPU baseline is the oracle (C restatement of the reference's
+traversal) on the same BVH and a bounded sample of the same rays, all host
+threads up to 16.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--width", type=int, default=1920)
+    p.add_argument("--height", type=int, default=1080)
+    p.add_argument("--freq", type=int, default=256)
+    p.add_argument("--seed", type=int, default=0xB0B)
+    p.add_argument("--cpu-seconds", type=float, default=10.0, help="budget for the CPU baseline sample")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_bounce_trace.json"),
+                   help="measured HBM bytes per launch of the dominant kernel (rocprofv3 --pmc summary)")
+    return p.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every GPU box here has one GPU",
+              file=sys.stderr)
+        sys.exit(2)
+    import bvhtracer_amd as B
+    from bvhtracer_amd import scenes
+
+    W, H = args.width, args.height
+    n_primary = W * H
+    t0 = time.time()
+    mesh = scenes.icosphere(args.freq, radius=32.0)
+    log(f"[bench] icosphere {mesh.shape[0]} triangles generated in {time.time() - t0:.1f}s")
+    ctx = B.Context(0)
+    t0 = time.time()
+    m = mesh.copy()
+    bvh = B.BvhBuilder(B.BUILD_SAH, 2, ctx=ctx).build_for(m)
+    build_s = time.time() - t0
+    log(f"[bench] GPU build {build_s * 1e3:.1f} ms, nodes_used {bvh.nodes_used()}, depth {bvh.depth()}")
+    cam = scenes.sphere_camera(32.0, W, H)
+
+    d_rays = ctx.alloc(n_primary * B.RAY_DTYPE.itemsize)
+    d_hits = ctx.alloc(n_primary * B.HIT_DTYPE.itemsize)
+    d_brays = ctx.alloc(n_primary * B.RAY_DTYPE.itemsize)
+    d_bhits = ctx.alloc(n_primary * B.HIT_DTYPE.itemsize)
+    d_count = ctx.alloc(4)
+    import ctypes
+
+    from bvhtracer_amd import _lib as L
+
+    lib = L.lib()
+    cam_rec = np.ascontiguousarray(cam)
+
+    def frame(seed, evs=None):
+        if evs:
+            ctx.record(evs[0])
+        L.check(lib.bvht_raygen(ctx.handle, L.ptr(cam_rec), W, H, B.ORDER_TILES, d_rays.ptr))
+        if evs:
+            ctx.record(evs[1])
+        L.check(lib.bvht_trace_blas(ctx.handle, bvh.handle, d_rays.ptr, n_primary, d_hits.ptr, None))
+        if evs:
+            ctx.record(evs[2])
+        L.check(lib.bvht_bounce(ctx.handle, bvh.handle, d_rays.ptr, d_hits.ptr, n_primary, seed, d_brays.ptr,
+                                d_count.ptr))
+        if evs:
+            ctx.record(evs[3])
+        L.check(lib.bvht_trace_blas_count(ctx.handle, bvh.handle, d_brays.ptr, d_count.ptr, n_primary,
+                                          d_bhits.ptr, None))
+        if evs:
+            ctx.record(evs[4])
+
+    for w in range(args.warmup):
+        frame(args.seed + w)
+    ctx.sync()
+    n_bounce = int(d_count.download(np.uint32, 1)[0])
+    log(f"[bench] warmup done; bounce rays per frame ~{n_bounce}")
+
+    evs = [[ctx.event() for _ in range(5)] for _ in range(args.steps)]
+    start, stop = ctx.event(), ctx.event()
+    ctx.sync()
+    ctx.record(start)
+    for s in range(args.steps):
+        frame(args.seed, evs[s])
+    ctx.record(stop)
+    ctx.sync()
+    total_ms = ctx.elapsed_ms(start, stop)
+    per = np.array([[ctx.elapsed_ms(e[k], e[k + 1]) for k in range(4)] for e in evs])
+    raygen_ms, prim_ms, bounce_gen_ms, bounce_ms = per.mean(axis=0)
+    ms_per_step = total_ms / args.steps
+    n_bounce = int(d_count.download(np.uint32, 1)[0])
+    rays_per_step = n_primary + n_bounce
+    value = rays_per_step / (ms_per_step * 1e-3) / 1e6
+    log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/frame (raygen {raygen_ms:.3f}, primary {prim_ms:.3f}, "
+        f"bounce-gen {bounce_gen_ms:.3f}, bounce {bounce_ms:.3f})")
+
+    # algorithmic bytes per bounce ray from the instrumented trace of the same rays (untimed)
+    d_stats = ctx.alloc(n_primary * B.STATS_DTYPE.itemsize)
+    L.check(lib.bvht_trace_blas(ctx.handle, bvh.handle, d_brays.ptr, n_bounce, d_bhits.ptr, d_stats.ptr))
+    st = d_stats.download(B.STATS_DTYPE, n_bounce)
+    P, T = st["pairs"].astype(np.float64).mean(), st["tests"].astype(np.float64).mean()
+    b_ray = 28 + 20 + 32 + 64 * P + 36 * T
+    L.check(lib.bvht_trace_blas(ctx.handle, bvh.handle, d_rays.ptr, n_primary, d_hits.ptr, d_stats.ptr))
+    stp = d_stats.download(B.STATS_DTYPE, n_primary)
+    Pp, Tp = stp["pairs"].astype(np.float64).mean(), stp["tests"].astype(np.float64).mean()
+    b_ray_p = 28 + 20 + 32 + 64 * Pp + 36 * Tp
+    achieved = b_ray * n_bounce / (bounce_ms * 1e-3) / 1e9
+    traffic = None
+    if args.traffic and os.path.exists(args.traffic):
+        with open(args.traffic) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    result = {
+        "metric": "Mrays/s primary+diffuse on 1 MI355X",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"displaced icosphere {mesh.shape[0]} tris (freq {args.freq}, R 32), {W}x{H} primary + "
+                        f"1 cosine diffuse bounce per hit, nearest hit, SAH BVH (8 bins, leaf<=2) built on GPU",
+            "primary_rays": n_primary,
+            "bounce_rays": n_bounce,
+        },
+        "primary_mrays_s": round(n_primary / (prim_ms * 1e-3) / 1e6, 2),
+        "bounce_mrays_s": round(n_bounce / (bounce_ms * 1e-3) / 1e6, 2),
+        "frame_breakdown_ms": {"raygen": round(raygen_ms, 4), "primary_trace": round(prim_ms, 4),
+                               "bounce_gen": round(bounce_gen_ms, 4), "bounce_trace": round(bounce_ms, 4)},
+        "blas_build_ms": round(build_s * 1e3, 1),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_trace_blas (bounce pass)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_ray": round(b_ray, 1),
+            "pairs_per_ray": round(P, 2),
+            "tests_per_ray": round(T, 3),
+            "primary_bytes_per_ray": round(b_ray_p, 1),
+            "primary_frac": round(b_ray_p * n_primary / (prim_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        },
+        "cpu_baseline": None,
+    }
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(ctx, bvh, d_brays, d_bhits, n_bounce, d_rays, d_hits, n_primary,
+                                              args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline(ctx, bvh, d_brays, d_bhits, n_bounce, d_rays, d_hits, n_primary, budget_s):
+    """The oracle (C restatement of the reference traversal, -O3) on the same
+    BVH and a bounded, interleaved sample of the same primary and bounce rays."""
+    import oracle
+
+    import bvhtracer_amd as B
+
+    nodes, used, tris = bvh.download()
+    cores = max(1, min(16, os.cpu_count() or 1))
+    rays = d_rays.download(B.RAY_DTYPE, n_primary)
+    brays = d_brays.download(B.RAY_DTYPE, n_bounce)
+    hits = d_hits.download(B.HIT_DTYPE, n_primary)
+    bhits = d_bhits.download(B.HIT_DTYPE, n_bounce)
+    # calibrate the sample on 20k rays, then size it to the budget
+    stride = 101
+    probe = np.concatenate([rays[::stride][:10000], brays[::stride][:10000]]).view(oracle.RAY_DTYPE)
+    t0 = time.perf_counter()
+    oracle.bvh_trace(nodes, tris, probe, nthreads=cores)
+    rate = probe.shape[0] / max(time.perf_counter() - t0, 1e-6)
+    want = int(min(rate * budget_s, n_primary + n_bounce))
+    frac = want / (n_primary + n_bounce)
+    kp = max(1, int(n_primary * frac))
+    kb = max(1, int(n_bounce * frac))
+    ip = np.linspace(0, n_primary - 1, kp).astype(np.int64)
+    ib = np.linspace(0, n_bounce - 1, kb).astype(np.int64)
+    sample = np.concatenate([rays[ip], brays[ib]]).view(oracle.RAY_DTYPE)
+    t0 = time.perf_counter()
+    ref = oracle.bvh_trace(nodes, tris, sample, nthreads=cores)
+    dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    single = np.concatenate([rays[ip[:20000]], brays[ib[:20000]]]).view(oracle.RAY_DTYPE)
+    oracle.bvh_trace(nodes, tris, single, nthreads=1)
+    dt1 = time.perf_counter() - t1
+    gpu = np.concatenate([hits[ip], bhits[ib]])
+    parity = bool(gpu.tobytes() == ref.tobytes())
+    return {
+        "value": round(sample.shape[0] / dt / 1e6, 3),
+        "unit": "Mrays/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{kp} primary + {kb} bounce rays (every {max(1, n_primary // kp)}th), same BVH; "
+                  f"{dt:.1f} s on {cores} threads",
+        "single_thread_mrays_s": round(single.shape[0] / dt1 / 1e6, 3),
+        "sample_hits_bit_identical": parity,
+    }
+
+
+if __name__ == "__main__":
+    main()

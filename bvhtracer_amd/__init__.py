@@ -1,0 +1,17 @@
+"""bvhtracer_amd -- MI355X (gfx950) ray/BVH traversal hot path.
+
+A drop-in for the traversal path of lambdaxymox/bvhtracer: hand-written HIP
+kernels behind the C ABI in include/bvht.h (libbvht_hip.so), with a host
+mirror of the reference's Bvh / BvhBuilder / Tlas / SceneObject / Ray API.
+"""
+from ._lib import (BUILD_REFERENCE, BUILD_SAH, CAMERA_DTYPE, HIT_DTYPE, INSTANCE_DTYPE, MISS, NODE_DTYPE,
+                   ORDER_ROWS, ORDER_TILES, RAY_DTYPE, STATS_DTYPE, TNODE_DTYPE, BvhtError)
+from .api import (Bvh, BvhBuilder, Context, DeviceArray, Ray, SceneObject, Tlas, TlasBuilder, bounce, bounce_tlas,
+                  default_context, raygen)
+
+__all__ = [
+    "BUILD_REFERENCE", "BUILD_SAH", "CAMERA_DTYPE", "HIT_DTYPE", "INSTANCE_DTYPE", "MISS", "NODE_DTYPE",
+    "ORDER_ROWS", "ORDER_TILES", "RAY_DTYPE", "STATS_DTYPE", "TNODE_DTYPE", "BvhtError",
+    "Bvh", "BvhBuilder", "Context", "DeviceArray", "Ray", "SceneObject", "Tlas", "TlasBuilder", "bounce",
+    "bounce_tlas", "default_context", "raygen",
+]

@@ -1,0 +1,107 @@
+"""ctypes binding of the C ABI in include/bvht.h (libbvht_hip.so).
+
+The library is loaded from the package directory only; if it is missing the
+import of any GPU entry point raises -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbvht_hip.so")
+
+OK, EINVAL, ENOMEM, EDEVICE, ESTACK = 0, -1, -2, -3, -4
+MISS = 0xFFFFFFFF
+ORDER_ROWS, ORDER_TILES = 0, 1
+BUILD_REFERENCE, BUILD_SAH = 0, 1
+
+NODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("left_first", "<u4"), ("bmax", "<f4", 3), ("count", "<u4")])
+TNODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("left_right", "<u4"), ("bmax", "<f4", 3), ("blas", "<u4")])
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmax", "<f4"), ("d", "<f4", 3), ("pad", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim", "<u4"), ("inst", "<u4"), ("packed", "<u4")])
+INSTANCE_DTYPE = np.dtype([("m", "<f4", 16), ("minv", "<f4", 16), ("blas", "<u4"), ("pad", "<u4", 3)])
+CAMERA_DTYPE = np.dtype([("tl", "<f4", 3), ("tr", "<f4", 3), ("bl", "<f4", 3), ("view_inv", "<f4", 16)])
+STATS_DTYPE = np.dtype([("pairs", "<u4"), ("tests", "<u4"), ("tlas_pairs", "<u4"), ("inst_visits", "<u4")])
+
+assert NODE_DTYPE.itemsize == 32 and TNODE_DTYPE.itemsize == 32 and RAY_DTYPE.itemsize == 32
+assert HIT_DTYPE.itemsize == 24 and INSTANCE_DTYPE.itemsize == 144 and CAMERA_DTYPE.itemsize == 100
+
+P = ctypes.c_void_p
+PP = ctypes.POINTER(ctypes.c_void_p)
+I32, U32, U64, F32, SZ = ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float, ctypes.c_size_t
+
+# name -> (restype, argtypes); every symbol declared in include/bvht.h
+SIGNATURES = {
+    "bvht_create": (I32, [I32, PP]),
+    "bvht_destroy": (I32, [P]),
+    "bvht_last_error": (ctypes.c_char_p, []),
+    "bvht_sync": (I32, [P]),
+    "bvht_stream": (P, [P]),
+    "bvht_device_alloc": (I32, [P, SZ, PP]),
+    "bvht_device_free": (I32, [P, P]),
+    "bvht_copy_to_device": (I32, [P, P, P, SZ]),
+    "bvht_copy_to_host": (I32, [P, P, P, SZ]),
+    "bvht_event_create": (I32, [P, PP]),
+    "bvht_event_record": (I32, [P, P]),
+    "bvht_event_elapsed": (I32, [P, P, ctypes.POINTER(F32)]),
+    "bvht_event_destroy": (I32, [P]),
+    "bvht_blas_build": (I32, [P, P, U32, I32, U32, PP]),
+    "bvht_blas_upload": (I32, [P, P, U32, U32, P, U32, PP]),
+    "bvht_blas_download": (I32, [P, P, P, P]),
+    "bvht_blas_info": (I32, [P, P, P, P, P]),
+    "bvht_blas_set_vertices": (I32, [P, P, P, U32]),
+    "bvht_blas_set_vertices_device": (I32, [P, P, P, U32]),
+    "bvht_blas_refit": (I32, [P, P]),
+    "bvht_blas_bounds": (I32, [P, P, P]),
+    "bvht_blas_destroy": (I32, [P]),
+    "bvht_trace_blas": (I32, [P, P, P, U32, P, P]),
+    "bvht_trace_blas_count": (I32, [P, P, P, P, U32, P, P]),
+    "bvht_raygen": (I32, [P, P, U32, U32, I32, P]),
+    "bvht_bounce": (I32, [P, P, P, P, U32, U64, P, P]),
+    "bvht_depth": (I32, [P, P, U32, P]),
+    "bvht_tlas_create": (I32, [P, P, U32, P, U32, PP]),
+    "bvht_tlas_set_objects": (I32, [P, P, P, U32, U32]),
+    "bvht_tlas_refresh_bounds": (I32, [P, P]),
+    "bvht_tlas_rebuild": (I32, [P, P]),
+    "bvht_tlas_download": (I32, [P, P, P, P]),
+    "bvht_tlas_destroy": (I32, [P]),
+    "bvht_trace_tlas": (I32, [P, P, P, U32, P, P]),
+    "bvht_trace_tlas_count": (I32, [P, P, P, P, U32, P, P]),
+    "bvht_bounce_tlas": (I32, [P, P, P, P, U32, U64, P, P]),
+}
+
+
+class BvhtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bvht error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libbvht_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m bvhtracer_amd.build` (no CPU fallback exists)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != OK:
+        msg = lib().bvht_last_error()
+        raise BvhtError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)

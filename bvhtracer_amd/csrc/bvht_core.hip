@@ -1,0 +1,756 @@
+// bvht_core.hip -- context, BLAS storage, refit and the BLAS trace kernels
+// for gfx950 (MI355X).
+//
+// Traversal kernels run one ray per lane (wave64).  Each lane keeps its
+// traversal stack in LDS, laid out slot-major (slot * WG + lane) so the 64
+// lanes of a wave hit 64 distinct banks; entries deeper than the LDS depth
+// spill to a per-ray region of a global buffer.  The visiting order is the
+// reference's exactly (bvh.rs:242-305): no root test, both children tested
+// against the current closest t, nearer first (ties to the right child), the
+// farther pushed only if hit, no re-cull on pop, first hit wins among equal t.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "bvht_device.hpp"
+#include "bvht_internal.hpp"
+#include "bvht_trace.hpp"
+
+using namespace bvht;
+
+namespace bvht {
+
+static thread_local std::string g_error;
+
+int32_t set_error(int32_t code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+int32_t ensure_scratch(bvht_ctx ctx, size_t bytes)
+{
+    if (ctx->scratch_bytes >= bytes)
+        return BVHT_OK;
+    if (ctx->scratch) {
+        BVHT_HIP(hipStreamSynchronize(ctx->stream));
+        BVHT_HIP(hipFree(ctx->scratch));
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+    }
+    size_t sz = std::max<size_t>(bytes, 1 << 20);
+    if (hipMalloc(&ctx->scratch, sz) != hipSuccess)
+        return set_error(BVHT_ENOMEM, "hipMalloc(%zu) for scratch failed", sz);
+    ctx->scratch_bytes = sz;
+    return BVHT_OK;
+}
+
+int32_t ensure_spill(bvht_ctx ctx, uint32_t** buf, size_t* words, size_t need)
+{
+    if (*words >= need)
+        return BVHT_OK;
+    if (*buf) {
+        BVHT_HIP(hipStreamSynchronize(ctx->stream));
+        BVHT_HIP(hipFree(*buf));
+        *buf = nullptr;
+        *words = 0;
+    }
+    if (hipMalloc(reinterpret_cast<void**>(buf), need * sizeof(uint32_t)) != hipSuccess)
+        return set_error(BVHT_ENOMEM, "hipMalloc(%zu) for the traversal spill failed", need * 4);
+    *words = need;
+    return BVHT_OK;
+}
+
+}  // namespace bvht
+
+// ------------------------------------------------------------- kernels --
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+// parent[] of every used node; interior nodes name their two children.
+__global__ void k_parents(const bvht_node* __restrict__ nodes, uint32_t nodes_used, uint32_t* __restrict__ parent)
+{
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nodes_used || k == 1)
+        return;
+    bvht_node nd = nodes[k];
+    if (nd.count == 0) {
+        parent[nd.left_first] = k;
+        parent[nd.left_first + 1] = k;
+    }
+}
+
+// Depth of the tree: every leaf walks to the root.
+__global__ void k_depth(const bvht_node* __restrict__ nodes, uint32_t nodes_used,
+                        const uint32_t* __restrict__ parent, uint32_t* __restrict__ depth)
+{
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nodes_used || k == 1)
+        return;
+    if (nodes[k].count == 0)
+        return;
+    uint32_t d = 1, p = parent[k];
+    while (p != BVHT_MISS && d < nodes_used + 1) {
+        ++d;
+        p = parent[p];
+    }
+    atomicMax(depth, d);
+}
+
+__device__ __forceinline__ float ld_agent(const float* p)
+{
+    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Bvh::refit (bvh.rs:469-493) bottom-up: leaves recompute their box from the
+// triangles starting at (+MAX, -MAX) (update_node_bounds, bvh.rs:317-330);
+// the second child to arrive at a parent writes the union of both children.
+// min/max are exact, so the result equals the reference's reverse sweep.
+__global__ void k_refit(bvht_node* __restrict__ nodes, const float* __restrict__ tris,
+                        const uint32_t* __restrict__ parent, uint32_t* __restrict__ flags,
+                        uint32_t nodes_used)
+{
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nodes_used || k == 1)
+        return;
+    bvht_node nd = nodes[k];
+    if (nd.count == 0)
+        return;
+    V3 lo = mk3(kFltMax, kFltMax, kFltMax), hi = mk3(-kFltMax, -kFltMax, -kFltMax);
+    for (uint32_t j = 0; j < nd.count; ++j) {
+        const float* t = tris + 9 * (size_t)(nd.left_first + j);
+        V3 v0 = mk3(t[0], t[1], t[2]), v1 = mk3(t[3], t[4], t[5]), v2 = mk3(t[6], t[7], t[8]);
+        lo = min3(lo, v0);
+        lo = min3(lo, v1);
+        lo = min3(lo, v2);
+        hi = max3(hi, v0);
+        hi = max3(hi, v1);
+        hi = max3(hi, v2);
+    }
+    bvht_node* out = nodes + k;
+    out->bmin[0] = lo.x; out->bmin[1] = lo.y; out->bmin[2] = lo.z;
+    out->bmax[0] = hi.x; out->bmax[1] = hi.y; out->bmax[2] = hi.z;
+    __threadfence();
+    uint32_t p = parent[k];
+    while (p != BVHT_MISS) {
+        uint32_t old = atomicAdd(&flags[p], 1u);
+        if (old == 0)
+            return;
+        __threadfence();
+        uint32_t l = nodes[p].left_first;
+        const bvht_node* a = nodes + l;
+        const bvht_node* b = nodes + l + 1;
+        V3 alo = mk3(ld_agent(&a->bmin[0]), ld_agent(&a->bmin[1]), ld_agent(&a->bmin[2]));
+        V3 ahi = mk3(ld_agent(&a->bmax[0]), ld_agent(&a->bmax[1]), ld_agent(&a->bmax[2]));
+        V3 blo = mk3(ld_agent(&b->bmin[0]), ld_agent(&b->bmin[1]), ld_agent(&b->bmin[2]));
+        V3 bhi = mk3(ld_agent(&b->bmax[0]), ld_agent(&b->bmax[1]), ld_agent(&b->bmax[2]));
+        V3 mn = min3(alo, blo), mx = max3(ahi, bhi);
+        bvht_node* o = nodes + p;
+        o->bmin[0] = mn.x; o->bmin[1] = mn.y; o->bmin[2] = mn.z;
+        o->bmax[0] = mx.x; o->bmax[1] = mx.y; o->bmax[2] = mx.z;
+        __threadfence();
+        p = parent[p];
+    }
+}
+
+template <int LDS_DEPTH, bool STATS>
+__global__ __launch_bounds__(kTraceWG) void k_trace_blas(const float4* __restrict__ nodes,
+                                                         const float* __restrict__ tris,
+                                                         const bvht_ray* __restrict__ rays,
+                                                         bvht_hit* __restrict__ hits, uint32_t n,
+                                                         const uint32_t* __restrict__ d_n,
+                                                         bvht_stats* __restrict__ stats,
+                                                         uint32_t* __restrict__ spill, uint32_t cap)
+{
+    __shared__ uint32_t lds[LDS_DEPTH * kTraceWG];
+    uint32_t i = blockIdx.x * kTraceWG + threadIdx.x;
+    uint32_t count = d_n ? min(*d_n, n) : n;
+    if (i >= count)
+        return;
+    Stack<LDS_DEPTH> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap};
+    DRay ray = load_ray(rays, i);
+    uint32_t pairs = 0, tests = 0;
+    uint32_t sp = 0;
+    BlasResult r = traverse_blas<LDS_DEPTH, STATS>(nodes, tris, ray, st, sp, pairs, tests);
+    store_hit(hits, i, r.found, r.t, r.u, r.v, r.prim, 0);
+    if (STATS) {
+        bvht_stats s;
+        s.pairs = pairs;
+        s.tests = tests;
+        s.tlas_pairs = 0;
+        s.inst_visits = 0;
+        stats[i] = s;
+    }
+}
+
+// Camera::get_ray_eye + get_ray_world (camera.rs:994-1010) with the
+// per-pixel mapping of PathTracer::evaluate (renderer.rs:353-362).
+__global__ void k_raygen(bvht_camera cam, uint32_t width, uint32_t height, int32_t order,
+                         bvht_ray* __restrict__ rays)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= width * height)
+        return;
+    uint32_t x, y;
+    if (order == 1) {
+        uint32_t tile = j >> 6, w = j & 63u, tiles_x = width >> 3;
+        x = (tile % tiles_x) * 8 + (w & 7u);
+        y = (tile / tiles_x) * 8 + (w >> 3);
+    } else {
+        x = j % width;
+        y = j / width;
+    }
+    float u = (float)x / (float)width;
+    float v = (float)y / (float)height;
+    V3 origin = mk3(0.0f, 0.0f, 0.0f);
+    V3 tl = mk3(cam.tl[0], cam.tl[1], cam.tl[2]);
+    V3 tr = mk3(cam.tr[0], cam.tr[1], cam.tr[2]);
+    V3 bl = mk3(cam.bl[0], cam.bl[1], cam.bl[2]);
+    V3 pixel = add3(add3(add3(origin, tl), scl3(sub3(tr, tl), u)), scl3(sub3(bl, tl), v));
+    V3 p = sub3(pixel, origin);
+    float nrm = sqrtf(dot3(p, p));
+    V3 dir = mk3(p.x / nrm, p.y / nrm, p.z / nrm);
+    V3 ow = mat_mul4(cam.view_inv, origin, 1.0f);
+    V3 dw = mat_mul4(cam.view_inv, dir, 0.0f);
+    float4* out = reinterpret_cast<float4*>(rays + j);
+    out[0] = make_float4(ow.x, ow.y, ow.z, kFltMax);
+    out[1] = make_float4(dw.x, dw.y, dw.z, 0.0f);
+}
+
+// Cosine-weighted bounce about the geometric normal of the hit triangle
+// (flipped against the incoming direction), origin offset 1e-4 along it.
+__device__ __forceinline__ void make_bounce(const float* tri, V3 o, V3 d, float t, uint64_t seed, uint32_t idx,
+                                            float4& a, float4& b)
+{
+    V3 v0 = mk3(tri[0], tri[1], tri[2]);
+    V3 e1 = sub3(mk3(tri[3], tri[4], tri[5]), v0);
+    V3 e2 = sub3(mk3(tri[6], tri[7], tri[8]), v0);
+    bounce_ray(cross3(e1, e2), add3(o, scl3(d, t)), d, seed, idx, a, b);
+}
+
+// Order-preserving compaction of hits: per-block counts, one-block scan, write.
+__global__ __launch_bounds__(256) void k_hit_counts(const bvht_hit* __restrict__ hits, uint32_t n,
+                                                    uint32_t* __restrict__ block_counts)
+{
+    __shared__ uint32_t wave_counts[4];
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    bool h = i < n && hits[i].prim != BVHT_MISS;
+    uint64_t m = __ballot(h);
+    if ((threadIdx.x & 63) == 0)
+        wave_counts[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        block_counts[blockIdx.x] = wave_counts[0] + wave_counts[1] + wave_counts[2] + wave_counts[3];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* __restrict__ counts, uint32_t nblocks,
+                                                      uint32_t* __restrict__ total)
+{
+    // exclusive scan in place; each thread owns a contiguous run
+    __shared__ uint32_t part[1024];
+    uint32_t per = (nblocks + 1023) / 1024;
+    uint32_t begin = threadIdx.x * per, end = min(begin + per, nblocks);
+    uint32_t s = 0;
+    for (uint32_t k = begin; k < end; ++k)
+        s += counts[k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    for (uint32_t k = begin; k < end; ++k) {
+        uint32_t c = counts[k];
+        counts[k] = run;
+        run += c;
+    }
+    if (threadIdx.x == 1023)
+        *total = part[1023];
+}
+
+__global__ __launch_bounds__(256) void k_bounce_write(const float* __restrict__ tris,
+                                                      const bvht_ray* __restrict__ rays,
+                                                      const bvht_hit* __restrict__ hits, uint32_t n,
+                                                      const uint32_t* __restrict__ block_offsets,
+                                                      uint64_t seed, bvht_ray* __restrict__ out)
+{
+    __shared__ uint32_t wave_counts[4];
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    bvht_hit h;
+    bool hit = false;
+    if (i < n) {
+        h = hits[i];
+        hit = h.prim != BVHT_MISS;
+    }
+    uint64_t m = __ballot(hit);
+    uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        wave_counts[wave] = __popcll(m);
+    __syncthreads();
+    uint32_t base = block_offsets[blockIdx.x];
+    for (uint32_t w = 0; w < wave; ++w)
+        base += wave_counts[w];
+    uint32_t rank = __popcll(m & ((1ull << lane) - 1ull));
+    if (!hit)
+        return;
+    DRay r = load_ray(rays, i);
+    float4 a, b;
+    make_bounce(tris + 9 * (size_t)h.prim, r.o, r.d, h.t, seed, i, a, b);
+    float4* o = reinterpret_cast<float4*>(out + base + rank);
+    o[0] = a;
+    o[1] = b;
+}
+
+__global__ void k_depth_acc(const bvht_hit* __restrict__ hits, uint32_t n, float* __restrict__ depth)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        depth[i] = hits[i].prim != BVHT_MISS ? hits[i].t : kFltMax;
+}
+
+// ------------------------------------------------------------ host side --
+
+namespace bvht {
+
+static inline uint32_t blocks_for(size_t n, uint32_t wg) { return (uint32_t)((n + wg - 1) / wg); }
+
+int32_t fill_u32(bvht_ctx ctx, uint32_t* p, uint32_t v, size_t n)
+{
+    if (n == 0)
+        return BVHT_OK;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, p, v, n);
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
+// parent array + depth on the device (after a build or an upload)
+int32_t compute_blas_topology(bvht_blas b, const bvht_node*)
+{
+    bvht_ctx ctx = b->ctx;
+    int32_t rc = fill_u32(ctx, b->d_parent, BVHT_MISS, b->node_count);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_parents, dim3(blocks_for(b->nodes_used, 256)), dim3(256), 0, ctx->stream, b->d_nodes,
+                       b->nodes_used, b->d_parent);
+    BVHT_HIP(hipGetLastError());
+    rc = ensure_scratch(ctx, 4);
+    if (rc)
+        return rc;
+    uint32_t* d_depth = static_cast<uint32_t*>(ctx->scratch);
+    rc = fill_u32(ctx, d_depth, 0, 1);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_depth, dim3(blocks_for(b->nodes_used, 256)), dim3(256), 0, ctx->stream, b->d_nodes,
+                       b->nodes_used, b->d_parent, d_depth);
+    BVHT_HIP(hipGetLastError());
+    uint32_t depth = 0;
+    BVHT_HIP(hipMemcpyAsync(&depth, d_depth, 4, hipMemcpyDeviceToHost, ctx->stream));
+    BVHT_HIP(hipStreamSynchronize(ctx->stream));
+    b->depth = depth;
+    return BVHT_OK;
+}
+
+// Host-side structural validation of a caller-supplied node array, so that
+// no kernel ever follows an out-of-range index: children must lie after
+// their parent (which also rules out cycles) and inside nodes_used, and leaf
+// ranges inside the triangle array.
+static int32_t validate_nodes(const bvht_node* nodes, uint32_t node_count, uint32_t nodes_used, uint32_t n)
+{
+    if (nodes_used > node_count || nodes_used == 0)
+        return set_error(BVHT_EINVAL, "nodes_used (%u) must be in [1, node_count=%u]", nodes_used, node_count);
+    if (node_count < 2)
+        return set_error(BVHT_EINVAL, "node array needs at least 2 entries (root + dummy)");
+    for (uint32_t k = 0; k < nodes_used; ++k) {
+        if (k == 1)
+            continue;
+        const bvht_node& nd = nodes[k];
+        if (nd.count > 0) {
+            if ((uint64_t)nd.left_first + nd.count > n)
+                return set_error(BVHT_EINVAL, "leaf %u range [%u, +%u) exceeds %u triangles", k, nd.left_first,
+                                 nd.count, n);
+        } else {
+            if (nd.left_first <= k || nd.left_first == 1 || (uint64_t)nd.left_first + 1 >= nodes_used)
+                return set_error(BVHT_EINVAL, "branch %u has invalid children %u,%u (nodes_used %u)", k,
+                                 nd.left_first, nd.left_first + 1, nodes_used);
+        }
+    }
+    return BVHT_OK;
+}
+
+int32_t blas_alloc(bvht_ctx ctx, uint32_t n, uint32_t node_count, bvht_blas* out)
+{
+    bvht_blas b = new (std::nothrow) bvht_blas_s();
+    if (!b)
+        return set_error(BVHT_ENOMEM, "host allocation failed");
+    b->ctx = ctx;
+    b->n_tris = n;
+    b->node_count = node_count;
+    bool ok = hipMalloc(&b->d_nodes, sizeof(bvht_node) * (size_t)node_count) == hipSuccess &&
+              hipMalloc(&b->d_tris, sizeof(float) * 9 * (size_t)n) == hipSuccess &&
+              hipMalloc(&b->d_parent, sizeof(uint32_t) * (size_t)node_count) == hipSuccess &&
+              hipMalloc(&b->d_flags, sizeof(uint32_t) * (size_t)node_count) == hipSuccess;
+    if (!ok) {
+        bvht_blas_destroy(b);
+        return set_error(BVHT_ENOMEM, "device allocation for a BLAS of %u triangles failed", n);
+    }
+    *out = b;
+    return BVHT_OK;
+}
+
+int32_t launch_trace_blas(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, uint32_t n,
+                          const uint32_t* d_n, bvht_hit* d_hits, bvht_stats* d_stats)
+{
+    if (n == 0)
+        return BVHT_OK;
+    // stack entries needed: at most depth - 1
+    uint32_t need = blas->depth > 0 ? blas->depth - 1 : 0;
+    int lds_depth = need <= 16 ? 16 : (need <= 32 ? 32 : 64);
+    uint32_t cap = need;
+    uint32_t* spill = nullptr;
+    if (need > (uint32_t)lds_depth) {
+        int32_t rc = ensure_spill(ctx, &blas->d_spill, &blas->spill_words, (size_t)(need - lds_depth) * n);
+        if (rc)
+            return rc;
+        spill = blas->d_spill;
+    }
+    const float4* nodes = reinterpret_cast<const float4*>(blas->d_nodes);
+    dim3 grid(blocks_for(n, kTraceWG)), block(kTraceWG);
+#define LAUNCH(D, S)                                                                                   \
+    hipLaunchKernelGGL((k_trace_blas<D, S>), grid, block, 0, ctx->stream, nodes, blas->d_tris, d_rays, d_hits, n, \
+                       d_n, d_stats, spill, cap)
+    if (d_stats) {
+        if (lds_depth == 16) LAUNCH(16, true);
+        else if (lds_depth == 32) LAUNCH(32, true);
+        else LAUNCH(64, true);
+    } else {
+        if (lds_depth == 16) LAUNCH(16, false);
+        else if (lds_depth == 32) LAUNCH(32, false);
+        else LAUNCH(64, false);
+    }
+#undef LAUNCH
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
+// Per-256-ray-block exclusive offsets of the hits (in ctx scratch) and the
+// total hit count in *d_count; the caller's write kernel packs the hits.
+int32_t hit_block_offsets(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, uint32_t* d_count, uint32_t** offsets)
+{
+    uint32_t nb = blocks_for(n, 256);
+    int32_t rc = ensure_scratch(ctx, sizeof(uint32_t) * std::max<uint32_t>(nb, 1));
+    if (rc)
+        return rc;
+    uint32_t* bc = static_cast<uint32_t*>(ctx->scratch);
+    *offsets = bc;
+    if (nb == 0)
+        return fill_u32(ctx, d_count, 0, 1);
+    hipLaunchKernelGGL(k_hit_counts, dim3(nb), dim3(256), 0, ctx->stream, d_hits, n, bc);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, ctx->stream, bc, nb, d_count);
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
+int32_t launch_bounce(bvht_ctx ctx, const float* d_tris, const bvht_ray* d_rays, const bvht_hit* d_hits, uint32_t n,
+                      uint64_t seed, bvht_ray* d_out, uint32_t* d_count)
+{
+    uint32_t* bc = nullptr;
+    int32_t rc = hit_block_offsets(ctx, d_hits, n, d_count, &bc);
+    if (rc || n == 0)
+        return rc;
+    uint32_t nb = blocks_for(n, 256);
+    hipLaunchKernelGGL(k_bounce_write, dim3(nb), dim3(256), 0, ctx->stream, d_tris, d_rays, d_hits, n, bc, seed,
+                       d_out);
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
+}  // namespace bvht
+
+// ------------------------------------------------------------- C ABI ----
+
+extern "C" {
+
+const char* bvht_last_error(void) { return bvht::g_error.c_str(); }
+
+int32_t bvht_create(int32_t device, bvht_ctx* out)
+{
+    BVHT_REQUIRE(out, "out is NULL");
+    int count = 0;
+    BVHT_HIP(hipGetDeviceCount(&count));
+    BVHT_REQUIRE(device >= 0 && device < count, "device %d out of range (%d devices)", device, count);
+    BVHT_HIP(hipSetDevice(device));
+    bvht_ctx c = new (std::nothrow) bvht_ctx_s();
+    if (!c)
+        return set_error(BVHT_ENOMEM, "host allocation failed");
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return set_error(BVHT_EDEVICE, "hipStreamCreate failed");
+    }
+    *out = c;
+    return BVHT_OK;
+}
+
+int32_t bvht_destroy(bvht_ctx ctx)
+{
+    if (!ctx)
+        return BVHT_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->scratch)
+        (void)hipFree(ctx->scratch);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return BVHT_OK;
+}
+
+int32_t bvht_sync(bvht_ctx ctx)
+{
+    BVHT_REQUIRE(ctx, "ctx is NULL");
+    BVHT_HIP(hipStreamSynchronize(ctx->stream));
+    return BVHT_OK;
+}
+
+void* bvht_stream(bvht_ctx ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int32_t bvht_device_alloc(bvht_ctx ctx, size_t bytes, void** out)
+{
+    BVHT_REQUIRE(ctx && out, "NULL argument");
+    if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess)
+        return set_error(BVHT_ENOMEM, "hipMalloc(%zu) failed", bytes);
+    return BVHT_OK;
+}
+
+int32_t bvht_device_free(bvht_ctx ctx, void* ptr)
+{
+    BVHT_REQUIRE(ctx, "ctx is NULL");
+    if (ptr) {
+        BVHT_HIP(hipStreamSynchronize(ctx->stream));
+        BVHT_HIP(hipFree(ptr));
+    }
+    return BVHT_OK;
+}
+
+int32_t bvht_copy_to_device(bvht_ctx ctx, void* dst, const void* src, size_t bytes)
+{
+    BVHT_REQUIRE(ctx && (bytes == 0 || (dst && src)), "NULL argument");
+    BVHT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    BVHT_HIP(hipStreamSynchronize(ctx->stream));
+    return BVHT_OK;
+}
+
+int32_t bvht_copy_to_host(bvht_ctx ctx, void* dst, const void* src, size_t bytes)
+{
+    BVHT_REQUIRE(ctx && (bytes == 0 || (dst && src)), "NULL argument");
+    BVHT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    BVHT_HIP(hipStreamSynchronize(ctx->stream));
+    return BVHT_OK;
+}
+
+int32_t bvht_event_create(bvht_ctx ctx, void** out)
+{
+    BVHT_REQUIRE(ctx && out, "NULL argument");
+    hipEvent_t ev;
+    BVHT_HIP(hipEventCreate(&ev));
+    *out = (void*)ev;
+    return BVHT_OK;
+}
+
+int32_t bvht_event_record(bvht_ctx ctx, void* ev)
+{
+    BVHT_REQUIRE(ctx && ev, "NULL argument");
+    BVHT_HIP(hipEventRecord((hipEvent_t)ev, ctx->stream));
+    return BVHT_OK;
+}
+
+int32_t bvht_event_elapsed(void* start, void* end, float* ms)
+{
+    BVHT_REQUIRE(start && end && ms, "NULL argument");
+    BVHT_HIP(hipEventSynchronize((hipEvent_t)end));
+    BVHT_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
+    return BVHT_OK;
+}
+
+int32_t bvht_event_destroy(void* ev)
+{
+    if (ev)
+        BVHT_HIP(hipEventDestroy((hipEvent_t)ev));
+    return BVHT_OK;
+}
+
+int32_t bvht_blas_upload(bvht_ctx ctx, const bvht_node* nodes, uint32_t node_count, uint32_t nodes_used,
+                         const float* tris, uint32_t n, bvht_blas* out)
+{
+    BVHT_REQUIRE(ctx && nodes && tris && out, "NULL argument");
+    BVHT_REQUIRE(n > 0, "empty triangle array (the reference panics on an empty mesh)");
+    int32_t rc = validate_nodes(nodes, node_count, nodes_used, n);
+    if (rc)
+        return rc;
+    bvht_blas b = nullptr;
+    rc = blas_alloc(ctx, n, node_count, &b);
+    if (rc)
+        return rc;
+    b->nodes_used = nodes_used;
+    if (hipMemcpyAsync(b->d_nodes, nodes, sizeof(bvht_node) * (size_t)node_count, hipMemcpyHostToDevice,
+                       ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(b->d_tris, tris, sizeof(float) * 9 * (size_t)n, hipMemcpyHostToDevice, ctx->stream) !=
+            hipSuccess) {
+        bvht_blas_destroy(b);
+        return set_error(BVHT_EDEVICE, "upload copy failed");
+    }
+    rc = compute_blas_topology(b, nodes);
+    if (rc) {
+        bvht_blas_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return BVHT_OK;
+}
+
+int32_t bvht_blas_download(bvht_ctx ctx, bvht_blas blas, bvht_node* nodes, float* tris)
+{
+    BVHT_REQUIRE(ctx && blas, "NULL argument");
+    if (nodes)
+        BVHT_HIP(hipMemcpyAsync(nodes, blas->d_nodes, sizeof(bvht_node) * (size_t)blas->node_count,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    if (tris)
+        BVHT_HIP(hipMemcpyAsync(tris, blas->d_tris, sizeof(float) * 9 * (size_t)blas->n_tris,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    BVHT_HIP(hipStreamSynchronize(ctx->stream));
+    return BVHT_OK;
+}
+
+int32_t bvht_blas_info(bvht_blas blas, uint32_t* tri_count, uint32_t* node_count, uint32_t* nodes_used,
+                       uint32_t* depth)
+{
+    BVHT_REQUIRE(blas, "blas is NULL");
+    if (tri_count) *tri_count = blas->n_tris;
+    if (node_count) *node_count = blas->node_count;
+    if (nodes_used) *nodes_used = blas->nodes_used;
+    if (depth) *depth = blas->depth;
+    return BVHT_OK;
+}
+
+int32_t bvht_blas_set_vertices(bvht_ctx ctx, bvht_blas blas, const float* tris, uint32_t n)
+{
+    BVHT_REQUIRE(ctx && blas && tris, "NULL argument");
+    BVHT_REQUIRE(n == blas->n_tris, "triangle count %u differs from the BLAS's %u", n, blas->n_tris);
+    BVHT_HIP(hipMemcpyAsync(blas->d_tris, tris, sizeof(float) * 9 * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    BVHT_HIP(hipStreamSynchronize(ctx->stream));
+    return BVHT_OK;
+}
+
+int32_t bvht_blas_set_vertices_device(bvht_ctx ctx, bvht_blas blas, const float* d_tris, uint32_t n)
+{
+    BVHT_REQUIRE(ctx && blas && d_tris, "NULL argument");
+    BVHT_REQUIRE(n == blas->n_tris, "triangle count %u differs from the BLAS's %u", n, blas->n_tris);
+    BVHT_HIP(hipMemcpyAsync(blas->d_tris, d_tris, sizeof(float) * 9 * (size_t)n, hipMemcpyDeviceToDevice,
+                            ctx->stream));
+    return BVHT_OK;
+}
+
+int32_t bvht_blas_refit(bvht_ctx ctx, bvht_blas blas)
+{
+    BVHT_REQUIRE(ctx && blas, "NULL argument");
+    int32_t rc = bvht::fill_u32(ctx, blas->d_flags, 0, blas->node_count);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_refit, dim3(bvht::blocks_for(blas->nodes_used, 256)), dim3(256), 0, ctx->stream,
+                       blas->d_nodes, blas->d_tris, blas->d_parent, blas->d_flags, blas->nodes_used);
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
+int32_t bvht_blas_bounds(bvht_ctx ctx, bvht_blas blas, float* out6)
+{
+    BVHT_REQUIRE(ctx && blas && out6, "NULL argument");
+    bvht_node root;
+    BVHT_HIP(hipMemcpyAsync(&root, blas->d_nodes, sizeof root, hipMemcpyDeviceToHost, ctx->stream));
+    BVHT_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(out6, root.bmin, 12);
+    std::memcpy(out6 + 3, root.bmax, 12);
+    return BVHT_OK;
+}
+
+int32_t bvht_blas_destroy(bvht_blas b)
+{
+    if (!b)
+        return BVHT_OK;
+    if (b->ctx)
+        (void)hipStreamSynchronize(b->ctx->stream);
+    (void)hipFree(b->d_nodes);
+    (void)hipFree(b->d_tris);
+    (void)hipFree(b->d_parent);
+    (void)hipFree(b->d_flags);
+    (void)hipFree(b->d_leaves);
+    (void)hipFree(b->d_spill);
+    delete b;
+    return BVHT_OK;
+}
+
+int32_t bvht_trace_blas(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, uint32_t n, bvht_hit* d_hits,
+                        bvht_stats* d_stats)
+{
+    BVHT_REQUIRE(ctx && blas, "NULL argument");
+    BVHT_REQUIRE(n == 0 || (d_rays && d_hits), "NULL ray or hit buffer");
+    return bvht::launch_trace_blas(ctx, blas, d_rays, n, nullptr, d_hits, d_stats);
+}
+
+int32_t bvht_trace_blas_count(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, const uint32_t* d_count,
+                              uint32_t max_n, bvht_hit* d_hits, bvht_stats* d_stats)
+{
+    BVHT_REQUIRE(ctx && blas && d_count, "NULL argument");
+    BVHT_REQUIRE(max_n == 0 || (d_rays && d_hits), "NULL ray or hit buffer");
+    return bvht::launch_trace_blas(ctx, blas, d_rays, max_n, d_count, d_hits, d_stats);
+}
+
+int32_t bvht_raygen(bvht_ctx ctx, const bvht_camera* cam, uint32_t width, uint32_t height, int32_t order,
+                    bvht_ray* d_rays)
+{
+    BVHT_REQUIRE(ctx && cam && d_rays, "NULL argument");
+    BVHT_REQUIRE(width > 0 && height > 0 && (uint64_t)width * height < (1ull << 32), "bad image size %ux%u",
+                 width, height);
+    BVHT_REQUIRE(order == BVHT_ORDER_ROWS || (order == BVHT_ORDER_TILES && width % 8 == 0 && height % 8 == 0),
+                 "order %d invalid for %ux%u (tile order needs multiples of 8)", order, width, height);
+    uint32_t n = width * height;
+    hipLaunchKernelGGL(k_raygen, dim3(bvht::blocks_for(n, 256)), dim3(256), 0, ctx->stream, *cam, width, height,
+                       order, d_rays);
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
+int32_t bvht_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, const bvht_hit* d_hits, uint32_t n,
+                    uint64_t seed, bvht_ray* d_out, uint32_t* d_count)
+{
+    BVHT_REQUIRE(ctx && blas && d_count, "NULL argument");
+    BVHT_REQUIRE(n == 0 || (d_rays && d_hits && d_out), "NULL buffer");
+    return bvht::launch_bounce(ctx, blas->d_tris, d_rays, d_hits, n, seed, d_out, d_count);
+}
+
+int32_t bvht_depth(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, float* d_depth)
+{
+    BVHT_REQUIRE(ctx, "ctx is NULL");
+    if (n == 0)
+        return BVHT_OK;
+    BVHT_REQUIRE(d_hits && d_depth, "NULL buffer");
+    hipLaunchKernelGGL(k_depth_acc, dim3(bvht::blocks_for(n, 256)), dim3(256), 0, ctx->stream, d_hits, n, d_depth);
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
+}  // extern "C"

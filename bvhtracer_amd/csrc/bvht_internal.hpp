@@ -1,0 +1,77 @@
+// bvht_internal.hpp -- host-side internals shared by the library's .hip files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/bvht.h"
+
+struct bvht_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // Reusable device scratch (grown on demand; freed with the context).
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+};
+
+struct bvht_blas_s {
+    bvht_ctx ctx = nullptr;
+    uint32_t n_tris = 0;
+    uint32_t node_count = 0;  // length of the node array (2N for built BLASes)
+    uint32_t nodes_used = 0;
+    uint32_t depth = 0;       // number of levels (root alone = 1)
+    bvht_node* d_nodes = nullptr;
+    float* d_tris = nullptr;  // 9 floats per triangle, build order
+    uint32_t* d_parent = nullptr;   // parent of each used node (root: 0xFFFFFFFF)
+    uint32_t* d_leaves = nullptr;   // indices of the leaf nodes
+    uint32_t n_leaves = 0;
+    uint32_t* d_flags = nullptr;    // refit arrival counters, one per node
+    uint32_t* d_spill = nullptr;    // traversal stack spill (grown on demand)
+    size_t spill_words = 0;
+};
+
+struct bvht_tlas_s {
+    bvht_ctx ctx = nullptr;
+    uint32_t n = 0;
+    std::vector<bvht_blas> blases;
+    void* d_blas_table = nullptr;   // per-BLAS {nodes, tris} device pointers
+    bvht_instance* d_objects = nullptr;
+    void* d_rows = nullptr;         // per-object 3x4 rows of M^-1 + BLAS id, as traced
+    float* d_bounds = nullptr;      // n * 6 world bounds
+    bvht_tnode* d_nodes = nullptr;  // 2n
+    int32_t* d_list = nullptr;      // rebuild work list (n)
+    uint32_t* d_meta = nullptr;     // {nodes_used, depth, status}
+    uint32_t depth = 0;             // TLAS levels after the last rebuild
+    uint32_t max_blas_depth = 0;
+    uint32_t* d_spill = nullptr;
+    size_t spill_words = 0;
+};
+
+namespace bvht {
+
+int32_t set_error(int32_t code, const char* fmt, ...);
+int32_t ensure_scratch(bvht_ctx ctx, size_t bytes);
+int32_t compute_blas_topology(bvht_blas b, const bvht_node* h_nodes);
+int32_t ensure_spill(bvht_ctx ctx, uint32_t** buf, size_t* words, size_t need);
+int32_t blas_alloc(bvht_ctx ctx, uint32_t n, uint32_t node_count, bvht_blas* out);
+int32_t fill_u32(bvht_ctx ctx, uint32_t* p, uint32_t v, size_t n);
+int32_t hit_block_offsets(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, uint32_t* d_count, uint32_t** offsets);
+
+}  // namespace bvht
+
+#define BVHT_HIP(expr)                                                                   \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess)                                                            \
+            return bvht::set_error(BVHT_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+#define BVHT_REQUIRE(cond, ...)                              \
+    do {                                                     \
+        if (!(cond))                                         \
+            return bvht::set_error(BVHT_EINVAL, __VA_ARGS__); \
+    } while (0)

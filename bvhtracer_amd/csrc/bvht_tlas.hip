@@ -1,0 +1,511 @@
+// bvht_tlas.hip -- scene objects, the TLAS rebuild and the two-level trace.
+//
+// Objects (SceneObject, scene_object.rs:11-18) carry M, M^-1 and a BLAS id.
+// Their world boxes come from the current BLAS root box through the 8
+// transformed corners (set_transform, scene_object.rs:60-75).  The TLAS is
+// rebuilt by agglomerative clustering (Tlas::rebuild, tlas.rs:179-250),
+// replayed by one workgroup: the clustering loop is inherently sequential,
+// each find_best_match scan is a block-wide argmin.  N is the object count
+// (16 per scene in the instancing workload), so the whole rebuild is a few
+// microseconds.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "bvht_device.hpp"
+#include "bvht_internal.hpp"
+#include "bvht_trace.hpp"
+
+using namespace bvht;
+
+namespace {
+
+inline uint32_t nblk(size_t n, uint32_t wg) { return (uint32_t)((n + wg - 1) / wg); }
+
+// World box of each object from its BLAS root box, and the traced rows of M^-1.
+__global__ void k_object_bounds(const bvht_instance* __restrict__ objs, uint32_t n, const BlasRef* __restrict__ blases,
+                                float* __restrict__ bounds, ObjRows* __restrict__ rows)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const bvht_instance& o = objs[i];
+    const float4* bn = blases[o.blas].nodes;
+    float4 a = bn[0], b = bn[1];
+    V3 lo = mk3(kFltMax, kFltMax, kFltMax), hi = mk3(-kFltMax, -kFltMax, -kFltMax);
+    for (int c = 0; c < 8; ++c) {
+        V3 p = mk3((c & 1) ? b.x : a.x, (c & 2) ? b.y : a.y, (c & 4) ? b.z : a.z);
+        V3 w = mat_mul4(o.m, p, 1.0f);
+        lo = min3(lo, w);
+        hi = max3(hi, w);
+    }
+    float* bo = bounds + 6 * (size_t)i;
+    bo[0] = lo.x; bo[1] = lo.y; bo[2] = lo.z;
+    bo[3] = hi.x; bo[4] = hi.y; bo[5] = hi.z;
+    ObjRows r;
+    r.r0 = make_float4(o.minv[0], o.minv[4], o.minv[8], o.minv[12]);
+    r.r1 = make_float4(o.minv[1], o.minv[5], o.minv[9], o.minv[13]);
+    r.r2 = make_float4(o.minv[2], o.minv[6], o.minv[10], o.minv[14]);
+    r.blas = o.blas;
+    r.pad[0] = r.pad[1] = r.pad[2] = 0;
+    rows[i] = r;
+}
+
+constexpr uint32_t kRebuildWG = 256;
+
+// Tlas::find_best_match (tlas.rs:179-202) as a block-wide argmin: the
+// sequential scan keeps the first b whose area is strictly below the running
+// minimum (starting at f32::MAX), i.e. the smallest index among the minimal
+// areas that are < f32::MAX.
+__device__ int32_t block_best_match(const bvht_tnode* nodes, const int32_t* list, int32_t count, int32_t a,
+                                    float* s_val, int32_t* s_idx)
+{
+    float best = kFltMax;
+    int32_t bi = -1;
+    const bvht_tnode& na = nodes[list[a]];
+    for (int32_t b = threadIdx.x; b < count; b += kRebuildWG) {
+        if (b == a)
+            continue;
+        const bvht_tnode& nb = nodes[list[b]];
+        V3 mx = max3(mk3(na.bmax[0], na.bmax[1], na.bmax[2]), mk3(nb.bmax[0], nb.bmax[1], nb.bmax[2]));
+        V3 mn = min3(mk3(na.bmin[0], na.bmin[1], na.bmin[2]), mk3(nb.bmin[0], nb.bmin[1], nb.bmin[2]));
+        V3 e = sub3(mx, mn);
+        float sa = e.x * e.y + e.y * e.z + e.z * e.x;
+        if (sa < best) {
+            best = sa;
+            bi = b;
+        }
+    }
+    s_val[threadIdx.x] = best;
+    s_idx[threadIdx.x] = bi;
+    __syncthreads();
+    for (uint32_t off = kRebuildWG / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            float v2 = s_val[threadIdx.x + off];
+            int32_t i2 = s_idx[threadIdx.x + off];
+            float v1 = s_val[threadIdx.x];
+            int32_t i1 = s_idx[threadIdx.x];
+            bool take = i2 >= 0 && (i1 < 0 || v2 < v1 || (!(v1 < v2) && i2 < i1));
+            if (take) {
+                s_val[threadIdx.x] = v2;
+                s_idx[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    int32_t r = s_idx[0];
+    __syncthreads();
+    return r;
+}
+
+// Tlas::rebuild (tlas.rs:204-250) + TlasBuilder::build_for (tlas.rs:273-279):
+// 2n default nodes (empty box), leaves 1..n in object order, mutual-best-match
+// merges with LeftRightIndex::new(a, b) = a + (b << 16), root = copy of the
+// last node.  meta = {nodes_used, depth, status}.
+__global__ __launch_bounds__(kRebuildWG) void k_tlas_rebuild(const float* __restrict__ bounds, uint32_t n,
+                                                             bvht_tnode* nodes, int32_t* list, uint32_t* meta)
+{
+    __shared__ float s_val[kRebuildWG];
+    __shared__ int32_t s_idx[kRebuildWG];
+    __shared__ int32_t sh_a, sh_b, sh_count, sh_used, sh_merged, sh_fail;
+    for (uint32_t i = threadIdx.x; i < 2 * n; i += kRebuildWG) {
+        bvht_tnode d;
+        d.bmin[0] = d.bmin[1] = d.bmin[2] = kFltMax;
+        d.bmax[0] = d.bmax[1] = d.bmax[2] = -kFltMax;
+        d.left_right = 0;
+        d.blas = 0;
+        nodes[i] = d;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += kRebuildWG) {
+        list[i] = (int32_t)(i + 1);
+        bvht_tnode l;
+        const float* b = bounds + 6 * (size_t)i;
+        l.bmin[0] = b[0]; l.bmin[1] = b[1]; l.bmin[2] = b[2];
+        l.bmax[0] = b[3]; l.bmax[1] = b[4]; l.bmax[2] = b[5];
+        l.left_right = 0;
+        l.blas = i;
+        nodes[i + 1] = l;
+    }
+    if (threadIdx.x == 0) {
+        sh_a = 0;
+        sh_count = (int32_t)n;
+        sh_used = (int32_t)n + 1;
+        sh_fail = 0;
+    }
+    __syncthreads();
+    int32_t b0 = block_best_match(nodes, list, sh_count, sh_a, s_val, s_idx);
+    if (threadIdx.x == 0)
+        sh_b = b0;
+    __syncthreads();
+    while (sh_count > 1) {
+        if (sh_b < 0) {  // every candidate area was >= f32::MAX (the reference panics)
+            if (threadIdx.x == 0)
+                sh_fail = 1;
+            __syncthreads();
+            break;
+        }
+        int32_t c = block_best_match(nodes, list, sh_count, sh_b, s_val, s_idx);
+        if (threadIdx.x == 0) {
+            int32_t a = sh_a, b = sh_b;
+            if (a == c) {
+                int32_t ia = list[a], ib = list[b];
+                bvht_tnode na = nodes[ia], nb = nodes[ib];
+                bvht_tnode nn;
+                nn.left_right = (uint32_t)ia + ((uint32_t)ib << 16);
+                V3 mn = min3(mk3(na.bmin[0], na.bmin[1], na.bmin[2]), mk3(nb.bmin[0], nb.bmin[1], nb.bmin[2]));
+                V3 mx = max3(mk3(na.bmax[0], na.bmax[1], na.bmax[2]), mk3(nb.bmax[0], nb.bmax[1], nb.bmax[2]));
+                nn.bmin[0] = mn.x; nn.bmin[1] = mn.y; nn.bmin[2] = mn.z;
+                nn.bmax[0] = mx.x; nn.bmax[1] = mx.y; nn.bmax[2] = mx.z;
+                nn.blas = 0;
+                nodes[sh_used] = nn;
+                list[a] = sh_used;
+                sh_used += 1;
+                list[b] = list[sh_count - 1];
+                sh_count -= 1;
+                sh_merged = 1;
+            } else {
+                sh_a = b;
+                sh_b = c;
+                sh_merged = 0;
+            }
+        }
+        __syncthreads();
+        if (sh_merged) {
+            int32_t nb = block_best_match(nodes, list, sh_count, sh_a, s_val, s_idx);
+            if (threadIdx.x == 0)
+                sh_b = nb;
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (!sh_fail) {
+            nodes[0] = nodes[list[sh_a]];
+            // depth: iterative walk, list[] reused as (node, level) stack
+            uint32_t best = 0, sp = 0;
+            uint32_t* st = reinterpret_cast<uint32_t*>(list);
+            st[sp++] = 0;
+            st[sp++] = 1;
+            while (sp) {
+                uint32_t lvl = st[--sp], ni = st[--sp];
+                best = max(best, lvl);
+                uint32_t lr = nodes[ni].left_right;
+                if (lr != 0 && sp + 4 <= 2 * max(n, 4u)) {
+                    st[sp++] = lr >> 16;
+                    st[sp++] = lvl + 1;
+                    st[sp++] = lr & 0xFFFFu;
+                    st[sp++] = lvl + 1;
+                }
+            }
+            meta[0] = (uint32_t)sh_used;
+            meta[1] = best;
+            meta[2] = 0;
+        } else {
+            meta[0] = (uint32_t)sh_used;
+            meta[1] = 0;
+            meta[2] = 1;
+        }
+    }
+}
+
+template <int LDS_DEPTH, bool STATS>
+__global__ __launch_bounds__(kTraceWG) void k_trace_tlas(const float4* __restrict__ tnodes,
+                                                         const ObjRows* __restrict__ rows,
+                                                         const BlasRef* __restrict__ blases,
+                                                         const bvht_ray* __restrict__ rays,
+                                                         bvht_hit* __restrict__ hits, uint32_t n,
+                                                         const uint32_t* __restrict__ d_n,
+                                                         bvht_stats* __restrict__ stats,
+                                                         uint32_t* __restrict__ spill, uint32_t cap)
+{
+    __shared__ uint32_t lds[LDS_DEPTH * kTraceWG];
+    uint32_t i = blockIdx.x * kTraceWG + threadIdx.x;
+    uint32_t count = d_n ? min(*d_n, n) : n;
+    if (i >= count)
+        return;
+    Stack<LDS_DEPTH> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap};
+    DRay ray = load_ray(rays, i);
+    uint32_t pairs = 0, tests = 0, tpairs = 0, visits = 0;
+    TlasResult r = traverse_tlas<LDS_DEPTH, STATS>(tnodes, rows, blases, ray, st, pairs, tests, tpairs, visits);
+    store_hit(hits, i, r.found, r.t, r.u, r.v, r.prim, r.inst);
+    if (STATS) {
+        bvht_stats s;
+        s.pairs = pairs;
+        s.tests = tests;
+        s.tlas_pairs = tpairs;
+        s.inst_visits = visits;
+        stats[i] = s;
+    }
+}
+
+// Bounce rays for TLAS hits: the geometric normal of the model-space
+// triangle is carried to world space by the transpose of M^-1.
+__global__ __launch_bounds__(256) void k_bounce_write_tlas(const ObjRows* __restrict__ rows,
+                                                           const BlasRef* __restrict__ blases,
+                                                           const bvht_ray* __restrict__ rays,
+                                                           const bvht_hit* __restrict__ hits, uint32_t n,
+                                                           const uint32_t* __restrict__ block_offsets, uint64_t seed,
+                                                           bvht_ray* __restrict__ out)
+{
+    __shared__ uint32_t wave_counts[4];
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    bvht_hit h;
+    bool hit = false;
+    if (i < n) {
+        h = hits[i];
+        hit = h.prim != BVHT_MISS;
+    }
+    uint64_t m = __ballot(hit);
+    uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        wave_counts[wave] = __popcll(m);
+    __syncthreads();
+    uint32_t base = block_offsets[blockIdx.x];
+    for (uint32_t w = 0; w < wave; ++w)
+        base += wave_counts[w];
+    uint32_t rank = __popcll(m & ((1ull << lane) - 1ull));
+    if (!hit)
+        return;
+    ObjRows ob = rows[h.inst];
+    const float* tri = blases[ob.blas].tris + 9 * (size_t)h.prim;
+    V3 v0 = mk3(tri[0], tri[1], tri[2]);
+    V3 e1 = sub3(mk3(tri[3], tri[4], tri[5]), v0);
+    V3 e2 = sub3(mk3(tri[6], tri[7], tri[8]), v0);
+    V3 nm = cross3(e1, e2);
+    V3 nw = mk3(ob.r0.x * nm.x + ob.r1.x * nm.y + ob.r2.x * nm.z, ob.r0.y * nm.x + ob.r1.y * nm.y + ob.r2.y * nm.z,
+                ob.r0.z * nm.x + ob.r1.z * nm.y + ob.r2.z * nm.z);
+    DRay r = load_ray(rays, i);
+    float4 a, b;
+    bounce_ray(nw, add3(r.o, scl3(r.d, h.t)), r.d, seed, i, a, b);
+    float4* o = reinterpret_cast<float4*>(out + base + rank);
+    o[0] = a;
+    o[1] = b;
+}
+
+void tlas_free(bvht_tlas t)
+{
+    (void)hipFree(t->d_blas_table);
+    (void)hipFree(t->d_objects);
+    (void)hipFree(t->d_rows);
+    (void)hipFree(t->d_bounds);
+    (void)hipFree(t->d_nodes);
+    (void)hipFree(t->d_list);
+    (void)hipFree(t->d_meta);
+    (void)hipFree(t->d_spill);
+    delete t;
+}
+
+int32_t validate_objects(bvht_tlas t, const bvht_instance* objs, uint32_t count)
+{
+    for (uint32_t i = 0; i < count; ++i)
+        if (objs[i].blas >= t->blases.size())
+            return set_error(BVHT_EINVAL, "object %u references BLAS %u of %zu", i, objs[i].blas, t->blases.size());
+    return BVHT_OK;
+}
+
+int32_t refresh(bvht_tlas t)
+{
+    hipLaunchKernelGGL(k_object_bounds, dim3(nblk(t->n, 64)), dim3(64), 0, t->ctx->stream, t->d_objects, t->n,
+                       static_cast<const BlasRef*>(t->d_blas_table), t->d_bounds, static_cast<ObjRows*>(t->d_rows));
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
+int32_t rebuild(bvht_tlas t, bool read_depth)
+{
+    hipLaunchKernelGGL(k_tlas_rebuild, dim3(1), dim3(kRebuildWG), 0, t->ctx->stream, t->d_bounds, t->n, t->d_nodes,
+                       t->d_list, t->d_meta);
+    BVHT_HIP(hipGetLastError());
+    if (read_depth) {
+        uint32_t meta[3];
+        BVHT_HIP(hipMemcpyAsync(meta, t->d_meta, sizeof meta, hipMemcpyDeviceToHost, t->ctx->stream));
+        BVHT_HIP(hipStreamSynchronize(t->ctx->stream));
+        if (meta[2])
+            return set_error(BVHT_EINVAL, "TLAS rebuild found no finite merge candidate (non-finite object bounds)");
+        t->depth = meta[1];
+    }
+    return BVHT_OK;
+}
+
+int32_t launch_trace_tlas(bvht_tlas t, const bvht_ray* d_rays, uint32_t n, const uint32_t* d_n, bvht_hit* d_hits,
+                          bvht_stats* d_stats)
+{
+    if (n == 0)
+        return BVHT_OK;
+    bvht_ctx ctx = t->ctx;
+    uint32_t need = (t->depth > 0 ? t->depth - 1 : 0) + (t->max_blas_depth > 0 ? t->max_blas_depth - 1 : 0);
+    int lds_depth = need <= 16 ? 16 : 32;
+    uint32_t* spill = nullptr;
+    if (need > (uint32_t)lds_depth) {
+        int32_t rc = ensure_spill(ctx, &t->d_spill, &t->spill_words, (size_t)(need - lds_depth) * n);
+        if (rc)
+            return rc;
+        spill = t->d_spill;
+    }
+    const float4* tn = reinterpret_cast<const float4*>(t->d_nodes);
+    const ObjRows* rows = static_cast<const ObjRows*>(t->d_rows);
+    const BlasRef* br = static_cast<const BlasRef*>(t->d_blas_table);
+    dim3 grid(nblk(n, kTraceWG)), block(kTraceWG);
+#define LAUNCH(D, S)                                                                                          \
+    hipLaunchKernelGGL((k_trace_tlas<D, S>), grid, block, 0, ctx->stream, tn, rows, br, d_rays, d_hits, n, d_n, \
+                       d_stats, spill, need)
+    if (d_stats) {
+        if (lds_depth == 16) LAUNCH(16, true);
+        else LAUNCH(32, true);
+    } else {
+        if (lds_depth == 16) LAUNCH(16, false);
+        else LAUNCH(32, false);
+    }
+#undef LAUNCH
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t bvht_tlas_create(bvht_ctx ctx, const bvht_blas* blases, uint32_t blas_count, const bvht_instance* objects,
+                         uint32_t n, bvht_tlas* out)
+{
+    BVHT_REQUIRE(ctx && out && blases && objects, "NULL argument");
+    BVHT_REQUIRE(n > 0, "empty object list (the reference panics on an empty TLAS)");
+    BVHT_REQUIRE(n <= 32767, "%u objects exceed the 16-bit LeftRightIndex (at most 32767)", n);
+    BVHT_REQUIRE(blas_count > 0, "no BLAS given");
+    bvht_tlas t = new (std::nothrow) bvht_tlas_s();
+    if (!t)
+        return set_error(BVHT_ENOMEM, "host allocation failed");
+    t->ctx = ctx;
+    t->n = n;
+    std::vector<BlasRef> table(blas_count);
+    for (uint32_t i = 0; i < blas_count; ++i) {
+        if (!blases[i]) {
+            delete t;
+            return set_error(BVHT_EINVAL, "BLAS %u is NULL", i);
+        }
+        t->blases.push_back(blases[i]);
+        table[i].nodes = reinterpret_cast<const float4*>(blases[i]->d_nodes);
+        table[i].tris = blases[i]->d_tris;
+        t->max_blas_depth = std::max(t->max_blas_depth, blases[i]->depth);
+    }
+    int32_t rc = validate_objects(t, objects, n);
+    if (rc) {
+        delete t;
+        return rc;
+    }
+    bool ok = hipMalloc(&t->d_blas_table, sizeof(BlasRef) * blas_count) == hipSuccess &&
+              hipMalloc(&t->d_objects, sizeof(bvht_instance) * n) == hipSuccess &&
+              hipMalloc(&t->d_rows, sizeof(ObjRows) * n) == hipSuccess &&
+              hipMalloc(&t->d_bounds, sizeof(float) * 6 * n) == hipSuccess &&
+              hipMalloc(&t->d_nodes, sizeof(bvht_tnode) * 2 * n) == hipSuccess &&
+              hipMalloc(&t->d_list, sizeof(int32_t) * std::max<uint32_t>(n, 4) * 2) == hipSuccess &&
+              hipMalloc(&t->d_meta, sizeof(uint32_t) * 4) == hipSuccess;
+    if (!ok) {
+        tlas_free(t);
+        return set_error(BVHT_ENOMEM, "device allocation for a TLAS of %u objects failed", n);
+    }
+    hipStream_t st = ctx->stream;
+    if (hipMemcpyAsync(t->d_blas_table, table.data(), sizeof(BlasRef) * blas_count, hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(t->d_objects, objects, sizeof(bvht_instance) * n, hipMemcpyHostToDevice, st) != hipSuccess) {
+        tlas_free(t);
+        return set_error(BVHT_EDEVICE, "TLAS upload failed");
+    }
+    rc = refresh(t);
+    if (!rc)
+        rc = rebuild(t, true);
+    if (rc) {
+        tlas_free(t);
+        return rc;
+    }
+    *out = t;
+    return BVHT_OK;
+}
+
+int32_t bvht_tlas_set_objects(bvht_ctx ctx, bvht_tlas tlas, const bvht_instance* objects, uint32_t first,
+                              uint32_t count)
+{
+    BVHT_REQUIRE(ctx && tlas && objects, "NULL argument");
+    BVHT_REQUIRE((uint64_t)first + count <= tlas->n, "objects [%u, +%u) out of range (%u)", first, count, tlas->n);
+    int32_t rc = validate_objects(tlas, objects, count);
+    if (rc)
+        return rc;
+    BVHT_HIP(hipMemcpyAsync(tlas->d_objects + first, objects, sizeof(bvht_instance) * count, hipMemcpyHostToDevice,
+                            ctx->stream));
+    BVHT_HIP(hipStreamSynchronize(ctx->stream));
+    return refresh(tlas);
+}
+
+int32_t bvht_tlas_refresh_bounds(bvht_ctx ctx, bvht_tlas tlas)
+{
+    BVHT_REQUIRE(ctx && tlas, "NULL argument");
+    return refresh(tlas);
+}
+
+int32_t bvht_tlas_rebuild(bvht_ctx ctx, bvht_tlas tlas)
+{
+    BVHT_REQUIRE(ctx && tlas, "NULL argument");
+    // the BLAS depths may have changed if a BLAS was rebuilt; refits keep them
+    for (bvht_blas b : tlas->blases)
+        tlas->max_blas_depth = std::max(tlas->max_blas_depth, b->depth);
+    return rebuild(tlas, true);
+}
+
+int32_t bvht_tlas_download(bvht_ctx ctx, bvht_tlas tlas, bvht_tnode* nodes, uint32_t* nodes_used)
+{
+    BVHT_REQUIRE(ctx && tlas, "NULL argument");
+    if (nodes)
+        BVHT_HIP(hipMemcpyAsync(nodes, tlas->d_nodes, sizeof(bvht_tnode) * 2 * tlas->n, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    uint32_t meta[3] = {0, 0, 0};
+    BVHT_HIP(hipMemcpyAsync(meta, tlas->d_meta, sizeof meta, hipMemcpyDeviceToHost, ctx->stream));
+    BVHT_HIP(hipStreamSynchronize(ctx->stream));
+    if (nodes_used)
+        *nodes_used = meta[0];
+    return BVHT_OK;
+}
+
+int32_t bvht_tlas_destroy(bvht_tlas tlas)
+{
+    if (!tlas)
+        return BVHT_OK;
+    (void)hipStreamSynchronize(tlas->ctx->stream);
+    tlas_free(tlas);
+    return BVHT_OK;
+}
+
+int32_t bvht_trace_tlas(bvht_ctx ctx, bvht_tlas tlas, const bvht_ray* d_rays, uint32_t n, bvht_hit* d_hits,
+                        bvht_stats* d_stats)
+{
+    BVHT_REQUIRE(ctx && tlas, "NULL argument");
+    BVHT_REQUIRE(n == 0 || (d_rays && d_hits), "NULL ray or hit buffer");
+    return launch_trace_tlas(tlas, d_rays, n, nullptr, d_hits, d_stats);
+}
+
+int32_t bvht_trace_tlas_count(bvht_ctx ctx, bvht_tlas tlas, const bvht_ray* d_rays, const uint32_t* d_count,
+                              uint32_t max_n, bvht_hit* d_hits, bvht_stats* d_stats)
+{
+    BVHT_REQUIRE(ctx && tlas && d_count, "NULL argument");
+    BVHT_REQUIRE(max_n == 0 || (d_rays && d_hits), "NULL ray or hit buffer");
+    return launch_trace_tlas(tlas, d_rays, max_n, d_count, d_hits, d_stats);
+}
+
+int32_t bvht_bounce_tlas(bvht_ctx ctx, bvht_tlas tlas, const bvht_ray* d_rays, const bvht_hit* d_hits, uint32_t n,
+                         uint64_t seed, bvht_ray* d_out, uint32_t* d_count)
+{
+    BVHT_REQUIRE(ctx && tlas && d_count, "NULL argument");
+    BVHT_REQUIRE(n == 0 || (d_rays && d_hits && d_out), "NULL buffer");
+    uint32_t* bc = nullptr;
+    int32_t rc = hit_block_offsets(ctx, d_hits, n, d_count, &bc);
+    if (rc || n == 0)
+        return rc;
+    hipLaunchKernelGGL(k_bounce_write_tlas, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream,
+                       static_cast<const ObjRows*>(tlas->d_rows), static_cast<const BlasRef*>(tlas->d_blas_table),
+                       d_rays, d_hits, n, bc, seed, d_out);
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,204 @@
+// bvht_trace.hpp -- the per-lane traversal loops (BLAS and two-level TLAS).
+#pragma once
+
+#include "bvht_device.hpp"
+
+namespace bvht {
+
+constexpr int kTraceWG = 256;
+
+// LDS (address space 3) pointer, so stack accesses compile to ds_read/ds_write
+// instead of flat accesses.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+// Per-lane traversal stack: the first LDS_DEPTH entries live in LDS at
+// lds[slot * kTraceWG] (lds already offset by the lane), deeper entries in a
+// per-ray column of a global buffer (spill[(slot - LDS_DEPTH) * stride]).
+// `cap` bounds the total; a push beyond it is dropped rather than written out
+// of bounds (cap is sized from the tree depth, so this never happens for a
+// structurally valid tree).
+template <int LDS_DEPTH>
+struct Stack {
+    lds_u32* lds;
+    uint32_t* spill;
+    size_t stride;
+    uint32_t cap;
+
+    __device__ __forceinline__ void push(uint32_t& sp, uint32_t v)
+    {
+        if (sp < (uint32_t)LDS_DEPTH) {
+            lds[sp * kTraceWG] = v;
+        } else {
+            if (sp >= cap)
+                return;
+            spill[(size_t)(sp - LDS_DEPTH) * stride] = v;
+        }
+        ++sp;
+    }
+    __device__ __forceinline__ uint32_t pop(uint32_t& sp)
+    {
+        --sp;
+        return sp < (uint32_t)LDS_DEPTH ? lds[sp * kTraceWG] : spill[(size_t)(sp - LDS_DEPTH) * stride];
+    }
+};
+
+struct BlasResult {
+    float t, u, v;
+    uint32_t prim;
+    bool found;
+};
+
+// Bvh::intersect_subtree (bvh.rs:242-305) for one ray, starting at node 0.
+// The stack is shared with an enclosing TLAS traversal: this call only uses
+// slots at or above sp_base and leaves the stack as it found it.
+template <int LDS_DEPTH, bool STATS>
+__device__ __forceinline__ BlasResult traverse_blas(const float4* __restrict__ nodes,
+                                                    const float* __restrict__ tris, const DRay& ray,
+                                                    Stack<LDS_DEPTH>& st, uint32_t sp_base, uint32_t& pairs,
+                                                    uint32_t& tests)
+{
+    BlasResult res{0.0f, 0.0f, 0.0f, 0u, false};
+    float closest = ray.t;
+    uint32_t sp = sp_base;
+    uint32_t lf = __float_as_uint(nodes[0].w);
+    uint32_t cnt = __float_as_uint(nodes[1].w);
+    while (true) {
+        if (cnt > 0) {
+            for (uint32_t k = 0; k < cnt; ++k) {
+                float t, u, v;
+                if (STATS)
+                    ++tests;
+                if (tri_test(tris + 9 * (size_t)(lf + k), ray, t, u, v) && t < closest) {
+                    closest = t;
+                    res.t = t;
+                    res.u = u;
+                    res.v = v;
+                    res.prim = lf + k;
+                    res.found = true;
+                }
+            }
+        } else {
+            if (STATS)
+                ++pairs;
+            const float4* c = nodes + 2 * (size_t)lf;
+            float4 a0 = c[0], a1 = c[1], b0 = c[2], b1 = c[3];
+            float dl, dr;
+            bool hl = slab(a0, a1, ray, closest, dl);
+            bool hr = slab(b0, b1, ray, closest, dr);
+            float kl = hl ? dl : kFltMax;
+            float kr = hr ? dr : kFltMax;
+            bool left_near = kl < kr;
+            bool near_hit = left_near ? hl : hr;
+            if (near_hit) {
+                bool far_hit = left_near ? hr : hl;
+                if (far_hit)
+                    st.push(sp, left_near ? lf + 1 : lf);
+                uint32_t nlf = __float_as_uint(left_near ? a0.w : b0.w);
+                cnt = __float_as_uint(left_near ? a1.w : b1.w);
+                lf = nlf;
+                continue;
+            }
+        }
+        if (sp == sp_base)
+            break;
+        uint32_t idx = st.pop(sp);
+        lf = __float_as_uint(nodes[2 * (size_t)idx].w);
+        cnt = __float_as_uint(nodes[2 * (size_t)idx + 1].w);
+    }
+    if (!(closest < kFltMax))
+        res.found = false;
+    return res;
+}
+
+// Per-BLAS device pointers used by the TLAS kernels.
+struct BlasRef {
+    const float4* nodes;
+    const float* tris;
+};
+
+// Object record as the TLAS kernels read it: the 3x4 row block of M^-1
+// ({m[0][i], m[1][i], m[2][i], m[3][i]} for rows i = 0..2) and the BLAS id.
+struct ObjRows {
+    float4 r0, r1, r2;
+    uint32_t blas;
+    uint32_t pad[3];
+};
+
+struct TlasResult {
+    float t, u, v;
+    uint32_t prim, inst;
+    bool found;
+};
+
+// Tlas::intersect (tlas.rs:123-177): the TLAS node loop with
+// SceneObject::intersect at the leaves (scene_object.rs:77-89).  Left child =
+// high 16 bits of the LeftRightIndex word (tlas.rs:24-32).
+template <int LDS_DEPTH, bool STATS>
+__device__ __forceinline__ TlasResult traverse_tlas(const float4* __restrict__ tnodes, const ObjRows* __restrict__ objs,
+                                                    const BlasRef* __restrict__ blases, const DRay& ray,
+                                                    Stack<LDS_DEPTH>& st, uint32_t& pairs, uint32_t& tests,
+                                                    uint32_t& tpairs, uint32_t& visits)
+{
+    TlasResult res{0.0f, 0.0f, 0.0f, 0u, 0u, false};
+    float closest = ray.t;
+    uint32_t sp = 0;
+    uint32_t cur = 0;
+    float4 c0 = tnodes[0], c1 = tnodes[1];
+    uint32_t lr = __float_as_uint(c0.w), blas_id = __float_as_uint(c1.w);
+    while (true) {
+        if (lr == 0) {
+            if (STATS)
+                ++visits;
+            const ObjRows* ob = objs + blas_id;
+            float4 q0 = ob->r0, q1 = ob->r1, q2 = ob->r2;
+            uint32_t model = ob->blas;
+            V3 om = rows_mul(q0, q1, q2, ray.o, 1.0f);
+            V3 dm = rows_mul(q0, q1, q2, ray.d, 0.0f);
+            DRay rm = make_ray(om, dm, closest);
+            BlasRef br = blases[model];
+            BlasResult r = traverse_blas<LDS_DEPTH, STATS>(br.nodes, br.tris, rm, st, sp, pairs, tests);
+            if (r.found && r.t < closest) {
+                closest = r.t;
+                res.t = r.t;
+                res.u = r.u;
+                res.v = r.v;
+                res.prim = r.prim;
+                res.inst = blas_id;
+                res.found = true;
+            }
+        } else {
+            if (STATS)
+                ++tpairs;
+            uint32_t li = (lr & 0xFFFF0000u) >> 16;
+            uint32_t ri = lr & 0x0000FFFFu;
+            float4 a0 = tnodes[2 * li], a1 = tnodes[2 * li + 1];
+            float4 b0 = tnodes[2 * ri], b1 = tnodes[2 * ri + 1];
+            float dl, dr;
+            bool hl = slab(a0, a1, ray, closest, dl);
+            bool hr = slab(b0, b1, ray, closest, dr);
+            float kl = hl ? dl : kFltMax;
+            float kr = hr ? dr : kFltMax;
+            bool left_near = kl < kr;
+            bool near_hit = left_near ? hl : hr;
+            if (near_hit) {
+                bool far_hit = left_near ? hr : hl;
+                if (far_hit)
+                    st.push(sp, left_near ? ri : li);
+                cur = left_near ? li : ri;
+                lr = __float_as_uint(left_near ? a0.w : b0.w);
+                blas_id = __float_as_uint(left_near ? a1.w : b1.w);
+                continue;
+            }
+        }
+        if (sp == 0)
+            break;
+        cur = st.pop(sp);
+        lr = __float_as_uint(tnodes[2 * cur].w);
+        blas_id = __float_as_uint(tnodes[2 * cur + 1].w);
+    }
+    if (!(closest < kFltMax))
+        res.found = false;
+    return res;
+}
+
+}  // namespace bvht

@@ -1,0 +1,249 @@
+/*
+ * bvht.h -- C ABI of the MI355X (gfx950) ray/BVH traversal library
+ * (libbvht_hip.so).  Drop-in boundary for the traversal hot path of the
+ * reference crate lambdaxymox/bvhtracer (Rust, CPU-only).  The reference has
+ * no FFI of its own; every entry point below names the reference API it
+ * replaces (paths relative to the reference repository root).
+ *
+ * Conventions
+ *   - Every function returns int32_t status: BVHT_OK (0) or a negative code;
+ *     bvht_last_error() gives a message.  Nothing panics, aborts or throws
+ *     across the ABI (the reference panics on empty meshes and TLASes,
+ *     bvhtracer/src/model/bvh.rs:530-535, bvhtracer/src/scene/tlas.rs:248).
+ *   - The unit of work is a batch of rays.  Ray and hit buffers passed to
+ *     trace entry points are DEVICE pointers (HBM-resident); triangle and
+ *     node arrays passed to build/upload/download are HOST pointers.
+ *   - All work is enqueued on the context's HIP stream; bvht_sync() or a
+ *     read-back is the fence.  One context per host thread.
+ *   - Arithmetic is IEEE binary32 in the reference's operation order, no FMA
+ *     contraction, correctly rounded division: hits are bit-identical to the
+ *     reference algorithm on the same BVH and rays.
+ */
+#ifndef BVHT_H
+#define BVHT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BVHT_OK 0
+#define BVHT_EINVAL (-1)   /* bad argument, empty input, size limit */
+#define BVHT_ENOMEM (-2)   /* device or host allocation failed */
+#define BVHT_EDEVICE (-3)  /* HIP runtime error */
+#define BVHT_ESTACK (-4)   /* traversal stack bound exceeded */
+
+#define BVHT_MISS 0xFFFFFFFFu
+
+/* ray orders produced by bvht_raygen */
+#define BVHT_ORDER_ROWS 0  /* index y*W + x */
+#define BVHT_ORDER_TILES 1 /* 8x8 tiles in PathTracer order: index ((ty*W/8 + tx)*64 + (y%8)*8 + x%8) */
+
+/* BLAS build modes */
+#define BVHT_BUILD_REFERENCE 0 /* BvhBuilder::build_for exactly (bvh.rs:333-541) */
+#define BVHT_BUILD_SAH 1       /* binned SAH with empty-initialised bins and a leaf cap */
+
+/* ---- records (byte layouts shared with hosts on both sides) ---------- */
+
+/* 32-byte BLAS node, bvh.rs:129-134: aabb, left_first (first primitive of a
+ * leaf / left child of a branch; the right child is left_first + 1), count
+ * (leaf iff > 0). */
+typedef struct {
+    float bmin[3];
+    uint32_t left_first;
+    float bmax[3];
+    uint32_t count;
+} bvht_node;
+
+/* 32-byte TLAS node, tlas.rs:41-46: aabb, LeftRightIndex word (leaf iff 0;
+ * left child = high 16 bits, right child = low 16 bits), scene-object id. */
+typedef struct {
+    float bmin[3];
+    uint32_t left_right;
+    float bmax[3];
+    uint32_t blas;
+} bvht_tnode;
+
+/* Ray, query/ray.rs:8-35: origin, t (f32::MAX for camera rays), direction.
+ * The reciprocal direction is recomputed on device (IEEE 1/d). */
+typedef struct {
+    float o[3];
+    float tmax;
+    float d[3];
+    float pad;
+} bvht_ray;
+
+/* Hit: t,u,v of the accepted SurfaceInteraction (query/intersection.rs:8-26),
+ * the true primitive index (position in the build-permuted triangle array)
+ * and scene-object index, and the reference's packed InstancePrimitiveIndex
+ * word (instance bits always 0 as in the reference, intersection.rs:41-47).
+ * Miss: t = FLT_MAX, u = v = 0, prim = inst = packed = BVHT_MISS. */
+typedef struct {
+    float t, u, v;
+    uint32_t prim;
+    uint32_t inst;
+    uint32_t packed;
+} bvht_hit;
+
+/* Scene object (scene_object.rs:11-18): transform M and its cached inverse
+ * (column-major 4x4, m[col*4+row], transform_component.rs:7-27) and the
+ * BLAS index (into the array passed to bvht_tlas_create). */
+typedef struct {
+    float m[16];
+    float minv[16];
+    uint32_t blas;
+    uint32_t pad[3];
+} bvht_instance;
+
+/* Pinhole camera (camera.rs:994-1010): eye-space viewport corners at the
+ * near plane and the eye-to-world matrix (column-major). */
+typedef struct {
+    float tl[3];
+    float tr[3];
+    float bl[3];
+    float view_inv[16];
+} bvht_camera;
+
+/* Per-ray work counters (optional instrumentation of the trace kernels). */
+typedef struct {
+    uint32_t pairs;       /* BLAS interior nodes expanded (child-pair fetches) */
+    uint32_t tests;       /* triangle tests */
+    uint32_t tlas_pairs;  /* TLAS interior nodes expanded */
+    uint32_t inst_visits; /* TLAS leaves visited (instance transforms) */
+} bvht_stats;
+
+typedef struct bvht_ctx_s* bvht_ctx;
+typedef struct bvht_blas_s* bvht_blas;
+typedef struct bvht_tlas_s* bvht_tlas;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Create a context on HIP device `device` with its own stream. */
+int32_t bvht_create(int32_t device, bvht_ctx* out);
+int32_t bvht_destroy(bvht_ctx ctx);
+/* Message for the last failing call on this thread ("" if none). */
+const char* bvht_last_error(void);
+int32_t bvht_sync(bvht_ctx ctx);
+/* The context's hipStream_t, for callers that order their own work on it. */
+void* bvht_stream(bvht_ctx ctx);
+/* Device memory helpers for hosts without their own allocator. */
+int32_t bvht_device_alloc(bvht_ctx ctx, size_t bytes, void** out);
+int32_t bvht_device_free(bvht_ctx ctx, void* ptr);
+int32_t bvht_copy_to_device(bvht_ctx ctx, void* dst, const void* src, size_t bytes);
+int32_t bvht_copy_to_host(bvht_ctx ctx, void* dst, const void* src, size_t bytes);
+
+/* HIP events on the context's stream, for timing enqueued work. */
+int32_t bvht_event_create(bvht_ctx ctx, void** out);
+int32_t bvht_event_record(bvht_ctx ctx, void* ev);
+/* Waits for `end`, then *ms = elapsed milliseconds between the two events. */
+int32_t bvht_event_elapsed(void* start, void* end, float* ms);
+int32_t bvht_event_destroy(void* ev);
+
+/* ---- BLAS ------------------------------------------------------------- */
+
+/* BvhBuilder::new().build_for(&mut [Triangle<f32>]) (bvh.rs:509-541) on the
+ * GPU.  tris: host float[9*n] (v0,v1,v2 per triangle), permuted IN PLACE into
+ * the build order like the reference's slice.  mode BVHT_BUILD_REFERENCE
+ * reproduces the reference node array and permutation (max_leaf ignored);
+ * BVHT_BUILD_SAH starts bins empty and splits every node with more than
+ * max_leaf primitives (median split when no plane separates them). */
+int32_t bvht_blas_build(bvht_ctx ctx, float* tris, uint32_t n, int32_t mode, uint32_t max_leaf,
+                        bvht_blas* out);
+/* Adopt an existing BVH (node array of node_count >= nodes_used nodes, and
+ * its triangle array in build order).  Used to trace a CPU-built BVH. */
+int32_t bvht_blas_upload(bvht_ctx ctx, const bvht_node* nodes, uint32_t node_count,
+                         uint32_t nodes_used, const float* tris, uint32_t n, bvht_blas* out);
+/* Copy back nodes[node_count] (2n for built BLASes) and tris[9n]; either may be NULL. */
+int32_t bvht_blas_download(bvht_ctx ctx, bvht_blas blas, bvht_node* nodes, float* tris);
+int32_t bvht_blas_info(bvht_blas blas, uint32_t* tri_count, uint32_t* node_count,
+                       uint32_t* nodes_used, uint32_t* depth);
+/* Replace the triangle vertices (host float[9n], build order), as
+ * Model::primitives_mut() does before ModelInstance::refit (model.rs:31-33,87-89). */
+int32_t bvht_blas_set_vertices(bvht_ctx ctx, bvht_blas blas, const float* tris, uint32_t n);
+/* Same from device memory (float[9n]). */
+int32_t bvht_blas_set_vertices_device(bvht_ctx ctx, bvht_blas blas, const float* d_tris, uint32_t n);
+/* Bvh::refit (bvh.rs:469-493) on the GPU. */
+int32_t bvht_blas_refit(bvht_ctx ctx, bvht_blas blas);
+/* Bvh::bounds (bvh.rs:499-501): out[6] = bmin, bmax of node 0 (host). */
+int32_t bvht_blas_bounds(bvht_ctx ctx, bvht_blas blas, float* out6);
+int32_t bvht_blas_destroy(bvht_blas blas);
+
+/* ---- traversal -------------------------------------------------------- */
+
+/* Bvh::intersect (bvh.rs:307-309, intersect_subtree :242-305) for a batch:
+ * d_rays[n] -> d_hits[n].  d_stats (device, may be NULL) receives per-ray
+ * work counters. */
+int32_t bvht_trace_blas(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, uint32_t n,
+                        bvht_hit* d_hits, bvht_stats* d_stats);
+
+/* Same, with the ray count read from device memory (*d_count, clamped to
+ * max_n) so a frame can be enqueued without a host round trip. */
+int32_t bvht_trace_blas_count(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays,
+                              const uint32_t* d_count, uint32_t max_n, bvht_hit* d_hits,
+                              bvht_stats* d_stats);
+
+/* ---- frame loop ------------------------------------------------------- */
+
+/* Camera::get_ray_world for every pixel (camera.rs:994-1010) with the
+ * per-pixel mapping u = x/W, v = y/H of PathTracer::evaluate
+ * (renderer.rs:345-368).  order BVHT_ORDER_TILES writes the rays in the
+ * PathTracer's 8x8-tile order (renderer.rs:353-361; W and H multiples of 8),
+ * so one wave64 traces one tile. */
+int32_t bvht_raygen(bvht_ctx ctx, const bvht_camera* cam, uint32_t width, uint32_t height,
+                    int32_t order, bvht_ray* d_rays);
+/* One cosine-weighted diffuse bounce per hit (no reference counterpart):
+ * origin = o + t d + 1e-4 n, n the geometric normal of the hit triangle
+ * flipped against d, direction from a PCG32 stream seeded by (seed, ray
+ * index).  Misses produce no ray; bounce rays are packed in ray order.
+ * *d_count (device uint32) receives the number of bounce rays.  The BLAS
+ * provides the triangles; for a TLAS use bvht_bounce_tlas. */
+int32_t bvht_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, const bvht_hit* d_hits,
+                    uint32_t n, uint64_t seed, bvht_ray* d_out, uint32_t* d_count);
+
+/* Depth accumulator (renderer.rs:176-194): d_depth[i] = hit t or f32::MAX. */
+int32_t bvht_depth(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, float* d_depth);
+
+/* ---- TLAS / instances ------------------------------------------------- */
+
+/* TlasBuilder::new().build_for(&[SceneObject]) (tlas.rs:253-279): objects
+ * are set with their transforms (SceneObjectBuilder::with_transform, world
+ * bounds from the current BLAS bounds, scene_object.rs:109-126) and the TLAS
+ * is built.  blases[] are the models the objects reference by index. */
+int32_t bvht_tlas_create(bvht_ctx ctx, const bvht_blas* blases, uint32_t blas_count,
+                         const bvht_instance* objects, uint32_t n, bvht_tlas* out);
+/* SceneObject::set_transform for objects [first, first+count) (host
+ * records; scene_object.rs:60-75): stores M, M^-1 and recomputes the world
+ * bounds from the referenced BLAS's current bounds.  Bounds are NOT refreshed
+ * by a BLAS refit alone, as in the reference. */
+int32_t bvht_tlas_set_objects(bvht_ctx ctx, bvht_tlas tlas, const bvht_instance* objects,
+                              uint32_t first, uint32_t count);
+/* Device-side variant: refresh every object's world bounds from its current
+ * M and the current BLAS bounds (set_transform with the stored transform). */
+int32_t bvht_tlas_refresh_bounds(bvht_ctx ctx, bvht_tlas tlas);
+/* Tlas::rebuild / Scene::rebuild (tlas.rs:204-250, scene.rs:36-38) on the GPU. */
+int32_t bvht_tlas_rebuild(bvht_ctx ctx, bvht_tlas tlas);
+/* Copy back nodes[2n] and nodes_used. */
+int32_t bvht_tlas_download(bvht_ctx ctx, bvht_tlas tlas, bvht_tnode* nodes, uint32_t* nodes_used);
+int32_t bvht_tlas_destroy(bvht_tlas tlas);
+/* Tlas::intersect / Scene::intersect (tlas.rs:123-177, scene.rs:32-34) with
+ * SceneObject::intersect at the leaves (scene_object.rs:77-89).  hit.inst is
+ * the true object index; hit.packed keeps the reference's instance bits 0. */
+int32_t bvht_trace_tlas(bvht_ctx ctx, bvht_tlas tlas, const bvht_ray* d_rays, uint32_t n,
+                        bvht_hit* d_hits, bvht_stats* d_stats);
+/* Same, with the ray count read from device memory (clamped to max_n). */
+int32_t bvht_trace_tlas_count(bvht_ctx ctx, bvht_tlas tlas, const bvht_ray* d_rays,
+                              const uint32_t* d_count, uint32_t max_n, bvht_hit* d_hits,
+                              bvht_stats* d_stats);
+/* Diffuse bounce rays for TLAS hits (see bvht_bounce); the normal is carried
+ * to world space by the transpose of the object's M^-1. */
+int32_t bvht_bounce_tlas(bvht_ctx ctx, bvht_tlas tlas, const bvht_ray* d_rays,
+                         const bvht_hit* d_hits, uint32_t n, uint64_t seed, bvht_ray* d_out,
+                         uint32_t* d_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,255 @@
+"""CPU oracle for the ray/BVH traversal hot path -- TEST INFRASTRUCTURE ONLY.
+
+A plain-C restatement (``bvht_oracle.c``) of the reference crate's CPU
+algorithms, loaded through ctypes.  Only the test-suite, the smoke check and
+the CPU-baseline leg of the benchmark use it, always as the checker or as the
+timed CPU path; the product library never loads it.
+
+Record layouts (numpy dtypes below) are byte-identical to the product's C ABI
+so the same buffers can be handed to both.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+NODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("left_first", "<u4"), ("bmax", "<f4", 3), ("count", "<u4")])
+TNODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("left_right", "<u4"), ("bmax", "<f4", 3), ("blas", "<u4")])
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmax", "<f4"), ("d", "<f4", 3), ("pad", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim", "<u4"), ("inst", "<u4"), ("packed", "<u4")])
+INSTANCE_DTYPE = np.dtype([("minv", "<f4", 16), ("blas", "<u4"), ("pad", "<u4", 3)])
+STATS_DTYPE = np.dtype([("pairs", "<u4"), ("tests", "<u4"), ("tlas_pairs", "<u4"), ("inst_visits", "<u4")])
+CAMERA_DTYPE = np.dtype([("tl", "<f4", 3), ("tr", "<f4", 3), ("bl", "<f4", 3), ("view_inv", "<f4", 16)])
+
+assert NODE_DTYPE.itemsize == 32 and TNODE_DTYPE.itemsize == 32
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 24 and INSTANCE_DTYPE.itemsize == 80
+
+_lib = None
+
+
+def build(force=False):
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+
+    if force and os.path.exists(_LIB_PATH):
+        os.remove(_LIB_PATH)
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run `make -C oracle`)")
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        u32, u64, i32, f32 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_float
+        sig = {
+            "orc_normalize": (None, [P, P]),
+            "orc_ray_recip": (None, [P, P]),
+            "orc_aabb_intersect": (i32, [P, P, P, P]),
+            "orc_triangle_intersect": (i32, [P, P, P, P, P]),
+            "orc_triangle_centroid": (None, [P, P]),
+            "orc_pack_index": (u32, [u32, u32]),
+            "orc_bvh_build": (i32, [P, u32, i32, u32, P, P]),
+            "orc_bvh_refit": (i32, [P, u32, P]),
+            "orc_bvh_depth": (u32, [P]),
+            "orc_bvh_intersect": (i32, [P, P, P, P, P]),
+            "orc_bvh_trace": (i32, [P, P, P, u64, P, P, i32]),
+            "orc_brute_trace": (i32, [P, u32, P, u64, P, P]),
+            "orc_transform_point": (None, [P, P, P]),
+            "orc_transform_vector": (None, [P, P, P]),
+            "orc_instance_bounds": (None, [P, P, P, P, P]),
+            "orc_tlas_build": (i32, [P, u32, P, P]),
+            "orc_tlas_intersect": (i32, [P, P, P, P, P, P, P]),
+            "orc_tlas_trace": (i32, [P, P, P, P, P, u64, P, P, i32]),
+            "orc_camera_ray": (None, [P, f32, f32, P]),
+            "orc_raygen": (i32, [P, u32, u32, P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f3(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float32).reshape(3))
+
+
+def make_ray(o, d, tmax=np.finfo(np.float32).max):
+    r = np.zeros(1, RAY_DTYPE)
+    r["o"][0] = _f3(o)
+    r["d"][0] = _f3(d)
+    r["tmax"][0] = np.float32(tmax)
+    return r
+
+
+def normalize(v):
+    out = np.zeros(3, np.float32)
+    lib().orc_normalize(_p(_f3(v)), _p(out))
+    return out
+
+
+def ray_recip(d):
+    out = np.zeros(3, np.float32)
+    lib().orc_ray_recip(_p(_f3(d)), _p(out))
+    return out
+
+
+def aabb_intersect(bmin, bmax, ray):
+    t = np.zeros(1, np.float32)
+    hit = lib().orc_aabb_intersect(_p(_f3(bmin)), _p(_f3(bmax)), _p(ray), _p(t))
+    return float(t[0]) if hit else None
+
+
+def triangle_intersect(tri, ray):
+    tri = np.ascontiguousarray(np.asarray(tri, np.float32).reshape(9))
+    out = np.zeros(3, np.float32)
+    hit = lib().orc_triangle_intersect(_p(tri), _p(ray), _p(out[0:]), _p(out[1:]), _p(out[2:]))
+    return (np.float32(out[0]), np.float32(out[1]), np.float32(out[2])) if hit else None
+
+
+def centroid(tri):
+    tri = np.ascontiguousarray(np.asarray(tri, np.float32).reshape(9))
+    out = np.zeros(3, np.float32)
+    lib().orc_triangle_centroid(_p(tri), _p(out))
+    return out
+
+
+def pack_index(inst, prim):
+    return int(lib().orc_pack_index(inst, prim))
+
+
+def bvh_build(tris, faithful=True, max_leaf=0):
+    """Returns (nodes[2N], nodes_used, permuted_tris[N,9]); the input is not modified."""
+    t = np.ascontiguousarray(np.asarray(tris, np.float32).reshape(-1, 9)).copy()
+    n = t.shape[0]
+    nodes = np.zeros(max(2 * n, 2), NODE_DTYPE)
+    used = np.zeros(1, np.uint32)
+    rc = lib().orc_bvh_build(_p(t), n, 1 if faithful else 0, max_leaf, _p(nodes), _p(used))
+    if rc != 0:
+        raise ValueError(f"orc_bvh_build failed rc={rc}")
+    return nodes, int(used[0]), t
+
+
+def bvh_refit(nodes, nodes_used, tris):
+    tris = np.ascontiguousarray(np.asarray(tris, np.float32).reshape(-1, 9))
+    rc = lib().orc_bvh_refit(_p(nodes), nodes_used, _p(tris))
+    if rc != 0:
+        raise ValueError(f"orc_bvh_refit failed rc={rc}")
+    return nodes
+
+
+def bvh_depth(nodes):
+    return int(lib().orc_bvh_depth(_p(nodes)))
+
+
+def bvh_trace(nodes, tris, rays, nthreads=1, stats=False):
+    tris = np.ascontiguousarray(np.asarray(tris, np.float32).reshape(-1, 9))
+    rays = np.ascontiguousarray(rays)
+    assert rays.dtype == RAY_DTYPE
+    hits = np.zeros(rays.shape[0], HIT_DTYPE)
+    st = np.zeros(rays.shape[0], STATS_DTYPE) if stats else None
+    rc = lib().orc_bvh_trace(_p(nodes), _p(tris), _p(rays), rays.shape[0], _p(hits),
+                             _p(st) if stats else None, nthreads)
+    if rc != 0:
+        raise ValueError(f"orc_bvh_trace failed rc={rc}")
+    return (hits, st) if stats else hits
+
+
+def brute_trace(tris, rays):
+    """Nearest hit over all triangles in array order (no BVH); returns (hits, ties)."""
+    tris = np.ascontiguousarray(np.asarray(tris, np.float32).reshape(-1, 9))
+    rays = np.ascontiguousarray(rays)
+    hits = np.zeros(rays.shape[0], HIT_DTYPE)
+    ties = np.zeros(rays.shape[0], np.uint32)
+    rc = lib().orc_brute_trace(_p(tris), tris.shape[0], _p(rays), rays.shape[0], _p(hits), _p(ties))
+    if rc != 0:
+        raise ValueError(f"orc_brute_trace failed rc={rc}")
+    return hits, ties
+
+
+def transform_point(m, p):
+    m = np.ascontiguousarray(np.asarray(m, np.float32).reshape(16))
+    out = np.zeros(3, np.float32)
+    lib().orc_transform_point(_p(m), _p(_f3(p)), _p(out))
+    return out
+
+
+def transform_vector(m, v):
+    m = np.ascontiguousarray(np.asarray(m, np.float32).reshape(16))
+    out = np.zeros(3, np.float32)
+    lib().orc_transform_vector(_p(m), _p(_f3(v)), _p(out))
+    return out
+
+
+def instance_bounds(m, model_bmin, model_bmax):
+    m = np.ascontiguousarray(np.asarray(m, np.float32).reshape(16))
+    bmin = np.zeros(3, np.float32)
+    bmax = np.zeros(3, np.float32)
+    lib().orc_instance_bounds(_p(m), _p(_f3(model_bmin)), _p(_f3(model_bmax)), _p(bmin), _p(bmax))
+    return bmin, bmax
+
+
+def tlas_build(bounds):
+    """bounds: (N, 6) float32 rows (bmin, bmax).  Returns (tnodes[2N], nodes_used)."""
+    b = np.ascontiguousarray(np.asarray(bounds, np.float32).reshape(-1, 6))
+    n = b.shape[0]
+    nodes = np.zeros(max(2 * n, 2), TNODE_DTYPE)
+    used = np.zeros(1, np.uint32)
+    rc = lib().orc_tlas_build(_p(b), n, _p(nodes), _p(used))
+    if rc != 0:
+        raise ValueError(f"orc_tlas_build failed rc={rc}")
+    return nodes, int(used[0])
+
+
+def _blas_ptr_arrays(blases):
+    """blases: list of (nodes, tris).  Returns ctypes pointer arrays + keep-alive list."""
+    keep = []
+    nptr = (ctypes.c_void_p * len(blases))()
+    tptr = (ctypes.c_void_p * len(blases))()
+    for i, (nodes, tris) in enumerate(blases):
+        tris = np.ascontiguousarray(np.asarray(tris, np.float32).reshape(-1, 9))
+        nodes = np.ascontiguousarray(nodes)
+        keep += [nodes, tris]
+        nptr[i] = nodes.ctypes.data
+        tptr[i] = tris.ctypes.data
+    return nptr, tptr, keep
+
+
+def tlas_trace(tnodes, instances, blases, rays, nthreads=1, stats=False):
+    rays = np.ascontiguousarray(rays)
+    instances = np.ascontiguousarray(instances)
+    nptr, tptr, keep = _blas_ptr_arrays(blases)
+    hits = np.zeros(rays.shape[0], HIT_DTYPE)
+    st = np.zeros(rays.shape[0], STATS_DTYPE) if stats else None
+    rc = lib().orc_tlas_trace(_p(tnodes), _p(instances), ctypes.cast(nptr, ctypes.c_void_p),
+                              ctypes.cast(tptr, ctypes.c_void_p), _p(rays), rays.shape[0], _p(hits),
+                              _p(st) if stats else None, nthreads)
+    del keep
+    if rc != 0:
+        raise ValueError(f"orc_tlas_trace failed rc={rc}")
+    return (hits, st) if stats else hits
+
+
+def camera_ray(cam, u, v):
+    out = np.zeros(1, RAY_DTYPE)
+    lib().orc_camera_ray(_p(cam), np.float32(u), np.float32(v), _p(out))
+    return out
+
+
+def raygen(cam, width, height):
+    rays = np.zeros(width * height, RAY_DTYPE)
+    rc = lib().orc_raygen(_p(cam), width, height, _p(rays))
+    if rc != 0:
+        raise ValueError("orc_raygen failed")
+    return rays

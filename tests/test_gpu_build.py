@@ -1,0 +1,127 @@
+"""GPU BLAS build and refit parity: node arrays, nodes_used and the triangle
+permutation equal the oracle's (bvh.rs:333-541, 469-493)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import bvhtracer_amd as B
+from bvhtracer_amd import scenes
+from helpers import assert_hits_equal, assert_nodes_equal
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def unit_triangle():
+    s3 = np.float32(1) / np.sqrt(np.float32(3))
+    return np.array([0, 0.5, 0, -s3, -0.5, 0, s3, -0.5, 0], np.float32)
+
+
+def cube_mesh():
+    with open(os.path.join(HERE, "golden", "cube_obj.json")) as f:
+        g = json.load(f)
+    v = np.array(g["vertices"], np.float32)
+    return v[np.array(g["faces"]) - 1].reshape(-1, 9).astype(np.float32)
+
+
+def meshes():
+    base = unit_triangle().reshape(3, 3)
+    diag = np.stack([(base + np.float32(i) * np.array([5, 5, 0], np.float32)).reshape(9) for i in range(-100, 100)])
+    stacked = np.stack([(base - np.array([0, 0, i], np.float32)).reshape(9) for i in range(100)])
+    same = np.tile(unit_triangle(), (40, 1))  # identical centroids: no usable axis
+    flat = scenes.triangle_soup(300, seed=8)
+    flat[:, 2::3] = 0.0  # every z is 0: the z axis is skipped
+    return {
+        "one": unit_triangle().reshape(1, 9),
+        "two": np.stack([(base - [5, 0, 0]).reshape(9), (base + [5, 0, 0]).reshape(9)]).astype(np.float32),
+        "cube": cube_mesh(),
+        "diagonal": diag,
+        "stacked": stacked,
+        "identical": same,
+        "flat": flat,
+        "soup33": scenes.triangle_soup(33, seed=4),
+        "soup1000": scenes.triangle_soup(1000, seed=6),
+        "soup": scenes.triangle_soup(),
+        "sphere16": scenes.icosphere(16, radius=32.0),
+    }
+
+
+@pytest.mark.parametrize("mode,max_leaf", [(B.BUILD_REFERENCE, 0), (B.BUILD_SAH, 2), (B.BUILD_SAH, 0),
+                                           (B.BUILD_SAH, 7)])
+@pytest.mark.parametrize("name", list(meshes().keys()))
+def test_build_matches_oracle(ctx, orc, name, mode, max_leaf):
+    mesh = meshes()[name]
+    faithful = mode == B.BUILD_REFERENCE
+    nodes, used, perm = orc.bvh_build(mesh, faithful, max_leaf)
+    m = np.ascontiguousarray(mesh.copy())
+    bvh = B.BvhBuilder(mode, max_leaf, ctx=ctx).build_for(m)
+    assert m.tobytes() == perm.tobytes(), "triangle permutation (written back in place) differs"
+    g_nodes, g_used, g_tris = bvh.download()
+    assert g_used == used == bvh.nodes_used()
+    assert_nodes_equal(g_nodes, nodes, used)
+    assert g_tris.tobytes() == perm.tobytes()
+    assert bvh.depth() == orc.bvh_depth(nodes)
+
+
+def test_cube_ids_through_gpu_build(ctx, orc):
+    """IDs 6 and 9 of bvhtracer/tests/test_scene_cube.rs through the GPU build."""
+    m = cube_mesh()
+    bvh = B.BvhBuilder(B.BUILD_REFERENCE, ctx=ctx).build_for(m)
+    for target, prim in [((0.5, 1.0, -0.5), 6), ((-0.5, 1.0, 0.5), 9)]:
+        o = np.array([0, 4, 0], np.float32)
+        t = np.array(target, np.float32)
+        # model space of the x2 scale, -1 translation object: o' = (o + 1) / 2
+        om = (o + np.float32(1)) * np.float32(0.5)
+        d = orc.normalize(t - o) * np.float32(0.5)
+        h = bvh.intersect(B.Ray.from_origin_dir(om, d))[0]
+        assert h["prim"] == prim
+
+
+@pytest.mark.parametrize("mode,max_leaf", [(B.BUILD_REFERENCE, 0), (B.BUILD_SAH, 2)])
+def test_sphere_1m_build_matches_oracle(ctx, orc, mode, max_leaf):
+    """Build parity at the 1,310,720-triangle size of the incoherent-ray workload."""
+    mesh = scenes.icosphere(256, radius=32.0)
+    faithful = mode == B.BUILD_REFERENCE
+    nodes, used, perm = orc.bvh_build(mesh, faithful, max_leaf)
+    m = mesh.copy()
+    bvh = B.BvhBuilder(mode, max_leaf, ctx=ctx).build_for(m)
+    g_nodes, g_used, g_tris = bvh.download()
+    assert g_used == used
+    assert_nodes_equal(g_nodes, nodes, used)
+    assert g_tris.tobytes() == perm.tobytes()
+
+
+@pytest.mark.parametrize("mode,max_leaf", [(B.BUILD_REFERENCE, 0), (B.BUILD_SAH, 2)])
+def test_refit_matches_oracle(ctx, orc, mode, max_leaf):
+    mesh = scenes.icosphere(40, radius=10.0)
+    m = mesh.copy()
+    bvh = B.BvhBuilder(mode, max_leaf, ctx=ctx).build_for(m)
+    nodes, used, perm = bvh.download()
+    for frame in (0, 5, 63):
+        deformed = scenes.deform_wave(perm, frame)
+        bvh.refit(deformed)
+        orc.bvh_refit(nodes, used, deformed)
+        g_nodes, g_used, _ = bvh.download()
+        assert_nodes_equal(g_nodes, nodes, used)
+    # refit on unchanged vertices is the identity (bvhtracer/tests/test_bvh_refit.rs:46-53)
+    before = bvh.download()[0]
+    bvh.refit()
+    assert bvh.download()[0].tobytes() == before.tobytes()
+
+
+def test_build_then_trace(ctx, orc):
+    mesh = scenes.triangle_soup()
+    m = mesh.copy()
+    bvh = B.BvhBuilder(B.BUILD_SAH, 2, ctx=ctx).build_for(m)
+    nodes, used, perm = orc.bvh_build(mesh, False, 2)
+    rays = orc.raygen(scenes.soup_camera(), 256, 256)
+    prays = np.zeros(rays.shape[0], B.RAY_DTYPE)
+    prays.view(np.uint8)[:] = rays.view(np.uint8)
+    assert_hits_equal(bvh.intersect(prays), orc.bvh_trace(nodes, perm, rays, nthreads=8), "built")
+
+
+def test_build_rejects_empty(ctx):
+    with pytest.raises(B.BvhtError):
+        B.BvhBuilder(ctx=ctx).build_for(np.zeros((0, 9), np.float32))

@@ -1,0 +1,136 @@
+"""GPU traversal parity: libbvht_hip.so against the oracle on identical BVHs
+and rays, bit for bit (t, u, v compared as raw binary32 words)."""
+import numpy as np
+import pytest
+
+import bvhtracer_amd as B
+from bvhtracer_amd import scenes
+from helpers import assert_hits_equal, comb_tree, oracle_rays_to_product, upload
+
+pytestmark = pytest.mark.gpu
+f32 = np.float32
+
+
+@pytest.mark.parametrize("faithful,max_leaf", [(True, 0), (False, 2)])
+def test_soup_primary_parity(ctx, orc, faithful, max_leaf):
+    """12,582-triangle soup, 640x640 camera rays (the CPU-runnable workload)."""
+    mesh = scenes.triangle_soup()
+    nodes, used, perm = orc.bvh_build(mesh, faithful, max_leaf)
+    bvh = upload(ctx, nodes, used, perm)
+    rays = orc.raygen(scenes.soup_camera(), 640, 640)
+    ref, st_ref = orc.bvh_trace(nodes, perm, rays, nthreads=8, stats=True)
+    got, st = bvh.intersect(oracle_rays_to_product(rays), stats=True)
+    assert_hits_equal(got, ref, "soup primary")
+    assert np.array_equal(st.view(np.uint32), st_ref.view(np.uint32)), "work counters differ"
+    assert (got["prim"] != B.MISS).sum() > 10000
+
+
+@pytest.mark.parametrize("order", [B.ORDER_ROWS, B.ORDER_TILES])
+def test_raygen_parity(ctx, orc, order):
+    cam = scenes.soup_camera()
+    ref = orc.raygen(cam, 640, 640)
+    d = B.raygen(ctx, cam, 640, 640, order)
+    got = d.download(B.RAY_DTYPE, 640 * 640)
+    if order == B.ORDER_TILES:
+        j = np.arange(640 * 640)
+        tile, w = j // 64, j % 64
+        x = (tile % 80) * 8 + w % 8
+        y = (tile // 80) * 8 + w // 8
+        ref = ref[y * 640 + x]
+    assert got.tobytes() == oracle_rays_to_product(ref).tobytes()
+
+
+def random_rays(n, seed, lo=-6.0, hi=6.0):
+    rng = scenes.Pcg32(seed, np.arange(n, dtype=np.uint64))
+    o = np.stack([rng.next_f32() for _ in range(3)], 1) * f32(hi - lo) + f32(lo)
+    d = np.stack([rng.next_f32() for _ in range(3)], 1) * f32(2) - f32(1)
+    r = np.zeros(n, B.RAY_DTYPE)
+    r["o"], r["d"] = o, d
+    r["tmax"] = np.where(np.arange(n) % 4 == 0, f32(3.0), np.finfo(np.float32).max)
+    return r
+
+
+def test_incoherent_rays_parity(ctx, orc):
+    mesh = scenes.triangle_soup(8000, seed=17)
+    nodes, used, perm = orc.bvh_build(mesh, False, 2)
+    bvh = upload(ctx, nodes, used, perm)
+    rays = random_rays(200000, 5)
+    ref = orc.bvh_trace(nodes, perm, rays.view(orc.RAY_DTYPE), nthreads=8)
+    assert_hits_equal(bvh.intersect(rays), ref, "incoherent")
+
+
+def test_axis_aligned_and_degenerate_rays(ctx, orc):
+    """Zero direction components (1/d = +-inf, NaN slabs), -0 components,
+    tiny tmax, rays starting inside boxes, and triangles under the 1e-4
+    determinant threshold."""
+    mesh = scenes.triangle_soup(500, seed=1)
+    mesh[:20, 6:9] = mesh[:20, 3:6]  # degenerate (zero area) triangles
+    nodes, used, perm = orc.bvh_build(mesh, False, 2)
+    bvh = upload(ctx, nodes, used, perm)
+    rays = []
+    for o in [(0, 0, -20), (0.5, -0.25, -20), (-3, 2, 0), (0, 0, 0), (1e-3, 1e-3, -1e-3)]:
+        for d in [(0, 0, 1), (0, 0, -1), (1, 0, 0), (0, -1, 0), (-0.0, 0.0, 1), (0, 1, 1), (1, 1, 1), (1e-20, 0, 1)]:
+            for t in [np.finfo(np.float32).max, 1e-5, 7.5, np.inf]:
+                r = np.zeros(1, B.RAY_DTYPE)
+                r["o"][0], r["d"][0], r["tmax"][0] = o, d, t
+                rays.append(r)
+    rays = np.concatenate(rays)
+    ref = orc.bvh_trace(nodes, perm, rays.view(orc.RAY_DTYPE))
+    assert_hits_equal(bvh.intersect(rays), ref, "edge rays")
+
+
+def test_deep_tree_uses_spill(ctx, orc):
+    nodes, used, tris = comb_tree(150)
+    depth = orc.bvh_depth(nodes)
+    assert depth == 150  # deeper than the largest LDS stack (64 entries)
+    bvh = upload(ctx, nodes, used, tris)
+    assert bvh.depth() == depth
+    n = 4096
+    k = np.arange(n)
+    rays = np.zeros(n, B.RAY_DTYPE)
+    rays["o"] = np.stack([np.full(n, -5.0), (k % 64) / 64.0 - 0.5, (k // 64) / 64.0 - 0.5], 1)
+    rays["d"] = np.stack([np.ones(n), np.full(n, 1e-3), np.full(n, -1e-3)], 1)
+    rays["tmax"] = np.where(k % 2 == 0, np.finfo(np.float32).max, f32(40.5))
+    ref, st = orc.bvh_trace(nodes, tris, rays.view(orc.RAY_DTYPE), stats=True)
+    assert st["pairs"].max() >= 149
+    assert_hits_equal(bvh.intersect(rays), ref, "deep tree")
+
+
+def test_empty_and_single(ctx, orc):
+    tri = np.array([[0, 0.5, 0, -0.5773503, -0.5, 0, 0.5773503, -0.5, 0]], np.float32)
+    nodes, used, perm = orc.bvh_build(tri)
+    bvh = upload(ctx, nodes, used, perm)
+    assert bvh.intersect(np.zeros(0, B.RAY_DTYPE)).shape == (0,)
+    r = B.Ray.from_origin_dir((0, 0, 5), (0, 0, -1))
+    h = bvh.intersect(r)[0]
+    assert h["prim"] == 0 and h["t"] == f32(5) and h["packed"] == 0
+    with pytest.raises(B.BvhtError):
+        B.Bvh.from_arrays(np.zeros(0, B.NODE_DTYPE), 0, np.zeros((0, 9), np.float32), ctx=ctx)
+    bad = nodes.copy()
+    bad[0]["count"] = 5  # leaf range beyond the triangle array: rejected on the host
+    with pytest.raises(B.BvhtError):
+        B.Bvh.from_arrays(bad, used, perm, ctx=ctx)
+
+
+def test_bounce_rays_parity(ctx, orc):
+    """Primary pass on the GPU, bounce rays generated on the GPU, then both
+    passes traced by the GPU and by the oracle on the same buffers."""
+    mesh = scenes.icosphere(24, radius=32.0)
+    nodes, used, perm = orc.bvh_build(mesh, False, 2)
+    bvh = upload(ctx, nodes, used, perm)
+    W, H = 320, 184
+    cam = scenes.sphere_camera(32.0, W, H)
+    d_rays = B.raygen(ctx, cam, W, H, B.ORDER_TILES)
+    d_hits = ctx.alloc(W * H * B.HIT_DTYPE.itemsize)
+    bvh.intersect_device(d_rays, W * H, d_hits)
+    rays = d_rays.download(B.RAY_DTYPE, W * H)
+    hits = d_hits.download(B.HIT_DTYPE, W * H)
+    assert_hits_equal(hits, orc.bvh_trace(nodes, perm, rays.view(orc.RAY_DTYPE), nthreads=8), "primary")
+    d_b, d_cnt = B.bounce(ctx, bvh, d_rays, d_hits, W * H, seed=1234)
+    cnt = int(d_cnt.download(np.uint32, 1)[0])
+    assert cnt == (hits["prim"] != B.MISS).sum() > 1000
+    brays = d_b.download(B.RAY_DTYPE, cnt)
+    # bounce origins sit 1e-4 off the surface on the incoming side
+    assert np.isfinite(brays["o"]).all() and np.allclose(np.linalg.norm(brays["d"], axis=1), 1, atol=1e-5)
+    got = bvh.intersect(brays)
+    assert_hits_equal(got, orc.bvh_trace(nodes, perm, brays.view(orc.RAY_DTYPE), nthreads=8), "bounce")
