@@ -454,7 +454,7 @@ __global__ __launch_bounds__(kWG) void k_part_scatter(const float4* __restrict__
                                                       const uint32_t* __restrict__ chunk_left,
                                                       uint32_t* __restrict__ kpos, uint32_t* __restrict__ sel,
                                                       float4* __restrict__ oc, float4* __restrict__ olo,
-                                                      float2* __restrict__ ohi)
+                                                      float2* __restrict__ ohi, uint32_t* __restrict__ err)
 {
     __shared__ uint32_t wc[4];
     Chunk ch = chunks[blockIdx.x];
@@ -505,23 +505,48 @@ __global__ __launch_bounds__(kWG) void k_part_scatter(const float4* __restrict__
             }
             continue;
         }
+        // every index below is range-checked: a violation flags the build as
+        // failed instead of touching memory outside the node's range
         if (!PHASE_B) {
-            if (k < a && !f)
-                kpos[first + (k - lb)] = k;
-            else if (k >= a && f)
-                sel[first + (nl - lb - 1)] = n - 1 - k;
+            if (k < a && !f) {
+                uint32_t mm = k - lb;
+                if (mm < n)
+                    kpos[first + mm] = k;
+                else
+                    atomicOr(err, 1u);
+            } else if (k >= a && f) {
+                uint32_t mm = nl - lb - 1;
+                if (mm < n)
+                    sel[first + mm] = n - 1 - k;
+                else
+                    atomicOr(err, 1u);
+            }
             continue;
         }
-        uint32_t dest;
+        uint32_t dest = n;
         if (k < a) {
             if (f) {
                 dest = k;
             } else {
                 uint32_t mm = k - lb;
-                dest = mm == 0 ? n - 1 : n - 2 - sel[first + mm - 1];
+                if (mm == 0) {
+                    dest = n - 1;
+                } else if (mm - 1 < n) {
+                    uint32_t sv = sel[first + mm - 1];
+                    dest = sv + 2 <= n ? n - 2 - sv : n;
+                }
             }
         } else {
-            dest = f ? kpos[first + (nl - lb - 1)] : k - 1;
+            if (f) {
+                uint32_t mm = nl - lb - 1;
+                dest = mm < n ? kpos[first + mm] : n;
+            } else {
+                dest = k - 1;
+            }
+        }
+        if (dest >= n) {
+            atomicOr(err, 2u);
+            continue;
         }
         lo = elo[e];
         hi = ehi[e];
@@ -589,7 +614,7 @@ __device__ __forceinline__ float comp4(float4 c, int axis) { return axis == 0 ? 
 __global__ void k_small_build(float4* __restrict__ ec, float4* __restrict__ elo, float2* __restrict__ ehi,
                               const Task* __restrict__ roots, const uint32_t* __restrict__ out_base, uint32_t nroots,
                               SNode* __restrict__ snodes, uint32_t* __restrict__ pairs_out, int faithful,
-                              uint32_t max_leaf)
+                              uint32_t max_leaf, uint32_t* __restrict__ err)
 {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nroots)
@@ -687,6 +712,10 @@ __global__ void k_small_build(float4* __restrict__ ec, float4* __restrict__ elo,
             }
             out[side ? rr : l] = SNode{{lo.x, lo.y, lo.z}, ranges[side][0], {hi.x, hi.y, hi.z}, ranges[side][1]};
         }
+        if (sp + 2 > kSmall + 2 || rr >= 2 * rt.count) {
+            atomicOr(err, 4u);
+            break;
+        }
         out[j].lf = p;
         out[j].count = 0;
         stack[sp++] = rr;
@@ -699,7 +728,8 @@ __global__ void k_small_build(float4* __restrict__ ec, float4* __restrict__ elo,
 // 2 + 2*(rank + p).  Internal nodes point at their pair's final index.
 __global__ void k_small_emit(const SNode* __restrict__ snodes, const uint32_t* __restrict__ out_base,
                              const uint32_t* __restrict__ pairs, const uint32_t* __restrict__ root_final,
-                             const uint32_t* __restrict__ rank, uint32_t nroots, bvht_node* __restrict__ nodes)
+                             const uint32_t* __restrict__ rank, uint32_t nroots, bvht_node* __restrict__ nodes,
+                             uint32_t node_count, uint32_t* __restrict__ err)
 {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nroots)
@@ -714,7 +744,29 @@ __global__ void k_small_emit(const SNode* __restrict__ snodes, const uint32_t* _
         o.bmax[0] = sn.bmax[0]; o.bmax[1] = sn.bmax[1]; o.bmax[2] = sn.bmax[2];
         o.count = sn.count;
         o.left_first = sn.count > 0 ? sn.lf : 2 + 2 * (rk + sn.lf);
+        if (fj >= node_count) {
+            atomicOr(err, 8u);
+            continue;
+        }
         nodes[fj] = o;
+    }
+}
+
+// Structural check of the finished node array (the same rules the upload
+// path checks on the host): children after their parent and inside
+// nodes_used, leaf ranges inside the triangle array.
+__global__ void k_validate_tree(const bvht_node* __restrict__ nodes, uint32_t nodes_used, uint32_t n,
+                                uint32_t* __restrict__ err)
+{
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nodes_used || k == 1)
+        return;
+    bvht_node nd = nodes[k];
+    if (nd.count > 0) {
+        if ((uint64_t)nd.left_first + nd.count > n)
+            atomicOr(err, 32u);
+    } else if (nd.left_first <= k || nd.left_first == 1 || nd.left_first + 1 >= nodes_used) {
+        atomicOr(err, 64u);
     }
 }
 
@@ -728,12 +780,16 @@ __global__ void k_scatter_nodes(const bvht_node* __restrict__ src, const uint32_
 
 // Build order triangle array: tris_out[k] = tris_in[index of element k].
 __global__ void k_gather_tris(const float* __restrict__ tin, const float4* __restrict__ ec, uint32_t n,
-                              float* __restrict__ tout)
+                              float* __restrict__ tout, uint32_t* __restrict__ err)
 {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n)
         return;
     uint32_t src = __float_as_uint(ec[k].w);
+    if (src >= n) {
+        atomicOr(err, 16u);
+        return;
+    }
     const float* a = tin + 9 * (size_t)src;
     float* b = tout + 9 * (size_t)k;
 #pragma unroll
@@ -812,7 +868,9 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
         ~Guard() { if (*b) bvht_blas_destroy(*b); }
     } guard{&b};
 
-    DevBuf tin, ec, elo, ehi, oc, olo, ohi, kpos, sel;
+    DevBuf tin, ec, elo, ehi, oc, olo, ohi, kpos, sel, d_err;
+    RC(d_err.alloc(sizeof(uint32_t)));
+    RC(fill(st, d_err.as<uint32_t>(), 0u, 1));
     RC(tin.alloc(sizeof(float) * 9 * (size_t)n));
     RC(ec.alloc(sizeof(float4) * (size_t)n));
     RC(elo.alloc(sizeof(float4) * (size_t)n));
@@ -910,11 +968,11 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
         hipLaunchKernelGGL((k_part_scatter<false>), dim3(C), dim3(kWG), 0, st, ec.as<float4>(), elo.as<float4>(),
                            ehi.as<float2>(), d_chunks.as<Chunk>(), d_tasks.as<Task>(), d_dec.as<Decision>(),
                            d_chunk_left.as<uint32_t>(), kpos.as<uint32_t>(), sel.as<uint32_t>(), oc.as<float4>(),
-                           olo.as<float4>(), ohi.as<float2>());
+                           olo.as<float4>(), ohi.as<float2>(), d_err.as<uint32_t>());
         hipLaunchKernelGGL((k_part_scatter<true>), dim3(C), dim3(kWG), 0, st, ec.as<float4>(), elo.as<float4>(),
                            ehi.as<float2>(), d_chunks.as<Chunk>(), d_tasks.as<Task>(), d_dec.as<Decision>(),
                            d_chunk_left.as<uint32_t>(), kpos.as<uint32_t>(), sel.as<uint32_t>(), oc.as<float4>(),
-                           olo.as<float4>(), ohi.as<float2>());
+                           olo.as<float4>(), ohi.as<float2>(), d_err.as<uint32_t>());
         hipLaunchKernelGGL(k_part_commit, dim3(C), dim3(kWG), 0, st, oc.as<float4>(), olo.as<float4>(),
                            ohi.as<float2>(), d_chunks.as<Chunk>(), d_tasks.as<Task>(), d_dec.as<Decision>(),
                            ec.as<float4>(), elo.as<float4>(), ehi.as<float2>(), d_child.as<uint32_t>());
@@ -991,7 +1049,7 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
         BVHT_HIP(hipMemcpyAsync(d_rbase.p, rbase.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_small_build, dim3(nblk(R, 64)), dim3(64), 0, st, ec.as<float4>(), elo.as<float4>(),
                            ehi.as<float2>(), d_rt.as<Task>(), d_rbase.as<uint32_t>(), R, d_snodes.as<SNode>(),
-                           d_pairs.as<uint32_t>(), faithful, max_leaf);
+                           d_pairs.as<uint32_t>(), faithful, max_leaf, d_err.as<uint32_t>());
         BVHT_HIP(hipGetLastError());
         BVHT_HIP(hipMemcpyAsync(pairs.data(), d_pairs.p, sizeof(uint32_t) * R, hipMemcpyDeviceToHost, st));
         BVHT_HIP(hipStreamSynchronize(st));
@@ -1042,6 +1100,8 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
                 o.left_first = hn.first;
                 o.count = hn.count;
             }
+            if (final_idx[h] >= 2 * n)
+                return set_error(BVHT_EDEVICE, "BLAS build: node index %u out of range", final_idx[h]);
             big_nodes.push_back(o);
             big_idx.push_back(final_idx[h]);
         }
@@ -1066,12 +1126,20 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
         BVHT_HIP(hipMemcpyAsync(d_rrank.p, rrank.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_small_emit, dim3(nblk(R, 64)), dim3(64), 0, st, d_snodes.as<SNode>(),
                            d_rbase.as<uint32_t>(), d_pairs.as<uint32_t>(), d_rfinal.as<uint32_t>(),
-                           d_rrank.as<uint32_t>(), R, b->d_nodes);
+                           d_rrank.as<uint32_t>(), R, b->d_nodes, 2 * n, d_err.as<uint32_t>());
         BVHT_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_gather_tris, dim3(nblk(n, 256)), dim3(256), 0, st, tin.as<float>(), ec.as<float4>(), n,
-                       b->d_tris);
+                       b->d_tris, d_err.as<uint32_t>());
+    hipLaunchKernelGGL(k_validate_tree, dim3(nblk(b->nodes_used, 256)), dim3(256), 0, st, b->d_nodes, b->nodes_used,
+                       n, d_err.as<uint32_t>());
     BVHT_HIP(hipGetLastError());
+    uint32_t errv = 0;
+    BVHT_HIP(hipMemcpyAsync(&errv, d_err.p, sizeof errv, hipMemcpyDeviceToHost, st));
+    BVHT_HIP(hipStreamSynchronize(st));
+    if (errv)
+        return set_error(BVHT_EDEVICE, "BLAS build self-check failed (flags 0x%x): inconsistent partition or tree",
+                         errv);
     BVHT_HIP(hipMemcpyAsync(h_tris, b->d_tris, sizeof(float) * 9 * (size_t)n, hipMemcpyDeviceToHost, st));
     BVHT_HIP(hipStreamSynchronize(st));
     RC(compute_blas_topology(b, nullptr));

@@ -243,11 +243,18 @@ __device__ __forceinline__ void make_bounce(const float* tri, V3 o, V3 d, float 
 
 // Order-preserving compaction of hits: per-block counts, one-block scan, write.
 __global__ __launch_bounds__(256) void k_hit_counts(const bvht_hit* __restrict__ hits, uint32_t n,
+                                                    uint32_t prim_limit, uint32_t inst_limit,
+                                                    const ObjRows* __restrict__ rows,
+                                                    const BlasRef* __restrict__ blases,
                                                     uint32_t* __restrict__ block_counts)
 {
     __shared__ uint32_t wave_counts[4];
     uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    bool h = i < n && hits[i].prim != BVHT_MISS;
+    bool h = false;
+    if (i < n) {
+        bvht_hit hh = hits[i];
+        h = bounce_hit(hh, prim_limit, inst_limit, rows, blases);
+    }
     uint64_t m = __ballot(h);
     if ((threadIdx.x & 63) == 0)
         wave_counts[threadIdx.x >> 6] = __popcll(m);
@@ -284,7 +291,7 @@ __global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* __restrict__ cou
         *total = part[1023];
 }
 
-__global__ __launch_bounds__(256) void k_bounce_write(const float* __restrict__ tris,
+__global__ __launch_bounds__(256) void k_bounce_write(const float* __restrict__ tris, uint32_t ntris,
                                                       const bvht_ray* __restrict__ rays,
                                                       const bvht_hit* __restrict__ hits, uint32_t n,
                                                       const uint32_t* __restrict__ block_offsets,
@@ -296,7 +303,7 @@ __global__ __launch_bounds__(256) void k_bounce_write(const float* __restrict__ 
     bool hit = false;
     if (i < n) {
         h = hits[i];
-        hit = h.prim != BVHT_MISS;
+        hit = bounce_hit(h, ntris, 0xFFFFFFFFu, nullptr, nullptr);
     }
     uint64_t m = __ballot(hit);
     uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -450,7 +457,8 @@ int32_t launch_trace_blas(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, 
 
 // Per-256-ray-block exclusive offsets of the hits (in ctx scratch) and the
 // total hit count in *d_count; the caller's write kernel packs the hits.
-int32_t hit_block_offsets(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, uint32_t* d_count, uint32_t** offsets)
+int32_t hit_block_offsets(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, uint32_t prim_limit, uint32_t inst_limit,
+                          const void* rows, const void* blases, uint32_t* d_count, uint32_t** offsets)
 {
     uint32_t nb = blocks_for(n, 256);
     int32_t rc = ensure_scratch(ctx, sizeof(uint32_t) * std::max<uint32_t>(nb, 1));
@@ -460,21 +468,22 @@ int32_t hit_block_offsets(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, uint
     *offsets = bc;
     if (nb == 0)
         return fill_u32(ctx, d_count, 0, 1);
-    hipLaunchKernelGGL(k_hit_counts, dim3(nb), dim3(256), 0, ctx->stream, d_hits, n, bc);
+    hipLaunchKernelGGL(k_hit_counts, dim3(nb), dim3(256), 0, ctx->stream, d_hits, n, prim_limit, inst_limit,
+                       static_cast<const ObjRows*>(rows), static_cast<const BlasRef*>(blases), bc);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, ctx->stream, bc, nb, d_count);
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }
 
-int32_t launch_bounce(bvht_ctx ctx, const float* d_tris, const bvht_ray* d_rays, const bvht_hit* d_hits, uint32_t n,
-                      uint64_t seed, bvht_ray* d_out, uint32_t* d_count)
+int32_t launch_bounce(bvht_ctx ctx, const float* d_tris, uint32_t ntris, const bvht_ray* d_rays,
+                      const bvht_hit* d_hits, uint32_t n, uint64_t seed, bvht_ray* d_out, uint32_t* d_count)
 {
     uint32_t* bc = nullptr;
-    int32_t rc = hit_block_offsets(ctx, d_hits, n, d_count, &bc);
+    int32_t rc = hit_block_offsets(ctx, d_hits, n, ntris, 0xFFFFFFFFu, nullptr, nullptr, d_count, &bc);
     if (rc || n == 0)
         return rc;
     uint32_t nb = blocks_for(n, 256);
-    hipLaunchKernelGGL(k_bounce_write, dim3(nb), dim3(256), 0, ctx->stream, d_tris, d_rays, d_hits, n, bc, seed,
+    hipLaunchKernelGGL(k_bounce_write, dim3(nb), dim3(256), 0, ctx->stream, d_tris, ntris, d_rays, d_hits, n, bc, seed,
                        d_out);
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
@@ -739,7 +748,7 @@ int32_t bvht_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, const 
 {
     BVHT_REQUIRE(ctx && blas && d_count, "NULL argument");
     BVHT_REQUIRE(n == 0 || (d_rays && d_hits && d_out), "NULL buffer");
-    return bvht::launch_bounce(ctx, blas->d_tris, d_rays, d_hits, n, seed, d_out, d_count);
+    return bvht::launch_bounce(ctx, blas->d_tris, blas->n_tris, d_rays, d_hits, n, seed, d_out, d_count);
 }
 
 int32_t bvht_depth(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, float* d_depth)

@@ -242,7 +242,7 @@ __global__ __launch_bounds__(kTraceWG) void k_trace_tlas(const float4* __restric
 
 // Bounce rays for TLAS hits: the geometric normal of the model-space
 // triangle is carried to world space by the transpose of M^-1.
-__global__ __launch_bounds__(256) void k_bounce_write_tlas(const ObjRows* __restrict__ rows,
+__global__ __launch_bounds__(256) void k_bounce_write_tlas(const ObjRows* __restrict__ rows, uint32_t n_obj,
                                                            const BlasRef* __restrict__ blases,
                                                            const bvht_ray* __restrict__ rays,
                                                            const bvht_hit* __restrict__ hits, uint32_t n,
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void k_bounce_write_tlas(const ObjRows* __rest
     bool hit = false;
     if (i < n) {
         h = hits[i];
-        hit = h.prim != BVHT_MISS;
+        hit = bounce_hit(h, 0xFFFFFFFFu, n_obj, rows, blases);
     }
     uint64_t m = __ballot(hit);
     uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -388,6 +388,7 @@ int32_t bvht_tlas_create(bvht_ctx ctx, const bvht_blas* blases, uint32_t blas_co
         t->blases.push_back(blases[i]);
         table[i].nodes = reinterpret_cast<const float4*>(blases[i]->d_nodes);
         table[i].tris = blases[i]->d_tris;
+        table[i].ntris = blases[i]->n_tris;
         t->max_blas_depth = std::max(t->max_blas_depth, blases[i]->depth);
     }
     int32_t rc = validate_objects(t, objects, n);
@@ -498,11 +499,13 @@ int32_t bvht_bounce_tlas(bvht_ctx ctx, bvht_tlas tlas, const bvht_ray* d_rays, c
     BVHT_REQUIRE(ctx && tlas && d_count, "NULL argument");
     BVHT_REQUIRE(n == 0 || (d_rays && d_hits && d_out), "NULL buffer");
     uint32_t* bc = nullptr;
-    int32_t rc = hit_block_offsets(ctx, d_hits, n, d_count, &bc);
+    int32_t rc = hit_block_offsets(ctx, d_hits, n, 0xFFFFFFFFu, tlas->n, tlas->d_rows, tlas->d_blas_table, d_count,
+                                   &bc);
     if (rc || n == 0)
         return rc;
     hipLaunchKernelGGL(k_bounce_write_tlas, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream,
-                       static_cast<const ObjRows*>(tlas->d_rows), static_cast<const BlasRef*>(tlas->d_blas_table),
+                       static_cast<const ObjRows*>(tlas->d_rows), tlas->n,
+                       static_cast<const BlasRef*>(tlas->d_blas_table),
                        d_rays, d_hits, n, bc, seed, d_out);
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;

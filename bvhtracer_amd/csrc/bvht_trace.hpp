@@ -114,6 +114,7 @@ __device__ __forceinline__ BlasResult traverse_blas(const float4* __restrict__ n
 struct BlasRef {
     const float4* nodes;
     const float* tris;
+    uint64_t ntris;
 };
 
 // Object record as the TLAS kernels read it: the 3x4 row block of M^-1
@@ -123,6 +124,20 @@ struct ObjRows {
     uint32_t blas;
     uint32_t pad[3];
 };
+
+// Whether a hit record is bounced off: a hit whose indices lie inside the
+// structure (prim < prim_limit and inst < inst_limit; for TLAS hits, with
+// rows/blases given, prim below the triangle count of the object's BLAS).
+// Hits outside are skipped, never dereferenced.
+__device__ __forceinline__ bool bounce_hit(const bvht_hit& h, uint32_t prim_limit, uint32_t inst_limit,
+                                           const ObjRows* rows, const BlasRef* blases)
+{
+    if (h.prim == BVHT_MISS || h.inst >= inst_limit)
+        return false;
+    if (rows)
+        return h.prim < blases[rows[h.inst].blas].ntris;
+    return h.prim < prim_limit;
+}
 
 struct TlasResult {
     float t, u, v;
