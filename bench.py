@@ -42,9 +42,29 @@ def parse():
     p.add_argument("--seed", type=int, default=0xB0B)
     p.add_argument("--cpu-seconds", type=float, default=10.0, help="budget for the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_bounce_trace.json"),
+    p.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_trace_blas_sphere1m_traffic.json"),
                    help="measured HBM bytes per launch of the dominant kernel (rocprofv3 --pmc summary)")
+    p.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
+                   help="only run the stream-copy kernel 5 times on BYTES (profiler counter calibration)")
     return p.parse_args()
+
+
+def stream_copy_gbs(ctx, nbytes=1 << 30, reps=10):
+    """Measured device copy bandwidth (read + write bytes / time)."""
+    from bvhtracer_amd import _lib as L
+
+    a, b = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    L.check(L.lib().bvht_stream_copy(ctx.handle, b.ptr, a.ptr, nbytes))
+    e0, e1 = ctx.event(), ctx.event()
+    ctx.sync()
+    ctx.record(e0)
+    for _ in range(reps):
+        L.check(L.lib().bvht_stream_copy(ctx.handle, b.ptr, a.ptr, nbytes))
+    ctx.record(e1)
+    ms = ctx.elapsed_ms(e0, e1)
+    a.free()
+    b.free()
+    return 2.0 * nbytes * reps / (ms * 1e-3) / 1e9
 
 
 def log(msg):
@@ -59,6 +79,17 @@ def main():
         sys.exit(2)
     import bvhtracer_amd as B
     from bvhtracer_amd import scenes
+
+    if args.copy_only:
+        from bvhtracer_amd import _lib as L
+
+        ctx = B.Context(0)
+        a, b = ctx.alloc(args.copy_only), ctx.alloc(args.copy_only)
+        for _ in range(5):
+            L.check(L.lib().bvht_stream_copy(ctx.handle, b.ptr, a.ptr, args.copy_only))
+        ctx.sync()
+        print(json.dumps({"copy_bytes": args.copy_only, "launches": 5}))
+        return
 
     W, H = args.width, args.height
     n_primary = W * H
@@ -137,7 +168,14 @@ def main():
     stp = d_stats.download(B.STATS_DTYPE, n_primary)
     Pp, Tp = stp["pairs"].astype(np.float64).mean(), stp["tests"].astype(np.float64).mean()
     b_ray_p = 28 + 20 + 32 + 64 * Pp + 36 * Tp
-    achieved = b_ray * n_bounce / (bounce_ms * 1e-3) / 1e9
+    # dominant kernel: k_trace_blas, two launches per frame (primary, bounce);
+    # average algorithmic bytes per launch over the average launch duration
+    bytes_per_launch = (b_ray_p * n_primary + b_ray * n_bounce) / 2.0
+    avg_launch_ms = (prim_ms + bounce_ms) / 2.0
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    bounce_achieved = b_ray * n_bounce / (bounce_ms * 1e-3) / 1e9
+    copy_gbs = stream_copy_gbs(ctx)
+    log(f"[bench] stream copy {copy_gbs:.0f} GB/s")
     traffic = None
     if args.traffic and os.path.exists(args.traffic):
         with open(args.traffic) as f:
@@ -169,17 +207,22 @@ def main():
         "blas_build_ms": round(build_s * 1e3, 1),
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_trace_blas (bounce pass)",
+            "kernel": "k_trace_blas (primary and bounce launches, averaged per launch)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "bytes_per_ray": round(b_ray, 1),
-            "pairs_per_ray": round(P, 2),
-            "tests_per_ray": round(T, 3),
-            "primary_bytes_per_ray": round(b_ray_p, 1),
-            "primary_frac": round(b_ray_p * n_primary / (prim_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes_per_launch": round(bytes_per_launch, 0),
+            "avg_launch_ms": round(avg_launch_ms, 4),
+            "bounce": {"bytes_per_ray": round(b_ray, 1), "pairs_per_ray": round(P, 2),
+                       "tests_per_ray": round(T, 3), "achieved": round(bounce_achieved, 1),
+                       "frac": round(bounce_achieved / HBM_PEAK_GBS, 4)},
+            "primary": {"bytes_per_ray": round(b_ray_p, 1), "pairs_per_ray": round(Pp, 2),
+                        "tests_per_ray": round(Tp, 3),
+                        "frac": round(b_ray_p * n_primary / (prim_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "stream_copy_gbs": round(copy_gbs, 1),
+            "frac_of_stream_copy": round(achieved / copy_gbs, 4),
         },
         "cpu_baseline": None,
     }

@@ -46,6 +46,7 @@ SIGNATURES = {
     "bvht_event_record": (I32, [P, P]),
     "bvht_event_elapsed": (I32, [P, P, ctypes.POINTER(F32)]),
     "bvht_event_destroy": (I32, [P]),
+    "bvht_stream_copy": (I32, [P, P, P, SZ]),
     "bvht_blas_build": (I32, [P, P, U32, I32, U32, PP]),
     "bvht_blas_upload": (I32, [P, P, U32, U32, P, U32, PP]),
     "bvht_blas_download": (I32, [P, P, P, P]),

@@ -324,6 +324,14 @@ __global__ __launch_bounds__(256) void k_bounce_write(const float* __restrict__ 
     o[1] = b;
 }
 
+__global__ __launch_bounds__(256) void k_stream_copy(float4* __restrict__ dst, const float4* __restrict__ src,
+                                                     size_t n)
+{
+    size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+        dst[i] = src[i];
+}
+
 __global__ void k_depth_acc(const bvht_hit* __restrict__ hits, uint32_t n, float* __restrict__ depth)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -599,6 +607,21 @@ int32_t bvht_event_destroy(void* ev)
 {
     if (ev)
         BVHT_HIP(hipEventDestroy((hipEvent_t)ev));
+    return BVHT_OK;
+}
+
+int32_t bvht_stream_copy(bvht_ctx ctx, void* d_dst, const void* d_src, size_t bytes)
+{
+    BVHT_REQUIRE(ctx && d_dst && d_src, "NULL argument");
+    BVHT_REQUIRE(bytes % 16 == 0 && ((uintptr_t)d_dst % 16) == 0 && ((uintptr_t)d_src % 16) == 0,
+                 "bvht_stream_copy needs 16-byte aligned pointers and a multiple of 16 bytes");
+    size_t n = bytes / 16;
+    if (n == 0)
+        return BVHT_OK;
+    uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 256 * 32);
+    hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, ctx->stream, static_cast<float4*>(d_dst),
+                       static_cast<const float4*>(d_src), n);
+    BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }
 

@@ -141,6 +141,12 @@ int32_t bvht_event_record(bvht_ctx ctx, void* ev);
 int32_t bvht_event_elapsed(void* start, void* end, float* ms);
 int32_t bvht_event_destroy(void* ev);
 
+/* Plain device-to-device copy of `bytes` (multiple of 16, 16-byte aligned
+ * pointers) by a streaming kernel.  Measurement aid with no reference
+ * counterpart: gives the achievable HBM copy bandwidth on the running GPU
+ * and calibrates byte counters read from hardware profilers. */
+int32_t bvht_stream_copy(bvht_ctx ctx, void* d_dst, const void* d_src, size_t bytes);
+
 /* ---- BLAS ------------------------------------------------------------- */
 
 /* BvhBuilder::new().build_for(&mut [Triangle<f32>]) (bvh.rs:509-541) on the
