@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_trace_blas_sphere1m_traffic.json"),
                    help="measured HBM bytes per launch of the dominant kernel (rocprofv3 --pmc summary)")
+    p.add_argument("--no-autotune", action="store_true", help="use the default trace configuration")
     p.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
                    help="only run the stream-copy kernel 5 times on BYTES (profiler counter calibration)")
     return p.parse_args()
@@ -139,6 +140,18 @@ def main():
     ctx.sync()
     n_bounce = int(d_count.download(np.uint32, 1)[0])
     log(f"[bench] warmup done; bounce rays per frame ~{n_bounce}")
+    tuned = None
+    if not args.no_autotune:
+        from bvhtracer_amd import autotune
+
+        def traces():
+            L.check(lib.bvht_trace_blas(ctx.handle, bvh.handle, d_rays.ptr, n_primary, d_hits.ptr, None))
+            L.check(lib.bvht_trace_blas(ctx.handle, bvh.handle, d_brays.ptr, n_bounce, d_bhits.ptr, None))
+
+        best, table = autotune.tune(ctx, traces, reps=3, log=log)
+        tuned = {"workgroup": best[0], "lds_stack_depth": best[1],
+                 "sweep_ms": {f"{k[0]}x{k[1]}": round(v, 4) for k, v in table.items()}}
+        log(f"[bench] autotuned trace config: workgroup {best[0]}, LDS depth {best[1]}")
 
     evs = [[ctx.event() for _ in range(5)] for _ in range(args.steps)]
     start, stop = ctx.event(), ctx.event()
@@ -205,6 +218,7 @@ def main():
         "frame_breakdown_ms": {"raygen": round(raygen_ms, 4), "primary_trace": round(prim_ms, 4),
                                "bounce_gen": round(bounce_gen_ms, 4), "bounce_trace": round(bounce_ms, 4)},
         "blas_build_ms": round(build_s * 1e3, 1),
+        "trace_config": tuned,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_trace_blas (primary and bounce launches, averaged per launch)",

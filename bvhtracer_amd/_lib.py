@@ -58,6 +58,8 @@ SIGNATURES = {
     "bvht_blas_destroy": (I32, [P]),
     "bvht_trace_blas": (I32, [P, P, P, U32, P, P]),
     "bvht_trace_blas_count": (I32, [P, P, P, P, U32, P, P]),
+    "bvht_set_trace_config": (I32, [P, I32, I32]),
+    "bvht_get_trace_config": (I32, [P, ctypes.POINTER(I32), ctypes.POINTER(I32)]),
     "bvht_raygen": (I32, [P, P, U32, U32, I32, P]),
     "bvht_bounce": (I32, [P, P, P, P, U32, U64, P, P]),
     "bvht_depth": (I32, [P, P, U32, P]),

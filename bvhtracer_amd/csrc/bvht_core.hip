@@ -166,25 +166,22 @@ __global__ void k_refit(bvht_node* __restrict__ nodes, const float* __restrict__
     }
 }
 
-template <int LDS_DEPTH, bool STATS>
-__global__ __launch_bounds__(kTraceWG) void k_trace_blas(const float4* __restrict__ nodes,
-                                                         const float* __restrict__ tris,
-                                                         const bvht_ray* __restrict__ rays,
-                                                         bvht_hit* __restrict__ hits, uint32_t n,
-                                                         const uint32_t* __restrict__ d_n,
-                                                         bvht_stats* __restrict__ stats,
-                                                         uint32_t* __restrict__ spill, uint32_t cap)
+template <int WG, int LDS_DEPTH, bool STATS>
+__global__ __launch_bounds__(WG) void k_trace_blas(const float4* __restrict__ nodes, const float* __restrict__ tris,
+                                                   const bvht_ray* __restrict__ rays, bvht_hit* __restrict__ hits,
+                                                   uint32_t n, const uint32_t* __restrict__ d_n,
+                                                   bvht_stats* __restrict__ stats, uint32_t* __restrict__ spill,
+                                                   uint32_t cap)
 {
-    __shared__ uint32_t lds[LDS_DEPTH * kTraceWG];
-    uint32_t i = blockIdx.x * kTraceWG + threadIdx.x;
+    __shared__ uint32_t lds[LDS_DEPTH * WG];
+    uint32_t i = blockIdx.x * WG + threadIdx.x;
     uint32_t count = d_n ? min(*d_n, n) : n;
     if (i >= count)
         return;
-    Stack<LDS_DEPTH> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap};
+    Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap};
     DRay ray = load_ray(rays, i);
     uint32_t pairs = 0, tests = 0;
-    uint32_t sp = 0;
-    BlasResult r = traverse_blas<LDS_DEPTH, STATS>(nodes, tris, ray, st, sp, pairs, tests);
+    BlasResult r = traverse_blas<STATS>(nodes, tris, ray, st, 0u, pairs, tests);
     store_hit(hits, i, r.found, r.t, r.u, r.v, r.prim, 0);
     if (STATS) {
         bvht_stats s;
@@ -194,6 +191,19 @@ __global__ __launch_bounds__(kTraceWG) void k_trace_blas(const float4* __restric
         s.inst_visits = 0;
         stats[i] = s;
     }
+}
+
+typedef void (*trace_blas_fn)(const float4*, const float*, const bvht_ray*, bvht_hit*, uint32_t, const uint32_t*,
+                              bvht_stats*, uint32_t*, uint32_t);
+
+static trace_blas_fn pick_trace_blas(int wg, int lds, bool stats)
+{
+#define X(W, D)                                      \
+    if (wg == W && lds == D)                         \
+        return stats ? k_trace_blas<W, D, true> : k_trace_blas<W, D, false>;
+    BVHT_TRACE_CONFIGS(X)
+#undef X
+    return nullptr;
 }
 
 // Camera::get_ray_eye + get_ray_world (camera.rs:994-1010) with the
@@ -428,6 +438,20 @@ int32_t blas_alloc(bvht_ctx ctx, uint32_t n, uint32_t node_count, bvht_blas* out
     return BVHT_OK;
 }
 
+// Trace configuration for a tree needing `need` stack entries: the context's
+// configuration if set (autotuned), else 256 lanes with the smallest LDS
+// depth that holds the whole stack (32 at most; deeper trees spill).
+void trace_config(bvht_ctx ctx, uint32_t need, int* wg, int* lds)
+{
+    if (ctx->trace_wg > 0) {
+        *wg = ctx->trace_wg;
+        *lds = ctx->trace_lds;
+        return;
+    }
+    *wg = 256;
+    *lds = need <= 8 ? 8 : (need <= 16 ? 16 : (need <= 24 ? 24 : 32));
+}
+
 int32_t launch_trace_blas(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, uint32_t n,
                           const uint32_t* d_n, bvht_hit* d_hits, bvht_stats* d_stats)
 {
@@ -435,30 +459,21 @@ int32_t launch_trace_blas(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, 
         return BVHT_OK;
     // stack entries needed: at most depth - 1
     uint32_t need = blas->depth > 0 ? blas->depth - 1 : 0;
-    int lds_depth = need <= 16 ? 16 : (need <= 32 ? 32 : 64);
-    uint32_t cap = need;
+    int wg, lds;
+    trace_config(ctx, need, &wg, &lds);
     uint32_t* spill = nullptr;
-    if (need > (uint32_t)lds_depth) {
-        int32_t rc = ensure_spill(ctx, &blas->d_spill, &blas->spill_words, (size_t)(need - lds_depth) * n);
+    if (need > (uint32_t)lds) {
+        int32_t rc = ensure_spill(ctx, &blas->d_spill, &blas->spill_words, (size_t)(need - lds) * n);
         if (rc)
             return rc;
         spill = blas->d_spill;
     }
-    const float4* nodes = reinterpret_cast<const float4*>(blas->d_nodes);
-    dim3 grid(blocks_for(n, kTraceWG)), block(kTraceWG);
-#define LAUNCH(D, S)                                                                                   \
-    hipLaunchKernelGGL((k_trace_blas<D, S>), grid, block, 0, ctx->stream, nodes, blas->d_tris, d_rays, d_hits, n, \
-                       d_n, d_stats, spill, cap)
-    if (d_stats) {
-        if (lds_depth == 16) LAUNCH(16, true);
-        else if (lds_depth == 32) LAUNCH(32, true);
-        else LAUNCH(64, true);
-    } else {
-        if (lds_depth == 16) LAUNCH(16, false);
-        else if (lds_depth == 32) LAUNCH(32, false);
-        else LAUNCH(64, false);
-    }
-#undef LAUNCH
+    trace_blas_fn fn = pick_trace_blas(wg, lds, d_stats != nullptr);
+    if (!fn)
+        return set_error(BVHT_EINVAL, "no trace kernel for workgroup %d, LDS depth %d", wg, lds);
+    hipLaunchKernelGGL(fn, dim3(blocks_for(n, wg)), dim3(wg), 0, ctx->stream,
+                       reinterpret_cast<const float4*>(blas->d_nodes), blas->d_tris, d_rays, d_hits, n, d_n, d_stats,
+                       spill, need);
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }
@@ -749,6 +764,29 @@ int32_t bvht_trace_blas_count(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_ra
     BVHT_REQUIRE(ctx && blas && d_count, "NULL argument");
     BVHT_REQUIRE(max_n == 0 || (d_rays && d_hits), "NULL ray or hit buffer");
     return bvht::launch_trace_blas(ctx, blas, d_rays, max_n, d_count, d_hits, d_stats);
+}
+
+int32_t bvht_set_trace_config(bvht_ctx ctx, int32_t wg, int32_t lds_depth)
+{
+    BVHT_REQUIRE(ctx, "ctx is NULL");
+    if (wg == 0) {
+        ctx->trace_wg = 0;
+        ctx->trace_lds = 0;
+        return BVHT_OK;
+    }
+    BVHT_REQUIRE(pick_trace_blas(wg, lds_depth, false) != nullptr,
+                 "unsupported trace configuration: workgroup %d, LDS depth %d", wg, lds_depth);
+    ctx->trace_wg = wg;
+    ctx->trace_lds = lds_depth;
+    return BVHT_OK;
+}
+
+int32_t bvht_get_trace_config(bvht_ctx ctx, int32_t* wg, int32_t* lds_depth)
+{
+    BVHT_REQUIRE(ctx && wg && lds_depth, "NULL argument");
+    *wg = ctx->trace_wg;
+    *lds_depth = ctx->trace_lds;
+    return BVHT_OK;
 }
 
 int32_t bvht_raygen(bvht_ctx ctx, const bvht_camera* cam, uint32_t width, uint32_t height, int32_t order,

@@ -16,6 +16,9 @@ struct bvht_ctx_s {
     // Reusable device scratch (grown on demand; freed with the context).
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
+    // trace kernel configuration (0 = choose from the tree depth)
+    int trace_wg = 0;
+    int trace_lds = 0;
 };
 
 struct bvht_blas_s {
@@ -59,6 +62,7 @@ int32_t compute_blas_topology(bvht_blas b, const bvht_node* h_nodes);
 int32_t ensure_spill(bvht_ctx ctx, uint32_t** buf, size_t* words, size_t need);
 int32_t blas_alloc(bvht_ctx ctx, uint32_t n, uint32_t node_count, bvht_blas* out);
 int32_t fill_u32(bvht_ctx ctx, uint32_t* p, uint32_t v, size_t n);
+void trace_config(bvht_ctx ctx, uint32_t need, int* wg, int* lds);
 int32_t hit_block_offsets(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, uint32_t prim_limit, uint32_t inst_limit,
                           const void* rows, const void* blases, uint32_t* d_count, uint32_t** offsets);
 

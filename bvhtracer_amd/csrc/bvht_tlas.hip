@@ -210,25 +210,23 @@ __global__ __launch_bounds__(kRebuildWG) void k_tlas_rebuild(const float* __rest
     }
 }
 
-template <int LDS_DEPTH, bool STATS>
-__global__ __launch_bounds__(kTraceWG) void k_trace_tlas(const float4* __restrict__ tnodes,
-                                                         const ObjRows* __restrict__ rows,
-                                                         const BlasRef* __restrict__ blases,
-                                                         const bvht_ray* __restrict__ rays,
-                                                         bvht_hit* __restrict__ hits, uint32_t n,
-                                                         const uint32_t* __restrict__ d_n,
-                                                         bvht_stats* __restrict__ stats,
-                                                         uint32_t* __restrict__ spill, uint32_t cap)
+template <int WG, int LDS_DEPTH, bool STATS>
+__global__ __launch_bounds__(WG) void k_trace_tlas(const float4* __restrict__ tnodes, const ObjRows* __restrict__ rows,
+                                                   const BlasRef* __restrict__ blases,
+                                                   const bvht_ray* __restrict__ rays, bvht_hit* __restrict__ hits,
+                                                   uint32_t n, const uint32_t* __restrict__ d_n,
+                                                   bvht_stats* __restrict__ stats, uint32_t* __restrict__ spill,
+                                                   uint32_t cap)
 {
-    __shared__ uint32_t lds[LDS_DEPTH * kTraceWG];
-    uint32_t i = blockIdx.x * kTraceWG + threadIdx.x;
+    __shared__ uint32_t lds[LDS_DEPTH * WG];
+    uint32_t i = blockIdx.x * WG + threadIdx.x;
     uint32_t count = d_n ? min(*d_n, n) : n;
     if (i >= count)
         return;
-    Stack<LDS_DEPTH> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap};
+    Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap};
     DRay ray = load_ray(rays, i);
     uint32_t pairs = 0, tests = 0, tpairs = 0, visits = 0;
-    TlasResult r = traverse_tlas<LDS_DEPTH, STATS>(tnodes, rows, blases, ray, st, pairs, tests, tpairs, visits);
+    TlasResult r = traverse_tlas<STATS>(tnodes, rows, blases, ray, st, pairs, tests, tpairs, visits);
     store_hit(hits, i, r.found, r.t, r.u, r.v, r.prim, r.inst);
     if (STATS) {
         bvht_stats s;
@@ -238,6 +236,19 @@ __global__ __launch_bounds__(kTraceWG) void k_trace_tlas(const float4* __restric
         s.inst_visits = visits;
         stats[i] = s;
     }
+}
+
+typedef void (*trace_tlas_fn)(const float4*, const ObjRows*, const BlasRef*, const bvht_ray*, bvht_hit*, uint32_t,
+                              const uint32_t*, bvht_stats*, uint32_t*, uint32_t);
+
+trace_tlas_fn pick_trace_tlas(int wg, int lds, bool stats)
+{
+#define X(W, D)                                      \
+    if (wg == W && lds == D)                         \
+        return stats ? k_trace_tlas<W, D, true> : k_trace_tlas<W, D, false>;
+    BVHT_TRACE_CONFIGS(X)
+#undef X
+    return nullptr;
 }
 
 // Bounce rays for TLAS hits: the geometric normal of the model-space
@@ -336,29 +347,21 @@ int32_t launch_trace_tlas(bvht_tlas t, const bvht_ray* d_rays, uint32_t n, const
         return BVHT_OK;
     bvht_ctx ctx = t->ctx;
     uint32_t need = (t->depth > 0 ? t->depth - 1 : 0) + (t->max_blas_depth > 0 ? t->max_blas_depth - 1 : 0);
-    int lds_depth = need <= 16 ? 16 : 32;
+    int wg, lds;
+    trace_config(ctx, need, &wg, &lds);
     uint32_t* spill = nullptr;
-    if (need > (uint32_t)lds_depth) {
-        int32_t rc = ensure_spill(ctx, &t->d_spill, &t->spill_words, (size_t)(need - lds_depth) * n);
+    if (need > (uint32_t)lds) {
+        int32_t rc = ensure_spill(ctx, &t->d_spill, &t->spill_words, (size_t)(need - lds) * n);
         if (rc)
             return rc;
         spill = t->d_spill;
     }
-    const float4* tn = reinterpret_cast<const float4*>(t->d_nodes);
-    const ObjRows* rows = static_cast<const ObjRows*>(t->d_rows);
-    const BlasRef* br = static_cast<const BlasRef*>(t->d_blas_table);
-    dim3 grid(nblk(n, kTraceWG)), block(kTraceWG);
-#define LAUNCH(D, S)                                                                                          \
-    hipLaunchKernelGGL((k_trace_tlas<D, S>), grid, block, 0, ctx->stream, tn, rows, br, d_rays, d_hits, n, d_n, \
-                       d_stats, spill, need)
-    if (d_stats) {
-        if (lds_depth == 16) LAUNCH(16, true);
-        else LAUNCH(32, true);
-    } else {
-        if (lds_depth == 16) LAUNCH(16, false);
-        else LAUNCH(32, false);
-    }
-#undef LAUNCH
+    trace_tlas_fn fn = pick_trace_tlas(wg, lds, d_stats != nullptr);
+    if (!fn)
+        return set_error(BVHT_EINVAL, "no TLAS trace kernel for workgroup %d, LDS depth %d", wg, lds);
+    hipLaunchKernelGGL(fn, dim3(nblk(n, wg)), dim3(wg), 0, ctx->stream, reinterpret_cast<const float4*>(t->d_nodes),
+                       static_cast<const ObjRows*>(t->d_rows), static_cast<const BlasRef*>(t->d_blas_table), d_rays,
+                       d_hits, n, d_n, d_stats, spill, need);
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }

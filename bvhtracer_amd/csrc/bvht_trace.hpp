@@ -5,19 +5,17 @@
 
 namespace bvht {
 
-constexpr int kTraceWG = 256;
-
 // LDS (address space 3) pointer, so stack accesses compile to ds_read/ds_write
 // instead of flat accesses.
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
 // Per-lane traversal stack: the first LDS_DEPTH entries live in LDS at
-// lds[slot * kTraceWG] (lds already offset by the lane), deeper entries in a
-// per-ray column of a global buffer (spill[(slot - LDS_DEPTH) * stride]).
-// `cap` bounds the total; a push beyond it is dropped rather than written out
-// of bounds (cap is sized from the tree depth, so this never happens for a
-// structurally valid tree).
-template <int LDS_DEPTH>
+// lds[slot * WG] (lds already offset by the lane: the lanes of a wave hit
+// distinct banks), deeper entries in a per-ray column of a global buffer
+// (spill[(slot - LDS_DEPTH) * stride]).  `cap` bounds the total; a push beyond
+// it is dropped rather than written out of bounds (cap is sized from the tree
+// depth, so this never happens for a structurally valid tree).
+template <int LDS_DEPTH, int WG>
 struct Stack {
     lds_u32* lds;
     uint32_t* spill;
@@ -27,7 +25,7 @@ struct Stack {
     __device__ __forceinline__ void push(uint32_t& sp, uint32_t v)
     {
         if (sp < (uint32_t)LDS_DEPTH) {
-            lds[sp * kTraceWG] = v;
+            lds[sp * WG] = v;
         } else {
             if (sp >= cap)
                 return;
@@ -38,9 +36,18 @@ struct Stack {
     __device__ __forceinline__ uint32_t pop(uint32_t& sp)
     {
         --sp;
-        return sp < (uint32_t)LDS_DEPTH ? lds[sp * kTraceWG] : spill[(size_t)(sp - LDS_DEPTH) * stride];
+        return sp < (uint32_t)LDS_DEPTH ? lds[sp * WG] : spill[(size_t)(sp - LDS_DEPTH) * stride];
     }
 };
+
+// Trace kernel configurations: workgroup size x LDS stack entries per lane.
+// The X-macro instantiates every kernel variant; the host picks one per
+// launch from the context's configuration (set by the autotuner or the
+// depth-based default).
+#define BVHT_TRACE_CONFIGS(X) \
+    X(64, 8) X(64, 16) X(64, 24) X(64, 32) \
+    X(128, 8) X(128, 16) X(128, 24) X(128, 32) \
+    X(256, 8) X(256, 16) X(256, 24) X(256, 32)
 
 struct BlasResult {
     float t, u, v;
@@ -51,10 +58,10 @@ struct BlasResult {
 // Bvh::intersect_subtree (bvh.rs:242-305) for one ray, starting at node 0.
 // The stack is shared with an enclosing TLAS traversal: this call only uses
 // slots at or above sp_base and leaves the stack as it found it.
-template <int LDS_DEPTH, bool STATS>
+template <bool STATS, class StackT>
 __device__ __forceinline__ BlasResult traverse_blas(const float4* __restrict__ nodes,
                                                     const float* __restrict__ tris, const DRay& ray,
-                                                    Stack<LDS_DEPTH>& st, uint32_t sp_base, uint32_t& pairs,
+                                                    StackT& st, uint32_t sp_base, uint32_t& pairs,
                                                     uint32_t& tests)
 {
     BlasResult res{0.0f, 0.0f, 0.0f, 0u, false};
@@ -148,10 +155,10 @@ struct TlasResult {
 // Tlas::intersect (tlas.rs:123-177): the TLAS node loop with
 // SceneObject::intersect at the leaves (scene_object.rs:77-89).  Left child =
 // high 16 bits of the LeftRightIndex word (tlas.rs:24-32).
-template <int LDS_DEPTH, bool STATS>
+template <bool STATS, class StackT>
 __device__ __forceinline__ TlasResult traverse_tlas(const float4* __restrict__ tnodes, const ObjRows* __restrict__ objs,
                                                     const BlasRef* __restrict__ blases, const DRay& ray,
-                                                    Stack<LDS_DEPTH>& st, uint32_t& pairs, uint32_t& tests,
+                                                    StackT& st, uint32_t& pairs, uint32_t& tests,
                                                     uint32_t& tpairs, uint32_t& visits)
 {
     TlasResult res{0.0f, 0.0f, 0.0f, 0u, 0u, false};
@@ -171,7 +178,7 @@ __device__ __forceinline__ TlasResult traverse_tlas(const float4* __restrict__ t
             V3 dm = rows_mul(q0, q1, q2, ray.d, 0.0f);
             DRay rm = make_ray(om, dm, closest);
             BlasRef br = blases[model];
-            BlasResult r = traverse_blas<LDS_DEPTH, STATS>(br.nodes, br.tris, rm, st, sp, pairs, tests);
+            BlasResult r = traverse_blas<STATS>(br.nodes, br.tris, rm, st, sp, pairs, tests);
             if (r.found && r.t < closest) {
                 closest = r.t;
                 res.t = r.t;

@@ -190,6 +190,14 @@ int32_t bvht_trace_blas_count(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_ra
                               const uint32_t* d_count, uint32_t max_n, bvht_hit* d_hits,
                               bvht_stats* d_stats);
 
+/* Trace kernel configuration for this context: workgroup size (64, 128 or
+ * 256 lanes) and per-lane LDS stack entries (8, 16, 24 or 32; deeper stacks
+ * spill to HBM).  wg = 0 restores the default (256 lanes, LDS depth from the
+ * tree depth).  Results never depend on the configuration; the autotuner
+ * (bvhtracer_amd/autotune.py) picks the fastest. */
+int32_t bvht_set_trace_config(bvht_ctx ctx, int32_t wg, int32_t lds_depth);
+int32_t bvht_get_trace_config(bvht_ctx ctx, int32_t* wg, int32_t* lds_depth);
+
 /* ---- frame loop ------------------------------------------------------- */
 
 /* Camera::get_ray_world for every pixel (camera.rs:994-1010) with the

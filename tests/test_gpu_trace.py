@@ -134,3 +134,34 @@ def test_bounce_rays_parity(ctx, orc):
     assert np.isfinite(brays["o"]).all() and np.allclose(np.linalg.norm(brays["d"], axis=1), 1, atol=1e-5)
     got = bvh.intersect(brays)
     assert_hits_equal(got, orc.bvh_trace(nodes, perm, brays.view(orc.RAY_DTYPE), nthreads=8), "bounce")
+
+
+def test_every_trace_config_is_bit_identical(ctx, orc):
+    """Each workgroup size x LDS depth variant (the autotuner's search space)
+    gives the oracle's hits, on a normal tree and on the 150-level comb tree
+    that spills past every LDS depth."""
+    from bvhtracer_amd import autotune
+
+    mesh = scenes.triangle_soup(3000, seed=23)
+    nodes, used, perm = orc.bvh_build(mesh, False, 2)
+    bvh = upload(ctx, nodes, used, perm)
+    rays = random_rays(30000, 8)
+    ref = orc.bvh_trace(nodes, perm, rays.view(orc.RAY_DTYPE), nthreads=8)
+    cnodes, cused, ctris = comb_tree(150)
+    comb = upload(ctx, cnodes, cused, ctris)
+    k = np.arange(2048)
+    crays = np.zeros(2048, B.RAY_DTYPE)
+    crays["o"] = np.stack([np.full(2048, -5.0), (k % 64) / 64.0 - 0.5, (k // 64) / 64.0 - 0.5], 1)
+    crays["d"] = np.stack([np.ones(2048), np.full(2048, 1e-3), np.full(2048, -1e-3)], 1)
+    crays["tmax"] = np.finfo(np.float32).max
+    cref = orc.bvh_trace(cnodes, ctris, crays.view(orc.RAY_DTYPE))
+    try:
+        for wg, d in autotune.configurations():
+            autotune.set_config(ctx, wg, d)
+            assert autotune.current(ctx) == (wg, d)
+            assert_hits_equal(bvh.intersect(rays), ref, f"config {wg}x{d}")
+            assert_hits_equal(comb.intersect(crays), cref, f"comb config {wg}x{d}")
+    finally:
+        autotune.set_config(ctx, 0, 0)
+    with pytest.raises(B.BvhtError):
+        autotune.set_config(ctx, 96, 16)
