@@ -45,6 +45,9 @@ def parse():
     p.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_trace_blas_sphere1m_traffic.json"),
                    help="measured HBM bytes per launch of the dominant kernel (rocprofv3 --pmc summary)")
     p.add_argument("--no-autotune", action="store_true", help="use the default trace configuration")
+    p.add_argument("--no-extras", action="store_true", help="skip the batched and instanced workloads")
+    p.add_argument("--batch-scenes", type=int, default=64)
+    p.add_argument("--instanced-frames", type=int, default=64)
     p.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
                    help="only run the stream-copy kernel 5 times on BYTES (profiler counter calibration)")
     return p.parse_args()
@@ -244,7 +247,172 @@ def main():
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(ctx, bvh, d_brays, d_bhits, n_bounce, d_rays, d_hits, n_primary,
                                               args.cpu_seconds)
+    if not args.no_extras:
+        for name, fn in (("instanced", bench_instanced), ("batched", bench_batched)):
+            try:
+                result[name] = fn(args, ctx)
+            except Exception as e:  # a failing extra must not hide the main measurement
+                result[name] = {"error": f"{type(e).__name__}: {e}"}
+                log(f"[bench] {name} failed: {e}")
     print(json.dumps(result), flush=True)
+
+
+def bench_instanced(args, ctx):
+    """16 instances of a 200k-triangle displaced icosphere; per frame: deform
+    the original vertices (y += 0.2 sin(4x + 0.1 frame)), BLAS refit, object
+    bounds, TLAS rebuild, 1280x720 primary + 1 bounce -- one hipGraph per
+    frame, replayed for --instanced-frames frames."""
+    import bvhtracer_amd as B
+    from bvhtracer_amd import _lib as L
+    from bvhtracer_amd import scenes
+
+    lib = L.lib()
+    W, H = 1280, 720
+    n = W * H
+    mesh = scenes.icosphere(100, radius=4.0)
+    m = mesh.copy()
+    bvh = B.BvhBuilder(B.BUILD_SAH, 2, ctx=ctx).build_for(m)
+    M, Minv = scenes.random_affines(16, seed=0x16, trans_range=20.0)
+    objs = [B.SceneObject(0, M[k].reshape(4, 4), Minv[k]) for k in range(16)]
+    tlas = B.TlasBuilder(ctx=ctx).build_for(objs, [bvh])
+    tlas.rebuild_async()
+    batch = B.Batch(ctx, [tlas])
+    cam = np.ascontiguousarray(scenes.camera(eye=(0, 0, -60), forward=(0, 0, 1), right=(1, 0, 0), up=(0, 1, 0),
+                                             fovy_deg=45, aspect=W / H))
+    d_orig = ctx.to_device(m)
+    d_frame = ctx.to_device(np.array([0], np.uint32))
+    d_rays, d_hits = ctx.alloc(n * 32), ctx.alloc(n * 24)
+    d_brays, d_bhits, d_cnt = ctx.alloc(n * 32), ctx.alloc(n * 24), ctx.alloc(4)
+
+    def update():
+        B.deform_wave(ctx, bvh, d_orig, 0.2, 4.0, 0.1, d_frame)
+        batch.refit()
+        batch.rebuild()
+
+    def trace():
+        L.check(lib.bvht_raygen(ctx.handle, L.ptr(cam), W, H, B.ORDER_TILES, d_rays.ptr))
+        L.check(lib.bvht_trace_tlas(ctx.handle, tlas.handle, d_rays.ptr, n, d_hits.ptr, None))
+        L.check(lib.bvht_bounce_tlas(ctx.handle, tlas.handle, d_rays.ptr, d_hits.ptr, n, 0xB0B, d_brays.ptr,
+                                     d_cnt.ptr))
+        L.check(lib.bvht_trace_tlas_count(ctx.handle, tlas.handle, d_brays.ptr, d_cnt.ptr, n, d_bhits.ptr, None))
+
+    def frame():
+        update()
+        trace()
+        B.counter_add(ctx, d_frame, 1)
+
+    frame()  # allocates scratch / spill before capture
+    ctx.sync()
+    e = [ctx.event() for _ in range(3)]
+    ctx.record(e[0])
+    update()
+    ctx.record(e[1])
+    trace()
+    ctx.record(e[2])
+    update_ms, trace_ms = ctx.elapsed_ms(e[0], e[1]), ctx.elapsed_ms(e[1], e[2])
+    graph = B.Graph.capture(ctx, frame)
+    graph.launch()
+    ctx.sync()
+    t0, t1 = ctx.event(), ctx.event()
+    ctx.record(t0)
+    for _ in range(args.instanced_frames):
+        graph.launch()
+    ctx.record(t1)
+    ms = ctx.elapsed_ms(t0, t1) / args.instanced_frames
+    nb = int(d_cnt.download(np.uint32, 1)[0])
+    graph.close()
+    return {"workload": "16 instances x 200,000-triangle icosphere, per-frame deform + refit + bounds + TLAS "
+                        "rebuild + 1280x720 primary + 1 bounce, one hipGraph per frame",
+            "frames": args.instanced_frames, "ms_per_frame": round(ms, 4),
+            "mrays_s": round((n + nb) / (ms * 1e-3) / 1e6, 2), "bounce_rays": nb,
+            "eager_update_ms": round(update_ms, 4), "eager_trace_ms": round(trace_ms, 4)}
+
+
+def bench_batched(args, ctx):
+    """64 uneven scenes (triangles log-uniform in [1e3, 1e6], 1-8 instances,
+    rays log-uniform in [4e3, 2e6]): one frame = deform + refit + bounds +
+    TLAS rebuild + trace of every scene.  Batched: one launch per stage (one
+    persistent trace fed by an atomic queue), captured in one hipGraph.
+    Plain: the same stages launched scene by scene.  Hits are compared."""
+    import bvhtracer_amd as B
+    from bvhtracer_amd import _lib as L
+    from bvhtracer_amd import autotune, scenes
+
+    lib = L.lib()
+    t0 = time.time()
+    work = scenes.batch_workload(args.batch_scenes)
+    log(f"[bench] batched workload generated in {time.time() - t0:.1f}s")
+    models, tlases, d_origs = [], [], []
+    for w in work:
+        m = w["mesh"].copy()
+        bvh = B.BvhBuilder(B.BUILD_SAH, 2, ctx=ctx).build_for(m)
+        objs = [B.SceneObject(0, w["M"][k].reshape(4, 4), w["Minv"][k]) for k in range(w["M"].shape[0])]
+        t = B.TlasBuilder(ctx=ctx).build_for(objs, [bvh])
+        t.rebuild_async()
+        models.append(bvh)
+        tlases.append(t)
+        d_origs.append(ctx.to_device(m))
+    rays = np.concatenate([w["rays"] for w in work])
+    offsets = np.concatenate([[0], np.cumsum([w["rays"].shape[0] for w in work])]).astype(np.uint32)
+    n = int(offsets[-1])
+    d_rays = ctx.to_device(rays)
+    d_hits, d_hits_plain = ctx.alloc(n * 24), ctx.alloc(n * 24)
+    d_frame = ctx.to_device(np.array([0], np.uint32))
+    batch = B.Batch(ctx, tlases)
+    batch.set_rays(offsets, 128)
+    best, table = autotune.tune_batch(batch, offsets, lambda: batch.trace(d_rays, d_hits), reps=2)
+    log(f"[bench] batched trace config {best}")
+
+    def deform():
+        for bvh, d_o in zip(models, d_origs):
+            B.deform_wave(ctx, bvh, d_o, 0.2, 4.0, 0.1, d_frame)
+
+    def batched_frame():
+        deform()
+        batch.refit()
+        batch.rebuild()
+        batch.trace(d_rays, d_hits)
+
+    def plain_frame():
+        deform()
+        for s, (bvh, t) in enumerate(zip(models, tlases)):
+            L.check(lib.bvht_blas_refit(ctx.handle, bvh.handle))
+            L.check(lib.bvht_tlas_refresh_bounds(ctx.handle, t.handle))
+            L.check(lib.bvht_tlas_rebuild_async(ctx.handle, t.handle))
+            cnt = int(offsets[s + 1] - offsets[s])
+            L.check(lib.bvht_trace_tlas(ctx.handle, t.handle, d_rays.offset(32 * int(offsets[s])), cnt,
+                                        d_hits_plain.offset(24 * int(offsets[s])), None))
+
+    batched_frame()
+    plain_frame()
+    ctx.sync()
+    graph = B.Graph.capture(ctx, batched_frame)
+
+    def timed(fn, reps):
+        fn()
+        a, b = ctx.event(), ctx.event()
+        ctx.sync()
+        ctx.record(a)
+        for _ in range(reps):
+            fn()
+        ctx.record(b)
+        return ctx.elapsed_ms(a, b) / reps
+
+    reps = 5
+    plain_ms = timed(plain_frame, reps)
+    batched_ms = timed(graph.launch, reps)
+    trace_only_ms = timed(lambda: batch.trace(d_rays, d_hits), reps)
+    ctx.sync()
+    same = d_hits.download(np.uint8).tobytes() == d_hits_plain.download(np.uint8).tobytes()
+    graph.close()
+    return {"workload": f"{len(work)} scenes, {sum(w['mesh'].shape[0] for w in work)} triangles, "
+                        f"{sum(w['M'].shape[0] for w in work)} instances, {n} rays (half camera, half random)",
+            "batched_ms_per_frame": round(batched_ms, 4), "plain_ms_per_frame": round(plain_ms, 4),
+            "batched_mrays_s": round(n / (batched_ms * 1e-3) / 1e6, 2),
+            "plain_mrays_s": round(n / (plain_ms * 1e-3) / 1e6, 2),
+            "speedup": round(plain_ms / batched_ms, 3), "batched_trace_only_ms": round(trace_only_ms, 4),
+            "hits_identical": bool(same),
+            "trace_config": {"workgroup": best[0], "lds_stack_depth": best[1], "rays_per_item": best[2]}}
 
 
 def cpu_baseline(ctx, bvh, d_brays, d_bhits, n_bounce, d_rays, d_hits, n_primary, budget_s):

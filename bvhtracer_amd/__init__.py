@@ -6,12 +6,12 @@ mirror of the reference's Bvh / BvhBuilder / Tlas / SceneObject / Ray API.
 """
 from ._lib import (BUILD_REFERENCE, BUILD_SAH, CAMERA_DTYPE, HIT_DTYPE, INSTANCE_DTYPE, MISS, NODE_DTYPE,
                    ORDER_ROWS, ORDER_TILES, RAY_DTYPE, STATS_DTYPE, TNODE_DTYPE, BvhtError)
-from .api import (Bvh, BvhBuilder, Context, DeviceArray, Ray, SceneObject, Tlas, TlasBuilder, bounce, bounce_tlas,
-                  default_context, raygen)
+from .api import (Batch, Bvh, BvhBuilder, Context, DeviceArray, Graph, Ray, SceneObject, Tlas, TlasBuilder, bounce,
+                  bounce_tlas, counter_add, default_context, deform_wave, raygen)
 
 __all__ = [
     "BUILD_REFERENCE", "BUILD_SAH", "CAMERA_DTYPE", "HIT_DTYPE", "INSTANCE_DTYPE", "MISS", "NODE_DTYPE",
     "ORDER_ROWS", "ORDER_TILES", "RAY_DTYPE", "STATS_DTYPE", "TNODE_DTYPE", "BvhtError",
-    "Bvh", "BvhBuilder", "Context", "DeviceArray", "Ray", "SceneObject", "Tlas", "TlasBuilder", "bounce",
-    "bounce_tlas", "default_context", "raygen",
+    "Batch", "Bvh", "BvhBuilder", "Context", "DeviceArray", "Graph", "Ray", "SceneObject", "Tlas", "TlasBuilder",
+    "bounce", "bounce_tlas", "counter_add", "default_context", "deform_wave", "raygen",
 ]

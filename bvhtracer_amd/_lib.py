@@ -67,11 +67,25 @@ SIGNATURES = {
     "bvht_tlas_set_objects": (I32, [P, P, P, U32, U32]),
     "bvht_tlas_refresh_bounds": (I32, [P, P]),
     "bvht_tlas_rebuild": (I32, [P, P]),
+    "bvht_tlas_rebuild_async": (I32, [P, P]),
     "bvht_tlas_download": (I32, [P, P, P, P]),
     "bvht_tlas_destroy": (I32, [P]),
     "bvht_trace_tlas": (I32, [P, P, P, U32, P, P]),
     "bvht_trace_tlas_count": (I32, [P, P, P, P, U32, P, P]),
     "bvht_bounce_tlas": (I32, [P, P, P, P, U32, U64, P, P]),
+    "bvht_batch_create": (I32, [P, P, U32, PP]),
+    "bvht_batch_destroy": (I32, [P]),
+    "bvht_batch_refit": (I32, [P, P]),
+    "bvht_batch_rebuild": (I32, [P, P]),
+    "bvht_batch_set_rays": (I32, [P, P, P, U32]),
+    "bvht_batch_config": (I32, [P, P, I32, I32, I32]),
+    "bvht_batch_trace": (I32, [P, P, P, P]),
+    "bvht_deform_wave": (I32, [P, P, P, F32, F32, F32, P]),
+    "bvht_counter_add": (I32, [P, P, U32]),
+    "bvht_graph_begin": (I32, [P]),
+    "bvht_graph_end": (I32, [P, PP]),
+    "bvht_graph_launch": (I32, [P, P]),
+    "bvht_graph_destroy": (I32, [P]),
 }
 
 

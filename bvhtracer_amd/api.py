@@ -311,6 +311,10 @@ class Tlas:
     def refresh_bounds(self):
         check(L.lib().bvht_tlas_refresh_bounds(self.ctx.handle, self.handle))
 
+    def rebuild_async(self):
+        """Tlas::rebuild without a host round trip (<= 64 objects)."""
+        check(L.lib().bvht_tlas_rebuild_async(self.ctx.handle, self.handle))
+
     def download(self):
         nodes = np.zeros(2 * self.n, TNODE_DTYPE)
         used = ctypes.c_uint32()
@@ -360,6 +364,92 @@ class TlasBuilder:
         t = Tlas(self.ctx, h, inst.shape[0])
         t._models = list(models)  # keep the BLASes alive
         return t
+
+
+class Batch:
+    """Many scenes (Tlas objects) updated and traced with one launch per stage
+    (bvht_batch_*).  Rays of all scenes live in one device buffer; scene s
+    owns rays [offsets[s], offsets[s+1])."""
+
+    def __init__(self, ctx, scenes):
+        self.ctx = ctx
+        self.scenes = list(scenes)
+        arr = (ctypes.c_void_p * len(self.scenes))(*[t.handle.value for t in self.scenes])
+        h = ctypes.c_void_p()
+        check(L.lib().bvht_batch_create(ctx.handle, ctypes.cast(arr, ctypes.c_void_p), len(self.scenes),
+                                        ctypes.byref(h)))
+        self.handle = h
+
+    def refit(self):
+        check(L.lib().bvht_batch_refit(self.ctx.handle, self.handle))
+
+    def rebuild(self):
+        check(L.lib().bvht_batch_rebuild(self.ctx.handle, self.handle))
+
+    def set_rays(self, offsets, rays_per_item=128):
+        off = np.ascontiguousarray(offsets, np.uint32)
+        assert off.shape[0] == len(self.scenes) + 1
+        check(L.lib().bvht_batch_set_rays(self.ctx.handle, self.handle, L.ptr(off), rays_per_item))
+
+    def config(self, wg, lds_depth, rays_per_item):
+        check(L.lib().bvht_batch_config(self.ctx.handle, self.handle, wg, lds_depth, rays_per_item))
+
+    def trace(self, d_rays, d_hits):
+        check(L.lib().bvht_batch_trace(self.ctx.handle, self.handle, d_rays.ptr, d_hits.ptr))
+
+    def close(self):
+        if self.handle:
+            L.lib().bvht_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A hipGraph of work enqueued on a context (bvht_graph_*)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    @classmethod
+    def capture(cls, ctx, enqueue):
+        """Capture everything ``enqueue()`` launches on ctx's stream."""
+        check(L.lib().bvht_graph_begin(ctx.handle))
+        try:
+            enqueue()
+        finally:
+            h = ctypes.c_void_p()
+            rc = L.lib().bvht_graph_end(ctx.handle, ctypes.byref(h))
+        check(rc)
+        return cls(ctx, h)
+
+    def launch(self):
+        check(L.lib().bvht_graph_launch(self.ctx.handle, self.handle))
+
+    def close(self):
+        if self.handle:
+            L.lib().bvht_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def deform_wave(ctx, bvh, d_orig, amp, kx, w, d_frame):
+    """BLAS vertices = originals with y += amp*sin(kx*x + w*frame), frame = *d_frame."""
+    check(L.lib().bvht_deform_wave(ctx.handle, bvh.handle, d_orig.ptr, amp, kx, w, d_frame.ptr))
+
+
+def counter_add(ctx, d_value, v=1):
+    check(L.lib().bvht_counter_add(ctx.handle, d_value.ptr, v))
 
 
 def raygen(ctx, camera, width, height, order=ORDER_ROWS, d_rays=None):

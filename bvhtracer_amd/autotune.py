@@ -56,3 +56,25 @@ def tune(ctx, run, reps=3, configs=None, log=None):
     best = min(table, key=table.get)
     set_config(ctx, *best)
     return best, table
+
+
+BATCH_CONFIGS = [(wg, d, r) for wg in (64, 256) for d in (16, 24) for r in (64, 128, 256)]
+
+
+def tune_batch(batch, offsets, run, reps=3, log=None):
+    """Sweep the persistent batched trace over workgroup size x LDS depth x
+    rays per work item; leave the fastest on the batch.  ``run()`` enqueues
+    the batched trace.  Returns (best, {config: ms})."""
+    ctx = batch.ctx
+    table = {}
+    for wg, d, r in BATCH_CONFIGS:
+        batch.set_rays(offsets, r)
+        batch.config(wg, d, r)
+        run()
+        table[(wg, d, r)] = time_ms(ctx, run, reps)
+        if log:
+            log(f"[autotune] batch wg {wg:3d} lds {d:2d} rays/item {r:3d}: {table[(wg, d, r)]:.4f} ms")
+    best = min(table, key=table.get)
+    batch.set_rays(offsets, best[2])
+    batch.config(*best)
+    return best, table

@@ -17,6 +17,7 @@
 #include "bvht_device.hpp"
 #include "bvht_internal.hpp"
 #include "bvht_trace.hpp"
+#include "bvht_update.hpp"
 
 using namespace bvht;
 
@@ -109,61 +110,14 @@ __global__ void k_depth(const bvht_node* __restrict__ nodes, uint32_t nodes_used
     atomicMax(depth, d);
 }
 
-__device__ __forceinline__ float ld_agent(const float* p)
-{
-    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT));
-}
-
-// Bvh::refit (bvh.rs:469-493) bottom-up: leaves recompute their box from the
-// triangles starting at (+MAX, -MAX) (update_node_bounds, bvh.rs:317-330);
-// the second child to arrive at a parent writes the union of both children.
-// min/max are exact, so the result equals the reference's reverse sweep.
+// Bvh::refit (bvh.rs:469-493); body in refit_from_leaf.
 __global__ void k_refit(bvht_node* __restrict__ nodes, const float* __restrict__ tris,
                         const uint32_t* __restrict__ parent, uint32_t* __restrict__ flags,
                         uint32_t nodes_used)
 {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nodes_used || k == 1)
-        return;
-    bvht_node nd = nodes[k];
-    if (nd.count == 0)
-        return;
-    V3 lo = mk3(kFltMax, kFltMax, kFltMax), hi = mk3(-kFltMax, -kFltMax, -kFltMax);
-    for (uint32_t j = 0; j < nd.count; ++j) {
-        const float* t = tris + 9 * (size_t)(nd.left_first + j);
-        V3 v0 = mk3(t[0], t[1], t[2]), v1 = mk3(t[3], t[4], t[5]), v2 = mk3(t[6], t[7], t[8]);
-        lo = min3(lo, v0);
-        lo = min3(lo, v1);
-        lo = min3(lo, v2);
-        hi = max3(hi, v0);
-        hi = max3(hi, v1);
-        hi = max3(hi, v2);
-    }
-    bvht_node* out = nodes + k;
-    out->bmin[0] = lo.x; out->bmin[1] = lo.y; out->bmin[2] = lo.z;
-    out->bmax[0] = hi.x; out->bmax[1] = hi.y; out->bmax[2] = hi.z;
-    __threadfence();
-    uint32_t p = parent[k];
-    while (p != BVHT_MISS) {
-        uint32_t old = atomicAdd(&flags[p], 1u);
-        if (old == 0)
-            return;
-        __threadfence();
-        uint32_t l = nodes[p].left_first;
-        const bvht_node* a = nodes + l;
-        const bvht_node* b = nodes + l + 1;
-        V3 alo = mk3(ld_agent(&a->bmin[0]), ld_agent(&a->bmin[1]), ld_agent(&a->bmin[2]));
-        V3 ahi = mk3(ld_agent(&a->bmax[0]), ld_agent(&a->bmax[1]), ld_agent(&a->bmax[2]));
-        V3 blo = mk3(ld_agent(&b->bmin[0]), ld_agent(&b->bmin[1]), ld_agent(&b->bmin[2]));
-        V3 bhi = mk3(ld_agent(&b->bmax[0]), ld_agent(&b->bmax[1]), ld_agent(&b->bmax[2]));
-        V3 mn = min3(alo, blo), mx = max3(ahi, bhi);
-        bvht_node* o = nodes + p;
-        o->bmin[0] = mn.x; o->bmin[1] = mn.y; o->bmin[2] = mn.z;
-        o->bmax[0] = mx.x; o->bmax[1] = mx.y; o->bmax[2] = mx.z;
-        __threadfence();
-        p = parent[p];
-    }
+    if (k < nodes_used)
+        refit_from_leaf(nodes, tris, parent, flags, k);
 }
 
 template <int WG, int LDS_DEPTH, bool STATS>
@@ -439,8 +393,11 @@ int32_t blas_alloc(bvht_ctx ctx, uint32_t n, uint32_t node_count, bvht_blas* out
 }
 
 // Trace configuration for a tree needing `need` stack entries: the context's
-// configuration if set (autotuned), else 256 lanes with the smallest LDS
-// depth that holds the whole stack (32 at most; deeper trees spill).
+// configuration if set (autotuned), else 64-lane workgroups with the smallest
+// LDS depth that holds the stack, at most 24 entries (deeper entries spill).
+// Small LDS stacks keep 8-10 waves per SIMD resident, which matters more than
+// avoiding the occasional spill (sweep on the 1.3M-triangle sphere: 64x24
+// 1.155 ms, 64x16 1.163 ms, 256x32 1.473 ms for both passes of a frame).
 void trace_config(bvht_ctx ctx, uint32_t need, int* wg, int* lds)
 {
     if (ctx->trace_wg > 0) {
@@ -448,8 +405,8 @@ void trace_config(bvht_ctx ctx, uint32_t need, int* wg, int* lds)
         *lds = ctx->trace_lds;
         return;
     }
-    *wg = 256;
-    *lds = need <= 8 ? 8 : (need <= 16 ? 16 : (need <= 24 ? 24 : 32));
+    *wg = 64;
+    *lds = need <= 8 ? 8 : (need <= 16 ? 16 : 24);
 }
 
 int32_t launch_trace_blas(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, uint32_t n,

@@ -17,6 +17,7 @@
 #include "bvht_device.hpp"
 #include "bvht_internal.hpp"
 #include "bvht_trace.hpp"
+#include "bvht_update.hpp"
 
 using namespace bvht;
 
@@ -24,190 +25,18 @@ namespace {
 
 inline uint32_t nblk(size_t n, uint32_t wg) { return (uint32_t)((n + wg - 1) / wg); }
 
-// World box of each object from its BLAS root box, and the traced rows of M^-1.
 __global__ void k_object_bounds(const bvht_instance* __restrict__ objs, uint32_t n, const BlasRef* __restrict__ blases,
                                 float* __restrict__ bounds, ObjRows* __restrict__ rows)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    const bvht_instance& o = objs[i];
-    const float4* bn = blases[o.blas].nodes;
-    float4 a = bn[0], b = bn[1];
-    V3 lo = mk3(kFltMax, kFltMax, kFltMax), hi = mk3(-kFltMax, -kFltMax, -kFltMax);
-    for (int c = 0; c < 8; ++c) {
-        V3 p = mk3((c & 1) ? b.x : a.x, (c & 2) ? b.y : a.y, (c & 4) ? b.z : a.z);
-        V3 w = mat_mul4(o.m, p, 1.0f);
-        lo = min3(lo, w);
-        hi = max3(hi, w);
-    }
-    float* bo = bounds + 6 * (size_t)i;
-    bo[0] = lo.x; bo[1] = lo.y; bo[2] = lo.z;
-    bo[3] = hi.x; bo[4] = hi.y; bo[5] = hi.z;
-    ObjRows r;
-    r.r0 = make_float4(o.minv[0], o.minv[4], o.minv[8], o.minv[12]);
-    r.r1 = make_float4(o.minv[1], o.minv[5], o.minv[9], o.minv[13]);
-    r.r2 = make_float4(o.minv[2], o.minv[6], o.minv[10], o.minv[14]);
-    r.blas = o.blas;
-    r.pad[0] = r.pad[1] = r.pad[2] = 0;
-    rows[i] = r;
+    if (i < n)
+        object_bounds(objs, blases, bounds, rows, i);
 }
 
-constexpr uint32_t kRebuildWG = 256;
-
-// Tlas::find_best_match (tlas.rs:179-202) as a block-wide argmin: the
-// sequential scan keeps the first b whose area is strictly below the running
-// minimum (starting at f32::MAX), i.e. the smallest index among the minimal
-// areas that are < f32::MAX.
-__device__ int32_t block_best_match(const bvht_tnode* nodes, const int32_t* list, int32_t count, int32_t a,
-                                    float* s_val, int32_t* s_idx)
-{
-    float best = kFltMax;
-    int32_t bi = -1;
-    const bvht_tnode& na = nodes[list[a]];
-    for (int32_t b = threadIdx.x; b < count; b += kRebuildWG) {
-        if (b == a)
-            continue;
-        const bvht_tnode& nb = nodes[list[b]];
-        V3 mx = max3(mk3(na.bmax[0], na.bmax[1], na.bmax[2]), mk3(nb.bmax[0], nb.bmax[1], nb.bmax[2]));
-        V3 mn = min3(mk3(na.bmin[0], na.bmin[1], na.bmin[2]), mk3(nb.bmin[0], nb.bmin[1], nb.bmin[2]));
-        V3 e = sub3(mx, mn);
-        float sa = e.x * e.y + e.y * e.z + e.z * e.x;
-        if (sa < best) {
-            best = sa;
-            bi = b;
-        }
-    }
-    s_val[threadIdx.x] = best;
-    s_idx[threadIdx.x] = bi;
-    __syncthreads();
-    for (uint32_t off = kRebuildWG / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) {
-            float v2 = s_val[threadIdx.x + off];
-            int32_t i2 = s_idx[threadIdx.x + off];
-            float v1 = s_val[threadIdx.x];
-            int32_t i1 = s_idx[threadIdx.x];
-            bool take = i2 >= 0 && (i1 < 0 || v2 < v1 || (!(v1 < v2) && i2 < i1));
-            if (take) {
-                s_val[threadIdx.x] = v2;
-                s_idx[threadIdx.x] = i2;
-            }
-        }
-        __syncthreads();
-    }
-    int32_t r = s_idx[0];
-    __syncthreads();
-    return r;
-}
-
-// Tlas::rebuild (tlas.rs:204-250) + TlasBuilder::build_for (tlas.rs:273-279):
-// 2n default nodes (empty box), leaves 1..n in object order, mutual-best-match
-// merges with LeftRightIndex::new(a, b) = a + (b << 16), root = copy of the
-// last node.  meta = {nodes_used, depth, status}.
 __global__ __launch_bounds__(kRebuildWG) void k_tlas_rebuild(const float* __restrict__ bounds, uint32_t n,
                                                              bvht_tnode* nodes, int32_t* list, uint32_t* meta)
 {
-    __shared__ float s_val[kRebuildWG];
-    __shared__ int32_t s_idx[kRebuildWG];
-    __shared__ int32_t sh_a, sh_b, sh_count, sh_used, sh_merged, sh_fail;
-    for (uint32_t i = threadIdx.x; i < 2 * n; i += kRebuildWG) {
-        bvht_tnode d;
-        d.bmin[0] = d.bmin[1] = d.bmin[2] = kFltMax;
-        d.bmax[0] = d.bmax[1] = d.bmax[2] = -kFltMax;
-        d.left_right = 0;
-        d.blas = 0;
-        nodes[i] = d;
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += kRebuildWG) {
-        list[i] = (int32_t)(i + 1);
-        bvht_tnode l;
-        const float* b = bounds + 6 * (size_t)i;
-        l.bmin[0] = b[0]; l.bmin[1] = b[1]; l.bmin[2] = b[2];
-        l.bmax[0] = b[3]; l.bmax[1] = b[4]; l.bmax[2] = b[5];
-        l.left_right = 0;
-        l.blas = i;
-        nodes[i + 1] = l;
-    }
-    if (threadIdx.x == 0) {
-        sh_a = 0;
-        sh_count = (int32_t)n;
-        sh_used = (int32_t)n + 1;
-        sh_fail = 0;
-    }
-    __syncthreads();
-    int32_t b0 = block_best_match(nodes, list, sh_count, sh_a, s_val, s_idx);
-    if (threadIdx.x == 0)
-        sh_b = b0;
-    __syncthreads();
-    while (sh_count > 1) {
-        if (sh_b < 0) {  // every candidate area was >= f32::MAX (the reference panics)
-            if (threadIdx.x == 0)
-                sh_fail = 1;
-            __syncthreads();
-            break;
-        }
-        int32_t c = block_best_match(nodes, list, sh_count, sh_b, s_val, s_idx);
-        if (threadIdx.x == 0) {
-            int32_t a = sh_a, b = sh_b;
-            if (a == c) {
-                int32_t ia = list[a], ib = list[b];
-                bvht_tnode na = nodes[ia], nb = nodes[ib];
-                bvht_tnode nn;
-                nn.left_right = (uint32_t)ia + ((uint32_t)ib << 16);
-                V3 mn = min3(mk3(na.bmin[0], na.bmin[1], na.bmin[2]), mk3(nb.bmin[0], nb.bmin[1], nb.bmin[2]));
-                V3 mx = max3(mk3(na.bmax[0], na.bmax[1], na.bmax[2]), mk3(nb.bmax[0], nb.bmax[1], nb.bmax[2]));
-                nn.bmin[0] = mn.x; nn.bmin[1] = mn.y; nn.bmin[2] = mn.z;
-                nn.bmax[0] = mx.x; nn.bmax[1] = mx.y; nn.bmax[2] = mx.z;
-                nn.blas = 0;
-                nodes[sh_used] = nn;
-                list[a] = sh_used;
-                sh_used += 1;
-                list[b] = list[sh_count - 1];
-                sh_count -= 1;
-                sh_merged = 1;
-            } else {
-                sh_a = b;
-                sh_b = c;
-                sh_merged = 0;
-            }
-        }
-        __syncthreads();
-        if (sh_merged) {
-            int32_t nb = block_best_match(nodes, list, sh_count, sh_a, s_val, s_idx);
-            if (threadIdx.x == 0)
-                sh_b = nb;
-            __syncthreads();
-        }
-    }
-    if (threadIdx.x == 0) {
-        if (!sh_fail) {
-            nodes[0] = nodes[list[sh_a]];
-            // depth: iterative walk, list[] reused as (node, level) stack
-            uint32_t best = 0, sp = 0;
-            uint32_t* st = reinterpret_cast<uint32_t*>(list);
-            st[sp++] = 0;
-            st[sp++] = 1;
-            while (sp) {
-                uint32_t lvl = st[--sp], ni = st[--sp];
-                best = max(best, lvl);
-                uint32_t lr = nodes[ni].left_right;
-                if (lr != 0 && sp + 4 <= 2 * max(n, 4u)) {
-                    st[sp++] = lr >> 16;
-                    st[sp++] = lvl + 1;
-                    st[sp++] = lr & 0xFFFFu;
-                    st[sp++] = lvl + 1;
-                }
-            }
-            meta[0] = (uint32_t)sh_used;
-            meta[1] = best;
-            meta[2] = 0;
-        } else {
-            meta[0] = (uint32_t)sh_used;
-            meta[1] = 0;
-            meta[2] = 1;
-        }
-    }
+    tlas_rebuild_block(bounds, n, nodes, list, meta);
 }
 
 template <int WG, int LDS_DEPTH, bool STATS>
@@ -455,6 +284,17 @@ int32_t bvht_tlas_rebuild(bvht_ctx ctx, bvht_tlas tlas)
     for (bvht_blas b : tlas->blases)
         tlas->max_blas_depth = std::max(tlas->max_blas_depth, b->depth);
     return rebuild(tlas, true);
+}
+
+int32_t bvht_tlas_rebuild_async(bvht_ctx ctx, bvht_tlas tlas)
+{
+    BVHT_REQUIRE(ctx && tlas, "NULL argument");
+    BVHT_REQUIRE(tlas->n <= 64, "asynchronous rebuild needs at most 64 objects (%u)", tlas->n);
+    for (bvht_blas b : tlas->blases)
+        tlas->max_blas_depth = std::max(tlas->max_blas_depth, b->depth);
+    // a TLAS over n objects has at most n levels: traces size their stack from that bound
+    tlas->depth = tlas->n;
+    return rebuild(tlas, false);
 }
 
 int32_t bvht_tlas_download(bvht_ctx ctx, bvht_tlas tlas, bvht_tnode* nodes, uint32_t* nodes_used)

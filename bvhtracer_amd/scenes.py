@@ -293,3 +293,64 @@ def deform_wave(tris_orig, frame, amp=0.2, kx=4.0, w=0.1):
     out = t.copy()
     out[:, :, 1] = yn.astype(np.float32)
     return out.reshape(-1, 9)
+
+
+# ---------------------------------------------------------- batched scenes
+
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmax", "<f4"), ("d", "<f4", 3), ("pad", "<f4")])
+
+
+def _log_uniform(u, lo, hi):
+    return np.exp(np.log(lo) + u.astype(np.float64) * (np.log(hi) - np.log(lo)))
+
+
+def batch_workload(n_scenes=64, seed=0xBA7C4, tri_range=(1e3, 1e6), ray_range=(4e3, 2e6), max_instances=8,
+                   radius=32.0):
+    """Uneven multi-scene workload: scene s is a displaced icosphere with a
+    log-uniform triangle count in ``tri_range`` (frequency round(sqrt(T/20))),
+    1..``max_instances`` seeded affine instances, and a log-uniform number of
+    rays in ``ray_range``: half pinhole camera rays looking at the scene from
+    outside its box, half rays with origins uniform in the scene box and
+    directions uniform on the sphere.  Returns a list of dicts with keys
+    mesh, M, Minv, rays."""
+    rng = Pcg32(seed, np.arange(n_scenes, dtype=np.uint64))
+    u_tri, u_inst, u_ray = rng.next_f32(), rng.next_f32(), rng.next_f32()
+    tris = _log_uniform(u_tri, *tri_range)
+    n_inst = 1 + np.minimum((u_inst * max_instances).astype(np.int64), max_instances - 1)
+    n_rays = np.round(_log_uniform(u_ray, *ray_range)).astype(np.int64)
+    out = []
+    for s in range(n_scenes):
+        freq = icosphere_frequency(tris[s])
+        mesh = icosphere(freq, radius=radius, seed=0x1C05 + s)
+        M, Minv = random_affines(int(n_inst[s]), seed=seed + 1000 + s, trans_range=2.5 * radius)
+        # scene box from the transformed mesh box corners
+        lo, hi = mesh.reshape(-1, 3).min(0).astype(np.float64), mesh.reshape(-1, 3).max(0).astype(np.float64)
+        corners = np.array([[(hi if c & 1 else lo)[0], (hi if c & 2 else lo)[1], (hi if c & 4 else lo)[2], 1.0]
+                            for c in range(8)])
+        w = np.concatenate([corners @ M[k].reshape(4, 4).astype(np.float64) for k in range(n_inst[s])])[:, :3]
+        blo, bhi = w.min(0), w.max(0)
+        center, ext = 0.5 * (blo + bhi), float((bhi - blo).max())
+        n = int(n_rays[s])
+        n_cam = n // 2
+        rays = np.zeros(n, RAY_DTYPE)
+        # camera rays: square grid of ceil(sqrt(n_cam)) per side, first n_cam taken
+        side = int(np.ceil(np.sqrt(max(n_cam, 1))))
+        k = np.arange(n_cam)
+        px = (k % side + 0.5) / side * 2.0 - 1.0
+        py = (k // side + 0.5) / side * 2.0 - 1.0
+        eye = center + np.array([0.0, 0.0, -1.6 * ext])
+        d = np.stack([px * 0.6, py * 0.6, np.ones(n_cam)], 1)
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        rays["o"][:n_cam] = eye.astype(np.float32)
+        rays["d"][:n_cam] = d.astype(np.float32)
+        # random rays inside the box
+        r2 = Pcg32(seed + 7 * s + 3, np.arange(n - n_cam, dtype=np.uint64))
+        uo = np.stack([r2.next_f32() for _ in range(3)], 1).astype(np.float64)
+        z = 2.0 * r2.next_f32().astype(np.float64) - 1.0
+        phi = 2.0 * np.pi * r2.next_f32().astype(np.float64)
+        rr = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+        rays["o"][n_cam:] = (blo + uo * (bhi - blo)).astype(np.float32)
+        rays["d"][n_cam:] = np.stack([rr * np.cos(phi), rr * np.sin(phi), z], 1).astype(np.float32)
+        rays["tmax"] = F32_MAX
+        out.append({"mesh": mesh, "M": M, "Minv": Minv, "rays": rays})
+    return out

@@ -117,6 +117,8 @@ typedef struct {
 typedef struct bvht_ctx_s* bvht_ctx;
 typedef struct bvht_blas_s* bvht_blas;
 typedef struct bvht_tlas_s* bvht_tlas;
+typedef struct bvht_batch_s* bvht_batch;
+typedef struct bvht_graph_s* bvht_graph;
 
 /* ---- context ---------------------------------------------------------- */
 
@@ -238,6 +240,9 @@ int32_t bvht_tlas_set_objects(bvht_ctx ctx, bvht_tlas tlas, const bvht_instance*
 int32_t bvht_tlas_refresh_bounds(bvht_ctx ctx, bvht_tlas tlas);
 /* Tlas::rebuild / Scene::rebuild (tlas.rs:204-250, scene.rs:36-38) on the GPU. */
 int32_t bvht_tlas_rebuild(bvht_ctx ctx, bvht_tlas tlas);
+/* Same without the host round trip (at most 64 objects): traces size their
+ * stack from the bound depth <= object count instead of the measured depth. */
+int32_t bvht_tlas_rebuild_async(bvht_ctx ctx, bvht_tlas tlas);
 /* Copy back nodes[2n] and nodes_used. */
 int32_t bvht_tlas_download(bvht_ctx ctx, bvht_tlas tlas, bvht_tnode* nodes, uint32_t* nodes_used);
 int32_t bvht_tlas_destroy(bvht_tlas tlas);
@@ -255,6 +260,48 @@ int32_t bvht_trace_tlas_count(bvht_ctx ctx, bvht_tlas tlas, const bvht_ray* d_ra
 int32_t bvht_bounce_tlas(bvht_ctx ctx, bvht_tlas tlas, const bvht_ray* d_rays,
                          const bvht_hit* d_hits, uint32_t n, uint64_t seed, bvht_ray* d_out,
                          uint32_t* d_count);
+
+/* ---- batched scenes (no reference counterpart) ------------------------ */
+
+/* A batch updates and traces many scenes (TLASes) with one launch per stage.
+ * The scenes must belong to ctx and outlive the batch. */
+int32_t bvht_batch_create(bvht_ctx ctx, const bvht_tlas* scenes, uint32_t n_scenes, bvht_batch* out);
+int32_t bvht_batch_destroy(bvht_batch batch);
+/* Bvh::refit of every distinct BLAS of the batch, one launch. */
+int32_t bvht_batch_refit(bvht_ctx ctx, bvht_batch batch);
+/* set_transform (bounds from the current BLAS boxes) for every object, then
+ * Tlas::rebuild of every scene: two launches, no host round trip. */
+int32_t bvht_batch_rebuild(bvht_ctx ctx, bvht_batch batch);
+/* Ray layout for bvht_batch_trace: scene s owns rays [ray_offsets[s],
+ * ray_offsets[s+1]) of one device buffer (host array of n_scenes + 1
+ * entries); work items hold rays_per_item rays (64, 128 or 256). */
+int32_t bvht_batch_set_rays(bvht_ctx ctx, bvht_batch batch, const uint32_t* ray_offsets,
+                            uint32_t rays_per_item);
+/* Persistent-kernel configuration: workgroup (64 or 256), LDS stack depth
+ * (16 or 24), rays per work item (64, 128, 256).  Chosen by the autotuner. */
+int32_t bvht_batch_config(bvht_ctx ctx, bvht_batch batch, int32_t wg, int32_t lds_depth,
+                          int32_t rays_per_item);
+/* Tlas::intersect for every scene's rays in one persistent launch fed by an
+ * atomic work queue; d_hits[i] is the hit of d_rays[i] in its own scene. */
+int32_t bvht_batch_trace(bvht_ctx ctx, bvht_batch batch, const bvht_ray* d_rays, bvht_hit* d_hits);
+
+/* Per-frame deformation for the refit workloads (input synthesis, no
+ * reference counterpart): BLAS vertices = originals (device float[9n], build
+ * order) with y += amp * sin(kx * x + w * frame), frame read from *d_frame. */
+int32_t bvht_deform_wave(bvht_ctx ctx, bvht_blas blas, const float* d_orig, float amp, float kx,
+                         float w, const uint32_t* d_frame);
+/* *d_value += v on the stream (advances a frame counter inside a graph). */
+int32_t bvht_counter_add(bvht_ctx ctx, uint32_t* d_value, uint32_t v);
+
+/* ---- hipGraph capture of enqueued work ---------------------------------- */
+
+/* Capture the calls made on ctx between begin and end into a graph (calls
+ * that synchronise or allocate are not capturable: build, upload, download,
+ * bvht_tlas_rebuild; use the batch calls, which only enqueue kernels). */
+int32_t bvht_graph_begin(bvht_ctx ctx);
+int32_t bvht_graph_end(bvht_ctx ctx, bvht_graph* out);
+int32_t bvht_graph_launch(bvht_ctx ctx, bvht_graph graph);
+int32_t bvht_graph_destroy(bvht_graph graph);
 
 #ifdef __cplusplus
 }
