@@ -2,8 +2,8 @@
 //
 // A batch is a set of scenes (TLASes, each over its own BLASes) updated and
 // traced with one launch per stage instead of one per scene:
-//   * refit: every distinct BLAS of the batch, one thread per node, one
-//     launch (refit_from_leaf on per-BLAS slices of one arrival-flag array);
+//   * refit: every distinct BLAS of the batch, one launch per tree level
+//     (the deepest level of all BLASes first), one thread per node;
 //   * object bounds: every object of every scene, one launch;
 //   * TLAS rebuild: one workgroup per scene, one launch;
 //   * trace: a persistent kernel.  The rays of all scenes sit in one buffer,
@@ -49,9 +49,6 @@ struct BatchScene {
 struct BatchBlas {
     bvht_node* nodes;
     const float* tris;
-    const uint32_t* parent;
-    uint32_t nodes_used;
-    uint32_t flag_off;
 };
 
 __device__ __forceinline__ uint32_t upper_index(const uint32_t* prefix, uint32_t n, uint32_t x)
@@ -68,17 +65,16 @@ __device__ __forceinline__ uint32_t upper_index(const uint32_t* prefix, uint32_t
     return lo;
 }
 
-__global__ void k_batch_refit(const BatchBlas* __restrict__ blases, const uint32_t* __restrict__ node_prefix,
-                              uint32_t n_blas, uint32_t* __restrict__ flags)
+// One tree level of every BLAS of the batch: entries are (BLAS, node) pairs.
+__global__ void k_batch_refit_level(const BatchBlas* __restrict__ blases, const uint2* __restrict__ entries,
+                                    uint32_t count)
 {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= node_prefix[n_blas])
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count)
         return;
-    uint32_t b = upper_index(node_prefix, n_blas, k);
-    BatchBlas bb = blases[b];
-    uint32_t local = k - node_prefix[b];
-    if (local < bb.nodes_used)
-        refit_from_leaf(bb.nodes, bb.tris, bb.parent, flags + bb.flag_off, local);
+    uint2 e = entries[i];
+    BatchBlas bb = blases[e.x];
+    refit_node(bb.nodes, bb.tris, e.y);
 }
 
 __global__ void k_batch_bounds(const BatchScene* __restrict__ scenes, const uint32_t* __restrict__ obj_prefix,
@@ -173,9 +169,9 @@ struct bvht_batch_s {
     std::vector<bvht_blas> blases;
     BatchScene* d_scenes = nullptr;
     BatchBlas* d_blases = nullptr;
-    uint32_t* d_node_prefix = nullptr;
+    uint2* d_refit = nullptr;           // (BLAS, node) of every used node, grouped by level
+    std::vector<uint32_t> refit_off;     // level L occupies [refit_off[L], refit_off[L+1])
     uint32_t* d_obj_prefix = nullptr;
-    uint32_t* d_flags = nullptr;
     uint32_t total_nodes = 0, total_objs = 0;
     uint32_t cap = 0;  // traversal stack bound: max over scenes of (objects - 1) + (BLAS depth - 1)
     // trace tables (bvht_batch_set_rays)
@@ -200,9 +196,8 @@ void batch_free(bvht_batch b)
 {
     (void)hipFree(b->d_scenes);
     (void)hipFree(b->d_blases);
-    (void)hipFree(b->d_node_prefix);
+    (void)hipFree(b->d_refit);
     (void)hipFree(b->d_obj_prefix);
-    (void)hipFree(b->d_flags);
     (void)hipFree(b->d_items);
     (void)hipFree(b->d_ray_off);
     (void)hipFree(b->d_counter);
@@ -266,23 +261,43 @@ int32_t bvht_batch_create(bvht_ctx ctx, const bvht_tlas* scenes, uint32_t n_scen
                 b->blases.push_back(bl);
     }
     std::vector<BatchBlas> hb(b->blases.size());
-    std::vector<uint32_t> node_prefix(b->blases.size() + 1, 0);
+    uint32_t max_depth = 0;
     for (size_t k = 0; k < b->blases.size(); ++k) {
         bvht_blas bl = b->blases[k];
         hb[k].nodes = bl->d_nodes;
         hb[k].tris = bl->d_tris;
-        hb[k].parent = bl->d_parent;
-        hb[k].nodes_used = bl->nodes_used;
-        hb[k].flag_off = node_prefix[k];
-        node_prefix[k + 1] = node_prefix[k] + bl->nodes_used;
+        max_depth = std::max(max_depth, bl->depth);
     }
-    b->total_nodes = node_prefix.back();
+    // combined level table: level L lists (BLAS, node) for every BLAS's level-L nodes
+    std::vector<std::vector<uint32_t>> orders(b->blases.size());
+    b->refit_off.assign(max_depth + 1, 0);
+    for (size_t k = 0; k < b->blases.size(); ++k) {
+        bvht_blas bl = b->blases[k];
+        orders[k].resize(bl->level_off.back());
+        if (hipMemcpy(orders[k].data(), bl->d_level_order, sizeof(uint32_t) * orders[k].size(),
+                      hipMemcpyDeviceToHost) != hipSuccess) {
+            delete b;
+            return set_error(BVHT_EDEVICE, "batch: level table download failed");
+        }
+        for (uint32_t L = 0; L < bl->depth; ++L)
+            b->refit_off[L + 1] += bl->level_off[L + 1] - bl->level_off[L];
+    }
+    for (uint32_t L = 0; L < max_depth; ++L)
+        b->refit_off[L + 1] += b->refit_off[L];
+    b->total_nodes = b->refit_off[max_depth];
+    std::vector<uint2> entries(b->total_nodes);
+    std::vector<uint32_t> cursor(b->refit_off.begin(), b->refit_off.end() - 1);
+    for (size_t k = 0; k < b->blases.size(); ++k) {
+        bvht_blas bl = b->blases[k];
+        for (uint32_t L = 0; L < bl->depth; ++L)
+            for (uint32_t i = bl->level_off[L]; i < bl->level_off[L + 1]; ++i)
+                entries[cursor[L]++] = make_uint2((uint32_t)k, orders[k][i]);
+    }
     b->total_objs = obj_prefix.back();
     bool ok = hipMalloc(&b->d_scenes, sizeof(BatchScene) * n_scenes) == hipSuccess &&
               hipMalloc(&b->d_blases, sizeof(BatchBlas) * hb.size()) == hipSuccess &&
-              hipMalloc(&b->d_node_prefix, sizeof(uint32_t) * node_prefix.size()) == hipSuccess &&
+              hipMalloc(&b->d_refit, sizeof(uint2) * std::max<size_t>(entries.size(), 1)) == hipSuccess &&
               hipMalloc(&b->d_obj_prefix, sizeof(uint32_t) * obj_prefix.size()) == hipSuccess &&
-              hipMalloc(&b->d_flags, sizeof(uint32_t) * std::max<uint32_t>(b->total_nodes, 1)) == hipSuccess &&
               hipMalloc(&b->d_items, sizeof(uint32_t) * (n_scenes + 1)) == hipSuccess &&
               hipMalloc(&b->d_ray_off, sizeof(uint32_t) * (n_scenes + 1)) == hipSuccess &&
               hipMalloc(&b->d_counter, sizeof(uint32_t)) == hipSuccess;
@@ -293,8 +308,7 @@ int32_t bvht_batch_create(bvht_ctx ctx, const bvht_tlas* scenes, uint32_t n_scen
     hipStream_t st = ctx->stream;
     if (hipMemcpyAsync(b->d_scenes, hs.data(), sizeof(BatchScene) * n_scenes, hipMemcpyHostToDevice, st) ||
         hipMemcpyAsync(b->d_blases, hb.data(), sizeof(BatchBlas) * hb.size(), hipMemcpyHostToDevice, st) ||
-        hipMemcpyAsync(b->d_node_prefix, node_prefix.data(), sizeof(uint32_t) * node_prefix.size(),
-                       hipMemcpyHostToDevice, st) ||
+        hipMemcpyAsync(b->d_refit, entries.data(), sizeof(uint2) * entries.size(), hipMemcpyHostToDevice, st) ||
         hipMemcpyAsync(b->d_obj_prefix, obj_prefix.data(), sizeof(uint32_t) * obj_prefix.size(),
                        hipMemcpyHostToDevice, st) ||
         hipStreamSynchronize(st)) {
@@ -322,11 +336,12 @@ int32_t bvht_batch_destroy(bvht_batch b)
 int32_t bvht_batch_refit(bvht_ctx ctx, bvht_batch b)
 {
     BVHT_REQUIRE(ctx && b && b->ctx == ctx, "NULL argument or foreign context");
-    int32_t rc = fill_u32(ctx, b->d_flags, 0, b->total_nodes);
-    if (rc)
-        return rc;
-    hipLaunchKernelGGL(k_batch_refit, dim3(nblk(b->total_nodes, 256)), dim3(256), 0, ctx->stream, b->d_blases,
-                       b->d_node_prefix, (uint32_t)b->blases.size(), b->d_flags);
+    for (size_t L = b->refit_off.size() - 1; L-- > 0;) {
+        uint32_t off = b->refit_off[L], cnt = b->refit_off[L + 1] - off;
+        if (cnt)
+            hipLaunchKernelGGL(k_batch_refit_level, dim3(nblk(cnt, 256)), dim3(256), 0, ctx->stream, b->d_blases,
+                               b->d_refit + off, cnt);
+    }
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }

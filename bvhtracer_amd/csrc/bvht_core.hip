@@ -93,31 +93,48 @@ __global__ void k_parents(const bvht_node* __restrict__ nodes, uint32_t nodes_us
     }
 }
 
-// Depth of the tree: every leaf walks to the root.
-__global__ void k_depth(const bvht_node* __restrict__ nodes, uint32_t nodes_used,
-                        const uint32_t* __restrict__ parent, uint32_t* __restrict__ depth)
+// Level of every used node (steps to the root) and the tree depth.
+__global__ void k_levels(uint32_t nodes_used, const uint32_t* __restrict__ parent, uint32_t* __restrict__ level,
+                         uint32_t* __restrict__ depth)
 {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nodes_used || k == 1)
+    if (k >= nodes_used)
         return;
-    if (nodes[k].count == 0)
+    if (k == 1) {
+        level[k] = BVHT_MISS;
         return;
-    uint32_t d = 1, p = parent[k];
-    while (p != BVHT_MISS && d < nodes_used + 1) {
+    }
+    uint32_t d = 0, p = parent[k];
+    while (p != BVHT_MISS && d < nodes_used) {
         ++d;
         p = parent[p];
     }
-    atomicMax(depth, d);
+    level[k] = d;
+    atomicMax(depth, d + 1);
 }
 
-// Bvh::refit (bvh.rs:469-493); body in refit_from_leaf.
-__global__ void k_refit(bvht_node* __restrict__ nodes, const float* __restrict__ tris,
-                        const uint32_t* __restrict__ parent, uint32_t* __restrict__ flags,
-                        uint32_t nodes_used)
+__global__ void k_level_hist(uint32_t nodes_used, const uint32_t* __restrict__ level, uint32_t* __restrict__ hist)
 {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nodes_used)
-        refit_from_leaf(nodes, tris, parent, flags, k);
+    if (k < nodes_used && k != 1)
+        atomicAdd(&hist[level[k]], 1u);
+}
+
+__global__ void k_level_scatter(uint32_t nodes_used, const uint32_t* __restrict__ level, uint32_t* __restrict__ cursor,
+                                uint32_t* __restrict__ order)
+{
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nodes_used && k != 1)
+        order[atomicAdd(&cursor[level[k]], 1u)] = k;
+}
+
+// Bvh::refit, one tree level (see refit_node).
+__global__ void k_refit_level(bvht_node* __restrict__ nodes, const float* __restrict__ tris,
+                              const uint32_t* __restrict__ order, uint32_t count)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        refit_node(nodes, tris, order[i]);
 }
 
 template <int WG, int LDS_DEPTH, bool STATS>
@@ -318,30 +335,66 @@ int32_t fill_u32(bvht_ctx ctx, uint32_t* p, uint32_t v, size_t n)
     return BVHT_OK;
 }
 
-// parent array + depth on the device (after a build or an upload)
+// Parent array, node levels, depth and the level-grouped node order on the
+// device (after a build or an upload); the per-level node counts come back to
+// the host to drive the level-by-level refit.
 int32_t compute_blas_topology(bvht_blas b, const bvht_node*)
 {
     bvht_ctx ctx = b->ctx;
+    hipStream_t st = ctx->stream;
     int32_t rc = fill_u32(ctx, b->d_parent, BVHT_MISS, b->node_count);
     if (rc)
         return rc;
-    hipLaunchKernelGGL(k_parents, dim3(blocks_for(b->nodes_used, 256)), dim3(256), 0, ctx->stream, b->d_nodes,
-                       b->nodes_used, b->d_parent);
-    BVHT_HIP(hipGetLastError());
-    rc = ensure_scratch(ctx, 4);
+    uint32_t nb = blocks_for(b->nodes_used, 256);
+    hipLaunchKernelGGL(k_parents, dim3(nb), dim3(256), 0, st, b->d_nodes, b->nodes_used, b->d_parent);
+    rc = ensure_scratch(ctx, sizeof(uint32_t) * (2 * (size_t)b->nodes_used + 2));
     if (rc)
         return rc;
     uint32_t* d_depth = static_cast<uint32_t*>(ctx->scratch);
     rc = fill_u32(ctx, d_depth, 0, 1);
     if (rc)
         return rc;
-    hipLaunchKernelGGL(k_depth, dim3(blocks_for(b->nodes_used, 256)), dim3(256), 0, ctx->stream, b->d_nodes,
-                       b->nodes_used, b->d_parent, d_depth);
+    hipLaunchKernelGGL(k_levels, dim3(nb), dim3(256), 0, st, b->nodes_used, b->d_parent, b->d_level, d_depth);
     BVHT_HIP(hipGetLastError());
     uint32_t depth = 0;
-    BVHT_HIP(hipMemcpyAsync(&depth, d_depth, 4, hipMemcpyDeviceToHost, ctx->stream));
-    BVHT_HIP(hipStreamSynchronize(ctx->stream));
+    BVHT_HIP(hipMemcpyAsync(&depth, d_depth, 4, hipMemcpyDeviceToHost, st));
+    BVHT_HIP(hipStreamSynchronize(st));
+    if (depth == 0 || depth > b->nodes_used)
+        return set_error(BVHT_EDEVICE, "BLAS topology: depth %u out of range", depth);
     b->depth = depth;
+    uint32_t* hist = d_depth + 1;
+    rc = fill_u32(ctx, hist, 0, depth);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_level_hist, dim3(nb), dim3(256), 0, st, b->nodes_used, b->d_level, hist);
+    std::vector<uint32_t> h(depth);
+    BVHT_HIP(hipMemcpyAsync(h.data(), hist, sizeof(uint32_t) * depth, hipMemcpyDeviceToHost, st));
+    BVHT_HIP(hipStreamSynchronize(st));
+    b->level_off.assign(depth + 1, 0);
+    for (uint32_t L = 0; L < depth; ++L)
+        b->level_off[L + 1] = b->level_off[L] + h[L];
+    if (b->level_off[depth] + 1 != b->nodes_used && !(b->nodes_used == 1))
+        return set_error(BVHT_EDEVICE, "BLAS topology: %u nodes grouped, %u expected", b->level_off[depth],
+                         b->nodes_used - 1);
+    BVHT_HIP(hipMemcpyAsync(hist, b->level_off.data(), sizeof(uint32_t) * depth, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_level_scatter, dim3(nb), dim3(256), 0, st, b->nodes_used, b->d_level, hist,
+                       b->d_level_order);
+    BVHT_HIP(hipGetLastError());
+    BVHT_HIP(hipStreamSynchronize(st));
+    return BVHT_OK;
+}
+
+// Bvh::refit: one launch per tree level, deepest first.
+int32_t launch_refit(bvht_ctx ctx, bvht_blas blas)
+{
+    for (size_t L = blas->depth; L-- > 0;) {
+        uint32_t off = blas->level_off[L], cnt = blas->level_off[L + 1] - off;
+        if (cnt == 0)
+            continue;
+        hipLaunchKernelGGL(k_refit_level, dim3(blocks_for(cnt, 256)), dim3(256), 0, ctx->stream, blas->d_nodes,
+                           blas->d_tris, blas->d_level_order + off, cnt);
+    }
+    BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }
 
@@ -383,7 +436,8 @@ int32_t blas_alloc(bvht_ctx ctx, uint32_t n, uint32_t node_count, bvht_blas* out
     bool ok = hipMalloc(&b->d_nodes, sizeof(bvht_node) * (size_t)node_count) == hipSuccess &&
               hipMalloc(&b->d_tris, sizeof(float) * 9 * (size_t)n) == hipSuccess &&
               hipMalloc(&b->d_parent, sizeof(uint32_t) * (size_t)node_count) == hipSuccess &&
-              hipMalloc(&b->d_flags, sizeof(uint32_t) * (size_t)node_count) == hipSuccess;
+              hipMalloc(&b->d_level, sizeof(uint32_t) * (size_t)node_count) == hipSuccess &&
+              hipMalloc(&b->d_level_order, sizeof(uint32_t) * (size_t)node_count) == hipSuccess;
     if (!ok) {
         bvht_blas_destroy(b);
         return set_error(BVHT_ENOMEM, "device allocation for a BLAS of %u triangles failed", n);
@@ -671,13 +725,7 @@ int32_t bvht_blas_set_vertices_device(bvht_ctx ctx, bvht_blas blas, const float*
 int32_t bvht_blas_refit(bvht_ctx ctx, bvht_blas blas)
 {
     BVHT_REQUIRE(ctx && blas, "NULL argument");
-    int32_t rc = bvht::fill_u32(ctx, blas->d_flags, 0, blas->node_count);
-    if (rc)
-        return rc;
-    hipLaunchKernelGGL(k_refit, dim3(bvht::blocks_for(blas->nodes_used, 256)), dim3(256), 0, ctx->stream,
-                       blas->d_nodes, blas->d_tris, blas->d_parent, blas->d_flags, blas->nodes_used);
-    BVHT_HIP(hipGetLastError());
-    return BVHT_OK;
+    return bvht::launch_refit(ctx, blas);
 }
 
 int32_t bvht_blas_bounds(bvht_ctx ctx, bvht_blas blas, float* out6)
@@ -700,8 +748,8 @@ int32_t bvht_blas_destroy(bvht_blas b)
     (void)hipFree(b->d_nodes);
     (void)hipFree(b->d_tris);
     (void)hipFree(b->d_parent);
-    (void)hipFree(b->d_flags);
-    (void)hipFree(b->d_leaves);
+    (void)hipFree(b->d_level);
+    (void)hipFree(b->d_level_order);
     (void)hipFree(b->d_spill);
     delete b;
     return BVHT_OK;

@@ -29,10 +29,10 @@ struct bvht_blas_s {
     uint32_t depth = 0;       // number of levels (root alone = 1)
     bvht_node* d_nodes = nullptr;
     float* d_tris = nullptr;  // 9 floats per triangle, build order
-    uint32_t* d_parent = nullptr;   // parent of each used node (root: 0xFFFFFFFF)
-    uint32_t* d_leaves = nullptr;   // indices of the leaf nodes
-    uint32_t n_leaves = 0;
-    uint32_t* d_flags = nullptr;    // refit arrival counters, one per node
+    uint32_t* d_parent = nullptr;       // parent of each used node (root: 0xFFFFFFFF)
+    uint32_t* d_level = nullptr;        // depth of each used node (root 0; dummy node 1: 0xFFFFFFFF)
+    uint32_t* d_level_order = nullptr;  // used nodes except the dummy, grouped by level
+    std::vector<uint32_t> level_off;    // level L occupies [level_off[L], level_off[L+1]) of d_level_order
     uint32_t* d_spill = nullptr;    // traversal stack spill (grown on demand)
     size_t spill_words = 0;
 };

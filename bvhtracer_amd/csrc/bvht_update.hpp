@@ -8,64 +8,39 @@
 
 namespace bvht {
 
-__device__ __forceinline__ float ld_agent(const float* p)
+// Bvh::refit (bvh.rs:469-493) for node k, run one tree level at a time from
+// the deepest level up (the levels are separate launches, so a node's
+// children are final when it is processed): a leaf recomputes its box from
+// its triangles starting at (+MAX, -MAX) (update_node_bounds, bvh.rs:317-330),
+// a branch takes the union of its two children.  min/max are exact and every
+// child has a larger index than its parent (bvh.rs:601-612), so the result
+// equals the reference's reverse index sweep bit for bit.
+__device__ __forceinline__ void refit_node(bvht_node* __restrict__ nodes, const float* __restrict__ tris, uint32_t k)
 {
-    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT));
-}
-
-// Bvh::refit (bvh.rs:469-493) bottom-up, one thread per node k: leaves
-// recompute their box from the triangles starting at (+MAX, -MAX)
-// (update_node_bounds, bvh.rs:317-330); the second child to arrive at a parent
-// writes the union of both children and continues upwards.  min/max are exact,
-// so the result equals the reference's reverse sweep.  flags[] (one per node)
-// must be zero on entry.
-__device__ __forceinline__ void refit_from_leaf(bvht_node* __restrict__ nodes, const float* __restrict__ tris,
-                                                const uint32_t* __restrict__ parent, uint32_t* __restrict__ flags,
-                                                uint32_t k)
-{
-    if (k == 1)
-        return;
     bvht_node nd = nodes[k];
-    if (nd.count == 0)
-        return;
-    V3 lo = mk3(kFltMax, kFltMax, kFltMax), hi = mk3(-kFltMax, -kFltMax, -kFltMax);
-    for (uint32_t j = 0; j < nd.count; ++j) {
-        const float* t = tris + 9 * (size_t)(nd.left_first + j);
-        V3 v0 = mk3(t[0], t[1], t[2]), v1 = mk3(t[3], t[4], t[5]), v2 = mk3(t[6], t[7], t[8]);
-        lo = min3(lo, v0);
-        lo = min3(lo, v1);
-        lo = min3(lo, v2);
-        hi = max3(hi, v0);
-        hi = max3(hi, v1);
-        hi = max3(hi, v2);
+    V3 lo, hi;
+    if (nd.count > 0) {
+        lo = mk3(kFltMax, kFltMax, kFltMax);
+        hi = mk3(-kFltMax, -kFltMax, -kFltMax);
+        for (uint32_t j = 0; j < nd.count; ++j) {
+            const float* t = tris + 9 * (size_t)(nd.left_first + j);
+            V3 v0 = mk3(t[0], t[1], t[2]), v1 = mk3(t[3], t[4], t[5]), v2 = mk3(t[6], t[7], t[8]);
+            lo = min3(lo, v0);
+            lo = min3(lo, v1);
+            lo = min3(lo, v2);
+            hi = max3(hi, v0);
+            hi = max3(hi, v1);
+            hi = max3(hi, v2);
+        }
+    } else {
+        bvht_node a = nodes[nd.left_first], c = nodes[nd.left_first + 1];
+        lo = min3(mk3(a.bmin[0], a.bmin[1], a.bmin[2]), mk3(c.bmin[0], c.bmin[1], c.bmin[2]));
+        hi = max3(mk3(a.bmax[0], a.bmax[1], a.bmax[2]), mk3(c.bmax[0], c.bmax[1], c.bmax[2]));
     }
-    bvht_node* out = nodes + k;
-    out->bmin[0] = lo.x; out->bmin[1] = lo.y; out->bmin[2] = lo.z;
-    out->bmax[0] = hi.x; out->bmax[1] = hi.y; out->bmax[2] = hi.z;
-    __threadfence();
-    uint32_t p = parent[k];
-    while (p != BVHT_MISS) {
-        uint32_t old = atomicAdd(&flags[p], 1u);
-        if (old == 0)
-            return;
-        __threadfence();
-        uint32_t l = nodes[p].left_first;
-        const bvht_node* a = nodes + l;
-        const bvht_node* b = nodes + l + 1;
-        V3 alo = mk3(ld_agent(&a->bmin[0]), ld_agent(&a->bmin[1]), ld_agent(&a->bmin[2]));
-        V3 ahi = mk3(ld_agent(&a->bmax[0]), ld_agent(&a->bmax[1]), ld_agent(&a->bmax[2]));
-        V3 blo = mk3(ld_agent(&b->bmin[0]), ld_agent(&b->bmin[1]), ld_agent(&b->bmin[2]));
-        V3 bhi = mk3(ld_agent(&b->bmax[0]), ld_agent(&b->bmax[1]), ld_agent(&b->bmax[2]));
-        V3 mn = min3(alo, blo), mx = max3(ahi, bhi);
-        bvht_node* o = nodes + p;
-        o->bmin[0] = mn.x; o->bmin[1] = mn.y; o->bmin[2] = mn.z;
-        o->bmax[0] = mx.x; o->bmax[1] = mx.y; o->bmax[2] = mx.z;
-        __threadfence();
-        p = parent[p];
-    }
+    float4* o = reinterpret_cast<float4*>(nodes + k);
+    o[0] = make_float4(lo.x, lo.y, lo.z, __uint_as_float(nd.left_first));
+    o[1] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(nd.count));
 }
-
 
 // SceneObject::set_transform (scene_object.rs:60-75) for object i: world box of
 // the 8 corners of its BLAS root box under M, and the traced rows of M^-1.
