@@ -48,6 +48,9 @@ def parse():
     p.add_argument("--no-extras", action="store_true", help="skip the batched and instanced workloads")
     p.add_argument("--batch-scenes", type=int, default=64)
     p.add_argument("--instanced-frames", type=int, default=64)
+    p.add_argument("--adjoint-steps", type=int, default=100)
+    p.add_argument("--only", choices=["adjoint", "instanced", "batched"], default=None,
+                   help="run one extra workload alone and print its JSON")
     p.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
                    help="only run the stream-copy kernel 5 times on BYTES (profiler counter calibration)")
     return p.parse_args()
@@ -93,6 +96,10 @@ def main():
             L.check(L.lib().bvht_stream_copy(ctx.handle, b.ptr, a.ptr, args.copy_only))
         ctx.sync()
         print(json.dumps({"copy_bytes": args.copy_only, "launches": 5}))
+        return
+    if args.only:
+        fn = {"adjoint": bench_adjoint, "instanced": bench_instanced, "batched": bench_batched}[args.only]
+        print(json.dumps({args.only: fn(args, B.Context(0))}), flush=True)
         return
 
     W, H = args.width, args.height
@@ -248,7 +255,7 @@ def main():
         result["cpu_baseline"] = cpu_baseline(ctx, bvh, d_brays, d_bhits, n_bounce, d_rays, d_hits, n_primary,
                                               args.cpu_seconds)
     if not args.no_extras:
-        for name, fn in (("instanced", bench_instanced), ("batched", bench_batched)):
+        for name, fn in (("instanced", bench_instanced), ("batched", bench_batched), ("adjoint", bench_adjoint)):
             try:
                 result[name] = fn(args, ctx)
             except Exception as e:  # a failing extra must not hide the main measurement
@@ -413,6 +420,105 @@ def bench_batched(args, ctx):
             "speedup": round(plain_ms / batched_ms, 3), "batched_trace_only_ms": round(trace_only_ms, 4),
             "hits_identical": bool(same),
             "trace_config": {"workgroup": best[0], "lds_stack_depth": best[1], "rays_per_item": best[2]}}
+
+
+def bench_adjoint(args, ctx):
+    """Differentiable depth/shading: the 81,920-face icosphere (frequency 64,
+    R 32) against a target with seeded radial vertex perturbation (sigma
+    0.05) rendered from 8 seeded cameras at 512x512 (2,097,152 rays).  One
+    step = forward (trace + shade), adjoint (dLoss/dvertex, dLoss/dM^-1,
+    loss), SGD on the vertices, gather + BVH refit + vertex normals; the step
+    is captured once as a hipGraph and replayed for --adjoint-steps steps.
+    fwd+bwd Mrays/s = rays / (forward + adjoint time) from eager per-phase
+    events; the adjoint's algorithmic bytes per contributing ray are
+    32 ray + 4 hit id + 8 target + 36 triangle + 12 vertex ids + 36 normals
+    + 36 scattered gradient (9 float atomics), 44 B for the others."""
+    import bvhtracer_amd as B
+    from bvhtracer_amd import scenes
+
+    w = scenes.diff_workload(64, width=512, n_cameras=8)
+    n1 = 512 * 512
+    n = 8 * n1
+    d_rays = ctx.alloc(n * B.RAY_DTYPE.itemsize)
+    for k in range(8):
+        B.raygen(ctx, w["cameras"][k:k + 1], 512, 512, B.ORDER_TILES,
+                 d_rays=B.DeviceArray.view(ctx, d_rays.value + k * n1 * 32, n1 * 32))
+
+    def make(V):
+        soup = np.ascontiguousarray(V[w["F"]].reshape(-1, 9))
+        return B.Mesh(B.BvhBuilder(B.BUILD_SAH, 2, ctx=ctx).build_for(soup), V, w["F"])
+
+    target_mesh = make(w["V_target"])
+    d_hits, d_img, d_target = ctx.alloc(n * 24), ctx.alloc(n * 8), ctx.alloc(n * 8)
+    B.trace_forward(target_mesh, w["Minv"], w["light"], d_rays, n, d_hits, d_target)
+    mesh = make(w["V"])
+    d_gv, d_gm, d_loss = ctx.alloc(mesh.n_vertices * 12), ctx.alloc(64), ctx.alloc(8)
+    lr, clip = scenes.DIFF_LR, scenes.DIFF_MAX_STEP
+
+    def fwd():
+        B.trace_forward(mesh, w["Minv"], w["light"], d_rays, n, d_hits, d_img)
+
+    def adj():
+        B.trace_adjoint(mesh, w["Minv"], w["light"], d_rays, n, d_hits, d_target, d_gv, d_gm, d_loss)
+
+    def upd():
+        mesh.sgd(d_gv, lr, clip)
+        mesh.update(normals=True)
+
+    def step():
+        fwd()
+        adj()
+        upd()
+
+    # step 0 eagerly, with per-phase events (also sizes the adjoint's scratch)
+    e = [ctx.event() for _ in range(4)]
+    ctx.sync()
+    ctx.record(e[0])
+    fwd()
+    ctx.record(e[1])
+    adj()
+    ctx.record(e[2])
+    upd()
+    ctx.record(e[3])
+    ctx.sync()
+    fwd_ms, adj_ms, upd_ms = (ctx.elapsed_ms(e[k], e[k + 1]) for k in range(3))
+    loss0 = float(d_loss.download(np.float64, 1)[0])
+    hits = d_hits.download(B.HIT_DTYPE, n)
+    tgt = d_target.download(np.float32, 2 * n).reshape(-1, 2)
+    contributing = int(((hits["prim"] != B.MISS) & (tgt[:, 0] < 3e38)).sum())
+    graph = B.Graph.capture(ctx, step)
+    steps = max(args.adjoint_steps - 1, 1)
+    t0, t1 = ctx.event(), ctx.event()
+    ctx.record(t0)
+    for _ in range(steps):
+        graph.launch()
+    ctx.record(t1)
+    ctx.sync()
+    step_ms = ctx.elapsed_ms(t0, t1) / steps
+    loss_last = float(d_loss.download(np.float64, 1)[0])
+    # eager phase times again on the evolved mesh (steady state)
+    ctx.record(e[0])
+    fwd()
+    ctx.record(e[1])
+    adj()
+    ctx.record(e[2])
+    ctx.sync()
+    fwd2, adj2 = ctx.elapsed_ms(e[0], e[1]), ctx.elapsed_ms(e[1], e[2])
+    graph.close()
+    adj_bytes = contributing * 200 + (n - contributing) * 44
+    return {"workload": f"icosphere {w['F'].shape[0]} faces / {w['V'].shape[0]} vertices, 8 cameras x 512x512 "
+                        f"({n} rays), depth + Lambert L2 loss, SGD lr {lr} (vertex step clip {clip}) + refit per "
+                        f"step, hipGraph per step",
+            "steps": steps + 1, "ms_per_step": round(step_ms, 4),
+            "fwd_bwd_mrays_s": round(n / ((fwd2 + adj2) * 1e-3) / 1e6, 2),
+            "step_mrays_s": round(n / (step_ms * 1e-3) / 1e6, 2),
+            "forward_ms": round(fwd2, 4), "adjoint_ms": round(adj2, 4), "update_ms_eager": round(upd_ms, 4),
+            "first_step_ms": {"forward": round(fwd_ms, 4), "adjoint": round(adj_ms, 4)},
+            "contributing_rays": contributing,
+            "adjoint_roofline": {"bound": "hbm", "algorithmic_bytes": adj_bytes,
+                                 "achieved": round(adj_bytes / (adj2 * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                                 "frac": round(adj_bytes / (adj2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "loss_first": loss0, "loss_last": loss_last}
 
 
 def cpu_baseline(ctx, bvh, d_brays, d_bhits, n_bounce, d_rays, d_hits, n_primary, budget_s):

@@ -6,12 +6,14 @@ mirror of the reference's Bvh / BvhBuilder / Tlas / SceneObject / Ray API.
 """
 from ._lib import (BUILD_REFERENCE, BUILD_SAH, CAMERA_DTYPE, HIT_DTYPE, INSTANCE_DTYPE, MISS, NODE_DTYPE,
                    ORDER_ROWS, ORDER_TILES, RAY_DTYPE, STATS_DTYPE, TNODE_DTYPE, BvhtError)
-from .api import (Batch, Bvh, BvhBuilder, Context, DeviceArray, Graph, Ray, SceneObject, Tlas, TlasBuilder, bounce,
-                  bounce_tlas, counter_add, default_context, deform_wave, raygen)
+from .api import (Batch, Bvh, BvhBuilder, Context, DeviceArray, Graph, Mesh, Ray, SceneObject, Tlas, TlasBuilder,
+                  bounce, bounce_tlas, counter_add, default_context, deform_wave, raygen, trace_adjoint, trace_forward,
+                  transform_gradient)
 
 __all__ = [
     "BUILD_REFERENCE", "BUILD_SAH", "CAMERA_DTYPE", "HIT_DTYPE", "INSTANCE_DTYPE", "MISS", "NODE_DTYPE",
     "ORDER_ROWS", "ORDER_TILES", "RAY_DTYPE", "STATS_DTYPE", "TNODE_DTYPE", "BvhtError",
-    "Batch", "Bvh", "BvhBuilder", "Context", "DeviceArray", "Graph", "Ray", "SceneObject", "Tlas", "TlasBuilder",
-    "bounce", "bounce_tlas", "counter_add", "default_context", "deform_wave", "raygen",
+    "Batch", "Bvh", "BvhBuilder", "Context", "DeviceArray", "Graph", "Mesh", "Ray", "SceneObject", "Tlas",
+    "TlasBuilder", "bounce", "bounce_tlas", "counter_add", "default_context", "deform_wave", "raygen",
+    "trace_adjoint", "trace_forward", "transform_gradient",
 ]

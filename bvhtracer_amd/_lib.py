@@ -15,6 +15,7 @@ OK, EINVAL, ENOMEM, EDEVICE, ESTACK = 0, -1, -2, -3, -4
 MISS = 0xFFFFFFFF
 ORDER_ROWS, ORDER_TILES = 0, 1
 BUILD_REFERENCE, BUILD_SAH = 0, 1
+SGD_ALONG_NORMAL = 1
 
 NODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("left_first", "<u4"), ("bmax", "<f4", 3), ("count", "<u4")])
 TNODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("left_right", "<u4"), ("bmax", "<f4", 3), ("blas", "<u4")])
@@ -53,6 +54,7 @@ SIGNATURES = {
     "bvht_blas_info": (I32, [P, P, P, P, P]),
     "bvht_blas_set_vertices": (I32, [P, P, P, U32]),
     "bvht_blas_set_vertices_device": (I32, [P, P, P, U32]),
+    "bvht_blas_permutation": (I32, [P, P, P]),
     "bvht_blas_refit": (I32, [P, P]),
     "bvht_blas_bounds": (I32, [P, P, P]),
     "bvht_blas_destroy": (I32, [P]),
@@ -82,6 +84,13 @@ SIGNATURES = {
     "bvht_batch_trace": (I32, [P, P, P, P]),
     "bvht_deform_wave": (I32, [P, P, P, F32, F32, F32, P]),
     "bvht_counter_add": (I32, [P, P, U32]),
+    "bvht_mesh_create": (I32, [P, P, P, U32, P, U32, P, PP]),
+    "bvht_mesh_destroy": (I32, [P]),
+    "bvht_mesh_buffers": (I32, [P, PP, PP]),
+    "bvht_mesh_update": (I32, [P, P, I32]),
+    "bvht_mesh_sgd": (I32, [P, P, P, F32, F32, I32]),
+    "bvht_trace_forward": (I32, [P, P, P, P, P, U32, P, P]),
+    "bvht_trace_adjoint": (I32, [P, P, P, P, P, U32, P, P, F32, F32, I32, P, P, P]),
     "bvht_graph_begin": (I32, [P]),
     "bvht_graph_end": (I32, [P, PP]),
     "bvht_graph_launch": (I32, [P, P]),

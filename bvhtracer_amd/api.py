@@ -18,6 +18,7 @@ instead of panicking.  There is no CPU path: every call runs on the GPU
 through libbvht_hip.so.
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -34,9 +35,19 @@ class DeviceArray:
     def __init__(self, ctx, nbytes):
         self.ctx = ctx
         self.nbytes = int(nbytes)
+        self.owned = True
         p = ctypes.c_void_p()
         check(L.lib().bvht_device_alloc(ctx.handle, self.nbytes, ctypes.byref(p)))
         self.ptr = p
+        ctx.adopt(self)
+
+    @classmethod
+    def view(cls, ctx, address, nbytes):
+        """Non-owning handle on device memory owned by the library (e.g. a
+        mesh's vertex buffer)."""
+        v = cls.__new__(cls)
+        v.ctx, v.nbytes, v.owned, v.ptr = ctx, int(nbytes), False, ctypes.c_void_p(address)
+        return v
 
     @property
     def value(self):
@@ -60,7 +71,7 @@ class DeviceArray:
         return out
 
     def free(self):
-        if self.ptr is not None and self.ptr.value:
+        if self.owned and self.ptr is not None and self.ptr.value:
             L.lib().bvht_device_free(self.ctx.handle, self.ptr)
         self.ptr = None
 
@@ -79,6 +90,14 @@ class Context:
         h = ctypes.c_void_p()
         check(L.lib().bvht_create(device, ctypes.byref(h)))
         self.handle = h
+        self._children = weakref.WeakSet()
+
+    def adopt(self, obj):
+        """Register an object holding library resources of this context;
+        close() releases it first (the C ABI requires objects to be destroyed
+        before their context)."""
+        self._children.add(obj)
+        return obj
 
     def alloc(self, nbytes):
         return DeviceArray(self, nbytes)
@@ -106,6 +125,8 @@ class Context:
 
     def close(self):
         if self.handle:
+            for obj in list(self._children):
+                obj.close() if not isinstance(obj, DeviceArray) else obj.free()
             L.lib().bvht_destroy(self.handle)
             self.handle = None
 
@@ -149,6 +170,7 @@ class Bvh:
     def __init__(self, ctx, handle):
         self.ctx = ctx
         self.handle = handle
+        self.ctx.adopt(self)
 
     @classmethod
     def from_arrays(cls, nodes, nodes_used, mesh, ctx=None):
@@ -199,6 +221,13 @@ class Bvh:
                 t = _as_tris(mesh)
                 check(L.lib().bvht_blas_set_vertices(self.ctx.handle, self.handle, L.ptr(t), t.shape[0]))
         check(L.lib().bvht_blas_refit(self.ctx.handle, self.handle))
+
+    def permutation(self):
+        """perm[k] = input index of the triangle at build position k (GPU
+        builds only; the reference keeps no permutation)."""
+        out = np.zeros(self.len_primitives(), np.uint32)
+        check(L.lib().bvht_blas_permutation(self.ctx.handle, self.handle, L.ptr(out)))
+        return out
 
     def intersect_device(self, d_rays, n, d_hits, d_stats=None):
         """Enqueue a trace of n device-resident rays into device hits."""
@@ -298,6 +327,7 @@ class Tlas:
     def __init__(self, ctx, handle, n):
         self.ctx = ctx
         self.handle = handle
+        self.ctx.adopt(self)
         self.n = n
 
     def rebuild(self, objects=None):
@@ -379,6 +409,7 @@ class Batch:
         check(L.lib().bvht_batch_create(ctx.handle, ctypes.cast(arr, ctypes.c_void_p), len(self.scenes),
                                         ctypes.byref(h)))
         self.handle = h
+        self.ctx.adopt(self)
 
     def refit(self):
         check(L.lib().bvht_batch_refit(self.ctx.handle, self.handle))
@@ -415,6 +446,7 @@ class Graph:
     def __init__(self, ctx, handle):
         self.ctx = ctx
         self.handle = handle
+        self.ctx.adopt(self)
 
     @classmethod
     def capture(cls, ctx, enqueue):
@@ -474,3 +506,103 @@ def bounce_tlas(ctx, tlas, d_rays, d_hits, n, seed, d_out=None, d_count=None):
     check(L.lib().bvht_bounce_tlas(ctx.handle, tlas.handle, d_rays.ptr, d_hits.ptr, n, seed, d_out.ptr,
                                    d_count.ptr))
     return d_out, d_count
+
+
+# ------------------------------------------------ differentiable depth/shading
+
+
+class Mesh:
+    """Indexed mesh (V (n, 3) float32, F (m, 3) uint32) bound to a BLAS built
+    from its face soup ``V[F].reshape(-1, 9)`` (face order), or to any BLAS
+    whose triangle k is face ``face_of[k]``.  Owns the device vertices and the
+    area-weighted vertex normals used by trace_forward / trace_adjoint."""
+
+    def __init__(self, bvh, vertices, faces, face_of=None):
+        self.ctx = bvh.ctx
+        self.bvh = bvh
+        V = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        F = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+        fo = None if face_of is None else np.ascontiguousarray(face_of, np.uint32)
+        if fo is not None and fo.shape[0] != bvh.len_primitives():
+            raise ValueError("face_of needs one entry per BLAS triangle")
+        h = ctypes.c_void_p()
+        check(L.lib().bvht_mesh_create(self.ctx.handle, bvh.handle, L.ptr(V), V.shape[0], L.ptr(F), F.shape[0],
+                                       L.ptr(fo) if fo is not None else None, ctypes.byref(h)))
+        self.handle = h
+        self.ctx.adopt(self)
+        self.n_vertices, self.n_faces = V.shape[0], F.shape[0]
+        self.faces = F
+        self.face_of = fo if fo is not None else bvh.permutation()
+        pv, pn = ctypes.c_void_p(), ctypes.c_void_p()
+        check(L.lib().bvht_mesh_buffers(h, ctypes.byref(pv), ctypes.byref(pn)))
+        nb = 12 * self.n_vertices
+        self.vertices = DeviceArray.view(self.ctx, pv.value, nb)
+        self.normals = DeviceArray.view(self.ctx, pn.value, nb)
+
+    def prim_vertices(self):
+        """(n_tris, 3) vertex ids of each BLAS triangle (build order)."""
+        return self.faces[self.face_of]
+
+    def update(self, normals=True):
+        """Publish changed vertices: gather into the BLAS, refit, and (if
+        ``normals``) recompute the vertex normals."""
+        check(L.lib().bvht_mesh_update(self.ctx.handle, self.handle, 1 if normals else 0))
+
+    def set_vertices(self, V, normals=True):
+        self.vertices.upload(np.ascontiguousarray(V, np.float32).reshape(-1, 3))
+        self.update(normals)
+
+    def sgd(self, d_grad, lr, max_step=0.0, along_normal=False):
+        """v -= lr * g_v per vertex (along_normal: only the component along
+        the vertex normal), the step clipped to max_step (<= 0: no clip)."""
+        check(L.lib().bvht_mesh_sgd(self.ctx.handle, self.handle, d_grad.ptr, lr, max_step,
+                                    L.SGD_ALONG_NORMAL if along_normal else 0))
+
+    def download(self):
+        """(vertices, normals) as (n, 3) float32 arrays."""
+        return (self.vertices.download(np.float32).reshape(-1, 3), self.normals.download(np.float32).reshape(-1, 3))
+
+    def close(self):
+        if self.handle:
+            L.lib().bvht_mesh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _mat16(m):
+    """16 floats, column-major m[col*4+row] (the SceneObject convention)."""
+    return np.ascontiguousarray(np.asarray(m, np.float32).reshape(16))
+
+
+def trace_forward(mesh, minv, light, d_rays, n, d_hits, d_image):
+    """Trace n device rays through the mesh's object (inverse transform
+    ``minv``: 16 floats, column-major) and shade:
+    d_hits[n], d_image[2n] = {depth, shade}."""
+    m = _mat16(minv)
+    lt = np.ascontiguousarray(light, np.float32).reshape(3)
+    check(L.lib().bvht_trace_forward(mesh.ctx.handle, mesh.handle, L.ptr(m), L.ptr(lt), d_rays.ptr, n, d_hits.ptr,
+                                     d_image.ptr))
+
+
+def trace_adjoint(mesh, minv, light, d_rays, n, d_hits, d_target, d_grad_vertices, d_grad_minv, d_loss,
+                  w_depth=1.0, w_shade=1.0, accumulate=False):
+    """dLoss/dvertex (float[3 n_vertices]), dLoss/dM^-1 (float[16],
+    column-major) and the loss (double) into device buffers."""
+    m = _mat16(minv)
+    lt = np.ascontiguousarray(light, np.float32).reshape(3)
+    check(L.lib().bvht_trace_adjoint(mesh.ctx.handle, mesh.handle, L.ptr(m), L.ptr(lt), d_rays.ptr, n, d_hits.ptr,
+                                     d_target.ptr, w_depth, w_shade, 1 if accumulate else 0, d_grad_vertices.ptr,
+                                     d_grad_minv.ptr, d_loss.ptr))
+
+
+def transform_gradient(minv, grad_minv):
+    """dLoss/dM from dLoss/dM^-1 (both flat column-major): -(M^-1)^T G (M^-1)^T,
+    returned flat column-major (float64)."""
+    A = np.asarray(minv, np.float64).reshape(4, 4).T  # A[r, c]
+    G = np.asarray(grad_minv, np.float64).reshape(4, 4).T
+    return np.ascontiguousarray((-(A.T @ G @ A.T)).T.reshape(16))

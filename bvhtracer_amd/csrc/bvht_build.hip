@@ -778,9 +778,10 @@ __global__ void k_scatter_nodes(const bvht_node* __restrict__ src, const uint32_
         nodes[idx[k]] = src[k];
 }
 
-// Build order triangle array: tris_out[k] = tris_in[index of element k].
+// Build order triangle array: tris_out[k] = tris_in[index of element k];
+// perm[k] records that index.
 __global__ void k_gather_tris(const float* __restrict__ tin, const float4* __restrict__ ec, uint32_t n,
-                              float* __restrict__ tout, uint32_t* __restrict__ err)
+                              float* __restrict__ tout, uint32_t* __restrict__ perm, uint32_t* __restrict__ err)
 {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n)
@@ -790,6 +791,7 @@ __global__ void k_gather_tris(const float* __restrict__ tin, const float4* __res
         atomicOr(err, 16u);
         return;
     }
+    perm[k] = src;
     const float* a = tin + 9 * (size_t)src;
     float* b = tout + 9 * (size_t)k;
 #pragma unroll
@@ -1129,8 +1131,10 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
                            d_rrank.as<uint32_t>(), R, b->d_nodes, 2 * n, d_err.as<uint32_t>());
         BVHT_HIP(hipGetLastError());
     }
+    if (hipMalloc(&b->d_perm, sizeof(uint32_t) * (size_t)n) != hipSuccess)
+        return set_error(BVHT_ENOMEM, "device allocation of the %u-entry build permutation failed", n);
     hipLaunchKernelGGL(k_gather_tris, dim3(nblk(n, 256)), dim3(256), 0, st, tin.as<float>(), ec.as<float4>(), n,
-                       b->d_tris, d_err.as<uint32_t>());
+                       b->d_tris, b->d_perm, d_err.as<uint32_t>());
     hipLaunchKernelGGL(k_validate_tree, dim3(nblk(b->nodes_used, 256)), dim3(256), 0, st, b->d_nodes, b->nodes_used,
                        n, d_err.as<uint32_t>());
     BVHT_HIP(hipGetLastError());

@@ -747,6 +747,7 @@ int32_t bvht_blas_destroy(bvht_blas b)
         (void)hipStreamSynchronize(b->ctx->stream);
     (void)hipFree(b->d_nodes);
     (void)hipFree(b->d_tris);
+    (void)hipFree(b->d_perm);
     (void)hipFree(b->d_parent);
     (void)hipFree(b->d_level);
     (void)hipFree(b->d_level_order);

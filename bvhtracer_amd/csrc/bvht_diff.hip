@@ -1,0 +1,618 @@
+// bvht_diff.hip -- differentiable depth/shading: indexed mesh bound to a
+// BLAS, forward trace + shade, and the closed-form Moeller-Trumbore adjoint.
+//
+// No reference counterpart: the reference has no differentiable path.  The
+// forward's intersection arithmetic is the reference's (SceneObject::intersect
+// scene_object.rs:77-89 around Bvh::intersect bvh.rs:242-309), so its hits
+// equal a one-object TLAS trace bit for bit.
+//
+// Adjoint.  A hit solves o' + t d' = v0 + u e1 + v e2, i.e. A x = s with
+// A = [-d' | e1 | e2], x = (t, u, v), s = o' - v0.  For g = dLoss/dx,
+//   lambda = A^-T g = [(e1 x e2) g_t + (e2 x -d') g_u + (-d' x e1) g_v] / det,
+// det = (-d').(e1 x e2) = e1.(d' x e2) (the Moeller-Trumbore determinant), and
+//   dLoss/do' = lambda,  dLoss/dd' = t lambda,
+//   dLoss/dv0 = -(1-u-v) lambda,  dLoss/dv1 = -u lambda,  dLoss/dv2 = -v lambda,
+//   dLoss/dM^-1[r][c] = lambda_r o_c + t lambda_r d_c   (o_3 = 1, d_3 = 0).
+// Shading s = (m.L)/|m|, m = (1-u-v) N0 + u N1 + v N2, adds
+//   g_u += g_s (dS/dm).(N1 - N0),  g_v += g_s (dS/dm).(N2 - N0),
+//   dS/dm = (L - s m/|m|) / |m|    (zero where the shade is clamped to 0).
+// The forward is binary32 (bitwise the reference arithmetic); the adjoint
+// solves each pixel in binary64 and scatters binary32.
+#include <algorithm>
+#include <vector>
+
+#include "bvht_internal.hpp"
+#include "bvht_trace.hpp"
+
+struct bvht_mesh_s {
+    bvht_ctx ctx = nullptr;
+    bvht_blas blas = nullptr;
+    uint32_t n_vertices = 0, n_faces = 0;
+    float* d_vertices = nullptr;   // 3 n_vertices
+    float* d_normals = nullptr;    // 3 n_vertices
+    uint32_t* d_faces = nullptr;   // 3 n_faces
+    uint32_t* d_vid = nullptr;     // 3 per BLAS triangle (build order): vertex ids of its corners
+    uint32_t* d_vf_off = nullptr;  // n_vertices + 1: incident faces of vertex i are d_vf_idx[off[i], off[i+1])
+    uint32_t* d_vf_idx = nullptr;  // 3 n_faces, ascending face index per vertex
+};
+
+namespace bvht {
+namespace {
+
+inline uint32_t nblk(size_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+__device__ __forceinline__ V3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
+
+// Lambert term of the barycentric-interpolated vertex normal:
+// max(0, m.L/|m|), m = ((1-u)-v) N0 + u N1 + v N2 (0 where m vanishes).
+__device__ __forceinline__ float shade_at(const float* __restrict__ normals, const uint32_t* __restrict__ vid,
+                                          uint32_t prim, float u, float v, V3 light)
+{
+    V3 n0 = ld3(normals + 3 * (size_t)vid[3 * (size_t)prim + 0]);
+    V3 n1 = ld3(normals + 3 * (size_t)vid[3 * (size_t)prim + 1]);
+    V3 n2 = ld3(normals + 3 * (size_t)vid[3 * (size_t)prim + 2]);
+    float w = (1.0f - u) - v;
+    V3 m = add3(add3(scl3(n0, w), scl3(n1, u)), scl3(n2, v));
+    float ml = dot3(m, light);
+    float len = sqrtf(dot3(m, m));
+    return (ml > 0.0f && len > 0.0f) ? ml / len : 0.0f;
+}
+
+struct Rows {
+    float4 r0, r1, r2;  // rows 0..2 of M^-1: {m[0][i], m[1][i], m[2][i], m[3][i]}
+};
+
+inline Rows rows_of(const float* minv)
+{
+    Rows r;
+    r.r0 = make_float4(minv[0], minv[4], minv[8], minv[12]);
+    r.r1 = make_float4(minv[1], minv[5], minv[9], minv[13]);
+    r.r2 = make_float4(minv[2], minv[6], minv[10], minv[14]);
+    return r;
+}
+
+// ------------------------------------------------------------- mesh kernels
+
+__global__ void k_mesh_gather(const float* __restrict__ verts, const uint32_t* __restrict__ vid, uint32_t n,
+                              float* __restrict__ tris)
+{
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n)
+        return;
+    float* out = tris + 9 * (size_t)k;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float* p = verts + 3 * (size_t)vid[3 * (size_t)k + c];
+        out[3 * c + 0] = p[0];
+        out[3 * c + 1] = p[1];
+        out[3 * c + 2] = p[2];
+    }
+}
+
+__global__ void k_mesh_check(const float* __restrict__ verts, const uint32_t* __restrict__ vid, uint32_t n,
+                             const float* __restrict__ tris, uint32_t* __restrict__ err)
+{
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n)
+        return;
+    const float* t = tris + 9 * (size_t)k;
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float* p = verts + 3 * (size_t)vid[3 * (size_t)k + c];
+        for (int j = 0; j < 3; ++j)
+            ok = ok && __float_as_uint(p[j]) == __float_as_uint(t[3 * c + j]);
+    }
+    if (!ok)
+        atomicOr(err, 1u);
+}
+
+// Area-weighted vertex normals: the cross products (v1-v0) x (v2-v0) of the
+// incident faces summed in ascending face order, then normalised (0 if the
+// sum vanishes).  Deterministic.
+__global__ void k_vertex_normals(const float* __restrict__ verts, const uint32_t* __restrict__ faces,
+                                 const uint32_t* __restrict__ off, const uint32_t* __restrict__ idx, uint32_t nv,
+                                 float* __restrict__ normals)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv)
+        return;
+    V3 acc = mk3(0.0f, 0.0f, 0.0f);
+    for (uint32_t k = off[i]; k < off[i + 1]; ++k) {
+        const uint32_t* f = faces + 3 * (size_t)idx[k];
+        V3 a = ld3(verts + 3 * (size_t)f[0]);
+        V3 b = ld3(verts + 3 * (size_t)f[1]);
+        V3 c = ld3(verts + 3 * (size_t)f[2]);
+        acc = add3(acc, cross3(sub3(b, a), sub3(c, a)));
+    }
+    float len = sqrtf(dot3(acc, acc));
+    V3 nrm = len > 0.0f ? mk3(acc.x / len, acc.y / len, acc.z / len) : mk3(0.0f, 0.0f, 0.0f);
+    normals[3 * (size_t)i + 0] = nrm.x;
+    normals[3 * (size_t)i + 1] = nrm.y;
+    normals[3 * (size_t)i + 2] = nrm.z;
+}
+
+// Per-vertex gradient step: v -= lr g_v (with along_normal, the component
+// of g_v along the vertex normal n: lr (g_v.n) n), the step shortened to
+// max_step if longer (max_step <= 0: no limit).
+__global__ void k_sgd(float* __restrict__ x, const float* __restrict__ g, const float* __restrict__ normals,
+                      uint32_t nv, float lr, float max_step, int along_normal)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv)
+        return;
+    V3 gv = ld3(g + 3 * (size_t)i);
+    if (along_normal) {
+        V3 nrm = ld3(normals + 3 * (size_t)i);
+        gv = scl3(nrm, dot3(gv, nrm));
+    }
+    V3 st = scl3(gv, lr);
+    if (max_step > 0.0f) {
+        float len = sqrtf(dot3(st, st));
+        if (len > max_step)
+            st = scl3(st, max_step / len);
+    }
+    float* p = x + 3 * (size_t)i;
+    p[0] = p[0] - st.x;
+    p[1] = p[1] - st.y;
+    p[2] = p[2] - st.z;
+}
+
+// ------------------------------------------------------------------ forward
+
+template <int WG, int LDS_DEPTH>
+__global__ __launch_bounds__(WG) void k_trace_forward(const float4* __restrict__ nodes, const float* __restrict__ tris,
+                                                      const uint32_t* __restrict__ vid,
+                                                      const float* __restrict__ normals, Rows rows, V3 light,
+                                                      const bvht_ray* __restrict__ rays, uint32_t n,
+                                                      bvht_hit* __restrict__ hits, float2* __restrict__ image,
+                                                      uint32_t* __restrict__ spill, uint32_t cap)
+{
+    __shared__ uint32_t lds[LDS_DEPTH * WG];
+    uint32_t i = blockIdx.x * WG + threadIdx.x;
+    if (i >= n)
+        return;
+    Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap};
+    const float4* rp = reinterpret_cast<const float4*>(rays + i);
+    float4 a = rp[0], b = rp[1];
+    V3 o = rows_mul(rows.r0, rows.r1, rows.r2, mk3(a.x, a.y, a.z), 1.0f);
+    V3 d = rows_mul(rows.r0, rows.r1, rows.r2, mk3(b.x, b.y, b.z), 0.0f);
+    DRay ray = make_ray(o, d, a.w);
+    uint32_t pairs = 0, tests = 0;
+    BlasResult r = traverse_blas<false>(nodes, tris, ray, st, 0u, pairs, tests);
+    store_hit(hits, i, r.found, r.t, r.u, r.v, r.prim, 0);
+    float depth = kFltMax, shade = 0.0f;
+    if (r.found) {
+        depth = r.t;
+        shade = shade_at(normals, vid, r.prim, r.u, r.v, light);
+    }
+    image[i] = make_float2(depth, shade);
+}
+
+typedef void (*forward_fn)(const float4*, const float*, const uint32_t*, const float*, Rows, V3, const bvht_ray*,
+                           uint32_t, bvht_hit*, float2*, uint32_t*, uint32_t);
+
+forward_fn pick_forward(int wg, int lds)
+{
+#define X(W, D)                      \
+    if (wg == W && lds == D)         \
+        return k_trace_forward<W, D>;
+    BVHT_TRACE_CONFIGS(X)
+#undef X
+    return nullptr;
+}
+
+// ------------------------------------------------------------------ adjoint
+
+constexpr int kAdjWG = 256;
+constexpr int kRed = 13;  // 12 entries of rows 0..2 of dLoss/dM^-1 (row-major r*4+c) + loss
+
+// Binary64 vector helpers for the adjoint's per-pixel solve.
+struct D3 {
+    double x, y, z;
+};
+__device__ __forceinline__ D3 dk(double x, double y, double z) { return D3{x, y, z}; }
+__device__ __forceinline__ D3 dof(V3 a) { return D3{(double)a.x, (double)a.y, (double)a.z}; }
+__device__ __forceinline__ D3 dsub(D3 a, D3 b) { return dk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ D3 dadd(D3 a, D3 b) { return dk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ D3 dscl(D3 a, double s) { return dk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 dcross(D3 a, D3 b)
+{
+    return dk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ D3 drows(const Rows& m, D3 p, double w)
+{
+    return dk((double)m.r0.x * p.x + (double)m.r0.y * p.y + (double)m.r0.z * p.z + (double)m.r0.w * w,
+              (double)m.r1.x * p.x + (double)m.r1.y * p.y + (double)m.r1.z * p.z + (double)m.r1.w * w,
+              (double)m.r2.x * p.x + (double)m.r2.y * p.y + (double)m.r2.z * p.z + (double)m.r2.w * w);
+}
+
+// Per pixel the solve runs in binary64 on the binary32 inputs: a grazing
+// ray (|cos| ~ 1e-3, determinant near the 1e-4 threshold) amplifies
+// binary32 rounding of t, u, v and lambda to percent-level errors in the
+// gradients of the vertices it touches, and such pixels carry the largest
+// gradients of all.  The scatter and the outputs stay binary32.
+__global__ __launch_bounds__(kAdjWG) void k_trace_adjoint(
+    const float* __restrict__ tris, uint32_t ntris, const uint32_t* __restrict__ vid,
+    const float* __restrict__ normals, Rows rows, V3 light, const bvht_ray* __restrict__ rays, uint32_t n,
+    const bvht_hit* __restrict__ hits, const float2* __restrict__ target, float w_depth, float w_shade,
+    float* __restrict__ grad_v, double* __restrict__ partials)
+{
+    __shared__ double red[kAdjWG / 64][kRed];
+    uint32_t i = blockIdx.x * kAdjWG + threadIdx.x;
+    double acc[kRed];
+#pragma unroll
+    for (int c = 0; c < kRed; ++c)
+        acc[c] = 0.0;
+    uint32_t prim = i < n ? hits[i].prim : BVHT_MISS;
+    float2 tg = i < n ? target[i] : make_float2(kFltMax, 0.0f);
+    uint32_t key = BVHT_MISS;  // the hit triangle, for contributing rays
+    uint32_t vid3[3] = {0, 0, 0};
+    float cv[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+        cv[c] = 0.0f;
+    if (prim < ntris && tg.x < kFltMax) {
+        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+        float4 ra = rp[0], rb = rp[1];
+        D3 ow = dk(ra.x, ra.y, ra.z), dw = dk(rb.x, rb.y, rb.z);
+        D3 o = drows(rows, ow, 1.0);
+        D3 d = drows(rows, dw, 0.0);
+        const float* tp = tris + 9 * (size_t)prim;
+        D3 v0 = dof(ld3(tp)), v1 = dof(ld3(tp + 3)), v2 = dof(ld3(tp + 6));
+        // Moeller-Trumbore (geometry/triangle.rs:41-72), binary64
+        D3 e1 = dsub(v1, v0), e2 = dsub(v2, v0);
+        D3 pn = dcross(d, e2);
+        double det = ddot(e1, pn);
+        double f = 1.0 / det;
+        D3 s = dsub(o, v0);
+        double u = f * ddot(s, pn);
+        D3 q = dcross(s, e1);
+        double v = f * ddot(d, q);
+        double t = f * ddot(e2, q);
+        const uint32_t* vv = vid + 3 * (size_t)prim;
+        D3 n0 = dof(ld3(normals + 3 * (size_t)vv[0]));
+        D3 n1 = dof(ld3(normals + 3 * (size_t)vv[1]));
+        D3 n2 = dof(ld3(normals + 3 * (size_t)vv[2]));
+        D3 lt = dof(light);
+        double w = 1.0 - u - v;
+        D3 m = dadd(dadd(dscl(n0, w), dscl(n1, u)), dscl(n2, v));
+        double ml = ddot(m, lt), len = sqrt(ddot(m, m));
+        bool lit = ml > 0.0 && len > 0.0;
+        double sh = lit ? ml / len : 0.0;
+        double rt = t - (double)tg.x;
+        double rs = sh - (double)tg.y;
+        double gt = 2.0 * (double)w_depth * rt;
+        double gs = 2.0 * (double)w_shade * rs;
+        double gu = 0.0, gv = 0.0;
+        if (lit) {
+            double il = 1.0 / len;
+            D3 dsdm = dscl(dsub(lt, dscl(m, sh * il)), il);
+            gu = gs * ddot(dsdm, dsub(n1, n0));
+            gv = gs * ddot(dsdm, dsub(n2, n0));
+        }
+        D3 a0 = dk(-d.x, -d.y, -d.z);
+        D3 lam = dscl(dadd(dadd(dscl(dcross(e1, e2), gt), dscl(dcross(e2, a0), gu)), dscl(dcross(a0, e1), gv)), f);
+        double bw[3] = {-w, -u, -v};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            cv[3 * k + 0] = (float)(lam.x * bw[k]);
+            cv[3 * k + 1] = (float)(lam.y * bw[k]);
+            cv[3 * k + 2] = (float)(lam.z * bw[k]);
+            vid3[k] = vv[k];
+        }
+        key = prim;
+        double lr[3] = {lam.x, lam.y, lam.z};
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            double tl = t * lr[r];
+            acc[4 * r + 0] = lr[r] * ow.x + tl * dw.x;
+            acc[4 * r + 1] = lr[r] * ow.y + tl * dw.y;
+            acc[4 * r + 2] = lr[r] * ow.z + tl * dw.z;
+            acc[4 * r + 3] = lr[r];
+        }
+        acc[12] = (double)w_depth * rt * rt + (double)w_shade * rs * rs;
+    }
+    // Vertex scatter.  Neighbouring pixels mostly hit the same triangle: the
+    // lanes of a wave that share one add their contributions first, and one
+    // leader per distinct triangle issues the nine atomics.
+    uint32_t lane_id = threadIdx.x & 63u;
+    unsigned long long todo = __ballot(key != BVHT_MISS);
+    while (todo) {
+        int leader = __ffsll(todo) - 1;
+        uint32_t k = __shfl(key, leader, 64);
+        bool mine = key == k;
+        unsigned long long members = __ballot(mine);
+        float sum[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            sum[c] = mine ? cv[c] : 0.0f;
+            for (int m = 32; m >= 1; m >>= 1)
+                sum[c] += __shfl_xor(sum[c], m, 64);
+        }
+        if (lane_id == (uint32_t)leader) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float* g = grad_v + 3 * (size_t)vid3[c];
+                atomicAdd(g + 0, sum[3 * c + 0]);
+                atomicAdd(g + 1, sum[3 * c + 1]);
+                atomicAdd(g + 2, sum[3 * c + 2]);
+            }
+        }
+        todo &= ~members;
+    }
+    // fixed-order reduction: butterfly within the wave, then the waves in order
+#pragma unroll
+    for (int c = 0; c < kRed; ++c)
+        for (int m = 32; m >= 1; m >>= 1)
+            acc[c] += __shfl_xor(acc[c], m, 64);
+    uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < kRed; ++c)
+            red[wave][c] = acc[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)kRed) {
+        double sum = 0.0;
+        for (int k = 0; k < kAdjWG / 64; ++k)
+            sum += red[k][threadIdx.x];
+        partials[(size_t)blockIdx.x * 16 + threadIdx.x] = sum;
+    }
+}
+
+// Sum the per-block partials in a fixed order; write (or add) dLoss/dM^-1
+// (column-major, bottom row 0) and the loss.
+__global__ __launch_bounds__(256) void k_adjoint_finish(const double* __restrict__ partials, uint32_t nblocks,
+                                                        int accumulate, float* __restrict__ grad_minv,
+                                                        double* __restrict__ loss)
+{
+    __shared__ double red[256];
+    for (int c = 0; c < kRed; ++c) {
+        double s = 0.0;
+        for (uint32_t b = threadIdx.x; b < nblocks; b += 256)
+            s += partials[(size_t)b * 16 + c];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (uint32_t h = 128; h > 0; h >>= 1) {
+            if (threadIdx.x < h)
+                red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            if (c < 12) {
+                int r = c / 4, col = c % 4;
+                float g = (float)red[0];
+                float* p = grad_minv + col * 4 + r;
+                *p = accumulate ? *p + g : g;
+            } else {
+                *loss = accumulate ? *loss + red[0] : red[0];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4 && !accumulate)
+        grad_minv[threadIdx.x * 4 + 3] = 0.0f;
+}
+
+}  // namespace
+}  // namespace bvht
+
+using namespace bvht;
+
+extern "C" int32_t bvht_blas_permutation(bvht_ctx ctx, bvht_blas blas, uint32_t* perm)
+{
+    BVHT_REQUIRE(ctx && blas && perm, "NULL argument");
+    BVHT_REQUIRE(blas->d_perm != nullptr, "this BLAS was adopted, not built: it has no build permutation");
+    BVHT_HIP(hipMemcpyAsync(perm, blas->d_perm, sizeof(uint32_t) * (size_t)blas->n_tris, hipMemcpyDeviceToHost,
+                            ctx->stream));
+    BVHT_HIP(hipStreamSynchronize(ctx->stream));
+    return BVHT_OK;
+}
+
+extern "C" int32_t bvht_mesh_destroy(bvht_mesh m)
+{
+    if (!m)
+        return BVHT_OK;
+    if (m->ctx)
+        (void)hipStreamSynchronize(m->ctx->stream);
+    (void)hipFree(m->d_vertices);
+    (void)hipFree(m->d_normals);
+    (void)hipFree(m->d_faces);
+    (void)hipFree(m->d_vid);
+    (void)hipFree(m->d_vf_off);
+    (void)hipFree(m->d_vf_idx);
+    delete m;
+    return BVHT_OK;
+}
+
+static int32_t launch_normals(bvht_ctx ctx, bvht_mesh m)
+{
+    hipLaunchKernelGGL(k_vertex_normals, dim3(nblk(m->n_vertices, 256)), dim3(256), 0, ctx->stream, m->d_vertices,
+                       m->d_faces, m->d_vf_off, m->d_vf_idx, m->n_vertices, m->d_normals);
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
+extern "C" int32_t bvht_mesh_create(bvht_ctx ctx, bvht_blas blas, const float* vertices, uint32_t n_vertices,
+                                    const uint32_t* faces, uint32_t n_faces, const uint32_t* face_of, bvht_mesh* out)
+{
+    BVHT_REQUIRE(ctx && blas && vertices && faces && out, "NULL argument");
+    BVHT_REQUIRE(blas->ctx == ctx, "the BLAS belongs to another context");
+    BVHT_REQUIRE(n_vertices > 0 && n_faces > 0, "empty mesh");
+    BVHT_REQUIRE(n_faces <= (1u << 30) && n_vertices <= (1u << 30), "mesh too large");
+    uint32_t nt = blas->n_tris;
+    for (size_t k = 0; k < 3 * (size_t)n_faces; ++k)
+        BVHT_REQUIRE(faces[k] < n_vertices, "face %zu references vertex %u of %u", k / 3, faces[k], n_vertices);
+    std::vector<uint32_t> fo(nt);
+    if (face_of) {
+        std::copy(face_of, face_of + nt, fo.begin());
+    } else {
+        BVHT_REQUIRE(blas->d_perm != nullptr, "face_of is NULL and the BLAS has no build permutation");
+        BVHT_REQUIRE(nt == n_faces, "the BLAS has %u triangles, the mesh %u faces", nt, n_faces);
+        BVHT_HIP(hipMemcpyAsync(fo.data(), blas->d_perm, sizeof(uint32_t) * nt, hipMemcpyDeviceToHost, ctx->stream));
+        BVHT_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    std::vector<uint32_t> vid(3 * (size_t)nt);
+    for (uint32_t k = 0; k < nt; ++k) {
+        BVHT_REQUIRE(fo[k] < n_faces, "face_of[%u] = %u out of range (%u faces)", k, fo[k], n_faces);
+        for (int c = 0; c < 3; ++c)
+            vid[3 * (size_t)k + c] = faces[3 * (size_t)fo[k] + c];
+    }
+    // vertex -> incident faces (ascending), by counting sort
+    std::vector<uint32_t> off(n_vertices + 1, 0), idx(3 * (size_t)n_faces);
+    for (size_t k = 0; k < 3 * (size_t)n_faces; ++k)
+        ++off[faces[k] + 1];
+    for (uint32_t i = 0; i < n_vertices; ++i)
+        off[i + 1] += off[i];
+    {
+        std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+        for (uint32_t f = 0; f < n_faces; ++f)
+            for (int c = 0; c < 3; ++c)
+                idx[cur[faces[3 * (size_t)f + c]]++] = f;
+    }
+    bvht_mesh m = new bvht_mesh_s;
+    m->ctx = ctx;
+    m->blas = blas;
+    m->n_vertices = n_vertices;
+    m->n_faces = n_faces;
+    size_t vb = sizeof(float) * 3 * (size_t)n_vertices;
+    bool ok = hipMalloc(&m->d_vertices, vb) == hipSuccess && hipMalloc(&m->d_normals, vb) == hipSuccess &&
+              hipMalloc(&m->d_faces, sizeof(uint32_t) * 3 * (size_t)n_faces) == hipSuccess &&
+              hipMalloc(&m->d_vid, sizeof(uint32_t) * vid.size()) == hipSuccess &&
+              hipMalloc(&m->d_vf_off, sizeof(uint32_t) * off.size()) == hipSuccess &&
+              hipMalloc(&m->d_vf_idx, sizeof(uint32_t) * idx.size()) == hipSuccess;
+    if (!ok) {
+        bvht_mesh_destroy(m);
+        return set_error(BVHT_ENOMEM, "device allocation for a mesh of %u vertices failed", n_vertices);
+    }
+    hipStream_t st = ctx->stream;
+    int32_t rc = BVHT_OK;
+    uint32_t* d_err = nullptr;
+    uint32_t err = 0;
+    if (hipMemcpyAsync(m->d_vertices, vertices, vb, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(m->d_faces, faces, sizeof(uint32_t) * 3 * (size_t)n_faces, hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(m->d_vid, vid.data(), sizeof(uint32_t) * vid.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(m->d_vf_off, off.data(), sizeof(uint32_t) * off.size(), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(m->d_vf_idx, idx.data(), sizeof(uint32_t) * idx.size(), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMalloc(&d_err, sizeof(uint32_t)) != hipSuccess) {
+        rc = set_error(BVHT_EDEVICE, "mesh upload failed");
+    }
+    if (!rc)
+        rc = fill_u32(ctx, d_err, 0u, 1);
+    if (!rc) {
+        hipLaunchKernelGGL(k_mesh_check, dim3(nblk(nt, 256)), dim3(256), 0, st, m->d_vertices, m->d_vid, nt,
+                           blas->d_tris, d_err);
+        rc = launch_normals(ctx, m);
+    }
+    if (!rc && (hipMemcpyAsync(&err, d_err, sizeof err, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess))
+        rc = set_error(BVHT_EDEVICE, "mesh check failed to run");
+    (void)hipFree(d_err);
+    if (!rc && err)
+        rc = set_error(BVHT_EINVAL, "the BLAS triangles are not the mesh faces in face_of order");
+    if (rc) {
+        bvht_mesh_destroy(m);
+        return rc;
+    }
+    *out = m;
+    return BVHT_OK;
+}
+
+extern "C" int32_t bvht_mesh_buffers(bvht_mesh mesh, float** d_vertices, float** d_normals)
+{
+    BVHT_REQUIRE(mesh, "NULL mesh");
+    if (d_vertices)
+        *d_vertices = mesh->d_vertices;
+    if (d_normals)
+        *d_normals = mesh->d_normals;
+    return BVHT_OK;
+}
+
+extern "C" int32_t bvht_mesh_update(bvht_ctx ctx, bvht_mesh mesh, int32_t normals)
+{
+    BVHT_REQUIRE(ctx && mesh && mesh->ctx == ctx, "NULL argument or mesh of another context");
+    bvht_blas b = mesh->blas;
+    hipLaunchKernelGGL(k_mesh_gather, dim3(nblk(b->n_tris, 256)), dim3(256), 0, ctx->stream, mesh->d_vertices,
+                       mesh->d_vid, b->n_tris, b->d_tris);
+    BVHT_HIP(hipGetLastError());
+    int32_t rc = bvht_blas_refit(ctx, b);
+    if (rc)
+        return rc;
+    return normals ? launch_normals(ctx, mesh) : BVHT_OK;
+}
+
+extern "C" int32_t bvht_mesh_sgd(bvht_ctx ctx, bvht_mesh mesh, const float* d_grad, float lr, float max_step,
+                                 int32_t flags)
+{
+    BVHT_REQUIRE(ctx && mesh && d_grad && mesh->ctx == ctx, "NULL argument or mesh of another context");
+    BVHT_REQUIRE((flags & ~BVHT_SGD_ALONG_NORMAL) == 0, "unknown SGD flags 0x%x", flags);
+    uint32_t n = mesh->n_vertices;
+    hipLaunchKernelGGL(k_sgd, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, mesh->d_vertices, d_grad,
+                       mesh->d_normals, n, lr, max_step, (int)(flags & BVHT_SGD_ALONG_NORMAL));
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
+extern "C" int32_t bvht_trace_forward(bvht_ctx ctx, bvht_mesh mesh, const float* minv, const float* light,
+                                      const bvht_ray* d_rays, uint32_t n, bvht_hit* d_hits, float* d_image)
+{
+    BVHT_REQUIRE(ctx && mesh && minv && light && mesh->ctx == ctx, "NULL argument or mesh of another context");
+    if (n == 0)
+        return BVHT_OK;
+    BVHT_REQUIRE(d_rays && d_hits && d_image, "NULL device buffer");
+    BVHT_REQUIRE(n <= 0xFFFFFF00u, "too many rays (%u)", n);
+    bvht_blas b = mesh->blas;
+    uint32_t need = b->depth > 0 ? b->depth - 1 : 0;
+    int wg, lds;
+    trace_config(ctx, need, &wg, &lds);
+    uint32_t* spill = nullptr;
+    if (need > (uint32_t)lds) {
+        int32_t rc = ensure_spill(ctx, &b->d_spill, &b->spill_words, (size_t)(need - lds) * n);
+        if (rc)
+            return rc;
+        spill = b->d_spill;
+    }
+    forward_fn fn = pick_forward(wg, lds);
+    if (!fn)
+        return set_error(BVHT_EINVAL, "no forward kernel for workgroup %d, LDS depth %d", wg, lds);
+    hipLaunchKernelGGL(fn, dim3(nblk(n, wg)), dim3(wg), 0, ctx->stream, reinterpret_cast<const float4*>(b->d_nodes),
+                       b->d_tris, mesh->d_vid, mesh->d_normals, rows_of(minv), V3{light[0], light[1], light[2]},
+                       d_rays, n, d_hits, reinterpret_cast<float2*>(d_image), spill, need);
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
+extern "C" int32_t bvht_trace_adjoint(bvht_ctx ctx, bvht_mesh mesh, const float* minv, const float* light,
+                                      const bvht_ray* d_rays, uint32_t n, const bvht_hit* d_hits,
+                                      const float* d_target, float w_depth, float w_shade, int32_t accumulate,
+                                      float* d_grad_vertices, float* d_grad_minv, double* d_loss)
+{
+    BVHT_REQUIRE(ctx && mesh && minv && light && mesh->ctx == ctx, "NULL argument or mesh of another context");
+    BVHT_REQUIRE(d_grad_vertices && d_grad_minv && d_loss, "NULL output buffer");
+    BVHT_REQUIRE(n == 0 || (d_rays && d_hits && d_target), "NULL device buffer");
+    BVHT_REQUIRE(n <= 0xFFFFFF00u, "too many rays (%u)", n);
+    if (!accumulate) {
+        int32_t rc = fill_u32(ctx, reinterpret_cast<uint32_t*>(d_grad_vertices), 0u, 3 * (size_t)mesh->n_vertices);
+        if (rc)
+            return rc;
+    }
+    uint32_t nb = std::max<uint32_t>(nblk(n, kAdjWG), 1);
+    int32_t rc = ensure_scratch(ctx, sizeof(double) * 16 * (size_t)nb);
+    if (rc)
+        return rc;
+    double* partials = reinterpret_cast<double*>(ctx->scratch);
+    bvht_blas b = mesh->blas;
+    hipLaunchKernelGGL(k_trace_adjoint, dim3(nb), dim3(kAdjWG), 0, ctx->stream, b->d_tris, b->n_tris, mesh->d_vid,
+                       mesh->d_normals, rows_of(minv), V3{light[0], light[1], light[2]}, d_rays, n, d_hits,
+                       reinterpret_cast<const float2*>(d_target), w_depth, w_shade, d_grad_vertices, partials);
+    BVHT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_adjoint_finish, dim3(1), dim3(256), 0, ctx->stream, partials, nb, (int)accumulate,
+                       d_grad_minv, d_loss);
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}

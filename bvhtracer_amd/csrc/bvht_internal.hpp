@@ -29,6 +29,7 @@ struct bvht_blas_s {
     uint32_t depth = 0;       // number of levels (root alone = 1)
     bvht_node* d_nodes = nullptr;
     float* d_tris = nullptr;  // 9 floats per triangle, build order
+    uint32_t* d_perm = nullptr;  // input index of the triangle at each build position (GPU builds only)
     uint32_t* d_parent = nullptr;       // parent of each used node (root: 0xFFFFFFFF)
     uint32_t* d_level = nullptr;        // depth of each used node (root 0; dummy node 1: 0xFFFFFFFF)
     uint32_t* d_level_order = nullptr;  // used nodes except the dummy, grouped by level

@@ -354,3 +354,78 @@ def batch_workload(n_scenes=64, seed=0xBA7C4, tri_range=(1e3, 1e6), ray_range=(4
         rays["tmax"] = F32_MAX
         out.append({"mesh": mesh, "M": M, "Minv": Minv, "rays": rays})
     return out
+
+
+# ------------------------------------------------ differentiable rendering
+
+def indexed(tris):
+    """(N, 9) triangle soup -> (V float32 (nV, 3), F uint32 (N, 3)), merging
+    vertices with identical coordinates."""
+    v = np.ascontiguousarray(np.asarray(tris, np.float32).reshape(-1, 3))
+    uniq, inv = np.unique(v.view(np.dtype((np.void, 12))).ravel(), return_index=False, return_inverse=True)
+    V = np.ascontiguousarray(uniq.view(np.float32).reshape(-1, 3))
+    return V, np.ascontiguousarray(inv.reshape(-1, 3).astype(np.uint32))
+
+
+def radial_perturb(V, sigma=0.05, seed=0xD1FF):
+    """Scale each vertex radially by (1 + sigma * g), g ~ N(0, 1) from PCG32
+    streams (Box-Muller)."""
+    rng = Pcg32(seed, np.arange(V.shape[0], dtype=np.uint64))
+    u1 = np.maximum(rng.next_f32().astype(np.float64), 1e-12)
+    u2 = rng.next_f32().astype(np.float64)
+    g = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+    return (V.astype(np.float64) * (1.0 + sigma * g)[:, None]).astype(np.float32)
+
+
+def vertex_normals(V, F):
+    """Area-weighted vertex normals (float64 accumulate, float32 result)."""
+    v = V.astype(np.float64)
+    n = np.cross(v[F[:, 1]] - v[F[:, 0]], v[F[:, 2]] - v[F[:, 0]])
+    acc = np.zeros_like(v)
+    for k in range(3):
+        np.add.at(acc, F[:, k], n)
+    return (acc / np.linalg.norm(acc, axis=1, keepdims=True)).astype(np.float32)
+
+
+def orbit_cameras(count=8, distance=96.0, fovy_deg=40.0, seed=0xCA3):
+    """``count`` seeded pinhole cameras on a sphere of radius ``distance``
+    looking at the origin (up vector +y, or +z near the poles)."""
+    rng = Pcg32(seed, np.arange(count, dtype=np.uint64))
+    z = 2.0 * rng.next_f32().astype(np.float64) - 1.0
+    phi = 2.0 * np.pi * rng.next_f32().astype(np.float64)
+    r = np.sqrt(1.0 - z * z)
+    cams = []
+    for k in range(count):
+        eye = distance * np.array([r[k] * np.cos(phi[k]), z[k], r[k] * np.sin(phi[k])])
+        fwd = -eye / np.linalg.norm(eye)
+        up0 = np.array([0.0, 1.0, 0.0]) if abs(fwd[1]) < 0.95 else np.array([0.0, 0.0, 1.0])
+        right = np.cross(fwd, up0)
+        right /= np.linalg.norm(right)
+        up = np.cross(right, fwd)
+        cams.append(camera(eye, fwd, right, up, fovy_deg=fovy_deg, aspect=1.0, near=2.0))
+    return np.concatenate(cams)
+
+
+DIFF_LIGHT = (0.48, 0.6, -0.64)  # unit model-space light direction
+
+
+def diff_workload(freq=64, radius=32.0, sigma=0.05, n_cameras=8, width=512, seed=0xD1FF):
+    """Differentiable depth/shading workload: the indexed icosphere of
+    ``freq`` (20 freq^2 faces; 64 -> 81,920) as the starting mesh, a target
+    mesh with seeded radial vertex perturbation ``sigma``, one object with a
+    seeded affine transform (rotation, scale in [0.9, 1.1], translation in
+    [-2, 2]^3), ``n_cameras`` seeded orbit cameras at width^2 pixels and the
+    model-space light direction.  Returns a dict."""
+    V, F = indexed(icosphere(freq, radius=radius, amp=0.0))
+    Vt = radial_perturb(V, sigma, seed)
+    M, Minv = random_affines(1, seed ^ 0x7F, scale_range=(0.9, 1.1), trans_range=2.0)
+    cams = orbit_cameras(n_cameras, distance=3.0 * radius, fovy_deg=45.0, seed=seed ^ 0xCA3)
+    return {"V": V, "F": F, "V_target": Vt, "M": M[0], "Minv": Minv[0], "cameras": cams, "width": width,
+            "light": np.array(DIFF_LIGHT, np.float32)}
+
+# SGD for the differentiable workload (loss = sum over pixels): step size
+# and per-vertex step clip (silhouette pixels carry gradients up to ~40x the
+# median vertex gradient).  Sweep over 100 steps: lr 1e-3 with a 0.05 clip
+# lowers the loss fastest among lr 1e-3..1e-1, clip 0.02..0.1.
+DIFF_LR = 1e-3
+DIFF_MAX_STEP = 0.05

@@ -14,7 +14,9 @@
  *     trace entry points are DEVICE pointers (HBM-resident); triangle and
  *     node arrays passed to build/upload/download are HOST pointers.
  *   - All work is enqueued on the context's HIP stream; bvht_sync() or a
- *     read-back is the fence.  One context per host thread.
+ *     read-back is the fence.  One context per host thread.  Destroy every
+ *     object made with a context (BLAS, TLAS, batch, mesh, graph, device
+ *     allocation) before the context itself.
  *   - Arithmetic is IEEE binary32 in the reference's operation order, no FMA
  *     contraction, correctly rounded division: hits are bit-identical to the
  *     reference algorithm on the same BVH and rays.
@@ -172,6 +174,12 @@ int32_t bvht_blas_info(bvht_blas blas, uint32_t* tri_count, uint32_t* node_count
 int32_t bvht_blas_set_vertices(bvht_ctx ctx, bvht_blas blas, const float* tris, uint32_t n);
 /* Same from device memory (float[9n]). */
 int32_t bvht_blas_set_vertices_device(bvht_ctx ctx, bvht_blas blas, const float* d_tris, uint32_t n);
+/* The build's triangle permutation (host uint32[n]): perm[k] = index, in the
+ * array passed to bvht_blas_build, of the triangle now at build position k.
+ * The reference reorders the caller's array in place and keeps no
+ * permutation (bvh.rs:420-429); this is what per-triangle attributes are
+ * permuted with.  BLASes adopted by bvht_blas_upload have none (EINVAL). */
+int32_t bvht_blas_permutation(bvht_ctx ctx, bvht_blas blas, uint32_t* perm);
 /* Bvh::refit (bvh.rs:469-493) on the GPU. */
 int32_t bvht_blas_refit(bvht_ctx ctx, bvht_blas blas);
 /* Bvh::bounds (bvh.rs:499-501): out[6] = bmin, bmax of node 0 (host). */
@@ -194,8 +202,8 @@ int32_t bvht_trace_blas_count(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_ra
 
 /* Trace kernel configuration for this context: workgroup size (64, 128 or
  * 256 lanes) and per-lane LDS stack entries (8, 16, 24 or 32; deeper stacks
- * spill to HBM).  wg = 0 restores the default (256 lanes, LDS depth from the
- * tree depth).  Results never depend on the configuration; the autotuner
+ * spill to HBM).  wg = 0 restores the default (64 lanes, LDS depth from the
+ * tree depth, at most 24).  Results never depend on the configuration; the autotuner
  * (bvhtracer_amd/autotune.py) picks the fastest. */
 int32_t bvht_set_trace_config(bvht_ctx ctx, int32_t wg, int32_t lds_depth);
 int32_t bvht_get_trace_config(bvht_ctx ctx, int32_t* wg, int32_t* lds_depth);
@@ -292,6 +300,68 @@ int32_t bvht_deform_wave(bvht_ctx ctx, bvht_blas blas, const float* d_orig, floa
                          float w, const uint32_t* d_frame);
 /* *d_value += v on the stream (advances a frame counter inside a graph). */
 int32_t bvht_counter_add(bvht_ctx ctx, uint32_t* d_value, uint32_t v);
+
+/* ---- differentiable depth/shading: forward/adjoint pair ----------------
+ * No reference counterpart.  A mesh binds an indexed triangle mesh (vertices
+ * V float[3 n_vertices], faces F uint32[3 n_faces]) to a BLAS built from its
+ * face soup, so that BLAS triangle k has corners V[F[face_of[k]][0..2]].  One
+ * object with inverse transform M^-1 (host float[16], column-major) is
+ * traced: rays go to model space as o' = M^-1 (o,1), d' = M^-1 (d,0) with the
+ * arithmetic of SceneObject::intersect (scene_object.rs:77-89).  Per pixel:
+ *   depth = hit t (f32::MAX on a miss),
+ *   shade = max(0, (m.L)/|m|), m = (1-u-v) N0 + u N1 + v N2 with N the
+ *           area-weighted vertex normals and L a model-space light direction
+ *           (0 on a miss).
+ * Loss = sum over pixels hit in both the image and the target of
+ *   w_depth (depth - depth*)^2 + w_shade (shade - shade*)^2.
+ * Visibility changes (silhouettes) are not differentiated, and the vertex
+ * normals are held fixed within a gradient evaluation. */
+typedef struct bvht_mesh_s* bvht_mesh;
+
+/* Bind V and F (host arrays, copied) to blas.  face_of (host uint32[n_tris]
+ * of the BLAS, may be NULL) maps build positions to faces; NULL takes the
+ * BLAS's build permutation (the BLAS was built by bvht_blas_build from the
+ * soup of F in face order).  Fails with EINVAL unless the BLAS triangles are
+ * bitwise the gathered faces.  Computes the vertex normals. */
+int32_t bvht_mesh_create(bvht_ctx ctx, bvht_blas blas, const float* vertices, uint32_t n_vertices,
+                         const uint32_t* faces, uint32_t n_faces, const uint32_t* face_of, bvht_mesh* out);
+int32_t bvht_mesh_destroy(bvht_mesh mesh);
+/* Device pointers to the mesh's vertices (float[3 n_vertices], writable by
+ * the caller; bvht_mesh_update publishes changes) and vertex normals
+ * (float[3 n_vertices]). */
+int32_t bvht_mesh_buffers(bvht_mesh mesh, float** d_vertices, float** d_normals);
+/* After the vertices changed: gather them into the BLAS triangles, Bvh::refit
+ * (bvh.rs:469-493), and, if normals != 0, recompute the vertex normals. */
+int32_t bvht_mesh_update(bvht_ctx ctx, bvht_mesh mesh, int32_t normals);
+/* Gradient step on the vertices: v -= lr * g_v for each vertex (d_grad:
+ * device float[3 n_vertices]); with BVHT_SGD_ALONG_NORMAL only the component
+ * along the current vertex normal, lr (g_v.n) n.  Each vertex's step is
+ * shortened to max_step if longer (max_step <= 0: unclipped): silhouette
+ * pixels carry gradients orders of magnitude above the rest. */
+#define BVHT_SGD_ALONG_NORMAL 1
+int32_t bvht_mesh_sgd(bvht_ctx ctx, bvht_mesh mesh, const float* d_grad, float lr, float max_step,
+                      int32_t flags);
+/* Forward: trace d_rays[n] through the object and shade.  d_hits[n] as
+ * bvht_trace_tlas with one object (hit.inst = 0); d_image[2n] = {depth,
+ * shade} per ray. */
+int32_t bvht_trace_forward(bvht_ctx ctx, bvht_mesh mesh, const float* minv, const float* light,
+                           const bvht_ray* d_rays, uint32_t n, bvht_hit* d_hits, float* d_image);
+/* Adjoint of the forward for the loss above against d_target[2n]: for each
+ * hit in d_hits (from bvht_trace_forward with the same mesh state and minv)
+ * recomputes t,u,v of the hit triangle with the Moeller-Trumbore arithmetic
+ * (in binary64, on the binary32 inputs: grazing rays make the binary32 solve
+ * ill-conditioned) and propagates dLoss/d(t,u,v) in closed form to the
+ * triangle's vertices
+ * (scattered with atomics into d_grad_vertices, float[3 n_vertices]) and
+ * through o', d' to M^-1 (d_grad_minv, float[16] column-major; the bottom
+ * row is 0).  *d_loss (device double) receives the loss.  accumulate = 0
+ * overwrites the three outputs, else adds to them.  The M^-1 gradient and the
+ * loss are reduced in a fixed order (deterministic); the vertex scatter is
+ * not.  dLoss/dM follows as -(M^-1)^T G (M^-1)^T. */
+int32_t bvht_trace_adjoint(bvht_ctx ctx, bvht_mesh mesh, const float* minv, const float* light,
+                           const bvht_ray* d_rays, uint32_t n, const bvht_hit* d_hits,
+                           const float* d_target, float w_depth, float w_shade, int32_t accumulate,
+                           float* d_grad_vertices, float* d_grad_minv, double* d_loss);
 
 /* ---- hipGraph capture of enqueued work ---------------------------------- */
 

@@ -68,6 +68,10 @@ def lib():
             "orc_tlas_trace": (i32, [P, P, P, P, P, u64, P, P, i32]),
             "orc_camera_ray": (None, [P, f32, f32, P]),
             "orc_raygen": (i32, [P, u32, u32, P]),
+            "orc_mesh_normals": (None, [P, u32, P, u32, P]),
+            "orc_diff_forward": (i32, [P, P, P, P, P, P, P, u64, P, P, i32]),
+            "orc_diff_grad64": (i32, [P, u32, P, u32, P, P, P, P, P, u64, P, ctypes.c_double, ctypes.c_double,
+                                      P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -253,3 +257,58 @@ def raygen(cam, width, height):
     if rc != 0:
         raise ValueError("orc_raygen failed")
     return rays
+
+
+# ---- differentiable depth/shading checker (no reference counterpart) ----
+
+
+def mesh_normals(V, F):
+    """Area-weighted vertex normals in binary32, faces accumulated in order."""
+    V = np.ascontiguousarray(V, np.float32).reshape(-1, 3)
+    F = np.ascontiguousarray(F, np.uint32).reshape(-1, 3)
+    N = np.zeros_like(V)
+    lib().orc_mesh_normals(_p(V), V.shape[0], _p(F), F.shape[0], _p(N))
+    return N
+
+
+def diff_forward(nodes, tris, vid, normals, minv, light, rays, nthreads=1):
+    """Binary32 forward: transform rays by minv (16 floats, column-major),
+    trace the BLAS, shade.  Returns (hits, image (n, 2) = depth, shade)."""
+    tris = np.ascontiguousarray(np.asarray(tris, np.float32).reshape(-1, 9))
+    vid = np.ascontiguousarray(vid, np.uint32)
+    normals = np.ascontiguousarray(normals, np.float32)
+    minv = np.ascontiguousarray(minv, np.float32).reshape(16)
+    light = np.ascontiguousarray(light, np.float32).reshape(3)
+    rays = np.ascontiguousarray(rays)
+    n = rays.shape[0]
+    hits = np.zeros(n, HIT_DTYPE)
+    image = np.zeros((n, 2), np.float32)
+    rc = lib().orc_diff_forward(_p(np.ascontiguousarray(nodes)), _p(tris), _p(vid), _p(normals), _p(minv), _p(light),
+                                _p(rays), n, _p(hits), _p(image), nthreads)
+    if rc != 0:
+        raise ValueError(f"orc_diff_forward failed rc={rc}")
+    return hits, image
+
+
+def diff_grad64(V, vid, N, minv, light, rays, prims, target, w_depth=1.0, w_shade=1.0):
+    """Binary64 loss and gradient at fixed hit triangles.  Returns
+    (loss, grad_V (nv, 3), grad_minv (16,), image (n, 2))."""
+    V = np.ascontiguousarray(V, np.float64).reshape(-1, 3)
+    vid = np.ascontiguousarray(vid, np.uint32).reshape(-1)
+    N = np.ascontiguousarray(N, np.float64).reshape(-1, 3)
+    minv = np.ascontiguousarray(minv, np.float64).reshape(16)
+    light = np.ascontiguousarray(light, np.float64).reshape(3)
+    rays = np.ascontiguousarray(rays)
+    prims = np.ascontiguousarray(prims, np.uint32)
+    target = np.ascontiguousarray(target, np.float32).reshape(-1, 2)
+    n = rays.shape[0]
+    loss = np.zeros(1, np.float64)
+    gv = np.zeros_like(V)
+    gm = np.zeros(16, np.float64)
+    image = np.zeros((n, 2), np.float64)
+    rc = lib().orc_diff_grad64(_p(V), V.shape[0], _p(vid), vid.shape[0] // 3, _p(N), _p(minv), _p(light), _p(rays),
+                               _p(prims), n, _p(target), float(w_depth), float(w_shade), _p(loss), _p(gv), _p(gm),
+                               _p(image))
+    if rc != 0:
+        raise ValueError(f"orc_diff_grad64 failed rc={rc}")
+    return float(loss[0]), gv, gm, image

@@ -895,7 +895,12 @@ int orc_tlas_intersect(const orc_tnode* tnodes, const orc_instance* inst,
 /* ------------------------------------------------------------------ */
 
 typedef struct {
-    int kind; /* 0 blas, 1 tlas */
+    int kind; /* 0 blas, 1 tlas, 2 differentiable forward */
+    const uint32_t* vid;
+    const float* normals;
+    const float* minv;
+    const float* light;
+    float* image;
     const orc_node* nodes;
     const float* tris;
     const orc_tnode* tnodes;
@@ -909,6 +914,8 @@ typedef struct {
     uint32_t depth, tdepth;
     int rc;
 } job_t;
+
+static void diff_forward_one(const job_t* j, uint64_t i, uint32_t* stack);
 
 static void* run_job(void* arg)
 {
@@ -926,7 +933,9 @@ static void* run_job(void* arg)
         orc_stats* st = j->stats ? &j->stats[i] : NULL;
         if (st)
             memset(st, 0, sizeof *st);
-        if (j->kind == 0) {
+        if (j->kind == 2) {
+            diff_forward_one(j, i, bs);
+        } else if (j->kind == 0) {
             blas_result res = bvh_traverse(j->nodes, j->tris, &r, bs, st);
             orc_hit* h = &j->hits[i];
             if (res.found) {
@@ -1098,5 +1107,208 @@ int orc_raygen(const orc_camera* cam, uint32_t width, uint32_t height, orc_ray* 
         for (uint32_t x = 0; x < width; ++x)
             orc_camera_ray(cam, (float)x / (float)width, (float)y / (float)height,
                            &rays[(size_t)y * width + x]);
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Differentiable depth/shading checker                                 */
+/* ------------------------------------------------------------------ */
+/* No reference counterpart (the reference has no differentiable path).
+ * These restate the GPU library's forward/adjoint pair so that it can be
+ * checked: the forward in binary32 with the same operation order (bitwise
+ * comparison), the gradient in binary64 (tolerance comparison; its own
+ * correctness is pinned by central finite differences in the tests). */
+
+void orc_mesh_normals(const float* V, uint32_t nv, const uint32_t* F, uint32_t nf, float* N)
+{
+    for (size_t k = 0; k < 3 * (size_t)nv; ++k)
+        N[k] = 0.0f;
+    for (uint32_t f = 0; f < nf; ++f) {
+        v3 a = ld3(V + 3 * (size_t)F[3 * (size_t)f]);
+        v3 b = ld3(V + 3 * (size_t)F[3 * (size_t)f + 1]);
+        v3 c = ld3(V + 3 * (size_t)F[3 * (size_t)f + 2]);
+        v3 n = cross(sub(b, a), sub(c, a));
+        for (int k = 0; k < 3; ++k) {
+            float* acc = N + 3 * (size_t)F[3 * (size_t)f + k];
+            v3 s = add(ld3(acc), n);
+            acc[0] = s.x;
+            acc[1] = s.y;
+            acc[2] = s.z;
+        }
+    }
+    for (uint32_t i = 0; i < nv; ++i) {
+        float* p = N + 3 * (size_t)i;
+        v3 a = ld3(p);
+        float len = sqrtf(dot(a, a));
+        if (len > 0.0f) {
+            p[0] = a.x / len;
+            p[1] = a.y / len;
+            p[2] = a.z / len;
+        } else {
+            p[0] = p[1] = p[2] = 0.0f;
+        }
+    }
+}
+
+/* Lambert term of the barycentric-interpolated vertex normal:
+ * max(0, m.L/|m|), m = ((1-u)-v) N0 + u N1 + v N2. */
+static float diff_shade(const float* normals, const uint32_t* vid, uint32_t prim, float u, float v, v3 light)
+{
+    v3 n0 = ld3(normals + 3 * (size_t)vid[3 * (size_t)prim]);
+    v3 n1 = ld3(normals + 3 * (size_t)vid[3 * (size_t)prim + 1]);
+    v3 n2 = ld3(normals + 3 * (size_t)vid[3 * (size_t)prim + 2]);
+    float w = (1.0f - u) - v;
+    v3 m = add(add(scl(n0, w), scl(n1, u)), scl(n2, v));
+    float ml = dot(m, light);
+    float len = sqrtf(dot(m, m));
+    return (ml > 0.0f && len > 0.0f) ? ml / len : 0.0f;
+}
+
+static void diff_forward_one(const job_t* j, uint64_t i, uint32_t* stack)
+{
+    const orc_ray* in = &j->rays[i];
+    v3 o = mat_mul4(j->minv, ld3(in->o), 1.0f);
+    v3 d = mat_mul4(j->minv, ld3(in->d), 0.0f);
+    ray_t r = ray_new(o, d, in->tmax);
+    blas_result res = bvh_traverse(j->nodes, j->tris, &r, stack, NULL);
+    orc_hit* h = &j->hits[i];
+    float* img = j->image + 2 * i;
+    if (res.found) {
+        h->t = res.t;
+        h->u = res.u;
+        h->v = res.v;
+        h->prim = res.prim;
+        h->inst = 0;
+        h->packed = orc_pack_index(0, res.prim);
+        img[0] = res.t;
+        img[1] = diff_shade(j->normals, j->vid, res.prim, res.u, res.v, ld3(j->light));
+    } else {
+        miss_hit(h);
+        img[0] = FLT_MAX;
+        img[1] = 0.0f;
+    }
+}
+
+int orc_diff_forward(const orc_node* nodes, const float* tris, const uint32_t* vid, const float* normals,
+                     const float minv[16], const float light[3], const orc_ray* rays, uint64_t n, orc_hit* hits,
+                     float* image, int nthreads)
+{
+    if (!nodes || !tris || !vid || !normals || !minv || !light || (n && (!rays || !hits || !image)))
+        return ORC_EINVAL;
+    job_t p;
+    memset(&p, 0, sizeof p);
+    p.kind = 2;
+    p.nodes = nodes;
+    p.tris = tris;
+    p.vid = vid;
+    p.normals = normals;
+    p.minv = minv;
+    p.light = light;
+    p.rays = rays;
+    p.hits = hits;
+    p.image = image;
+    p.depth = orc_bvh_depth(nodes);
+    p.tdepth = 1;
+    return run_jobs(p, n, nthreads);
+}
+
+typedef struct {
+    double x, y, z;
+} d3;
+
+static inline d3 dk(double x, double y, double z) { d3 v = {x, y, z}; return v; }
+static inline d3 dld(const double* p) { return dk(p[0], p[1], p[2]); }
+static inline d3 dsub(d3 a, d3 b) { return dk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline d3 dadd(d3 a, d3 b) { return dk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline d3 dscl(d3 a, double s) { return dk(a.x * s, a.y * s, a.z * s); }
+static inline double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline d3 dcross(d3 a, d3 b)
+{
+    return dk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline d3 dmul4(const double* m, d3 p, double w)
+{
+    return dk(m[0] * p.x + m[4] * p.y + m[8] * p.z + m[12] * w, m[1] * p.x + m[5] * p.y + m[9] * p.z + m[13] * w,
+              m[2] * p.x + m[6] * p.y + m[10] * p.z + m[14] * w);
+}
+
+/* Loss and gradient in binary64 for fixed per-ray hit triangles prims[i]
+ * (BLAS build order; 0xFFFFFFFF = miss).  Ray i contributes iff it hits and
+ * its target depth is below FLT_MAX:
+ *   loss += w_depth (t - t*)^2 + w_shade (s - s*)^2
+ * with t, u, v the exact solution of o' + t d' = v0 + u e1 + v e2
+ * (o' = M^-1 (o,1), d' = M^-1 (d,0)) and s the Lambert term of the
+ * interpolated normals N (held fixed).  grad_v (3 nv) and grad_minv (16,
+ * column-major; bottom row 0) receive dLoss/dV and dLoss/dM^-1 by the
+ * closed-form derivatives of that linear solve; image (2n, may be NULL)
+ * receives {t, s} (0 where the ray does not contribute). */
+int orc_diff_grad64(const double* V, uint32_t nv, const uint32_t* vid, uint32_t ntris, const double* N,
+                    const double minv[16], const double light[3], const orc_ray* rays, const uint32_t* prims,
+                    uint64_t n, const float* target, double w_depth, double w_shade, double* loss, double* grad_v,
+                    double* grad_minv, double* image)
+{
+    if (!V || !vid || !N || !minv || !light || !loss || !grad_v || !grad_minv || (n && (!rays || !prims || !target)))
+        return ORC_EINVAL;
+    for (size_t k = 0; k < 3 * (size_t)ntris; ++k)
+        if (vid[k] >= nv)
+            return ORC_EINVAL;
+    memset(grad_v, 0, sizeof(double) * 3 * (size_t)nv);
+    memset(grad_minv, 0, sizeof(double) * 16);
+    double L = 0.0;
+    d3 lt = dld(light);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (image) {
+            image[2 * i] = 0.0;
+            image[2 * i + 1] = 0.0;
+        }
+        uint32_t p = prims[i];
+        if (p >= ntris || !(target[2 * i] < FLT_MAX))
+            continue;
+        d3 ow = dk(rays[i].o[0], rays[i].o[1], rays[i].o[2]);
+        d3 dw = dk(rays[i].d[0], rays[i].d[1], rays[i].d[2]);
+        d3 o = dmul4(minv, ow, 1.0), d = dmul4(minv, dw, 0.0);
+        const uint32_t* tv = vid + 3 * (size_t)p;
+        d3 v0 = dld(V + 3 * (size_t)tv[0]), v1 = dld(V + 3 * (size_t)tv[1]), v2 = dld(V + 3 * (size_t)tv[2]);
+        d3 e1 = dsub(v1, v0), e2 = dsub(v2, v0), s = dsub(o, v0);
+        d3 a0 = dscl(d, -1.0);
+        d3 c12 = dcross(e1, e2), c20 = dcross(e2, a0), c01 = dcross(a0, e1);
+        double det = ddot(a0, c12);
+        if (det == 0.0)
+            continue;
+        /* Cramer: x_k = (row k of A^-1) . s, rows = c12/det, c20/det, c01/det */
+        double t = ddot(c12, s) / det, u = ddot(c20, s) / det, v = ddot(c01, s) / det;
+        d3 n0 = dld(N + 3 * (size_t)tv[0]), n1 = dld(N + 3 * (size_t)tv[1]), n2 = dld(N + 3 * (size_t)tv[2]);
+        double w = 1.0 - u - v;
+        d3 m = dadd(dadd(dscl(n0, w), dscl(n1, u)), dscl(n2, v));
+        double ml = ddot(m, lt), len = sqrt(ddot(m, m));
+        int lit = ml > 0.0 && len > 0.0;
+        double sh = lit ? ml / len : 0.0;
+        double rt = t - (double)target[2 * i], rs = sh - (double)target[2 * i + 1];
+        L += w_depth * rt * rt + w_shade * rs * rs;
+        if (image) {
+            image[2 * i] = t;
+            image[2 * i + 1] = sh;
+        }
+        double gt = 2.0 * w_depth * rt, gs = 2.0 * w_shade * rs, gu = 0.0, gv = 0.0;
+        if (lit) {
+            d3 dsdm = dscl(dsub(lt, dscl(m, sh / len)), 1.0 / len);
+            gu = gs * ddot(dsdm, dsub(n1, n0));
+            gv = gs * ddot(dsdm, dsub(n2, n0));
+        }
+        d3 lam = dscl(dadd(dadd(dscl(c12, gt), dscl(c20, gu)), dscl(c01, gv)), 1.0 / det);
+        double cw[3] = {-w, -u, -v};
+        for (int k = 0; k < 3; ++k) {
+            double* g = grad_v + 3 * (size_t)tv[k];
+            g[0] += cw[k] * lam.x;
+            g[1] += cw[k] * lam.y;
+            g[2] += cw[k] * lam.z;
+        }
+        double lr[3] = {lam.x, lam.y, lam.z};
+        double oc[4] = {ow.x, ow.y, ow.z, 1.0}, dc[4] = {dw.x, dw.y, dw.z, 0.0};
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c)
+                grad_minv[c * 4 + r] += lr[r] * oc[c] + t * lr[r] * dc[c];
+    }
+    *loss = L;
     return ORC_OK;
 }

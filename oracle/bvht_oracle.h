@@ -136,6 +136,17 @@ int orc_tlas_trace(const orc_tnode* tnodes, const orc_instance* inst,
 void orc_camera_ray(const orc_camera* cam, float u, float v, orc_ray* out);
 int orc_raygen(const orc_camera* cam, uint32_t width, uint32_t height, orc_ray* rays);
 
+/* differentiable depth/shading checker (no reference counterpart; restates
+ * the library's forward/adjoint pair: binary32 forward, binary64 gradient) */
+void orc_mesh_normals(const float* V, uint32_t nv, const uint32_t* F, uint32_t nf, float* N);
+int orc_diff_forward(const orc_node* nodes, const float* tris, const uint32_t* vid, const float* normals,
+                     const float minv[16], const float light[3], const orc_ray* rays, uint64_t n, orc_hit* hits,
+                     float* image, int nthreads);
+int orc_diff_grad64(const double* V, uint32_t nv, const uint32_t* vid, uint32_t ntris, const double* N,
+                    const double minv[16], const double light[3], const orc_ray* rays, const uint32_t* prims,
+                    uint64_t n, const float* target, double w_depth, double w_shade, double* loss, double* grad_v,
+                    double* grad_minv, double* image);
+
 #ifdef __cplusplus
 }
 #endif
