@@ -10,10 +10,18 @@ from .api import (Batch, Bvh, BvhBuilder, Context, DeviceArray, Graph, Mesh, Ray
                   bounce, bounce_tlas, counter_add, default_context, deform_wave, raygen, trace_adjoint, trace_forward,
                   transform_gradient)
 
+from .render import (DepthAccumulator, DepthMappingShader, IntersectionAccumulator, IntersectionShader, PathTracer,
+                     RadianceToRgbShader, Renderer, RendererState, UvMappingAccumulator)
+from .scene import (ModelBuilder, ModelInstance, Scene, SceneBuilder, SceneObjectBuilder, Triangle,
+                    intersect_pairs)
+
 __all__ = [
     "BUILD_REFERENCE", "BUILD_SAH", "CAMERA_DTYPE", "HIT_DTYPE", "INSTANCE_DTYPE", "MISS", "NODE_DTYPE",
     "ORDER_ROWS", "ORDER_TILES", "RAY_DTYPE", "STATS_DTYPE", "TNODE_DTYPE", "BvhtError",
     "Batch", "Bvh", "BvhBuilder", "Context", "DeviceArray", "Graph", "Mesh", "Ray", "SceneObject", "Tlas",
     "TlasBuilder", "bounce", "bounce_tlas", "counter_add", "default_context", "deform_wave", "raygen",
     "trace_adjoint", "trace_forward", "transform_gradient",
+    "DepthAccumulator", "DepthMappingShader", "IntersectionAccumulator", "IntersectionShader", "PathTracer",
+    "RadianceToRgbShader", "Renderer", "RendererState", "UvMappingAccumulator",
+    "ModelBuilder", "ModelInstance", "Scene", "SceneBuilder", "SceneObjectBuilder", "Triangle", "intersect_pairs",
 ]

@@ -16,6 +16,10 @@ MISS = 0xFFFFFFFF
 ORDER_ROWS, ORDER_TILES = 0, 1
 BUILD_REFERENCE, BUILD_SAH = 0, 1
 SGD_ALONG_NORMAL = 1
+ACC_DEPTH, ACC_INTERSECTION, ACC_UV = 0, 1, 2
+SHADE_RADIANCE_TO_RGB, SHADE_INTERSECTION, SHADE_DEPTH_MAPPING = 0, 1, 2
+ACCUM_PARAMS_DTYPE = np.dtype([("hit_value", "<f4", 3), ("miss_value", "<f4", 3)])
+SHADER_PARAMS_DTYPE = np.dtype([("scale", "<f4"), ("offset", "<f4"), ("hit_rgba", "u1", 4), ("miss_rgba", "u1", 4)])
 
 NODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("left_first", "<u4"), ("bmax", "<f4", 3), ("count", "<u4")])
 TNODE_DTYPE = np.dtype([("bmin", "<f4", 3), ("left_right", "<u4"), ("bmax", "<f4", 3), ("blas", "<u4")])
@@ -58,6 +62,7 @@ SIGNATURES = {
     "bvht_blas_refit": (I32, [P, P]),
     "bvht_blas_bounds": (I32, [P, P, P]),
     "bvht_blas_destroy": (I32, [P]),
+    "bvht_intersect_pairs": (I32, [P, P, P, U32, P]),
     "bvht_trace_blas": (I32, [P, P, P, U32, P, P]),
     "bvht_trace_blas_count": (I32, [P, P, P, P, U32, P, P]),
     "bvht_set_trace_config": (I32, [P, I32, I32]),
@@ -65,6 +70,8 @@ SIGNATURES = {
     "bvht_raygen": (I32, [P, P, U32, U32, I32, P]),
     "bvht_bounce": (I32, [P, P, P, P, U32, U64, P, P]),
     "bvht_depth": (I32, [P, P, U32, P]),
+    "bvht_accumulate": (I32, [P, P, U32, U32, I32, I32, P, P]),
+    "bvht_shade": (I32, [P, P, U32, I32, P, P]),
     "bvht_tlas_create": (I32, [P, P, U32, P, U32, PP]),
     "bvht_tlas_set_objects": (I32, [P, P, P, U32, U32]),
     "bvht_tlas_refresh_bounds": (I32, [P, P]),

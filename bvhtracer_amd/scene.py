@@ -1,0 +1,191 @@
+"""Host mirror of the reference's model and scene types over the GPU library.
+
+Model / ModelInstance / ModelBuilder (bvhtracer/src/model/model.rs:15-150),
+SceneObjectBuilder (scene/scene_object.rs:92-135), Scene / SceneBuilder
+(scene/scene.rs:7-96) and Triangle (geometry/triangle.rs:9-108) with the same
+names and argument meaning.  Every query runs on the GPU (libbvht_hip.so);
+these classes only hold host-side bookkeeping (meshes, transforms, object
+lists) and device handles.
+"""
+import numpy as np
+
+from . import _lib as L
+from .api import (BUILD_REFERENCE, HIT_DTYPE, MISS, RAY_DTYPE, BvhBuilder, SceneObject, TlasBuilder, _as_tris,
+                  default_context)
+
+
+class ModelInstance:
+    """A shared model: mesh (triangles in build order) + its BVH on the GPU.
+    Python references give the sharing of Rc<RefCell<Model>>
+    (model.rs:15-58): every SceneObject built from one ModelInstance uses
+    the same mesh and the same BLAS."""
+
+    def __init__(self, mesh, bvh):
+        self._mesh = mesh
+        self.bvh = bvh
+
+    def intersect(self, rays):
+        """Model::intersect (model.rs:69-71) for a batch of model-space rays."""
+        return self.bvh.intersect(rays)
+
+    def refit(self):
+        """Model::refit (model.rs:73-75): Bvh::refit against the current
+        primitives (edit them through primitives_mut())."""
+        self.bvh.refit(self._mesh)
+
+    def bounds(self):
+        """Model-space bounds (model.rs:77-80): (bmin, bmax)."""
+        return self.bvh.bounds()
+
+    def model(self):
+        return self
+
+    def primitives(self):
+        view = self._mesh.view()
+        view.flags.writeable = False
+        return view
+
+    def primitives_mut(self):
+        return self._mesh
+
+    def len(self):
+        return self._mesh.shape[0]
+
+    def len_primitives(self):
+        return self._mesh.shape[0]
+
+
+class ModelBuilder:
+    """ModelBuilder (model.rs:113-150).  build() builds the BVH on the GPU and
+    permutes the mesh's triangles into build order, as
+    BvhBuilder::build_for(mesh.primitives_mut()) does.  with_build() selects
+    the builder (BUILD_REFERENCE reproduces the reference exactly)."""
+
+    def __init__(self, ctx=None):
+        self.ctx = ctx or default_context()
+        self.mesh = np.zeros((0, 9), np.float32)
+        self.mode, self.max_leaf = BUILD_REFERENCE, 0
+
+    def with_mesh(self, mesh):
+        self.mesh = _as_tris(mesh)
+        return self
+
+    def with_build(self, mode, max_leaf=0):
+        self.mode, self.max_leaf = mode, max_leaf
+        return self
+
+    def build(self):
+        bvh = BvhBuilder(self.mode, self.max_leaf, ctx=self.ctx).build_for(self.mesh)
+        return ModelInstance(self.mesh, bvh)
+
+
+class SceneObjectBuilder:
+    """SceneObjectBuilder (scene_object.rs:92-135): a ModelInstance placed by
+    a transform M (column-major 4x4; identity by default)."""
+
+    def __init__(self, model):
+        self.model = model
+        self.transform = None
+
+    def with_transform(self, transform):
+        self.transform = np.asarray(transform, np.float32).reshape(16)
+        return self
+
+    def build(self):
+        obj = SceneObject(-1, self.transform)
+        obj.instance = self.model
+        return obj
+
+
+class Scene:
+    """Scene (scene.rs:7-58): objects + TLAS + active camera record."""
+
+    def __init__(self, tlas, objects, camera):
+        self.tlas = tlas
+        self.objects = objects
+        self.camera = camera
+
+    def get_unchecked(self, index):
+        return self.objects[index]
+
+    def get_mut_unchecked(self, index):
+        return self.objects[index]
+
+    def active_camera(self):
+        return self.camera
+
+    def intersect(self, rays):
+        """Scene::intersect (scene.rs:32-34) for a batch of world-space rays."""
+        return self.tlas.intersect(rays)
+
+    def intersect_device(self, d_rays, n, d_hits, d_stats=None):
+        self.tlas.intersect_device(d_rays, n, d_hits, d_stats)
+
+    def rebuild(self):
+        """Scene::rebuild (scene.rs:36-38): object transforms and world bounds
+        from the current models' bounds, then Tlas::rebuild."""
+        self.tlas.rebuild(self.objects)
+
+
+class SceneBuilder:
+    """SceneBuilder (scene.rs:61-96).  Objects built by SceneObjectBuilder
+    share BLASes by ModelInstance identity."""
+
+    def __init__(self, camera=None, ctx=None):
+        self.camera = camera
+        self.objects = []
+        self.ctx = ctx or default_context()
+
+    def with_object(self, obj):
+        self.objects.append(obj)
+        return self
+
+    def with_objects(self, objects):
+        self.objects = list(objects)
+        return self
+
+    def build(self):
+        models, index = [], {}
+        for o in self.objects:
+            mi = getattr(o, "instance", None)
+            if mi is None:
+                raise ValueError("scene objects must come from SceneObjectBuilder (they name their model)")
+            if id(mi) not in index:
+                index[id(mi)] = len(models)
+                models.append(mi)
+            o.model = index[id(mi)]
+        tlas = TlasBuilder(ctx=self.ctx).build_for(self.objects, [m.bvh for m in models])
+        scene = Scene(tlas, self.objects, self.camera)
+        scene.models = models
+        return scene
+
+
+class Triangle:
+    """Triangle (geometry/triangle.rs:9-108).  intersect() tests triangle
+    i against ray i on the GPU for arrays of triangles ((N, 9) float32) and
+    rays; a single triangle is tested against every ray."""
+
+    def __init__(self, vertex0, vertex1, vertex2):
+        self.tri = np.concatenate([np.asarray(v, np.float32).reshape(3) for v in (vertex0, vertex1, vertex2)])
+
+    def intersect(self, rays, ctx=None):
+        """(t, u, v) per ray, or None where it misses (Triangle::intersect)."""
+        rays = np.ascontiguousarray(rays, RAY_DTYPE).reshape(-1)
+        hits = intersect_pairs(np.tile(self.tri, (rays.shape[0], 1)), rays, ctx)
+        return [None if h["prim"] == MISS else (float(h["t"]), float(h["u"]), float(h["v"])) for h in hits]
+
+
+def intersect_pairs(tris, rays, ctx=None):
+    """Triangle::intersect / intersect_mut for n (triangle, ray) pairs on the
+    GPU: hit i = {min(ray.t, t), u, v, prim = i} or a miss record."""
+    ctx = ctx or default_context()
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE).reshape(-1)
+    if tris.shape[0] != rays.shape[0]:
+        raise ValueError("one triangle per ray")
+    n = rays.shape[0]
+    if n == 0:
+        return np.zeros(0, HIT_DTYPE)
+    d_t, d_r, d_h = ctx.to_device(tris), ctx.to_device(rays), ctx.alloc(n * HIT_DTYPE.itemsize)
+    L.check(L.lib().bvht_intersect_pairs(ctx.handle, d_t.ptr, d_r.ptr, n, d_h.ptr))
+    return d_h.download(HIT_DTYPE, n)

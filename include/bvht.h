@@ -188,6 +188,12 @@ int32_t bvht_blas_destroy(bvht_blas blas);
 
 /* ---- traversal -------------------------------------------------------- */
 
+/* Triangle::intersect / intersect_mut (geometry/triangle.rs:41-108) for n
+ * independent (triangle, ray) pairs: d_hits[i] = {min(ray.t, t), u, v,
+ * prim = i} when ray i hits triangle i (d_tris float[9n]), else a miss. */
+int32_t bvht_intersect_pairs(bvht_ctx ctx, const float* d_tris, const bvht_ray* d_rays, uint32_t n,
+                             bvht_hit* d_hits);
+
 /* Bvh::intersect (bvh.rs:307-309, intersect_subtree :242-305) for a batch:
  * d_rays[n] -> d_hits[n].  d_stats (device, may be NULL) receives per-ray
  * work counters. */
@@ -228,6 +234,35 @@ int32_t bvht_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, const 
 
 /* Depth accumulator (renderer.rs:176-194): d_depth[i] = hit t or f32::MAX. */
 int32_t bvht_depth(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, float* d_depth);
+
+/* Accumulators (renderer.rs:134-235) evaluated from the hits of a frame's
+ * rays (d_hits[W*H] in ray order `order`, as produced by bvht_raygen + a
+ * trace) and stored at each ray's pixel address y*W + x of the accumulation
+ * buffer d_accum (float[3*W*H], AccumulationBuffer, renderer.rs:15-33), as
+ * PathTracer::evaluate does (renderer.rs:345-368). */
+#define BVHT_ACC_DEPTH 0        /* DepthAccumulator: t (f32::MAX on a miss) in all three channels */
+#define BVHT_ACC_INTERSECTION 1 /* IntersectionAccumulator: hit_value or miss_value */
+#define BVHT_ACC_UV 2           /* UvMappingAccumulator: (u, v, 1 - (u + v)), 0 on a miss */
+typedef struct {
+    float hit_value[3];
+    float miss_value[3];
+} bvht_accum_params;
+int32_t bvht_accumulate(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t width, uint32_t height,
+                        int32_t order, int32_t kind, const bvht_accum_params* params, float* d_accum);
+
+/* Pixel shaders (renderer.rs:113-222) over an accumulation buffer of
+ * n_pixels into the RGBA8 frame buffer d_rgba (uint8[4*n_pixels], row-major
+ * like FrameBuffer / TextureBuffer2D, texture_buffer.rs:216-240). */
+#define BVHT_SHADE_RADIANCE_TO_RGB 0 /* RadianceToRgbShader */
+#define BVHT_SHADE_INTERSECTION 1    /* IntersectionShader: hit_rgba iff the radiance is non-zero */
+#define BVHT_SHADE_DEPTH_MAPPING 2   /* DepthMappingShader(scale, offset), wrapping u32 arithmetic */
+typedef struct {
+    float scale, offset;  /* DepthMappingShader::new(scale, offset) */
+    uint8_t hit_rgba[4];  /* IntersectionShader::new(hit_value, miss_value) */
+    uint8_t miss_rgba[4];
+} bvht_shader_params;
+int32_t bvht_shade(bvht_ctx ctx, const float* d_accum, uint32_t n_pixels, int32_t kind,
+                   const bvht_shader_params* params, uint8_t* d_rgba);
 
 /* ---- TLAS / instances ------------------------------------------------- */
 
