@@ -10,12 +10,20 @@ timed region; every step regenerates its rays on the GPU.
 
 value = (primary + bounce rays) / frame time.  Primary and bounce Mrays/s are
 reported separately from their own kernel times.  The roofline object prices
-the bounce trace (the dominant kernel) at the algorithmic bytes per ray
-28 + 20 + 32 + 64 P + 36 T (ray, hit, root node, P child-pair fetches, T
-triangle tests; P and T from the instrumented trace of the same rays) against
-8 TB/s.  The CPU baseline is the oracle (C restatement of the reference's
-traversal) on the same BVH and a bounded sample of the same rays, all host
-threads up to 16.
+the dominant kernel, k_trace_blas (two launches per frame), at its
+algorithmic bytes per launch -- per ray 28 + 20 + 32 + 64 P + 36 T (ray, hit,
+root node, P child-pair fetches, T triangle tests; P and T from the
+instrumented trace of the same rays), averaged over the primary and bounce
+launches -- divided by the mean launch duration from HIP events, against
+8 TB/s; `traffic` is the measured HBM bytes per launch from rocprofv3
+FETCH_SIZE/WRITE_SIZE (profiles/).  The CPU baseline is the oracle (C
+restatement of the reference's traversal) on the same BVH and the same
+frame's rays, on up to 16 host threads, repeated to fill --cpu-seconds.
+
+Extras (not part of `value`): 16 instances with per-frame deform/refit/TLAS
+rebuild in one hipGraph; 64 uneven scenes traced by one persistent launch vs
+64 plain launches; and the differentiable depth/shading loop (forward +
+adjoint + SGD + refit, 100 steps on the 81,920-face icosphere).
 """
 import argparse
 import json
@@ -45,6 +53,8 @@ def parse():
     p.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_trace_blas_sphere1m_traffic.json"),
                    help="measured HBM bytes per launch of the dominant kernel (rocprofv3 --pmc summary)")
     p.add_argument("--no-autotune", action="store_true", help="use the default trace configuration")
+    p.add_argument("--trace-config", default=None, metavar="WGxLDS",
+                   help="fix the trace kernel configuration (e.g. 64x24) instead of autotuning")
     p.add_argument("--no-extras", action="store_true", help="skip the batched and instanced workloads")
     p.add_argument("--batch-scenes", type=int, default=64)
     p.add_argument("--instanced-frames", type=int, default=64)
@@ -151,7 +161,13 @@ def main():
     n_bounce = int(d_count.download(np.uint32, 1)[0])
     log(f"[bench] warmup done; bounce rays per frame ~{n_bounce}")
     tuned = None
-    if not args.no_autotune:
+    if args.trace_config:
+        from bvhtracer_amd import autotune
+
+        wg, d = (int(x) for x in args.trace_config.lower().split("x"))
+        autotune.set_config(ctx, wg, d)
+        tuned = {"workgroup": wg, "lds_stack_depth": d, "fixed": True}
+    elif not args.no_autotune:
         from bvhtracer_amd import autotune
 
         def traces():
@@ -540,16 +556,21 @@ def cpu_baseline(ctx, bvh, d_brays, d_bhits, n_bounce, d_rays, d_hits, n_primary
     t0 = time.perf_counter()
     oracle.bvh_trace(nodes, tris, probe, nthreads=cores)
     rate = probe.shape[0] / max(time.perf_counter() - t0, 1e-6)
-    want = int(min(rate * budget_s, n_primary + n_bounce))
-    frac = want / (n_primary + n_bounce)
+    total = n_primary + n_bounce
+    want = int(rate * budget_s)
+    frac = min(want, total) / total
     kp = max(1, int(n_primary * frac))
     kb = max(1, int(n_bounce * frac))
+    # a whole frame takes less than the budget on many cores: trace it again
+    reps = max(1, min(64, round(want / total))) if want > total else 1
     ip = np.linspace(0, n_primary - 1, kp).astype(np.int64)
     ib = np.linspace(0, n_bounce - 1, kb).astype(np.int64)
     sample = np.concatenate([rays[ip], brays[ib]]).view(oracle.RAY_DTYPE)
     t0 = time.perf_counter()
     ref = oracle.bvh_trace(nodes, tris, sample, nthreads=cores)
-    dt = time.perf_counter() - t0
+    for _ in range(reps - 1):
+        oracle.bvh_trace(nodes, tris, sample, nthreads=cores)
+    dt = (time.perf_counter() - t0) / reps
     t1 = time.perf_counter()
     single = np.concatenate([rays[ip[:20000]], brays[ib[:20000]]]).view(oracle.RAY_DTYPE)
     oracle.bvh_trace(nodes, tris, single, nthreads=1)
@@ -561,8 +582,8 @@ def cpu_baseline(ctx, bvh, d_brays, d_bhits, n_bounce, d_rays, d_hits, n_primary
         "unit": "Mrays/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{kp} primary + {kb} bounce rays (every {max(1, n_primary // kp)}th), same BVH; "
-                  f"{dt:.1f} s on {cores} threads",
+        "sample": f"{kp} primary + {kb} bounce rays of one frame (every {max(1, n_primary // kp)}th), same BVH, "
+                  f"traced {reps}x; {dt * reps:.1f} s on {cores} threads",
         "single_thread_mrays_s": round(single.shape[0] / dt1 / 1e6, 3),
         "sample_hits_bit_identical": parity,
     }

@@ -113,19 +113,61 @@ __global__ void k_levels(uint32_t nodes_used, const uint32_t* __restrict__ paren
     atomicMax(depth, d + 1);
 }
 
+// Per-level node counts and the level-grouped node order.  Levels below
+// kLevelsLds are counted in LDS first, so each workgroup issues one global
+// atomic per level instead of one per node (a few dozen counters shared by
+// millions of nodes serialise otherwise); deeper levels (degenerate trees)
+// go straight to the global counters.  Workgroups stride over the nodes.
+// The order within a level is arbitrary: the nodes of one level are refit
+// independently.
+constexpr uint32_t kLevelsLds = 64;
+
 __global__ void k_level_hist(uint32_t nodes_used, const uint32_t* __restrict__ level, uint32_t* __restrict__ hist)
 {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nodes_used && k != 1)
-        atomicAdd(&hist[level[k]], 1u);
+    __shared__ uint32_t sh[kLevelsLds];
+    for (uint32_t i = threadIdx.x; i < kLevelsLds; i += blockDim.x)
+        sh[i] = 0;
+    __syncthreads();
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nodes_used; k += gridDim.x * blockDim.x) {
+        if (k == 1)
+            continue;
+        uint32_t L = level[k];
+        if (L < kLevelsLds)
+            atomicAdd(&sh[L], 1u);
+        else
+            atomicAdd(&hist[L], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kLevelsLds; i += blockDim.x)
+        if (sh[i])
+            atomicAdd(&hist[i], sh[i]);
 }
 
 __global__ void k_level_scatter(uint32_t nodes_used, const uint32_t* __restrict__ level, uint32_t* __restrict__ cursor,
                                 uint32_t* __restrict__ order)
 {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nodes_used && k != 1)
-        order[atomicAdd(&cursor[level[k]], 1u)] = k;
+    __shared__ uint32_t cnt[kLevelsLds], base[kLevelsLds];
+    uint32_t stride = gridDim.x * blockDim.x;
+    // one pass per block-sized slice of nodes, so the local ranks fit in registers
+    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < nodes_used; k0 += stride) {
+        for (uint32_t i = threadIdx.x; i < kLevelsLds; i += blockDim.x)
+            cnt[i] = 0;
+        __syncthreads();
+        uint32_t k = k0 + threadIdx.x;
+        bool valid = k < nodes_used && k != 1;
+        uint32_t L = valid ? level[k] : 0u;
+        uint32_t local = 0;
+        if (valid && L < kLevelsLds)
+            local = atomicAdd(&cnt[L], 1u);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < kLevelsLds; i += blockDim.x)
+            if (cnt[i])
+                base[i] = atomicAdd(&cursor[i], cnt[i]);
+        __syncthreads();
+        if (valid)
+            order[L < kLevelsLds ? base[L] + local : atomicAdd(&cursor[L], 1u)] = k;
+        __syncthreads();
+    }
 }
 
 // Bvh::refit, one tree level (see refit_node).
@@ -366,7 +408,8 @@ int32_t compute_blas_topology(bvht_blas b, const bvht_node*)
     rc = fill_u32(ctx, hist, 0, depth);
     if (rc)
         return rc;
-    hipLaunchKernelGGL(k_level_hist, dim3(nb), dim3(256), 0, st, b->nodes_used, b->d_level, hist);
+    uint32_t nb_levels = std::min<uint32_t>(nb, 512);
+    hipLaunchKernelGGL(k_level_hist, dim3(nb_levels), dim3(256), 0, st, b->nodes_used, b->d_level, hist);
     std::vector<uint32_t> h(depth);
     BVHT_HIP(hipMemcpyAsync(h.data(), hist, sizeof(uint32_t) * depth, hipMemcpyDeviceToHost, st));
     BVHT_HIP(hipStreamSynchronize(st));
@@ -377,7 +420,7 @@ int32_t compute_blas_topology(bvht_blas b, const bvht_node*)
         return set_error(BVHT_EDEVICE, "BLAS topology: %u nodes grouped, %u expected", b->level_off[depth],
                          b->nodes_used - 1);
     BVHT_HIP(hipMemcpyAsync(hist, b->level_off.data(), sizeof(uint32_t) * depth, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_level_scatter, dim3(nb), dim3(256), 0, st, b->nodes_used, b->d_level, hist,
+    hipLaunchKernelGGL(k_level_scatter, dim3(nb_levels), dim3(256), 0, st, b->nodes_used, b->d_level, hist,
                        b->d_level_order);
     BVHT_HIP(hipGetLastError());
     BVHT_HIP(hipStreamSynchronize(st));
