@@ -23,7 +23,8 @@ frame's rays, on up to 16 host threads, repeated to fill --cpu-seconds.
 Extras (not part of `value`): 16 instances with per-frame deform/refit/TLAS
 rebuild in one hipGraph; 64 uneven scenes traced by one persistent launch vs
 64 plain launches; and the differentiable depth/shading loop (forward +
-adjoint + SGD + refit, 100 steps on the 81,920-face icosphere).
+adjoint + SGD + refit, 100 steps on the 81,920-face icosphere); and the
+reference's own build/refit benchmarks on its unity.tri and bigben.tri.
 """
 import argparse
 import json
@@ -59,7 +60,7 @@ def parse():
     p.add_argument("--batch-scenes", type=int, default=64)
     p.add_argument("--instanced-frames", type=int, default=64)
     p.add_argument("--adjoint-steps", type=int, default=100)
-    p.add_argument("--only", choices=["adjoint", "instanced", "batched"], default=None,
+    p.add_argument("--only", choices=["adjoint", "instanced", "batched", "meshes"], default=None,
                    help="run one extra workload alone and print its JSON")
     p.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
                    help="only run the stream-copy kernel 5 times on BYTES (profiler counter calibration)")
@@ -108,7 +109,8 @@ def main():
         print(json.dumps({"copy_bytes": args.copy_only, "launches": 5}))
         return
     if args.only:
-        fn = {"adjoint": bench_adjoint, "instanced": bench_instanced, "batched": bench_batched}[args.only]
+        fn = {"adjoint": bench_adjoint, "instanced": bench_instanced, "batched": bench_batched,
+              "meshes": bench_meshes}[args.only]
         print(json.dumps({args.only: fn(args, B.Context(0))}), flush=True)
         return
 
@@ -271,7 +273,8 @@ def main():
         result["cpu_baseline"] = cpu_baseline(ctx, bvh, d_brays, d_bhits, n_bounce, d_rays, d_hits, n_primary,
                                               args.cpu_seconds)
     if not args.no_extras:
-        for name, fn in (("instanced", bench_instanced), ("batched", bench_batched), ("adjoint", bench_adjoint)):
+        for name, fn in (("instanced", bench_instanced), ("batched", bench_batched), ("adjoint", bench_adjoint),
+                         ("meshes", bench_meshes)):
             try:
                 result[name] = fn(args, ctx)
             except Exception as e:  # a failing extra must not hide the main measurement
@@ -535,6 +538,75 @@ def bench_adjoint(args, ctx):
                                  "achieved": round(adj_bytes / (adj2 * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                                  "frac": round(adj_bytes / (adj2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
             "loss_first": loss0, "loss_last": loss_last}
+
+
+def bench_meshes(args, ctx):
+    """The reference's criterion benchmarks on its own assets
+    (tests/golden/): bvh_construction = BvhBuilder on unity.tri
+    (benches/bench_bvh_build.rs); bvh_rebuild = a fresh build of bigben.tri
+    after animate(0.05), and bvh_refit = Bvh::refit of that animated mesh
+    (benches/bench_bvh_refit_rebuild.rs).  The reference builder (8 bins,
+    zero-initialised) on the GPU; the oracle on one host thread beside it.
+    Build times are wall clock around the synchronous build call; refit time
+    from HIP events, averaged over 100 refits."""
+    import oracle
+
+    import bvhtracer_amd as B
+    from bvhtracer_amd import mesh as M
+    from bvhtracer_amd import scenes
+
+    golden = os.path.join(ROOT, "tests", "golden")
+    unity = M.load_tri(os.path.join(golden, "unity.tri.gz"))
+    bigben = M.load_tri(os.path.join(golden, "bigben.tri.gz"))
+
+    def gpu_build(tris, reps=10):
+        times = []
+        for _ in range(reps):
+            t = tris.copy()
+            t0 = time.perf_counter()
+            bvh = B.BvhBuilder(B.BUILD_REFERENCE, ctx=ctx).build_for(t)
+            times.append(time.perf_counter() - t0)
+            last = (bvh, t)
+        return float(np.median(times)) * 1e3, last
+
+    def cpu_build(tris, reps=3):
+        times = []
+        for _ in range(reps):
+            t = tris.copy()
+            t0 = time.perf_counter()
+            out = oracle.bvh_build(t, True, 0)
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times)) * 1e3, out
+
+    build_gpu, (ubvh, ut) = gpu_build(unity)
+    build_cpu, (un, uu, utris) = cpu_build(unity)
+    same_build = ubvh.download()[2].tobytes() == utris.tobytes()
+    # the refit benchmark: build the original mesh, animate(0.05), refit
+    bb = bigben.copy()
+    bvh = B.BvhBuilder(B.BUILD_REFERENCE, ctx=ctx).build_for(bb)
+    animated = scenes.animate(bb, 0.05)
+    bvh.refit(animated)
+    e0, e1 = ctx.event(), ctx.event()
+    ctx.sync()
+    ctx.record(e0)
+    for _ in range(100):
+        bvh.refit()
+    ctx.record(e1)
+    refit_gpu = ctx.elapsed_ms(e0, e1) / 100
+    nodes0, used, _ = oracle.bvh_build(bb.copy(), True, 0)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        ref_nodes = oracle.bvh_refit(nodes0.copy(), used, animated)
+    refit_cpu = (time.perf_counter() - t0) / 20 * 1e3
+    same_refit = bvh.download()[0][:used].tobytes() == ref_nodes[:used].tobytes()
+    rebuild_gpu, _ = gpu_build(scenes.animate(bigben, 0.05))
+    rebuild_cpu, _ = cpu_build(scenes.animate(bigben, 0.05))
+    return {"workload": f"unity.tri ({unity.shape[0]} tris) build; bigben.tri ({bigben.shape[0]} tris) "
+                        f"animate(0.05) rebuild and refit; reference builder",
+            "bvh_construction_ms": {"gpu": round(build_gpu, 3), "cpu_oracle_1_thread": round(build_cpu, 3)},
+            "bvh_rebuild_ms": {"gpu": round(rebuild_gpu, 3), "cpu_oracle_1_thread": round(rebuild_cpu, 3)},
+            "bvh_refit_ms": {"gpu": round(refit_gpu, 4), "cpu_oracle_1_thread": round(refit_cpu, 4)},
+            "build_identical_to_oracle": bool(same_build), "refit_identical_to_oracle": bool(same_refit)}
 
 
 def cpu_baseline(ctx, bvh, d_brays, d_bhits, n_bounce, d_rays, d_hits, n_primary, budget_s):

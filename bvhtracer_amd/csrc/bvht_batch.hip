@@ -77,6 +77,18 @@ __global__ void k_batch_refit_level(const BatchBlas* __restrict__ blases, const 
     refit_node(bb.nodes, bb.tris, e.y);
 }
 
+// Big leaves of every BLAS of the batch, one wave per (BLAS, leaf) entry.
+__global__ __launch_bounds__(256) void k_batch_refit_big(const BatchBlas* __restrict__ blases,
+                                                         const uint2* __restrict__ entries, uint32_t count)
+{
+    uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= count)
+        return;
+    uint2 e = entries[w];
+    BatchBlas bb = blases[e.x];
+    refit_big_leaf(bb.nodes, bb.tris, e.y, threadIdx.x & 63u);
+}
+
 __global__ void k_batch_bounds(const BatchScene* __restrict__ scenes, const uint32_t* __restrict__ obj_prefix,
                                uint32_t n_scenes)
 {
@@ -170,6 +182,8 @@ struct bvht_batch_s {
     BatchScene* d_scenes = nullptr;
     BatchBlas* d_blases = nullptr;
     uint2* d_refit = nullptr;           // (BLAS, node) of every used node, grouped by level
+    uint2* d_refit_big = nullptr;       // (BLAS, leaf) of every big leaf
+    uint32_t n_refit_big = 0;
     std::vector<uint32_t> refit_off;     // level L occupies [refit_off[L], refit_off[L+1])
     uint32_t* d_obj_prefix = nullptr;
     uint32_t total_nodes = 0, total_objs = 0;
@@ -197,6 +211,7 @@ void batch_free(bvht_batch b)
     (void)hipFree(b->d_scenes);
     (void)hipFree(b->d_blases);
     (void)hipFree(b->d_refit);
+    (void)hipFree(b->d_refit_big);
     (void)hipFree(b->d_obj_prefix);
     (void)hipFree(b->d_items);
     (void)hipFree(b->d_ray_off);
@@ -293,10 +308,24 @@ int32_t bvht_batch_create(bvht_ctx ctx, const bvht_tlas* scenes, uint32_t n_scen
             for (uint32_t i = bl->level_off[L]; i < bl->level_off[L + 1]; ++i)
                 entries[cursor[L]++] = make_uint2((uint32_t)k, orders[k][i]);
     }
+    std::vector<uint2> big;
+    for (size_t k = 0; k < b->blases.size(); ++k) {
+        bvht_blas bl = b->blases[k];
+        std::vector<uint32_t> lst(bl->n_big_leaves);
+        if (!lst.empty() && hipMemcpy(lst.data(), bl->d_big_leaves, sizeof(uint32_t) * lst.size(),
+                                      hipMemcpyDeviceToHost) != hipSuccess) {
+            delete b;
+            return set_error(BVHT_EDEVICE, "batch: big-leaf list download failed");
+        }
+        for (uint32_t leaf : lst)
+            big.push_back(make_uint2((uint32_t)k, leaf));
+    }
+    b->n_refit_big = (uint32_t)big.size();
     b->total_objs = obj_prefix.back();
     bool ok = hipMalloc(&b->d_scenes, sizeof(BatchScene) * n_scenes) == hipSuccess &&
               hipMalloc(&b->d_blases, sizeof(BatchBlas) * hb.size()) == hipSuccess &&
               hipMalloc(&b->d_refit, sizeof(uint2) * std::max<size_t>(entries.size(), 1)) == hipSuccess &&
+              hipMalloc(&b->d_refit_big, sizeof(uint2) * std::max<size_t>(big.size(), 1)) == hipSuccess &&
               hipMalloc(&b->d_obj_prefix, sizeof(uint32_t) * obj_prefix.size()) == hipSuccess &&
               hipMalloc(&b->d_items, sizeof(uint32_t) * (n_scenes + 1)) == hipSuccess &&
               hipMalloc(&b->d_ray_off, sizeof(uint32_t) * (n_scenes + 1)) == hipSuccess &&
@@ -309,6 +338,8 @@ int32_t bvht_batch_create(bvht_ctx ctx, const bvht_tlas* scenes, uint32_t n_scen
     if (hipMemcpyAsync(b->d_scenes, hs.data(), sizeof(BatchScene) * n_scenes, hipMemcpyHostToDevice, st) ||
         hipMemcpyAsync(b->d_blases, hb.data(), sizeof(BatchBlas) * hb.size(), hipMemcpyHostToDevice, st) ||
         hipMemcpyAsync(b->d_refit, entries.data(), sizeof(uint2) * entries.size(), hipMemcpyHostToDevice, st) ||
+        (!big.empty() &&
+         hipMemcpyAsync(b->d_refit_big, big.data(), sizeof(uint2) * big.size(), hipMemcpyHostToDevice, st)) ||
         hipMemcpyAsync(b->d_obj_prefix, obj_prefix.data(), sizeof(uint32_t) * obj_prefix.size(),
                        hipMemcpyHostToDevice, st) ||
         hipStreamSynchronize(st)) {
@@ -336,6 +367,9 @@ int32_t bvht_batch_destroy(bvht_batch b)
 int32_t bvht_batch_refit(bvht_ctx ctx, bvht_batch b)
 {
     BVHT_REQUIRE(ctx && b && b->ctx == ctx, "NULL argument or foreign context");
+    if (b->n_refit_big)
+        hipLaunchKernelGGL(k_batch_refit_big, dim3(nblk(b->n_refit_big, 4)), dim3(256), 0, ctx->stream, b->d_blases,
+                           b->d_refit_big, b->n_refit_big);
     for (size_t L = b->refit_off.size() - 1; L-- > 0;) {
         uint32_t off = b->refit_off[L], cnt = b->refit_off[L + 1] - off;
         if (cnt)

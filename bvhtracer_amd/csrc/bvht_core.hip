@@ -170,6 +170,27 @@ __global__ void k_level_scatter(uint32_t nodes_used, const uint32_t* __restrict_
     }
 }
 
+// Leaves with more than kBigLeaf triangles: counted, then listed.
+__global__ void k_big_leaves(const bvht_node* __restrict__ nodes, uint32_t nodes_used, uint32_t* __restrict__ counter,
+                             uint32_t* __restrict__ list)
+{
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nodes_used || k == 1 || nodes[k].count <= kBigLeaf)
+        return;
+    uint32_t slot = atomicAdd(counter, 1u);
+    if (list)
+        list[slot] = k;
+}
+
+// Bvh::refit of the big leaves, one wave per leaf.
+__global__ __launch_bounds__(256) void k_refit_big_leaves(bvht_node* __restrict__ nodes, const float* __restrict__ tris,
+                                                          const uint32_t* __restrict__ list, uint32_t count)
+{
+    uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w < count)
+        refit_big_leaf(nodes, tris, list[w], threadIdx.x & 63u);
+}
+
 // Bvh::refit, one tree level (see refit_node).
 __global__ void k_refit_level(bvht_node* __restrict__ nodes, const float* __restrict__ tris,
                               const uint32_t* __restrict__ order, uint32_t count)
@@ -423,6 +444,27 @@ int32_t compute_blas_topology(bvht_blas b, const bvht_node*)
     hipLaunchKernelGGL(k_level_scatter, dim3(nb_levels), dim3(256), 0, st, b->nodes_used, b->d_level, hist,
                        b->d_level_order);
     BVHT_HIP(hipGetLastError());
+    // big leaves: count, allocate, list
+    rc = fill_u32(ctx, d_depth, 0, 1);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_big_leaves, dim3(nb), dim3(256), 0, st, b->d_nodes, b->nodes_used, d_depth, nullptr);
+    uint32_t nbig = 0;
+    BVHT_HIP(hipMemcpyAsync(&nbig, d_depth, 4, hipMemcpyDeviceToHost, st));
+    BVHT_HIP(hipStreamSynchronize(st));
+    (void)hipFree(b->d_big_leaves);
+    b->d_big_leaves = nullptr;
+    b->n_big_leaves = nbig;
+    if (nbig) {
+        if (hipMalloc(&b->d_big_leaves, sizeof(uint32_t) * nbig) != hipSuccess)
+            return set_error(BVHT_ENOMEM, "device allocation of the big-leaf list failed");
+        rc = fill_u32(ctx, d_depth, 0, 1);
+        if (rc)
+            return rc;
+        hipLaunchKernelGGL(k_big_leaves, dim3(nb), dim3(256), 0, st, b->d_nodes, b->nodes_used, d_depth,
+                           b->d_big_leaves);
+        BVHT_HIP(hipGetLastError());
+    }
     BVHT_HIP(hipStreamSynchronize(st));
     return BVHT_OK;
 }
@@ -430,6 +472,9 @@ int32_t compute_blas_topology(bvht_blas b, const bvht_node*)
 // Bvh::refit: one launch per tree level, deepest first.
 int32_t launch_refit(bvht_ctx ctx, bvht_blas blas)
 {
+    if (blas->n_big_leaves)
+        hipLaunchKernelGGL(k_refit_big_leaves, dim3(blocks_for(blas->n_big_leaves, 4)), dim3(256), 0, ctx->stream,
+                           blas->d_nodes, blas->d_tris, blas->d_big_leaves, blas->n_big_leaves);
     for (size_t L = blas->depth; L-- > 0;) {
         uint32_t off = blas->level_off[L], cnt = blas->level_off[L + 1] - off;
         if (cnt == 0)
@@ -794,6 +839,7 @@ int32_t bvht_blas_destroy(bvht_blas b)
     (void)hipFree(b->d_parent);
     (void)hipFree(b->d_level);
     (void)hipFree(b->d_level_order);
+    (void)hipFree(b->d_big_leaves);
     (void)hipFree(b->d_spill);
     delete b;
     return BVHT_OK;

@@ -34,6 +34,8 @@ struct bvht_blas_s {
     uint32_t* d_level = nullptr;        // depth of each used node (root 0; dummy node 1: 0xFFFFFFFF)
     uint32_t* d_level_order = nullptr;  // used nodes except the dummy, grouped by level
     std::vector<uint32_t> level_off;    // level L occupies [level_off[L], level_off[L+1]) of d_level_order
+    uint32_t* d_big_leaves = nullptr;   // leaves with more than kBigLeaf triangles (refit a wave each)
+    uint32_t n_big_leaves = 0;
     uint32_t* d_spill = nullptr;    // traversal stack spill (grown on demand)
     size_t spill_words = 0;
 };

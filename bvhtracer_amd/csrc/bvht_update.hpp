@@ -15,9 +15,16 @@ namespace bvht {
 // a branch takes the union of its two children.  min/max are exact and every
 // child has a larger index than its parent (bvh.rs:601-612), so the result
 // equals the reference's reverse index sweep bit for bit.
+// Leaves with more than kBigLeaf triangles (the reference builder makes
+// leaves of thousands) are refit by a whole wave each (refit_big_leaf), before
+// the level passes; refit_node leaves them alone.
+constexpr uint32_t kBigLeaf = 64;
+
 __device__ __forceinline__ void refit_node(bvht_node* __restrict__ nodes, const float* __restrict__ tris, uint32_t k)
 {
     bvht_node nd = nodes[k];
+    if (nd.count > kBigLeaf)
+        return;
     V3 lo, hi;
     if (nd.count > 0) {
         lo = mk3(kFltMax, kFltMax, kFltMax);
@@ -40,6 +47,33 @@ __device__ __forceinline__ void refit_node(bvht_node* __restrict__ nodes, const 
     float4* o = reinterpret_cast<float4*>(nodes + k);
     o[0] = make_float4(lo.x, lo.y, lo.z, __uint_as_float(nd.left_first));
     o[1] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(nd.count));
+}
+
+// Leaf k's box from its triangles by the 64 lanes of one wave: lanes take
+// every 64th triangle, then min/max are combined across the wave (exact, so
+// the order does not change the box; only the sign of a zero bound can
+// differ from the sequential sweep).  Every lane of the wave must call it.
+__device__ __forceinline__ void refit_big_leaf(bvht_node* __restrict__ nodes, const float* __restrict__ tris,
+                                               uint32_t k, uint32_t lane)
+{
+    bvht_node nd = nodes[k];
+    V3 lo = mk3(kFltMax, kFltMax, kFltMax);
+    V3 hi = mk3(-kFltMax, -kFltMax, -kFltMax);
+    for (uint32_t j = lane; j < nd.count; j += 64) {
+        const float* t = tris + 9 * (size_t)(nd.left_first + j);
+        V3 v0 = mk3(t[0], t[1], t[2]), v1 = mk3(t[3], t[4], t[5]), v2 = mk3(t[6], t[7], t[8]);
+        lo = min3(min3(min3(lo, v0), v1), v2);
+        hi = max3(max3(max3(hi, v0), v1), v2);
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+        lo = min3(lo, mk3(__shfl_xor(lo.x, m, 64), __shfl_xor(lo.y, m, 64), __shfl_xor(lo.z, m, 64)));
+        hi = max3(hi, mk3(__shfl_xor(hi.x, m, 64), __shfl_xor(hi.y, m, 64), __shfl_xor(hi.z, m, 64)));
+    }
+    if (lane == 0) {
+        float4* o = reinterpret_cast<float4*>(nodes + k);
+        o[0] = make_float4(lo.x, lo.y, lo.z, __uint_as_float(nd.left_first));
+        o[1] = make_float4(hi.x, hi.y, hi.z, __uint_as_float(nd.count));
+    }
 }
 
 // SceneObject::set_transform (scene_object.rs:60-75) for object i: world box of

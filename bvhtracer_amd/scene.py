@@ -20,9 +20,15 @@ class ModelInstance:
     (model.rs:15-58): every SceneObject built from one ModelInstance uses
     the same mesh and the same BLAS."""
 
-    def __init__(self, mesh, bvh):
+    def __init__(self, mesh, bvh, attributes=None):
         self._mesh = mesh
         self.bvh = bvh
+        self._attributes = attributes  # the decoded Mesh, attributes in build order
+
+    def mesh(self):
+        """The Mesh (texture coordinates and normals permuted with the
+        triangles), when the model was built from one."""
+        return self._attributes
 
     def intersect(self, rays):
         """Model::intersect (model.rs:69-71) for a batch of model-space rays."""
@@ -64,10 +70,18 @@ class ModelBuilder:
     def __init__(self, ctx=None):
         self.ctx = ctx or default_context()
         self.mesh = np.zeros((0, 9), np.float32)
+        self.attributes = None
         self.mode, self.max_leaf = BUILD_REFERENCE, 0
 
     def with_mesh(self, mesh):
-        self.mesh = _as_tris(mesh)
+        """``mesh``: a decoded Mesh (bvhtracer_amd.mesh) or an (N, 9) float32
+        triangle array."""
+        if hasattr(mesh, "primitives_mut"):
+            self.attributes = mesh
+            self.mesh = mesh.primitives_mut()
+        else:
+            self.attributes = None
+            self.mesh = _as_tris(mesh)
         return self
 
     def with_build(self, mode, max_leaf=0):
@@ -76,7 +90,10 @@ class ModelBuilder:
 
     def build(self):
         bvh = BvhBuilder(self.mode, self.max_leaf, ctx=self.ctx).build_for(self.mesh)
-        return ModelInstance(self.mesh, bvh)
+        attrs = getattr(self, "attributes", None)
+        if attrs is not None:
+            attrs.permute(bvh.permutation())
+        return ModelInstance(self.mesh, bvh, attrs)
 
 
 class SceneObjectBuilder:

@@ -429,3 +429,19 @@ def diff_workload(freq=64, radius=32.0, sigma=0.05, n_cameras=8, width=512, seed
 # lowers the loss fastest among lr 1e-3..1e-1, clip 0.02..0.1.
 DIFF_LR = 1e-3
 DIFF_MAX_STEP = 0.05
+
+
+def animate(tris, r):
+    """The refit benchmark's deformation (benches/bench_bvh_refit_rebuild.rs:
+    animate): every vertex rotated about z by s = (a (y - 0.2)) 0.2 with
+    a = sin(r) 0.5, in binary32 (numpy's float32 sin/cos; input synthesis,
+    the same buffer goes to the GPU and to the oracle)."""
+    t = np.asarray(tris, np.float32).reshape(-1, 3, 3)
+    a = np.float32(np.sin(np.float32(r))) * np.float32(0.5)
+    x, y = t[..., 0], t[..., 1]
+    s = (a * (y - np.float32(0.2))) * np.float32(0.2)
+    c, sn = np.cos(s), np.sin(s)
+    out = t.copy()
+    out[..., 0] = x * c - y * sn
+    out[..., 1] = x * sn + y * c
+    return np.ascontiguousarray(out.reshape(-1, 9))

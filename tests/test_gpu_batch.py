@@ -10,12 +10,12 @@ from helpers import assert_hits_equal
 pytestmark = pytest.mark.gpu
 
 
-def build_batch(ctx, n_scenes=6, seed=0x51, tri_range=(500, 20000), ray_range=(1000, 20000)):
+def build_batch(ctx, n_scenes=6, seed=0x51, tri_range=(500, 20000), ray_range=(1000, 20000), mode=B.BUILD_SAH):
     work = scenes.batch_workload(n_scenes, seed=seed, tri_range=tri_range, ray_range=ray_range)
     models, tlases, origs = [], [], []
     for w in work:
         m = w["mesh"].copy()
-        bvh = B.BvhBuilder(B.BUILD_SAH, 2, ctx=ctx).build_for(m)
+        bvh = B.BvhBuilder(mode, 2 if mode == B.BUILD_SAH else 0, ctx=ctx).build_for(m)
         objs = [B.SceneObject(0, w["M"][k].reshape(4, 4), w["Minv"][k]) for k in range(w["M"].shape[0])]
         models.append(bvh)
         origs.append(m)
@@ -69,8 +69,11 @@ def test_every_batch_config_is_bit_identical(ctx):
         assert_hits_equal(d_hits.download(B.HIT_DTYPE, rays.shape[0]), ref, f"config {wg}x{d}x{r}")
 
 
-def test_batched_refit_and_rebuild_equal_per_scene(ctx):
-    work, models, tlases, origs, rays, offsets = build_batch(ctx, n_scenes=5, seed=0x99)
+@pytest.mark.parametrize("mode", [B.BUILD_SAH, B.BUILD_REFERENCE])
+def test_batched_refit_and_rebuild_equal_per_scene(ctx, mode):
+    """Both builders: the reference builder's leaves of hundreds of
+    triangles go through the wave-per-leaf refit."""
+    work, models, tlases, origs, rays, offsets = build_batch(ctx, n_scenes=5, seed=0x99, mode=mode)
     batch = B.Batch(ctx, tlases)
     for bvh, orig in zip(models, origs):
         bvh.refit(scenes.deform_wave(orig, 3))
