@@ -10,8 +10,9 @@ from .api import (Batch, Bvh, BvhBuilder, Context, DeviceArray, Graph, Mesh, Ray
                   bounce, bounce_tlas, counter_add, default_context, deform_wave, raygen, trace_adjoint, trace_forward,
                   transform_gradient)
 
-from .render import (DepthAccumulator, DepthMappingShader, IntersectionAccumulator, IntersectionShader, PathTracer,
-                     RadianceToRgbShader, Renderer, RendererState, UvMappingAccumulator)
+from .render import (DepthAccumulator, DepthMappingShader, IntersectionAccumulator, IntersectionShader,
+                     NormalMappingAccumulator, PathTracer, RadianceToRgbShader, Renderer, RendererState,
+                     TextureMaterialAccumulator, UvMappingAccumulator)
 from .scene import (ModelBuilder, ModelInstance, Scene, SceneBuilder, SceneObjectBuilder, Triangle,
                     intersect_pairs)
 
@@ -21,7 +22,8 @@ __all__ = [
     "Batch", "Bvh", "BvhBuilder", "Context", "DeviceArray", "Graph", "Mesh", "Ray", "SceneObject", "Tlas",
     "TlasBuilder", "bounce", "bounce_tlas", "counter_add", "default_context", "deform_wave", "raygen",
     "trace_adjoint", "trace_forward", "transform_gradient",
-    "DepthAccumulator", "DepthMappingShader", "IntersectionAccumulator", "IntersectionShader", "PathTracer",
-    "RadianceToRgbShader", "Renderer", "RendererState", "UvMappingAccumulator",
+    "DepthAccumulator", "DepthMappingShader", "IntersectionAccumulator", "IntersectionShader",
+    "NormalMappingAccumulator", "PathTracer", "RadianceToRgbShader", "Renderer", "RendererState",
+    "TextureMaterialAccumulator", "UvMappingAccumulator",
     "ModelBuilder", "ModelInstance", "Scene", "SceneBuilder", "SceneObjectBuilder", "Triangle", "intersect_pairs",
 ]

@@ -16,7 +16,7 @@ MISS = 0xFFFFFFFF
 ORDER_ROWS, ORDER_TILES = 0, 1
 BUILD_REFERENCE, BUILD_SAH = 0, 1
 SGD_ALONG_NORMAL = 1
-ACC_DEPTH, ACC_INTERSECTION, ACC_UV = 0, 1, 2
+ACC_DEPTH, ACC_INTERSECTION, ACC_UV, ACC_NORMAL, ACC_TEXTURE = 0, 1, 2, 3, 4
 SHADE_RADIANCE_TO_RGB, SHADE_INTERSECTION, SHADE_DEPTH_MAPPING = 0, 1, 2
 ACCUM_PARAMS_DTYPE = np.dtype([("hit_value", "<f4", 3), ("miss_value", "<f4", 3)])
 SHADER_PARAMS_DTYPE = np.dtype([("scale", "<f4"), ("offset", "<f4"), ("hit_rgba", "u1", 4), ("miss_rgba", "u1", 4)])
@@ -72,6 +72,9 @@ SIGNATURES = {
     "bvht_depth": (I32, [P, P, U32, P]),
     "bvht_accumulate": (I32, [P, P, U32, U32, I32, I32, P, P]),
     "bvht_shade": (I32, [P, P, U32, I32, P, P]),
+    "bvht_blas_set_attributes": (I32, [P, P, P, P, P, U32, U32]),
+    "bvht_accumulate_blas": (I32, [P, P, P, U32, U32, I32, I32, P]),
+    "bvht_accumulate_tlas": (I32, [P, P, P, U32, U32, I32, I32, P]),
     "bvht_tlas_create": (I32, [P, P, U32, P, U32, PP]),
     "bvht_tlas_set_objects": (I32, [P, P, P, U32, U32]),
     "bvht_tlas_refresh_bounds": (I32, [P, P]),

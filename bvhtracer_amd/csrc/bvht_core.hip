@@ -840,6 +840,10 @@ int32_t bvht_blas_destroy(bvht_blas b)
     (void)hipFree(b->d_level);
     (void)hipFree(b->d_level_order);
     (void)hipFree(b->d_big_leaves);
+    (void)hipFree(b->d_attr_normals);
+    (void)hipFree(b->d_attr_tex);
+    (void)hipFree(b->d_texture);
+    (void)hipFree(b->d_attr_ref);
     (void)hipFree(b->d_spill);
     delete b;
     return BVHT_OK;

@@ -21,6 +21,16 @@ struct bvht_ctx_s {
     int trace_lds = 0;
 };
 
+// Per-model shading attributes as the attribute accumulators read them
+// (device pointers; NULL where absent), triangles in build order.
+struct AttrRef {
+    const float* normals;     // 9 floats per triangle (three corner normals)
+    const float* tex_coords;  // 6 floats per triangle
+    const uint8_t* texture;   // RGB8, row-major, tex_w x tex_h
+    uint32_t tex_w, tex_h;
+    uint32_t ntris, pad;
+};
+
 struct bvht_blas_s {
     bvht_ctx ctx = nullptr;
     uint32_t n_tris = 0;
@@ -36,6 +46,13 @@ struct bvht_blas_s {
     std::vector<uint32_t> level_off;    // level L occupies [level_off[L], level_off[L+1]) of d_level_order
     uint32_t* d_big_leaves = nullptr;   // leaves with more than kBigLeaf triangles (refit a wave each)
     uint32_t n_big_leaves = 0;
+    // shading attributes (bvht_blas_set_attributes)
+    float* d_attr_normals = nullptr;
+    float* d_attr_tex = nullptr;
+    uint8_t* d_texture = nullptr;
+    uint32_t tex_w = 0, tex_h = 0;
+    uint64_t attr_version = 0;
+    AttrRef* d_attr_ref = nullptr;      // this BLAS's AttrRef on the device
     uint32_t* d_spill = nullptr;    // traversal stack spill (grown on demand)
     size_t spill_words = 0;
 };
@@ -49,12 +66,14 @@ struct bvht_tlas_s {
     void* d_rows = nullptr;         // per-object 3x4 rows of M^-1 + BLAS id, as traced
     float* d_bounds = nullptr;      // n * 6 world bounds
     bvht_tnode* d_nodes = nullptr;  // 2n
-    int32_t* d_list = nullptr;      // rebuild work list (n)
+    int32_t* d_list = nullptr;      // rebuild work list (n), reused as the depth-walk stack: 2 max(n, 4) entries
     uint32_t* d_meta = nullptr;     // {nodes_used, depth, status}
     uint32_t depth = 0;             // TLAS levels after the last rebuild
     uint32_t max_blas_depth = 0;
     uint32_t* d_spill = nullptr;
     size_t spill_words = 0;
+    AttrRef* d_attr_table = nullptr;        // per-BLAS attributes for the attribute accumulators
+    std::vector<uint64_t> attr_versions;    // BLAS attribute versions the table was built from
 };
 
 namespace bvht {

@@ -127,6 +127,7 @@ __global__ __launch_bounds__(256) void k_bounce_write_tlas(const ObjRows* __rest
 void tlas_free(bvht_tlas t)
 {
     (void)hipFree(t->d_blas_table);
+    (void)hipFree(t->d_attr_table);
     (void)hipFree(t->d_objects);
     (void)hipFree(t->d_rows);
     (void)hipFree(t->d_bounds);

@@ -2,8 +2,9 @@
 the whole frame on the GPU: camera rays in 8x8-tile order (bvht_raygen),
 trace, accumulator into the device accumulation buffer at each ray's pixel
 (bvht_accumulate), pixel shader into the device RGBA8 frame buffer
-(bvht_shade).  Names and argument meaning follow renderer.rs; the
-accumulators and shaders are the ones that depend only on the hit record.
+(bvht_shade).  Names and argument meaning follow renderer.rs.  The
+normal-mapping and texture accumulators read the models' attributes (set by
+ModelBuilder from a decoded Mesh and a texture).
 """
 import numpy as np
 
@@ -36,6 +37,22 @@ class IntersectionAccumulator:
 class UvMappingAccumulator:
     """renderer.rs:225-244."""
     kind = L.ACC_UV
+
+    def params(self):
+        return None
+
+
+class NormalMappingAccumulator:
+    """renderer.rs:248-290 (needs the models' normals)."""
+    kind = L.ACC_NORMAL
+
+    def params(self):
+        return None
+
+
+class TextureMaterialAccumulator:
+    """renderer.rs:293-334 (needs the models' texture coordinates and texture)."""
+    kind = L.ACC_TEXTURE
 
     def params(self):
         return None
@@ -115,9 +132,20 @@ class PathTracer:
         n = W * H
         raygen(ctx, cam, W, H, ORDER_TILES, d_rays=state.d_rays)
         scene.intersect_device(state.d_rays, n, state.d_hits)
-        ap = state.accumulator.params()
-        L.check(L.lib().bvht_accumulate(ctx.handle, state.d_hits.ptr, W, H, ORDER_TILES, state.accumulator.kind,
-                                        L.ptr(ap) if ap is not None else None, state.d_accum.ptr))
+        kind = state.accumulator.kind
+        if kind in (L.ACC_NORMAL, L.ACC_TEXTURE):
+            tlas = getattr(scene, "tlas", scene if hasattr(scene, "rebuild_async") else None)
+            if tlas is not None:
+                L.check(L.lib().bvht_accumulate_tlas(ctx.handle, tlas.handle, state.d_hits.ptr, W, H, ORDER_TILES,
+                                                     kind, state.d_accum.ptr))
+            else:
+                bvh = getattr(scene, "bvh", scene)
+                L.check(L.lib().bvht_accumulate_blas(ctx.handle, bvh.handle, state.d_hits.ptr, W, H, ORDER_TILES,
+                                                     kind, state.d_accum.ptr))
+        else:
+            ap = state.accumulator.params()
+            L.check(L.lib().bvht_accumulate(ctx.handle, state.d_hits.ptr, W, H, ORDER_TILES, kind,
+                                            L.ptr(ap) if ap is not None else None, state.d_accum.ptr))
         sp = state.pixel_shader.params()
         L.check(L.lib().bvht_shade(ctx.handle, state.d_accum.ptr, n, state.pixel_shader.kind,
                                    L.ptr(sp) if sp is not None else None, state.d_frame.ptr))

@@ -30,9 +30,28 @@ class ModelInstance:
         triangles), when the model was built from one."""
         return self._attributes
 
+    def set_attributes(self, normals=None, tex_coords=None, texture=None):
+        """Upload shading attributes (build order): normals (n, 3, 3),
+        tex_coords (n, 3, 2), texture (H, W, 3) uint8."""
+        nrm = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1)
+        tc = None if tex_coords is None else np.ascontiguousarray(tex_coords, np.float32).reshape(-1)
+        tx = None if texture is None else np.ascontiguousarray(texture, np.uint8)
+        L.check(L.lib().bvht_blas_set_attributes(self.bvh.ctx.handle, self.bvh.handle,
+                                                 L.ptr(nrm) if nrm is not None else None,
+                                                 L.ptr(tc) if tc is not None else None,
+                                                 L.ptr(tx) if tx is not None else None,
+                                                 0 if tx is None else tx.shape[1], 0 if tx is None else tx.shape[0]))
+        self._texture = tx
+
+    def texture(self):
+        return getattr(self, "_texture", None)
+
     def intersect(self, rays):
         """Model::intersect (model.rs:69-71) for a batch of model-space rays."""
         return self.bvh.intersect(rays)
+
+    def intersect_device(self, d_rays, n, d_hits, d_stats=None):
+        self.bvh.intersect_device(d_rays, n, d_hits, d_stats)
 
     def refit(self):
         """Model::refit (model.rs:73-75): Bvh::refit against the current
@@ -88,12 +107,25 @@ class ModelBuilder:
         self.mode, self.max_leaf = mode, max_leaf
         return self
 
+    def with_texture(self, texture):
+        """ModelBuilder::with_texture (model.rs:135-139): an (H, W, 3) uint8
+        RGB image (TextureMaterial)."""
+        self.texture = np.ascontiguousarray(texture, np.uint8)
+        if self.texture.ndim != 3 or self.texture.shape[2] != 3:
+            raise ValueError("texture must be (H, W, 3) uint8 RGB")
+        return self
+
     def build(self):
         bvh = BvhBuilder(self.mode, self.max_leaf, ctx=self.ctx).build_for(self.mesh)
         attrs = getattr(self, "attributes", None)
         if attrs is not None:
             attrs.permute(bvh.permutation())
-        return ModelInstance(self.mesh, bvh, attrs)
+        model = ModelInstance(self.mesh, bvh, attrs)
+        tex = getattr(self, "texture", None)
+        if attrs is not None or tex is not None:
+            model.set_attributes(attrs.normals() if attrs is not None else None,
+                                 attrs.tex_coords() if attrs is not None else None, tex)
+        return model
 
 
 class SceneObjectBuilder:

@@ -243,12 +243,39 @@ int32_t bvht_depth(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, float* d_de
 #define BVHT_ACC_DEPTH 0        /* DepthAccumulator: t (f32::MAX on a miss) in all three channels */
 #define BVHT_ACC_INTERSECTION 1 /* IntersectionAccumulator: hit_value or miss_value */
 #define BVHT_ACC_UV 2           /* UvMappingAccumulator: (u, v, 1 - (u + v)), 0 on a miss */
+#define BVHT_ACC_NORMAL 3       /* NormalMappingAccumulator (bvht_accumulate_blas / _tlas) */
+#define BVHT_ACC_TEXTURE 4      /* TextureMaterialAccumulator (bvht_accumulate_blas / _tlas) */
 typedef struct {
     float hit_value[3];
     float miss_value[3];
 } bvht_accum_params;
 int32_t bvht_accumulate(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t width, uint32_t height,
                         int32_t order, int32_t kind, const bvht_accum_params* params, float* d_accum);
+
+/* Shading attributes of a model (Mesh::normals / tex_coords, mesh/mesh.rs;
+ * Model::texture, model/model.rs:108-110), host arrays copied to the device,
+ * triangles in BUILD order (permute them with bvht_blas_permutation; the
+ * reference leaves them in input order, misaligned with the triangles):
+ * normals float[9 n] (three corner normals per triangle), tex_coords
+ * float[6 n] (three corner (u, v)), texture RGB8 tex_width x tex_height,
+ * row-major.  NULL removes that attribute. */
+int32_t bvht_blas_set_attributes(bvht_ctx ctx, bvht_blas blas, const float* normals,
+                                 const float* tex_coords, const uint8_t* texture, uint32_t tex_width,
+                                 uint32_t tex_height);
+/* NormalMappingAccumulator (renderer.rs:248-290) and
+ * TextureMaterialAccumulator (renderer.rs:293-334, with
+ * TextureMaterial::evaluate, materials/material.rs:44-52) for the hits of a
+ * frame, stored at each ray's pixel like bvht_accumulate.  kind is
+ * BVHT_ACC_NORMAL or BVHT_ACC_TEXTURE.  The hit's true primitive and object
+ * indices select the attributes (the reference's packed word has object bits
+ * 0, so it reads object 0's model for every hit).  bvht_accumulate_blas: hits
+ * of bvht_trace_blas, object transform the identity; bvht_accumulate_tlas:
+ * hits of bvht_trace_tlas, normals carried to world space by each object's
+ * M.  EINVAL if a model lacks the attribute. */
+int32_t bvht_accumulate_blas(bvht_ctx ctx, bvht_blas blas, const bvht_hit* d_hits, uint32_t width,
+                             uint32_t height, int32_t order, int32_t kind, float* d_accum);
+int32_t bvht_accumulate_tlas(bvht_ctx ctx, bvht_tlas tlas, const bvht_hit* d_hits, uint32_t width,
+                             uint32_t height, int32_t order, int32_t kind, float* d_accum);
 
 /* Pixel shaders (renderer.rs:113-222) over an accumulation buffer of
  * n_pixels into the RGBA8 frame buffer d_rgba (uint8[4*n_pixels], row-major

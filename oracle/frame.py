@@ -84,3 +84,70 @@ def shade(accum, kind, scale=0.0, offset=0.0, hit_rgba=(0, 0, 0, 0), miss_rgba=(
     else:
         raise ValueError(kind)
     return out
+
+
+def _usize(x):
+    """Rust `f32 as usize` (64-bit), then kept as Python ints."""
+    x = np.asarray(x, np.float64)
+    out = []
+    for v in x.ravel():
+        if not v > 0:
+            out.append(0)
+        elif v >= 2.0**64:
+            out.append(2**64 - 1)
+        else:
+            out.append(int(v))
+    return out
+
+
+def accumulate_attributes(hits, width, height, kind, models, objects=None, order="tiles"):
+    """NormalMappingAccumulator (renderer.rs:248-290) and
+    TextureMaterialAccumulator (renderer.rs:293-334, TextureMaterial::evaluate
+    materials/material.rs:44-52) per ray, stored at its pixel address.
+    models: list of dicts with 'normals' (n, 3, 3), 'tex_coords' (n, 3, 2),
+    'texture' (H, W, 3) uint8, triangles in build order.  objects: list of
+    (M as 16 column-major floats, model index), or None for a single-BLAS
+    trace (identity transform, model 0).  Indices are the hits' true prim and
+    inst.  binary32 in the reference's operation order."""
+    hits = np.asarray(hits)
+    n = width * height
+    val = np.zeros((n, 3), np.float32)
+    one, half = np.float32(1.0), np.float32(0.5)
+    ident = np.eye(4, dtype=np.float32).reshape(16)
+    for j in range(n):
+        h = hits[j]
+        if h["prim"] == MISS:
+            continue
+        if objects is None:
+            m16, mi = ident, 0
+        else:
+            if h["inst"] >= len(objects):
+                continue
+            m16, mi = objects[h["inst"]]
+        mdl = models[mi]
+        u, v = np.float32(h["u"]), np.float32(h["v"])
+        w = (one - u) - v
+        p = int(h["prim"])
+        if kind == "normal":
+            c = np.asarray(mdl["normals"], np.float32)[p]
+            m = ((c[0] * w) + (c[1] * u)) + (c[2] * v)
+            M = np.asarray(m16, np.float32)
+            ws = np.array([((M[0 + r] * m[0] + M[4 + r] * m[1]) + M[8 + r] * m[2]) + M[12 + r] * np.float32(0)
+                           for r in range(3)], np.float32)
+            ln = np.sqrt((ws[0] * ws[0] + ws[1] * ws[1]) + ws[2] * ws[2])
+            val[j] = (ws / ln + one) * half
+        elif kind == "texture":
+            c = np.asarray(mdl["tex_coords"], np.float32)[p]
+            tex = np.asarray(mdl["texture"], np.uint8)
+            th, tw = tex.shape[0], tex.shape[1]
+            uv = ((c[0] * w) + (c[1] * u)) + (c[2] * v)
+            iu = _usize(uv[0] * np.float32(tw))[0] % tw
+            iv = _usize(uv[1] * np.float32(th))[0] % th
+            s = np.float32(1.0) / np.float32(256.0)
+            val[j] = tex[iv, iu].astype(np.float32) * s
+        else:
+            raise ValueError(kind)
+    out = np.zeros((n, 3), np.float32)
+    addr = tile_pixel_order(width, height) if order == "tiles" else np.arange(n)
+    out[addr] = val
+    return out

@@ -118,3 +118,76 @@ def test_scene_mirror_equals_oracle(ctx, orc):
     got2 = scene.intersect(rays.view(B.RAY_DTYPE))
     assert_hits_equal(got2, oracle_hits(), "scene after refit + rebuild")
     assert got2.tobytes() != got.tobytes()
+
+
+def test_cube_scene_normal_mapping_known_answer(ctx):
+    """test_scene_cube.rs:139-175,230-265: rays from (0, 4, 0) toward
+    (0.5, 1, -0.5) and (-0.5, 1, 0.5) hit the top face of cube.obj scaled by 2
+    and moved by (-1, -1, -1); the world normal of the hit triangle's corners
+    is +y.  With the attributes permuted like the triangles, the
+    normal-mapping accumulator gives (0.5, 1, 0.5) for both (the reference
+    reads the unpermuted normal array there)."""
+    import os
+
+    from bvhtracer_amd import mesh as M
+
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    model = ModelBuilder(ctx).with_mesh(M.open_mesh(os.path.join(golden, "cube.obj"))).build()
+    T = np.diag([2.0, 2.0, 2.0, 1.0]).astype(np.float32)
+    T[3, :3] = -1.0
+    scene = SceneBuilder(None, ctx=ctx).with_object(SceneObjectBuilder(model).with_transform(T.reshape(16)).build()) \
+        .build()
+    o = np.array([0.0, 4.0, 0.0], np.float32)
+    rays = np.zeros(2, B.RAY_DTYPE)
+    for k, target in enumerate(([0.5, 1.0, -0.5], [-0.5, 1.0, 0.5])):
+        d = np.asarray(target, np.float32) - o
+        rays[k]["o"], rays[k]["d"], rays[k]["tmax"] = o, d / np.sqrt(np.float32((d * d).sum())), F32_MAX
+    hits = scene.intersect(rays)
+    assert hits["prim"].tolist() == [6, 9]  # test_scene_cube.rs:124-135,215-226
+    d_hits = ctx.to_device(hits)
+    d_acc = ctx.alloc(2 * 12)
+    B._lib.check(B._lib.lib().bvht_accumulate_tlas(ctx.handle, scene.tlas.handle, d_hits.ptr, 2, 1, B.ORDER_ROWS,
+                                                   B._lib.ACC_NORMAL, d_acc.ptr))
+    assert d_acc.download(np.float32).reshape(2, 3).tolist() == [[0.5, 1.0, 0.5], [0.5, 1.0, 0.5]]
+
+
+def sphere_attributes(tris):
+    """Corner normals (radial) and spherical texture coordinates of a
+    sphere-like mesh, binary32."""
+    v = tris.reshape(-1, 3, 3).astype(np.float64)
+    n = v / np.linalg.norm(v, axis=2, keepdims=True)
+    uv = np.stack([0.5 + np.arctan2(n[..., 2], n[..., 0]) / (2 * np.pi), 0.5 - np.arcsin(n[..., 1]) / np.pi], -1)
+    return n.astype(np.float32), (uv * 3.0).astype(np.float32)  # texture repeats 3x
+
+
+@pytest.mark.parametrize("acc", ["normal", "texture"])
+def test_attribute_accumulators_frame_equal_oracle(ctx, acc):
+    """Two models (different spheres, own textures), five objects with
+    rotations and non-uniform scales: the frame's accumulation buffer equals
+    the oracle's, bit for bit, through the PathTracer and the TLAS."""
+    from bvhtracer_amd.mesh import Mesh
+
+    rng = np.random.default_rng(11)
+    models, attrs = [], []
+    for k, (freq, seed) in enumerate(((8, 1), (5, 2))):
+        tris = scenes.icosphere(freq, radius=2.0, seed=seed)
+        nrm, uv = sphere_attributes(tris)
+        tex = rng.integers(0, 256, (16 + 8 * k, 32, 3), dtype=np.uint8)
+        m = ModelBuilder(ctx).with_mesh(Mesh(tris, uv, nrm)).with_texture(tex).with_build(B.BUILD_SAH, 2).build()
+        models.append(m)
+        attrs.append({"normals": m.mesh().normals(), "tex_coords": m.mesh().tex_coords(), "texture": tex})
+    M, _ = scenes.random_affines(5, 0xA77, scale_range=(0.6, 1.4), trans_range=5.0)
+    M = M.reshape(5, 4, 4).copy()
+    M[:, 0, :3] *= np.float32(1.3)  # non-uniform scale: normals still go through M, as in the reference
+    objs = [SceneObjectBuilder(models[k % 2]).with_transform(M[k].reshape(16)).build() for k in range(5)]
+    cam = scenes.camera(eye=(0, 0, -14), forward=(0, 0, 1), right=(1, 0, 0), up=(0, 1, 0), fovy_deg=50)
+    scene = SceneBuilder(cam, ctx=ctx).with_objects(objs).build()
+    accum = render.NormalMappingAccumulator() if acc == "normal" else render.TextureMaterialAccumulator()
+    state = render.RendererState(accum, render.RadianceToRgbShader(), 64, 64, ctx=ctx)
+    render.PathTracer().evaluate(state, scene)
+    hits = state.d_hits.download(B.HIT_DTYPE, 64 * 64)
+    assert (hits["prim"] != B.MISS).sum() > 500 and len(set(hits["inst"][hits["prim"] != B.MISS])) >= 3
+    ref = oframe.accumulate_attributes(hits, 64, 64, acc, attrs, [(o.m, o.model) for o in objs], order="tiles")
+    got = state.accumulation_buffer().reshape(-1, 3)
+    assert got.tobytes() == ref.tobytes()
+    assert np.array_equal(state.frame_buffer().reshape(-1, 4), oframe.shade(ref, "radiance"))

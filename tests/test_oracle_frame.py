@@ -64,3 +64,37 @@ def test_accumulators_on_hand_made_hits():
     assert uv[1].tolist() == [0.25, 0.5, 0.25] and uv[0].tolist() == [0, 0, 0]
     it = frame.accumulate(hits, 8, 8, "intersection", (1, 1, 1), (0, 0, 0.5))
     assert it[0].tolist() == [0, 0, 0.5] and it[1].tolist() == [1, 1, 1]
+
+
+HIT = [("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim", "<u4"), ("inst", "<u4"), ("packed", "<u4")]
+
+
+def test_normal_mapping_closed_form():
+    """Corner normals all +y under M = 2 I (+ a translation, which vectors
+    ignore): world normal (0, 2, 0) -> (0, 1, 0) -> (0.5, 1, 0.5) exactly;
+    a miss gives 0."""
+    hits = np.zeros(2, HIT)
+    hits["prim"] = [0, 0xFFFFFFFF]
+    hits["u"], hits["v"] = 0.25, 0.5
+    model = {"normals": np.tile(np.array([0, 1, 0], np.float32), (1, 3, 1))}
+    M = np.diag([2.0, 2.0, 2.0, 1.0]).astype(np.float32)
+    M[3, :3] = -1.0  # column 3 = translation in the column-major layout
+    got = frame.accumulate_attributes(hits, 2, 1, "normal", [model], [(M.reshape(16), 0)], order="rows")
+    assert got.tolist() == [[0.5, 1.0, 0.5], [0.0, 0.0, 0.0]]
+
+
+def test_texture_lookup_closed_form():
+    """Nearest texel at ((u W) as usize % W, (v H) as usize % H): corner
+    coordinates equal, so the lookup sees them exactly; negative and NaN
+    coordinates go to texel 0, coordinates past 1 wrap."""
+    tex = np.arange(4 * 2 * 3, dtype=np.uint8).reshape(2, 4, 3)  # H = 2, W = 4
+    uvs = [(0.3, 0.6), (-0.5, 0.2), (1.3, 2.75), (np.nan, 0.9), (0.999, 0.0)]
+    want_px = [(1, 1), (0, 0), (1, 1), (0, 1), (3, 0)]  # (iu, iv)
+    hits = np.zeros(len(uvs), HIT)
+    hits["prim"] = np.arange(len(uvs))
+    hits["u"], hits["v"] = 0.2, 0.3
+    tcs = np.array([[uv, uv, uv] for uv in uvs], np.float32)
+    got = frame.accumulate_attributes(hits, len(uvs), 1, "texture", [{"tex_coords": tcs, "texture": tex}],
+                                      order="rows")
+    for k, (iu, iv) in enumerate(want_px):
+        assert got[k].tolist() == (tex[iv, iu].astype(np.float32) / 256).tolist(), (uvs[k], got[k])
