@@ -4,8 +4,9 @@ Workload (one "step" = one frame): a 1,310,720-triangle displaced icosphere
 (frequency 256, radius 32, 4-octave value-noise displacement of 0.15 R),
 BVH built on the GPU (binned SAH, 8 bins, leaf size <= 2), 1920x1080 pinhole
 primary rays in 8x8-tile order, nearest-hit trace, one cosine-weighted
-diffuse bounce per hit (PCG32-seeded, origin offset 1e-4 along the normal),
-nearest-hit trace of the bounce rays.  Inputs are resident in HBM before the
+diffuse bounce per hit (PCG32-seeded, origin offset 1e-4 along the normal;
+generated grouped by direction bin with each ray's source pixel, see
+bvht_bounce_sorted), nearest-hit trace of the bounce rays.  Inputs are resident in HBM before the
 timed region; every step regenerates its rays on the GPU.
 
 value = (primary + bounce rays) / frame time.  Primary and bounce Mrays/s are
@@ -131,6 +132,7 @@ def main():
     d_hits = ctx.alloc(n_primary * B.HIT_DTYPE.itemsize)
     d_brays = ctx.alloc(n_primary * B.RAY_DTYPE.itemsize)
     d_bhits = ctx.alloc(n_primary * B.HIT_DTYPE.itemsize)
+    d_bsrc = ctx.alloc(n_primary * 4)
     d_count = ctx.alloc(4)
     import ctypes
 
@@ -148,8 +150,8 @@ def main():
         L.check(lib.bvht_trace_blas(ctx.handle, bvh.handle, d_rays.ptr, n_primary, d_hits.ptr, None))
         if evs:
             ctx.record(evs[2])
-        L.check(lib.bvht_bounce(ctx.handle, bvh.handle, d_rays.ptr, d_hits.ptr, n_primary, seed, d_brays.ptr,
-                                d_count.ptr))
+        L.check(lib.bvht_bounce_sorted(ctx.handle, bvh.handle, d_rays.ptr, d_hits.ptr, n_primary, seed, d_brays.ptr,
+                                       d_bsrc.ptr, d_count.ptr))
         if evs:
             ctx.record(evs[3])
         L.check(lib.bvht_trace_blas_count(ctx.handle, bvh.handle, d_brays.ptr, d_count.ptr, n_primary,
@@ -237,7 +239,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"displaced icosphere {mesh.shape[0]} tris (freq {args.freq}, R 32), {W}x{H} primary + "
-                        f"1 cosine diffuse bounce per hit, nearest hit, SAH BVH (8 bins, leaf<=2) built on GPU",
+                        f"1 cosine diffuse bounce per hit (grouped by direction), nearest hit, SAH BVH (8 bins, "
+                        f"leaf<=2) built on GPU",
             "primary_rays": n_primary,
             "bounce_rays": n_bounce,
         },

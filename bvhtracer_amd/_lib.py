@@ -69,6 +69,7 @@ SIGNATURES = {
     "bvht_get_trace_config": (I32, [P, ctypes.POINTER(I32), ctypes.POINTER(I32)]),
     "bvht_raygen": (I32, [P, P, U32, U32, I32, P]),
     "bvht_bounce": (I32, [P, P, P, P, U32, U64, P, P]),
+    "bvht_bounce_sorted": (I32, [P, P, P, P, U32, U64, P, P, P]),
     "bvht_depth": (I32, [P, P, U32, P]),
     "bvht_accumulate": (I32, [P, P, U32, U32, I32, I32, P, P]),
     "bvht_shade": (I32, [P, P, U32, I32, P, P]),
@@ -85,6 +86,7 @@ SIGNATURES = {
     "bvht_trace_tlas": (I32, [P, P, P, U32, P, P]),
     "bvht_trace_tlas_count": (I32, [P, P, P, P, U32, P, P]),
     "bvht_bounce_tlas": (I32, [P, P, P, P, U32, U64, P, P]),
+    "bvht_bounce_tlas_sorted": (I32, [P, P, P, P, U32, U64, P, P, P]),
     "bvht_batch_create": (I32, [P, P, U32, PP]),
     "bvht_batch_destroy": (I32, [P]),
     "bvht_batch_refit": (I32, [P, P]),

@@ -500,6 +500,26 @@ def bounce(ctx, bvh, d_rays, d_hits, n, seed, d_out=None, d_count=None):
     return d_out, d_count
 
 
+def bounce_sorted(ctx, bvh, d_rays, d_hits, n, seed, d_out=None, d_src=None, d_count=None):
+    """bounce() with the rays grouped by direction bin (ray order inside a
+    bin); d_src[k] = index of the ray that spawned bounce ray k."""
+    d_out = d_out or ctx.alloc(max(n, 1) * RAY_DTYPE.itemsize)
+    d_src = d_src or ctx.alloc(max(n, 1) * 4)
+    d_count = d_count or ctx.alloc(4)
+    check(L.lib().bvht_bounce_sorted(ctx.handle, bvh.handle, d_rays.ptr, d_hits.ptr, n, seed, d_out.ptr, d_src.ptr,
+                                     d_count.ptr))
+    return d_out, d_src, d_count
+
+
+def bounce_tlas_sorted(ctx, tlas, d_rays, d_hits, n, seed, d_out=None, d_src=None, d_count=None):
+    d_out = d_out or ctx.alloc(max(n, 1) * RAY_DTYPE.itemsize)
+    d_src = d_src or ctx.alloc(max(n, 1) * 4)
+    d_count = d_count or ctx.alloc(4)
+    check(L.lib().bvht_bounce_tlas_sorted(ctx.handle, tlas.handle, d_rays.ptr, d_hits.ptr, n, seed, d_out.ptr,
+                                          d_src.ptr, d_count.ptr))
+    return d_out, d_src, d_count
+
+
 def bounce_tlas(ctx, tlas, d_rays, d_hits, n, seed, d_out=None, d_count=None):
     d_out = d_out or ctx.alloc(max(n, 1) * RAY_DTYPE.itemsize)
     d_count = d_count or ctx.alloc(4)

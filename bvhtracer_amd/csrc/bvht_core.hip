@@ -368,6 +368,157 @@ __global__ __launch_bounds__(256) void k_bounce_write(const float* __restrict__ 
     o[1] = b;
 }
 
+// ---- bounce rays grouped by direction ------------------------------------
+// 24 direction bins: dominant axis (first of equal magnitudes) and its sign,
+// then the signs of the other two components (in axis order after it).  A
+// wave whose rays share a bin takes the same near/far child decisions more
+// often; the traced rays and their hits are the same as in ray order.
+constexpr uint32_t kDirBins = 24;
+
+__device__ __forceinline__ uint32_t dir_bin(float dx, float dy, float dz)
+{
+    float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    uint32_t axis = (ax >= ay && ax >= az) ? 0u : (ay >= az ? 1u : 2u);
+    float c0 = axis == 0 ? dx : (axis == 1 ? dy : dz);
+    float c1 = axis == 0 ? dy : (axis == 1 ? dz : dx);
+    float c2 = axis == 0 ? dz : (axis == 1 ? dx : dy);
+    return (axis * 2u + (c0 < 0.0f ? 1u : 0u)) * 4u + (c1 >= 0.0f ? 2u : 0u) + (c2 >= 0.0f ? 1u : 0u);
+}
+
+// Bounce ray of hit i for a BLAS trace (rows == nullptr) or a TLAS trace
+// (normal carried to world space by the transpose of the object's M^-1);
+// false where the hit is a miss or out of range.
+struct BounceSrc {
+    const float* tris;
+    uint32_t ntris;
+    const ObjRows* rows;
+    const BlasRef* blases;
+    uint32_t n_obj;
+};
+
+__device__ __forceinline__ bool bounce_for(const BounceSrc& s, const bvht_ray* __restrict__ rays, uint32_t n,
+                                           const bvht_hit* __restrict__ hits, uint32_t i, uint64_t seed, float4& a,
+                                           float4& b)
+{
+    if (i >= n)
+        return false;
+    bvht_hit h = hits[i];
+    if (!bounce_hit(h, s.ntris, s.rows ? s.n_obj : 0xFFFFFFFFu, s.rows, s.blases))
+        return false;
+    DRay r = load_ray(rays, i);
+    if (!s.rows) {
+        make_bounce(s.tris + 9 * (size_t)h.prim, r.o, r.d, h.t, seed, i, a, b);
+        return true;
+    }
+    ObjRows ob = s.rows[h.inst];
+    const float* tri = s.blases[ob.blas].tris + 9 * (size_t)h.prim;
+    V3 v0 = mk3(tri[0], tri[1], tri[2]);
+    V3 nm = cross3(sub3(mk3(tri[3], tri[4], tri[5]), v0), sub3(mk3(tri[6], tri[7], tri[8]), v0));
+    V3 nw = mk3(ob.r0.x * nm.x + ob.r1.x * nm.y + ob.r2.x * nm.z, ob.r0.y * nm.x + ob.r1.y * nm.y + ob.r2.y * nm.z,
+                ob.r0.z * nm.x + ob.r1.z * nm.y + ob.r2.z * nm.z);
+    bounce_ray(nw, add3(r.o, scl3(r.d, h.t)), r.d, seed, i, a, b);
+    return true;
+}
+
+// Per (bin, 256-ray block) counts and each wave's rank inside its bin.
+__device__ __forceinline__ void bin_ranks(uint32_t bin, bool valid, uint32_t (*wc)[kDirBins], uint32_t& rank)
+{
+    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    unsigned long long below = (1ull << lane) - 1ull;
+    rank = 0;
+    for (uint32_t k = 0; k < kDirBins; ++k) {
+        unsigned long long m = __ballot(valid && bin == k);
+        if (lane == 0)
+            wc[wave][k] = (uint32_t)__popcll(m);
+        if (valid && bin == k)
+            rank = (uint32_t)__popcll(m & below);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bounce_bin_counts(BounceSrc src, const bvht_ray* __restrict__ rays,
+                                                           const bvht_hit* __restrict__ hits, uint32_t n,
+                                                           uint64_t seed, uint32_t nb, uint32_t* __restrict__ counts)
+{
+    __shared__ uint32_t wc[4][kDirBins];
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    float4 a, b;
+    bool valid = bounce_for(src, rays, n, hits, i, seed, a, b);
+    uint32_t bin = valid ? dir_bin(b.x, b.y, b.z) : 0u, rank;
+    bin_ranks(bin, valid, wc, rank);
+    __syncthreads();
+    if (threadIdx.x < kDirBins)
+        counts[(size_t)threadIdx.x * nb + blockIdx.x] =
+            wc[0][threadIdx.x] + wc[1][threadIdx.x] + wc[2][threadIdx.x] + wc[3][threadIdx.x];
+}
+
+// Exclusive scan of one bin's per-block counts (one workgroup per bin, each
+// thread a contiguous run); totals[bin] = the bin's size.
+__global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* __restrict__ counts, uint32_t nb,
+                                                   uint32_t* __restrict__ totals)
+{
+    __shared__ uint32_t part[1024];
+    uint32_t* c = counts + (size_t)blockIdx.x * nb;
+    uint32_t per = (nb + 1023) / 1024;
+    uint32_t begin = min(threadIdx.x * per, nb), end = min(begin + per, nb);
+    uint32_t s = 0;
+    for (uint32_t k = begin; k < end; ++k)
+        s += c[k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    for (uint32_t k = begin; k < end; ++k) {
+        uint32_t v = c[k];
+        c[k] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023)
+        totals[blockIdx.x] = part[1023];
+}
+
+__global__ __launch_bounds__(256) void k_bounce_bin_write(BounceSrc src, const bvht_ray* __restrict__ rays,
+                                                          const bvht_hit* __restrict__ hits, uint32_t n,
+                                                          uint64_t seed, uint32_t nb,
+                                                          const uint32_t* __restrict__ offsets,
+                                                          const uint32_t* __restrict__ totals,
+                                                          bvht_ray* __restrict__ out, uint32_t* __restrict__ src_index,
+                                                          uint32_t* __restrict__ d_count)
+{
+    __shared__ uint32_t wc[4][kDirBins];
+    __shared__ uint32_t bin_base[kDirBins + 1];
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t k = 0; k < kDirBins; ++k) {
+            bin_base[k] = acc;
+            acc += totals[k];
+        }
+        bin_base[kDirBins] = acc;
+        if (blockIdx.x == 0)
+            *d_count = acc;
+    }
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    float4 a, b;
+    bool valid = bounce_for(src, rays, n, hits, i, seed, a, b);
+    uint32_t bin = valid ? dir_bin(b.x, b.y, b.z) : 0u, rank;
+    bin_ranks(bin, valid, wc, rank);
+    __syncthreads();
+    if (!valid)
+        return;
+    uint32_t pos = bin_base[bin] + offsets[(size_t)bin * nb + blockIdx.x] + rank;
+    for (uint32_t w = 0; w < (threadIdx.x >> 6); ++w)
+        pos += wc[w][bin];
+    float4* o = reinterpret_cast<float4*>(out + pos);
+    o[0] = a;
+    o[1] = b;
+    if (src_index)
+        src_index[pos] = i;
+}
+
 __global__ __launch_bounds__(256) void k_stream_copy(float4* __restrict__ dst, const float4* __restrict__ src,
                                                      size_t n)
 {
@@ -593,6 +744,31 @@ int32_t hit_block_offsets(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, uint
     hipLaunchKernelGGL(k_hit_counts, dim3(nb), dim3(256), 0, ctx->stream, d_hits, n, prim_limit, inst_limit,
                        static_cast<const ObjRows*>(rows), static_cast<const BlasRef*>(blases), bc);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, ctx->stream, bc, nb, d_count);
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
+// Bounce rays grouped by direction bin, ray order inside a bin: counts per
+// (bin, block), one scan (bin-major), write.  rows/blases select the TLAS form.
+int32_t launch_bounce_sorted(bvht_ctx ctx, const float* d_tris, uint32_t ntris, const void* rows, const void* blases,
+                             uint32_t n_obj, const bvht_ray* d_rays, const bvht_hit* d_hits, uint32_t n,
+                             uint64_t seed, bvht_ray* d_out, uint32_t* d_src, uint32_t* d_count)
+{
+    if (n == 0)
+        return fill_u32(ctx, d_count, 0, 1);
+    uint32_t nb = blocks_for(n, 256);
+    size_t entries = (size_t)kDirBins * nb;
+    int32_t rc = ensure_scratch(ctx, sizeof(uint32_t) * (entries + kDirBins));
+    if (rc)
+        return rc;
+    uint32_t* counts = static_cast<uint32_t*>(ctx->scratch);
+    uint32_t* totals = counts + entries;
+    BounceSrc src{d_tris, ntris, static_cast<const ObjRows*>(rows), static_cast<const BlasRef*>(blases), n_obj};
+    hipLaunchKernelGGL(k_bounce_bin_counts, dim3(nb), dim3(256), 0, ctx->stream, src, d_rays, d_hits, n, seed, nb,
+                       counts);
+    hipLaunchKernelGGL(k_bin_scan, dim3(kDirBins), dim3(1024), 0, ctx->stream, counts, nb, totals);
+    hipLaunchKernelGGL(k_bounce_bin_write, dim3(nb), dim3(256), 0, ctx->stream, src, d_rays, d_hits, n, seed, nb,
+                       counts, totals, d_out, d_src, d_count);
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }
@@ -909,6 +1085,15 @@ int32_t bvht_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, const 
     BVHT_REQUIRE(ctx && blas && d_count, "NULL argument");
     BVHT_REQUIRE(n == 0 || (d_rays && d_hits && d_out), "NULL buffer");
     return bvht::launch_bounce(ctx, blas->d_tris, blas->n_tris, d_rays, d_hits, n, seed, d_out, d_count);
+}
+
+int32_t bvht_bounce_sorted(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, const bvht_hit* d_hits, uint32_t n,
+                           uint64_t seed, bvht_ray* d_out, uint32_t* d_src, uint32_t* d_count)
+{
+    BVHT_REQUIRE(ctx && blas && d_count, "NULL argument");
+    BVHT_REQUIRE(n == 0 || (d_rays && d_hits && d_out), "NULL buffer");
+    return bvht::launch_bounce_sorted(ctx, blas->d_tris, blas->n_tris, nullptr, nullptr, 0, d_rays, d_hits, n, seed,
+                                      d_out, d_src, d_count);
 }
 
 int32_t bvht_depth(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, float* d_depth)

@@ -85,6 +85,9 @@ int32_t ensure_spill(bvht_ctx ctx, uint32_t** buf, size_t* words, size_t need);
 int32_t blas_alloc(bvht_ctx ctx, uint32_t n, uint32_t node_count, bvht_blas* out);
 int32_t fill_u32(bvht_ctx ctx, uint32_t* p, uint32_t v, size_t n);
 void trace_config(bvht_ctx ctx, uint32_t need, int* wg, int* lds);
+int32_t launch_bounce_sorted(bvht_ctx ctx, const float* d_tris, uint32_t ntris, const void* rows, const void* blases,
+                             uint32_t n_obj, const bvht_ray* d_rays, const bvht_hit* d_hits, uint32_t n,
+                             uint64_t seed, bvht_ray* d_out, uint32_t* d_src, uint32_t* d_count);
 int32_t hit_block_offsets(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, uint32_t prim_limit, uint32_t inst_limit,
                           const void* rows, const void* blases, uint32_t* d_count, uint32_t** offsets);
 

@@ -355,4 +355,13 @@ int32_t bvht_bounce_tlas(bvht_ctx ctx, bvht_tlas tlas, const bvht_ray* d_rays, c
     return BVHT_OK;
 }
 
+int32_t bvht_bounce_tlas_sorted(bvht_ctx ctx, bvht_tlas tlas, const bvht_ray* d_rays, const bvht_hit* d_hits,
+                                uint32_t n, uint64_t seed, bvht_ray* d_out, uint32_t* d_src, uint32_t* d_count)
+{
+    BVHT_REQUIRE(ctx && tlas && d_count && tlas->ctx == ctx, "NULL argument or TLAS of another context");
+    BVHT_REQUIRE(n == 0 || (d_rays && d_hits && d_out), "NULL buffer");
+    return bvht::launch_bounce_sorted(ctx, nullptr, 0xFFFFFFFFu, tlas->d_rows, tlas->d_blas_table, tlas->n, d_rays,
+                                      d_hits, n, seed, d_out, d_src, d_count);
+}
+
 }  // extern "C"

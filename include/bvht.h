@@ -232,6 +232,15 @@ int32_t bvht_raygen(bvht_ctx ctx, const bvht_camera* cam, uint32_t width, uint32
 int32_t bvht_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, const bvht_hit* d_hits,
                     uint32_t n, uint64_t seed, bvht_ray* d_out, uint32_t* d_count);
 
+/* The same bounce rays grouped by direction (24 bins: dominant axis and its
+ * sign x the signs of the other two components), ray order inside a bin, so
+ * that a wave's rays take the same child order more often.  d_src[k]
+ * (device uint32, may be NULL) receives the index of the ray whose hit
+ * spawned bounce ray k.  The rays and their hits are those of bvht_bounce;
+ * only the order differs. */
+int32_t bvht_bounce_sorted(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, const bvht_hit* d_hits,
+                           uint32_t n, uint64_t seed, bvht_ray* d_out, uint32_t* d_src, uint32_t* d_count);
+
 /* Depth accumulator (renderer.rs:176-194): d_depth[i] = hit t or f32::MAX. */
 int32_t bvht_depth(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, float* d_depth);
 
@@ -330,6 +339,11 @@ int32_t bvht_trace_tlas_count(bvht_ctx ctx, bvht_tlas tlas, const bvht_ray* d_ra
 int32_t bvht_bounce_tlas(bvht_ctx ctx, bvht_tlas tlas, const bvht_ray* d_rays,
                          const bvht_hit* d_hits, uint32_t n, uint64_t seed, bvht_ray* d_out,
                          uint32_t* d_count);
+
+/* bvht_bounce_sorted for TLAS hits. */
+int32_t bvht_bounce_tlas_sorted(bvht_ctx ctx, bvht_tlas tlas, const bvht_ray* d_rays,
+                                const bvht_hit* d_hits, uint32_t n, uint64_t seed, bvht_ray* d_out,
+                                uint32_t* d_src, uint32_t* d_count);
 
 /* ---- batched scenes (no reference counterpart) ------------------------ */
 

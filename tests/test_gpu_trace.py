@@ -165,3 +165,55 @@ def test_every_trace_config_is_bit_identical(ctx, orc):
         autotune.set_config(ctx, 0, 0)
     with pytest.raises(B.BvhtError):
         autotune.set_config(ctx, 96, 16)
+
+
+def dir_bins(d):
+    """24 direction bins: dominant axis (first of equal magnitudes) and its
+    sign, then the signs of the next two components in axis order."""
+    a = np.abs(d)
+    axis = np.where((a[:, 0] >= a[:, 1]) & (a[:, 0] >= a[:, 2]), 0, np.where(a[:, 1] >= a[:, 2], 1, 2))
+    r = np.arange(d.shape[0])
+    c0, c1, c2 = d[r, axis], d[r, (axis + 1) % 3], d[r, (axis + 2) % 3]
+    return (axis * 2 + (c0 < 0)) * 4 + 2 * (c1 >= 0) + (c2 >= 0)
+
+
+@pytest.mark.parametrize("tlas", [False, True])
+def test_sorted_bounce_is_a_grouping_of_the_ray_order_bounce(ctx, orc, tlas):
+    """bvht_bounce_sorted emits exactly bvht_bounce's rays, grouped by
+    direction bin with ray order kept inside each bin; d_src names each ray's
+    source; the sorted rays trace to the oracle's hits."""
+    mesh = scenes.icosphere(24, radius=32.0)
+    nodes, used, perm = orc.bvh_build(mesh, False, 2)
+    bvh = upload(ctx, nodes, used, perm)
+    W, H = 320, 184
+    cam = scenes.sphere_camera(32.0, W, H)
+    d_rays = B.raygen(ctx, cam, W, H, B.ORDER_TILES)
+    d_hits = ctx.alloc(W * H * B.HIT_DTYPE.itemsize)
+    if tlas:
+        M, Minv = scenes.random_affines(1, 3, scale_range=(1, 1), trans_range=0.5)
+        t = B.TlasBuilder(ctx=ctx).build_for([B.SceneObject(0, M[0].reshape(4, 4), Minv[0])], [bvh])
+        t.intersect_device(d_rays, W * H, d_hits)
+        d_b, d_cnt = B.bounce_tlas(ctx, t, d_rays, d_hits, W * H, seed=99)
+        d_s, d_src, d_cnt2 = B.bounce_tlas_sorted(ctx, t, d_rays, d_hits, W * H, seed=99)
+    else:
+        bvh.intersect_device(d_rays, W * H, d_hits)
+        d_b, d_cnt = B.bounce(ctx, bvh, d_rays, d_hits, W * H, seed=99)
+        d_s, d_src, d_cnt2 = B.bounce_sorted(ctx, bvh, d_rays, d_hits, W * H, seed=99)
+    cnt = int(d_cnt.download(np.uint32, 1)[0])
+    assert cnt == int(d_cnt2.download(np.uint32, 1)[0]) > 1000
+    plain = d_b.download(B.RAY_DTYPE, cnt)
+    srt = d_s.download(B.RAY_DTYPE, cnt)
+    src = d_src.download(np.uint32, cnt)
+    hits = d_hits.download(B.HIT_DTYPE, W * H)
+    hit_idx = np.nonzero(hits["prim"] != B.MISS)[0]
+    assert np.array_equal(np.sort(src), hit_idx)
+    pos = np.searchsorted(hit_idx, src)          # where each sorted ray sits in ray order
+    assert srt.tobytes() == plain[pos].tobytes()
+    bins = dir_bins(srt["d"])
+    assert (np.diff(bins) >= 0).all()
+    for b in np.unique(bins):
+        assert (np.diff(src[bins == b]) > 0).all()
+    assert len(np.unique(bins)) >= 8
+    if not tlas:
+        assert_hits_equal(bvh.intersect(srt), orc.bvh_trace(nodes, perm, srt.view(orc.RAY_DTYPE), nthreads=8),
+                          "sorted bounce")
