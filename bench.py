@@ -7,7 +7,10 @@ primary rays in 8x8-tile order, nearest-hit trace, one cosine-weighted
 diffuse bounce per hit (PCG32-seeded, origin offset 1e-4 along the normal;
 generated grouped by direction bin with each ray's source pixel, see
 bvht_bounce_sorted), nearest-hit trace of the bounce rays.  Inputs are resident in HBM before the
-timed region; every step regenerates its rays on the GPU.
+timed region; every step regenerates its rays on the GPU.  The timed steps
+replay the frame captured once as a hipGraph (the frame's calls only enqueue
+kernels); a second set of eager frames with HIP events between the launches
+gives the per-kernel breakdown.
 
 value = (primary + bounce rays) / frame time.  Primary and bounce Mrays/s are
 reported separately from their own kernel times.  The roofline object prices
@@ -183,6 +186,7 @@ def main():
                  "sweep_ms": {f"{k[0]}x{k[1]}": round(v, 4) for k, v in table.items()}}
         log(f"[bench] autotuned trace config: workgroup {best[0]}, LDS depth {best[1]}")
 
+    # per-kernel breakdown: K eager frames with HIP events between the launches
     evs = [[ctx.event() for _ in range(5)] for _ in range(args.steps)]
     start, stop = ctx.event(), ctx.event()
     ctx.sync()
@@ -191,15 +195,26 @@ def main():
         frame(args.seed, evs[s])
     ctx.record(stop)
     ctx.sync()
-    total_ms = ctx.elapsed_ms(start, stop)
+    eager_ms = ctx.elapsed_ms(start, stop) / args.steps
     per = np.array([[ctx.elapsed_ms(e[k], e[k + 1]) for k in range(4)] for e in evs])
     raygen_ms, prim_ms, bounce_gen_ms, bounce_ms = per.mean(axis=0)
-    ms_per_step = total_ms / args.steps
+    # timed region: the frame captured once as a hipGraph, K replays
+    graph = B.Graph.capture(ctx, lambda: frame(args.seed))
+    graph.launch()
+    ctx.sync()
+    start, stop = ctx.event(), ctx.event()
+    ctx.record(start)
+    for s in range(args.steps):
+        graph.launch()
+    ctx.record(stop)
+    ctx.sync()
+    ms_per_step = ctx.elapsed_ms(start, stop) / args.steps
+    graph.close()
     n_bounce = int(d_count.download(np.uint32, 1)[0])
     rays_per_step = n_primary + n_bounce
     value = rays_per_step / (ms_per_step * 1e-3) / 1e6
-    log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/frame (raygen {raygen_ms:.3f}, primary {prim_ms:.3f}, "
-        f"bounce-gen {bounce_gen_ms:.3f}, bounce {bounce_ms:.3f})")
+    log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/frame as a graph, {eager_ms:.3f} eager (raygen "
+        f"{raygen_ms:.3f}, primary {prim_ms:.3f}, bounce-gen {bounce_gen_ms:.3f}, bounce {bounce_ms:.3f})")
 
     # algorithmic bytes per bounce ray from the instrumented trace of the same rays (untimed)
     d_stats = ctx.alloc(n_primary * B.STATS_DTYPE.itemsize)
@@ -246,6 +261,7 @@ def main():
         },
         "primary_mrays_s": round(n_primary / (prim_ms * 1e-3) / 1e6, 2),
         "bounce_mrays_s": round(n_bounce / (bounce_ms * 1e-3) / 1e6, 2),
+        "eager_ms_per_step": round(eager_ms, 4),
         "frame_breakdown_ms": {"raygen": round(raygen_ms, 4), "primary_trace": round(prim_ms, 4),
                                "bounce_gen": round(bounce_gen_ms, 4), "bounce_trace": round(bounce_ms, 4)},
         "blas_build_ms": round(build_s * 1e3, 1),

@@ -49,6 +49,20 @@ struct Stack {
     X(128, 8) X(128, 16) X(128, 24) X(128, 32) \
     X(256, 8) X(256, 16) X(256, 24) X(256, 32)
 
+// A BLAS stack entry carries the pushed node's (left_first, count) itself
+// when they fit -- bit 31 set, count in bits 25..30, left_first in bits
+// 0..24 -- so a pop needs no node fetch; otherwise it is the node index
+// (bit 31 clear: node indices stay below 2^31) and the pop reads the node.
+// The visiting order is unchanged.
+constexpr uint32_t kPackFlag = 0x80000000u;
+constexpr uint32_t kPackShift = 25;
+constexpr uint32_t kPackMask = (1u << kPackShift) - 1u;
+
+__device__ __forceinline__ uint32_t pack_entry(uint32_t idx, uint32_t lf, uint32_t cnt)
+{
+    return (cnt < 64u && lf <= kPackMask) ? (kPackFlag | (cnt << kPackShift) | lf) : idx;
+}
+
 struct BlasResult {
     float t, u, v;
     uint32_t prim;
@@ -99,7 +113,8 @@ __device__ __forceinline__ BlasResult traverse_blas(const float4* __restrict__ n
             if (near_hit) {
                 bool far_hit = left_near ? hr : hl;
                 if (far_hit)
-                    st.push(sp, left_near ? lf + 1 : lf);
+                    st.push(sp, left_near ? pack_entry(lf + 1, __float_as_uint(b0.w), __float_as_uint(b1.w))
+                                          : pack_entry(lf, __float_as_uint(a0.w), __float_as_uint(a1.w)));
                 uint32_t nlf = __float_as_uint(left_near ? a0.w : b0.w);
                 cnt = __float_as_uint(left_near ? a1.w : b1.w);
                 lf = nlf;
@@ -108,9 +123,14 @@ __device__ __forceinline__ BlasResult traverse_blas(const float4* __restrict__ n
         }
         if (sp == sp_base)
             break;
-        uint32_t idx = st.pop(sp);
-        lf = __float_as_uint(nodes[2 * (size_t)idx].w);
-        cnt = __float_as_uint(nodes[2 * (size_t)idx + 1].w);
+        uint32_t e = st.pop(sp);
+        if (e & kPackFlag) {
+            lf = e & kPackMask;
+            cnt = (e >> kPackShift) & 63u;
+        } else {
+            lf = __float_as_uint(nodes[2 * (size_t)e].w);
+            cnt = __float_as_uint(nodes[2 * (size_t)e + 1].w);
+        }
     }
     if (!(closest < kFltMax))
         res.found = false;
