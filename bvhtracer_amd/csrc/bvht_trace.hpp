@@ -63,6 +63,13 @@ __device__ __forceinline__ uint32_t pack_entry(uint32_t idx, uint32_t lf, uint32
     return (cnt < 64u && lf <= kPackMask) ? (kPackFlag | (cnt << kPackShift) | lf) : idx;
 }
 
+// Lanes of a wave that must still be descending for the node-pair loop to
+// take another step (see traverse_blas).
+#ifndef BVHT_DESCEND_MIN
+#define BVHT_DESCEND_MIN 24
+#endif
+constexpr uint32_t kDescendMin = BVHT_DESCEND_MIN;
+
 struct BlasResult {
     float t, u, v;
     uint32_t prim;
@@ -83,22 +90,17 @@ __device__ __forceinline__ BlasResult traverse_blas(const float4* __restrict__ n
     uint32_t sp = sp_base;
     uint32_t lf = __float_as_uint(nodes[0].w);
     uint32_t cnt = __float_as_uint(nodes[1].w);
+    // pend: this lane has finished its current node (both children missed) or
+    // leaf and pops next.
+    bool pend = false;
     while (true) {
-        if (cnt > 0) {
-            for (uint32_t k = 0; k < cnt; ++k) {
-                float t, u, v;
-                if (STATS)
-                    ++tests;
-                if (tri_test(tris + 9 * (size_t)(lf + k), ray, t, u, v) && t < closest) {
-                    closest = t;
-                    res.t = t;
-                    res.u = u;
-                    res.v = v;
-                    res.prim = lf + k;
-                    res.found = true;
-                }
-            }
-        } else {
+        // Node-pair steps while the lane is at an interior node.  After each
+        // step the wave leaves this loop once fewer than kDescendMin of its
+        // lanes are still descending, instead of holding the lanes that wait at
+        // a leaf or a pop until the longest descent ends; the lanes still
+        // descending resume here on the next pass.  Every lane's own sequence of
+        // node steps, leaf tests and pops is that of the one-loop traversal.
+        while (!pend && cnt == 0) {
             if (STATS)
                 ++pairs;
             const float4* c = nodes + 2 * (size_t)lf;
@@ -118,19 +120,46 @@ __device__ __forceinline__ BlasResult traverse_blas(const float4* __restrict__ n
                 uint32_t nlf = __float_as_uint(left_near ? a0.w : b0.w);
                 cnt = __float_as_uint(left_near ? a1.w : b1.w);
                 lf = nlf;
-                continue;
+            } else {
+                pend = true;
             }
+            if (__popcll(__builtin_amdgcn_ballot_w64(!pend && cnt == 0)) < kDescendMin)
+                break;
         }
-        if (sp == sp_base)
-            break;
-        uint32_t e = st.pop(sp);
-        if (e & kPackFlag) {
-            lf = e & kPackMask;
-            cnt = (e >> kPackShift) & 63u;
-        } else {
-            lf = __float_as_uint(nodes[2 * (size_t)e].w);
-            cnt = __float_as_uint(nodes[2 * (size_t)e + 1].w);
+        if (!pend && cnt > 0) {
+            for (uint32_t k = 0; k < cnt; ++k) {
+                float t, u, v;
+                if (STATS)
+                    ++tests;
+                if (tri_test(tris + 9 * (size_t)(lf + k), ray, t, u, v) && t < closest) {
+                    closest = t;
+                    res.t = t;
+                    res.u = u;
+                    res.v = v;
+                    res.prim = lf + k;
+                    res.found = true;
+                }
+            }
+            pend = true;
         }
+        if (pend) {
+            if (sp == sp_base)
+                break;
+            uint32_t e = st.pop(sp);
+            if (e & kPackFlag) {
+                lf = e & kPackMask;
+                cnt = (e >> kPackShift) & 63u;
+            } else {
+                lf = __float_as_uint(nodes[2 * (size_t)e].w);
+                cnt = __float_as_uint(nodes[2 * (size_t)e + 1].w);
+            }
+            pend = false;
+        }
+        // One loop latch for every lane: without this convergent point the
+        // compiler gives the no-pop path a back edge of its own and nests the
+        // loops so that pops wait for every lane of the wave to finish its
+        // node or leaf.
+        __builtin_amdgcn_wave_barrier();
     }
     if (!(closest < kFltMax))
         res.found = false;
