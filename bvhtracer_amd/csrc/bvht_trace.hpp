@@ -204,6 +204,16 @@ struct TlasResult {
 // Tlas::intersect (tlas.rs:123-177): the TLAS node loop with
 // SceneObject::intersect at the leaves (scene_object.rs:77-89).  Left child =
 // high 16 bits of the LeftRightIndex word (tlas.rs:24-32).
+//
+// Both levels run in one loop: a lane is either in the TLAS (world ray) or in
+// the BLAS of the object whose leaf it reached (object-space ray), and the
+// node-pair steps of the two levels share one step body, so the lanes of a
+// wave descend their instances' BLASes together instead of one instance
+// visit at a time.  Each lane's sequence of TLAS steps, object visits (with
+// the BLAS traversal of traverse_blas inside) and pops is that of the nested
+// traversal; a visit's nearest hit is committed when its BLAS stack empties,
+// under the same conditions as the nested form (found, t < FLT_MAX, t below
+// the committed distance).
 template <bool STATS, class StackT>
 __device__ __forceinline__ TlasResult traverse_tlas(const float4* __restrict__ tnodes, const ObjRows* __restrict__ objs,
                                                     const BlasRef* __restrict__ blases, const DRay& ray,
@@ -211,61 +221,138 @@ __device__ __forceinline__ TlasResult traverse_tlas(const float4* __restrict__ t
                                                     uint32_t& tpairs, uint32_t& visits)
 {
     TlasResult res{0.0f, 0.0f, 0.0f, 0u, 0u, false};
-    float closest = ray.t;
-    uint32_t sp = 0;
-    uint32_t cur = 0;
-    float4 c0 = tnodes[0], c1 = tnodes[1];
-    uint32_t lr = __float_as_uint(c0.w), blas_id = __float_as_uint(c1.w);
+    float closest = ray.t;  // committed nearest distance (world)
+    // Current level: in_blas = false -> TLAS with the world ray; true -> the
+    // BLAS of object `obj` with the object-space ray cr.
+    bool in_blas = false;
+    DRay cr = ray;
+    float cclosest = closest;  // culling distance of the current level
+    const float4* bnodes = tnodes;
+    const float* btris = nullptr;
+    uint32_t obj = 0, sp = 0, sp_base = 0;
+    float bt = 0.0f, bu = 0.0f, bv = 0.0f;
+    uint32_t bprim = 0;
+    bool bfound = false;
+    // Current node: TLAS -> (left_right, object index), leaf when x == 0;
+    // BLAS -> (left_first, count), leaf when y > 0.
+    uint32_t x = __float_as_uint(tnodes[0].w);
+    uint32_t y = __float_as_uint(tnodes[1].w);
+    bool pend = false;
     while (true) {
-        if (lr == 0) {
-            if (STATS)
-                ++visits;
-            const ObjRows* ob = objs + blas_id;
-            float4 q0 = ob->r0, q1 = ob->r1, q2 = ob->r2;
-            uint32_t model = ob->blas;
-            V3 om = rows_mul(q0, q1, q2, ray.o, 1.0f);
-            V3 dm = rows_mul(q0, q1, q2, ray.d, 0.0f);
-            DRay rm = make_ray(om, dm, closest);
-            BlasRef br = blases[model];
-            BlasResult r = traverse_blas<STATS>(br.nodes, br.tris, rm, st, sp, pairs, tests);
-            if (r.found && r.t < closest) {
-                closest = r.t;
-                res.t = r.t;
-                res.u = r.u;
-                res.v = r.v;
-                res.prim = r.prim;
-                res.inst = blas_id;
-                res.found = true;
+        while (!pend && (in_blas ? y == 0 : x != 0)) {
+            if (STATS) {
+                if (in_blas)
+                    ++pairs;
+                else
+                    ++tpairs;
             }
-        } else {
-            if (STATS)
-                ++tpairs;
-            uint32_t li = (lr & 0xFFFF0000u) >> 16;
-            uint32_t ri = lr & 0x0000FFFFu;
-            float4 a0 = tnodes[2 * li], a1 = tnodes[2 * li + 1];
-            float4 b0 = tnodes[2 * ri], b1 = tnodes[2 * ri + 1];
+            const float4* pl;
+            const float4* pr;
+            if (in_blas) {
+                pl = bnodes + 2 * (size_t)x;
+                pr = pl + 2;
+            } else {
+                pl = tnodes + 2 * (size_t)((x & 0xFFFF0000u) >> 16);
+                pr = tnodes + 2 * (size_t)(x & 0x0000FFFFu);
+            }
+            float4 a0 = pl[0], a1 = pl[1], b0 = pr[0], b1 = pr[1];
             float dl, dr;
-            bool hl = slab(a0, a1, ray, closest, dl);
-            bool hr = slab(b0, b1, ray, closest, dr);
+            bool hl = slab(a0, a1, cr, cclosest, dl);
+            bool hr = slab(b0, b1, cr, cclosest, dr);
             float kl = hl ? dl : kFltMax;
             float kr = hr ? dr : kFltMax;
             bool left_near = kl < kr;
             bool near_hit = left_near ? hl : hr;
             if (near_hit) {
                 bool far_hit = left_near ? hr : hl;
-                if (far_hit)
-                    st.push(sp, left_near ? ri : li);
-                cur = left_near ? li : ri;
-                lr = __float_as_uint(left_near ? a0.w : b0.w);
-                blas_id = __float_as_uint(left_near ? a1.w : b1.w);
-                continue;
+                if (far_hit) {
+                    uint32_t e;
+                    if (in_blas)
+                        e = left_near ? pack_entry(x + 1, __float_as_uint(b0.w), __float_as_uint(b1.w))
+                                      : pack_entry(x, __float_as_uint(a0.w), __float_as_uint(a1.w));
+                    else
+                        e = left_near ? (x & 0x0000FFFFu) : ((x & 0xFFFF0000u) >> 16);
+                    st.push(sp, e);
+                }
+                uint32_t nx = __float_as_uint(left_near ? a0.w : b0.w);
+                y = __float_as_uint(left_near ? a1.w : b1.w);
+                x = nx;
+            } else {
+                pend = true;
             }
+            if (__popcll(__builtin_amdgcn_ballot_w64(!pend && (in_blas ? y == 0 : x != 0))) < kDescendMin)
+                break;
         }
-        if (sp == 0)
-            break;
-        cur = st.pop(sp);
-        lr = __float_as_uint(tnodes[2 * cur].w);
-        blas_id = __float_as_uint(tnodes[2 * cur + 1].w);
+        if (!pend && in_blas && y > 0) {
+            for (uint32_t k = 0; k < y; ++k) {
+                float t, u, v;
+                if (STATS)
+                    ++tests;
+                if (tri_test(btris + 9 * (size_t)(x + k), cr, t, u, v) && t < cclosest) {
+                    cclosest = t;
+                    bt = t;
+                    bu = u;
+                    bv = v;
+                    bprim = x + k;
+                    bfound = true;
+                }
+            }
+            pend = true;
+        }
+        if (!pend && !in_blas && x == 0) {
+            // object leaf: enter its BLAS with the object-space ray
+            if (STATS)
+                ++visits;
+            const ObjRows* ob = objs + y;
+            float4 q0 = ob->r0, q1 = ob->r1, q2 = ob->r2;
+            uint32_t model = ob->blas;
+            V3 om = rows_mul(q0, q1, q2, ray.o, 1.0f);
+            V3 dm = rows_mul(q0, q1, q2, ray.d, 0.0f);
+            cr = make_ray(om, dm, closest);
+            BlasRef br = blases[model];
+            bnodes = br.nodes;
+            btris = br.tris;
+            obj = y;
+            sp_base = sp;
+            cclosest = closest;
+            bfound = false;
+            in_blas = true;
+            x = __float_as_uint(bnodes[0].w);
+            y = __float_as_uint(bnodes[1].w);
+        }
+        if (pend) {
+            if (in_blas && sp == sp_base) {
+                // the object's BLAS traversal is over: commit its nearest hit
+                if (bfound && cclosest < kFltMax && cclosest < closest) {
+                    closest = cclosest;
+                    res.t = bt;
+                    res.u = bu;
+                    res.v = bv;
+                    res.prim = bprim;
+                    res.inst = obj;
+                    res.found = true;
+                }
+                in_blas = false;
+                cr = ray;
+                cclosest = closest;
+            }
+            if (!in_blas && sp == 0)
+                break;
+            uint32_t e = st.pop(sp);
+            if (!in_blas) {
+                x = __float_as_uint(tnodes[2 * (size_t)e].w);
+                y = __float_as_uint(tnodes[2 * (size_t)e + 1].w);
+            } else if (e & kPackFlag) {
+                x = e & kPackMask;
+                y = (e >> kPackShift) & 63u;
+            } else {
+                x = __float_as_uint(bnodes[2 * (size_t)e].w);
+                y = __float_as_uint(bnodes[2 * (size_t)e + 1].w);
+            }
+            pend = false;
+        }
+        // one loop latch for every lane (see traverse_blas)
+        __builtin_amdgcn_wave_barrier();
     }
     if (!(closest < kFltMax))
         res.found = false;
