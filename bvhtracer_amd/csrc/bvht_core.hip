@@ -170,12 +170,19 @@ __global__ void k_level_scatter(uint32_t nodes_used, const uint32_t* __restrict_
     }
 }
 
-// Leaves with more than kBigLeaf triangles: counted, then listed.
+// Leaves with more than kBigLeaf triangles: counted, then listed.  `wide`
+// (when given) is set if some leaf has 64 or more triangles, too many for a
+// packed stack entry.
 __global__ void k_big_leaves(const bvht_node* __restrict__ nodes, uint32_t nodes_used, uint32_t* __restrict__ counter,
-                             uint32_t* __restrict__ list)
+                             uint32_t* __restrict__ list, uint32_t* __restrict__ wide)
 {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nodes_used || k == 1 || nodes[k].count <= kBigLeaf)
+    if (k >= nodes_used || k == 1)
+        return;
+    uint32_t c = nodes[k].count;
+    if (wide && c >= 64u)
+        *wide = 1u;
+    if (c <= kBigLeaf)
         return;
     uint32_t slot = atomicAdd(counter, 1u);
     if (list)
@@ -200,7 +207,7 @@ __global__ void k_refit_level(bvht_node* __restrict__ nodes, const float* __rest
         refit_node(nodes, tris, order[i]);
 }
 
-template <int WG, int LDS_DEPTH, bool STATS>
+template <int WG, int LDS_DEPTH, bool STATS, bool PACKED>
 __global__ __launch_bounds__(WG) void k_trace_blas(const float4* __restrict__ nodes, const float* __restrict__ tris,
                                                    const bvht_ray* __restrict__ rays, bvht_hit* __restrict__ hits,
                                                    uint32_t n, const uint32_t* __restrict__ d_n,
@@ -215,7 +222,7 @@ __global__ __launch_bounds__(WG) void k_trace_blas(const float4* __restrict__ no
     Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap};
     DRay ray = load_ray(rays, i);
     uint32_t pairs = 0, tests = 0;
-    BlasResult r = traverse_blas<STATS>(nodes, tris, ray, st, 0u, pairs, tests);
+    BlasResult r = traverse_blas<STATS, PACKED>(nodes, tris, ray, st, 0u, pairs, tests);
     store_hit(hits, i, r.found, r.t, r.u, r.v, r.prim, 0);
     if (STATS) {
         bvht_stats s;
@@ -230,11 +237,14 @@ __global__ __launch_bounds__(WG) void k_trace_blas(const float4* __restrict__ no
 typedef void (*trace_blas_fn)(const float4*, const float*, const bvht_ray*, bvht_hit*, uint32_t, const uint32_t*,
                               bvht_stats*, uint32_t*, uint32_t);
 
-static trace_blas_fn pick_trace_blas(int wg, int lds, bool stats)
+static trace_blas_fn pick_trace_blas(int wg, int lds, bool stats, bool packed)
 {
-#define X(W, D)                                      \
-    if (wg == W && lds == D)                         \
-        return stats ? k_trace_blas<W, D, true> : k_trace_blas<W, D, false>;
+#define X(W, D)                                                                             \
+    if (wg == W && lds == D) {                                                              \
+        if (packed)                                                                         \
+            return stats ? k_trace_blas<W, D, true, true> : k_trace_blas<W, D, false, true>;  \
+        return stats ? k_trace_blas<W, D, true, false> : k_trace_blas<W, D, false, false>;    \
+    }
     BVHT_TRACE_CONFIGS(X)
 #undef X
     return nullptr;
@@ -595,14 +605,18 @@ int32_t compute_blas_topology(bvht_blas b, const bvht_node*)
     hipLaunchKernelGGL(k_level_scatter, dim3(nb_levels), dim3(256), 0, st, b->nodes_used, b->d_level, hist,
                        b->d_level_order);
     BVHT_HIP(hipGetLastError());
-    // big leaves: count, allocate, list
-    rc = fill_u32(ctx, d_depth, 0, 1);
+    // big leaves: count, allocate, list; and whether every node fits a packed
+    // stack entry
+    rc = fill_u32(ctx, d_depth, 0, 2);
     if (rc)
         return rc;
-    hipLaunchKernelGGL(k_big_leaves, dim3(nb), dim3(256), 0, st, b->d_nodes, b->nodes_used, d_depth, nullptr);
-    uint32_t nbig = 0;
-    BVHT_HIP(hipMemcpyAsync(&nbig, d_depth, 4, hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_big_leaves, dim3(nb), dim3(256), 0, st, b->d_nodes, b->nodes_used, d_depth, nullptr,
+                       d_depth + 1);
+    uint32_t hb[2] = {0, 0};
+    BVHT_HIP(hipMemcpyAsync(hb, d_depth, 8, hipMemcpyDeviceToHost, st));
     BVHT_HIP(hipStreamSynchronize(st));
+    uint32_t nbig = hb[0];
+    b->packed_stack = hb[1] == 0 && b->nodes_used <= kPackMask + 1u && b->n_tris <= kPackMask + 1u;
     (void)hipFree(b->d_big_leaves);
     b->d_big_leaves = nullptr;
     b->n_big_leaves = nbig;
@@ -613,7 +627,7 @@ int32_t compute_blas_topology(bvht_blas b, const bvht_node*)
         if (rc)
             return rc;
         hipLaunchKernelGGL(k_big_leaves, dim3(nb), dim3(256), 0, st, b->d_nodes, b->nodes_used, d_depth,
-                           b->d_big_leaves);
+                           b->d_big_leaves, nullptr);
         BVHT_HIP(hipGetLastError());
     }
     BVHT_HIP(hipStreamSynchronize(st));
@@ -718,7 +732,7 @@ int32_t launch_trace_blas(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, 
             return rc;
         spill = blas->d_spill;
     }
-    trace_blas_fn fn = pick_trace_blas(wg, lds, d_stats != nullptr);
+    trace_blas_fn fn = pick_trace_blas(wg, lds, d_stats != nullptr, blas->packed_stack);
     if (!fn)
         return set_error(BVHT_EINVAL, "no trace kernel for workgroup %d, LDS depth %d", wg, lds);
     hipLaunchKernelGGL(fn, dim3(blocks_for(n, wg)), dim3(wg), 0, ctx->stream,
@@ -1049,7 +1063,7 @@ int32_t bvht_set_trace_config(bvht_ctx ctx, int32_t wg, int32_t lds_depth)
         ctx->trace_lds = 0;
         return BVHT_OK;
     }
-    BVHT_REQUIRE(pick_trace_blas(wg, lds_depth, false) != nullptr,
+    BVHT_REQUIRE(pick_trace_blas(wg, lds_depth, false, false) != nullptr,
                  "unsupported trace configuration: workgroup %d, LDS depth %d", wg, lds_depth);
     ctx->trace_wg = wg;
     ctx->trace_lds = lds_depth;

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(WG) void k_trace_forward(const float4* __restrict__
     V3 d = rows_mul(rows.r0, rows.r1, rows.r2, mk3(b.x, b.y, b.z), 0.0f);
     DRay ray = make_ray(o, d, a.w);
     uint32_t pairs = 0, tests = 0;
-    BlasResult r = traverse_blas<false>(nodes, tris, ray, st, 0u, pairs, tests);
+    BlasResult r = traverse_blas<false, false>(nodes, tris, ray, st, 0u, pairs, tests);
     store_hit(hits, i, r.found, r.t, r.u, r.v, r.prim, 0);
     float depth = kFltMax, shade = 0.0f;
     if (r.found) {

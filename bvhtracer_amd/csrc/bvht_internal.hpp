@@ -46,6 +46,7 @@ struct bvht_blas_s {
     std::vector<uint32_t> level_off;    // level L occupies [level_off[L], level_off[L+1]) of d_level_order
     uint32_t* d_big_leaves = nullptr;   // leaves with more than kBigLeaf triangles (refit a wave each)
     uint32_t n_big_leaves = 0;
+    bool packed_stack = false;  // every node fits a packed stack entry (see bvht_trace.hpp)
     // shading attributes (bvht_blas_set_attributes)
     float* d_attr_normals = nullptr;
     float* d_attr_tex = nullptr;

@@ -76,10 +76,17 @@ struct BlasResult {
     bool found;
 };
 
+// cnt value of a lane that has finished its node (both children missed) or
+// leaf and pops next.  A leaf's count is below the triangle count, so it
+// never takes this value.
+constexpr uint32_t kPopNext = 0xFFFFFFFFu;
+
 // Bvh::intersect_subtree (bvh.rs:242-305) for one ray, starting at node 0.
 // The stack is shared with an enclosing TLAS traversal: this call only uses
-// slots at or above sp_base and leaves the stack as it found it.
-template <bool STATS, class StackT>
+// slots at or above sp_base and leaves the stack as it found it.  PACKED:
+// every node of the tree fits a packed stack entry (no leaf of 64 or more
+// triangles, left_first below 2^25), so entries are packed without a check.
+template <bool STATS, bool PACKED, class StackT>
 __device__ __forceinline__ BlasResult traverse_blas(const float4* __restrict__ nodes,
                                                     const float* __restrict__ tris, const DRay& ray,
                                                     StackT& st, uint32_t sp_base, uint32_t& pairs,
@@ -90,17 +97,15 @@ __device__ __forceinline__ BlasResult traverse_blas(const float4* __restrict__ n
     uint32_t sp = sp_base;
     uint32_t lf = __float_as_uint(nodes[0].w);
     uint32_t cnt = __float_as_uint(nodes[1].w);
-    // pend: this lane has finished its current node (both children missed) or
-    // leaf and pops next.
-    bool pend = false;
     while (true) {
-        // Node-pair steps while the lane is at an interior node.  After each
-        // step the wave leaves this loop once fewer than kDescendMin of its
-        // lanes are still descending, instead of holding the lanes that wait at
-        // a leaf or a pop until the longest descent ends; the lanes still
-        // descending resume here on the next pass.  Every lane's own sequence of
-        // node steps, leaf tests and pops is that of the one-loop traversal.
-        while (!pend && cnt == 0) {
+        // Node-pair steps while the lane is at an interior node (cnt == 0).
+        // After each step the wave leaves this loop once fewer than
+        // kDescendMin of its lanes are still descending, instead of holding
+        // the lanes that wait at a leaf or a pop until the longest descent
+        // ends; the lanes still descending resume here on the next pass.
+        // Every lane's own sequence of node steps, leaf tests and pops is
+        // that of the one-loop traversal.
+        while (cnt == 0) {
             if (STATS)
                 ++pairs;
             const float4* c = nodes + 2 * (size_t)lf;
@@ -114,19 +119,22 @@ __device__ __forceinline__ BlasResult traverse_blas(const float4* __restrict__ n
             bool near_hit = left_near ? hl : hr;
             if (near_hit) {
                 bool far_hit = left_near ? hr : hl;
-                if (far_hit)
-                    st.push(sp, left_near ? pack_entry(lf + 1, __float_as_uint(b0.w), __float_as_uint(b1.w))
-                                          : pack_entry(lf, __float_as_uint(a0.w), __float_as_uint(a1.w)));
+                if (far_hit) {
+                    uint32_t flf = __float_as_uint(left_near ? b0.w : a0.w);
+                    uint32_t fcnt = __float_as_uint(left_near ? b1.w : a1.w);
+                    st.push(sp, PACKED ? (kPackFlag | (fcnt << kPackShift) | flf)
+                                       : pack_entry(left_near ? lf + 1 : lf, flf, fcnt));
+                }
                 uint32_t nlf = __float_as_uint(left_near ? a0.w : b0.w);
                 cnt = __float_as_uint(left_near ? a1.w : b1.w);
                 lf = nlf;
             } else {
-                pend = true;
+                cnt = kPopNext;
             }
-            if (__popcll(__builtin_amdgcn_ballot_w64(!pend && cnt == 0)) < kDescendMin)
+            if ((uint32_t)__popcll(__builtin_amdgcn_ballot_w64(cnt == 0)) < kDescendMin)
                 break;
         }
-        if (!pend && cnt > 0) {
+        if (cnt != 0 && cnt != kPopNext) {
             for (uint32_t k = 0; k < cnt; ++k) {
                 float t, u, v;
                 if (STATS)
@@ -140,20 +148,19 @@ __device__ __forceinline__ BlasResult traverse_blas(const float4* __restrict__ n
                     res.found = true;
                 }
             }
-            pend = true;
+            cnt = kPopNext;
         }
-        if (pend) {
+        if (cnt == kPopNext) {
             if (sp == sp_base)
                 break;
             uint32_t e = st.pop(sp);
-            if (e & kPackFlag) {
+            if (PACKED || (e & kPackFlag)) {
                 lf = e & kPackMask;
                 cnt = (e >> kPackShift) & 63u;
             } else {
                 lf = __float_as_uint(nodes[2 * (size_t)e].w);
                 cnt = __float_as_uint(nodes[2 * (size_t)e + 1].w);
             }
-            pend = false;
         }
         // One loop latch for every lane: without this convergent point the
         // compiler gives the no-pop path a back edge of its own and nests the
@@ -280,7 +287,7 @@ __device__ __forceinline__ TlasResult traverse_tlas(const float4* __restrict__ t
             } else {
                 pend = true;
             }
-            if (__popcll(__builtin_amdgcn_ballot_w64(!pend && (in_blas ? y == 0 : x != 0))) < kDescendMin)
+            if ((uint32_t)__popcll(__builtin_amdgcn_ballot_w64(!pend && (in_blas ? y == 0 : x != 0))) < kDescendMin)
                 break;
         }
         if (!pend && in_blas && y > 0) {
