@@ -15,6 +15,7 @@ from .render import (DepthAccumulator, DepthMappingShader, IntersectionAccumulat
                      TextureMaterialAccumulator, UvMappingAccumulator)
 from .scene import (ModelBuilder, ModelInstance, Scene, SceneBuilder, SceneObjectBuilder, Triangle,
                     intersect_pairs)
+from .physics import Gravity, NullForce, RigidBody, RigidBodyInstance, Spring, World
 
 __all__ = [
     "BUILD_REFERENCE", "BUILD_SAH", "CAMERA_DTYPE", "HIT_DTYPE", "INSTANCE_DTYPE", "MISS", "NODE_DTYPE",
@@ -26,4 +27,5 @@ __all__ = [
     "NormalMappingAccumulator", "PathTracer", "RadianceToRgbShader", "Renderer", "RendererState",
     "TextureMaterialAccumulator", "UvMappingAccumulator",
     "ModelBuilder", "ModelInstance", "Scene", "SceneBuilder", "SceneObjectBuilder", "Triangle", "intersect_pairs",
+    "Gravity", "NullForce", "RigidBody", "RigidBodyInstance", "Spring", "World",
 ]

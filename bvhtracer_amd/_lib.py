@@ -28,9 +28,19 @@ HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim", "<u4"),
 INSTANCE_DTYPE = np.dtype([("m", "<f4", 16), ("minv", "<f4", 16), ("blas", "<u4"), ("pad", "<u4", 3)])
 CAMERA_DTYPE = np.dtype([("tl", "<f4", 3), ("tr", "<f4", 3), ("bl", "<f4", 3), ("view_inv", "<f4", 16)])
 STATS_DTYPE = np.dtype([("pairs", "<u4"), ("tests", "<u4"), ("tlas_pairs", "<u4"), ("inst_visits", "<u4")])
+BODY_DTYPE = np.dtype([
+    ("inverse_mass", "<f4"), ("inverse_inertia_tensor", "<f4", 9), ("linear_drag", "<f4"), ("angular_drag", "<f4"),
+    ("position", "<f4", 3), ("orientation", "<f4", 4), ("velocity", "<f4", 3), ("rotation", "<f4", 3),
+    ("inverse_inertia_tensor_world", "<f4", 9), ("motion", "<f4"), ("is_awake", "<u4"), ("can_sleep", "<u4"),
+    ("transform", "<f4", 16), ("force_accumulator", "<f4", 3), ("torque_accumulator", "<f4", 3),
+    ("acceleration", "<f4", 3), ("last_frame_acceleration", "<f4", 3), ("pad", "<f4", 3)])
+FORCE_DTYPE = np.dtype([("kind", "<u4"), ("body", "<u4"), ("other", "<u4"), ("pad", "<u4"), ("v0", "<f4", 3),
+                        ("v1", "<f4", 3), ("spring_constant", "<f4"), ("rest_length", "<f4")])
+FORCE_NULL, FORCE_GRAVITY, FORCE_SPRING = 0, 1, 2
 
 assert NODE_DTYPE.itemsize == 32 and TNODE_DTYPE.itemsize == 32 and RAY_DTYPE.itemsize == 32
 assert HIT_DTYPE.itemsize == 24 and INSTANCE_DTYPE.itemsize == 144 and CAMERA_DTYPE.itemsize == 100
+assert BODY_DTYPE.itemsize == 272 and FORCE_DTYPE.itemsize == 48
 
 P = ctypes.c_void_p
 PP = ctypes.POINTER(ctypes.c_void_p)
@@ -103,6 +113,15 @@ SIGNATURES = {
     "bvht_mesh_sgd": (I32, [P, P, P, F32, F32, I32]),
     "bvht_trace_forward": (I32, [P, P, P, P, P, U32, P, P]),
     "bvht_trace_adjoint": (I32, [P, P, P, P, P, U32, P, P, F32, F32, I32, P, P, P]),
+    "bvht_world_create": (I32, [P, P, U32, P, U32, P]),
+    "bvht_world_destroy": (I32, [P, P]),
+    "bvht_world_set_bodies": (I32, [P, P, P, U32, U32]),
+    "bvht_world_get_bodies": (I32, [P, P, P, U32, U32]),
+    "bvht_world_start_frame": (I32, [P, P]),
+    "bvht_world_run_physics": (I32, [P, P, F32]),
+    "bvht_tlas_bind_world": (I32, [P, P, P, P, P]),
+    "bvht_tlas_get_objects": (I32, [P, P, P, U32, U32]),
+    "bvht_scene_run_physics": (I32, [P, P, F32]),
     "bvht_graph_begin": (I32, [P]),
     "bvht_graph_end": (I32, [P, PP]),
     "bvht_graph_launch": (I32, [P, P]),

@@ -75,6 +75,10 @@ struct bvht_tlas_s {
     size_t spill_words = 0;
     AttrRef* d_attr_table = nullptr;        // per-BLAS attributes for the attribute accumulators
     std::vector<uint64_t> attr_versions;    // BLAS attribute versions the table was built from
+    // physics binding (bvht_tlas_bind_world)
+    bvht_world world = nullptr;
+    uint32_t* d_object_body = nullptr;  // body of each object (BVHT_MISS: none)
+    float* d_transform_init = nullptr;  // 16 per object, column-major
 };
 
 namespace bvht {
@@ -91,6 +95,11 @@ int32_t launch_bounce_sorted(bvht_ctx ctx, const float* d_tris, uint32_t ntris, 
                              uint64_t seed, bvht_ray* d_out, uint32_t* d_src, uint32_t* d_count);
 int32_t hit_block_offsets(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, uint32_t prim_limit, uint32_t inst_limit,
                           const void* rows, const void* blases, uint32_t* d_count, uint32_t** offsets);
+// World bounds and traced rows of every object from its current M, M^-1
+// (SceneObject::set_transform on the device).
+int32_t tlas_refresh(bvht_tlas t);
+// World::run_physics on the device (bvht_physics.hip).
+int32_t world_run_physics(bvht_world w, float duration);
 
 }  // namespace bvht
 

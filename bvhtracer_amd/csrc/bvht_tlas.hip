@@ -135,6 +135,8 @@ void tlas_free(bvht_tlas t)
     (void)hipFree(t->d_list);
     (void)hipFree(t->d_meta);
     (void)hipFree(t->d_spill);
+    (void)hipFree(t->d_object_body);
+    (void)hipFree(t->d_transform_init);
     delete t;
 }
 
@@ -197,6 +199,11 @@ int32_t launch_trace_tlas(bvht_tlas t, const bvht_ray* d_rays, uint32_t n, const
 }
 
 }  // namespace
+
+int32_t bvht::tlas_refresh(bvht_tlas t)
+{
+    return refresh(t);
+}
 
 extern "C" {
 
@@ -270,6 +277,16 @@ int32_t bvht_tlas_set_objects(bvht_ctx ctx, bvht_tlas tlas, const bvht_instance*
                             ctx->stream));
     BVHT_HIP(hipStreamSynchronize(ctx->stream));
     return refresh(tlas);
+}
+
+int32_t bvht_tlas_get_objects(bvht_ctx ctx, bvht_tlas tlas, bvht_instance* out, uint32_t first, uint32_t count)
+{
+    BVHT_REQUIRE(ctx && tlas && (count == 0 || out), "NULL argument");
+    BVHT_REQUIRE((uint64_t)first + count <= tlas->n, "objects [%u, +%u) out of range (%u)", first, count, tlas->n);
+    BVHT_HIP(hipMemcpyAsync(out, tlas->d_objects + first, sizeof(bvht_instance) * count, hipMemcpyDeviceToHost,
+                            ctx->stream));
+    BVHT_HIP(hipStreamSynchronize(ctx->stream));
+    return BVHT_OK;
 }
 
 int32_t bvht_tlas_refresh_bounds(bvht_ctx ctx, bvht_tlas tlas)

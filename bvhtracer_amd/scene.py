@@ -10,8 +10,8 @@ lists) and device handles.
 import numpy as np
 
 from . import _lib as L
-from .api import (BUILD_REFERENCE, HIT_DTYPE, MISS, RAY_DTYPE, BvhBuilder, SceneObject, TlasBuilder, _as_tris,
-                  default_context)
+from .api import (BUILD_REFERENCE, HIT_DTYPE, INSTANCE_DTYPE, MISS, RAY_DTYPE, BvhBuilder, SceneObject, TlasBuilder,
+                  _as_tris, default_context)
 
 
 class ModelInstance:
@@ -130,10 +130,14 @@ class ModelBuilder:
 
 class SceneObjectBuilder:
     """SceneObjectBuilder (scene_object.rs:92-135): a ModelInstance placed by
-    a transform M (column-major 4x4; identity by default)."""
+    a transform M (column-major 4x4; identity by default), following the
+    rigid body ``rigid_body`` (a RigidBodyInstance of the scene's World) if
+    given: SceneObject::update_transform then sets M = body transform x this
+    transform."""
 
-    def __init__(self, model):
+    def __init__(self, model, rigid_body=None):
         self.model = model
+        self.rigid_body = rigid_body
         self.transform = None
 
     def with_transform(self, transform):
@@ -143,21 +147,39 @@ class SceneObjectBuilder:
     def build(self):
         obj = SceneObject(-1, self.transform)
         obj.instance = self.model
+        obj.rigid_body = self.rigid_body
+        obj.transform_init = obj.m.copy()
         return obj
 
 
 class Scene:
-    """Scene (scene.rs:7-58): objects + TLAS + active camera record."""
+    """Scene (scene.rs:7-58): objects + TLAS + active camera record + World.
+    After run_physics the objects' transforms live on the device; the host
+    records are refreshed when an object is read (get_unchecked)."""
 
-    def __init__(self, tlas, objects, camera):
+    def __init__(self, tlas, objects, camera, physics=None):
         self.tlas = tlas
         self.objects = objects
         self.camera = camera
+        self.physics = physics
+        self._bound = False
+        self._device_ahead = False
+
+    def _sync(self):
+        if self._device_ahead:
+            n = len(self.objects)
+            recs = np.zeros(n, INSTANCE_DTYPE)
+            L.check(L.lib().bvht_tlas_get_objects(self.tlas.ctx.handle, self.tlas.handle, L.ptr(recs), 0, n))
+            for o, r in zip(self.objects, recs):
+                o.m, o.minv = r["m"].copy(), r["minv"].copy()
+            self._device_ahead = False
 
     def get_unchecked(self, index):
+        self._sync()
         return self.objects[index]
 
     def get_mut_unchecked(self, index):
+        self._sync()
         return self.objects[index]
 
     def active_camera(self):
@@ -173,7 +195,37 @@ class Scene:
     def rebuild(self):
         """Scene::rebuild (scene.rs:36-38): object transforms and world bounds
         from the current models' bounds, then Tlas::rebuild."""
-        self.tlas.rebuild(self.objects)
+        if self._device_ahead:
+            self.tlas.rebuild()
+        else:
+            self.tlas.rebuild(self.objects)
+
+    def _bind(self):
+        if not self._bound:
+            body = np.array([MISS if getattr(o, "rigid_body", None) is None else o.rigid_body.index
+                             for o in self.objects], np.uint32)
+            init = np.stack([getattr(o, "transform_init", o.m) for o in self.objects]).astype(np.float32)
+            for o in self.objects:
+                rb = getattr(o, "rigid_body", None)
+                if rb is not None and rb.world is not self.physics:
+                    raise ValueError("a scene object follows a body of another world")
+            L.check(L.lib().bvht_tlas_bind_world(self.tlas.ctx.handle, self.tlas.handle, self.physics.device(),
+                                                 L.ptr(body), L.ptr(init)))
+            self._bound = True
+
+    def run_physics(self, elapsed):
+        """Scene::run_physics (scene.rs:40-45) on the device: World::run_physics,
+        then SceneObject::update_transform of every object with a body."""
+        if self.physics is None or len(self.physics) == 0:
+            return
+        self._bind()
+        L.check(L.lib().bvht_scene_run_physics(self.tlas.ctx.handle, self.tlas.handle, np.float32(elapsed)))
+        self._device_ahead = True
+
+    def run(self, elapsed):
+        """Scene::run (scene.rs:47-50)."""
+        self.run_physics(elapsed)
+        self.rebuild()
 
 
 class SceneBuilder:
@@ -183,6 +235,7 @@ class SceneBuilder:
     def __init__(self, camera=None, ctx=None):
         self.camera = camera
         self.objects = []
+        self.physics = None
         self.ctx = ctx or default_context()
 
     def with_object(self, obj):
@@ -191,6 +244,11 @@ class SceneBuilder:
 
     def with_objects(self, objects):
         self.objects = list(objects)
+        return self
+
+    def with_physics(self, physics):
+        """SceneBuilder::with_physics (scene.rs:79-83): a physics.World."""
+        self.physics = physics
         return self
 
     def build(self):
@@ -204,7 +262,7 @@ class SceneBuilder:
                 models.append(mi)
             o.model = index[id(mi)]
         tlas = TlasBuilder(ctx=self.ctx).build_for(self.objects, [m.bvh for m in models])
-        scene = Scene(tlas, self.objects, self.camera)
+        scene = Scene(tlas, self.objects, self.camera, self.physics)
         scene.models = models
         return scene
 

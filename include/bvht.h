@@ -15,8 +15,9 @@
  *     node arrays passed to build/upload/download are HOST pointers.
  *   - All work is enqueued on the context's HIP stream; bvht_sync() or a
  *     read-back is the fence.  One context per host thread.  Destroy every
- *     object made with a context (BLAS, TLAS, batch, mesh, graph, device
- *     allocation) before the context itself.
+ *     object made with a context (BLAS, TLAS, batch, mesh, world, graph,
+ *     device allocation) before the context itself, and a world after the
+ *     TLASes bound to it.
  *   - Arithmetic is IEEE binary32 in the reference's operation order, no FMA
  *     contraction, correctly rounded division: hits are bit-identical to the
  *     reference algorithm on the same BVH and rays.
@@ -314,6 +315,9 @@ int32_t bvht_tlas_create(bvht_ctx ctx, const bvht_blas* blases, uint32_t blas_co
  * by a BLAS refit alone, as in the reference. */
 int32_t bvht_tlas_set_objects(bvht_ctx ctx, bvht_tlas tlas, const bvht_instance* objects,
                               uint32_t first, uint32_t count);
+/* Copy back objects [first, first+count) as the device holds them (after
+ * bvht_scene_run_physics, their M and M^-1 are the bodies'). */
+int32_t bvht_tlas_get_objects(bvht_ctx ctx, bvht_tlas tlas, bvht_instance* out, uint32_t first, uint32_t count);
 /* Device-side variant: refresh every object's world bounds from its current
  * M and the current BLAS bounds (set_transform with the stored transform). */
 int32_t bvht_tlas_refresh_bounds(bvht_ctx ctx, bvht_tlas tlas);
@@ -438,6 +442,82 @@ int32_t bvht_trace_adjoint(bvht_ctx ctx, bvht_mesh mesh, const float* minv, cons
                            const bvht_ray* d_rays, uint32_t n, const bvht_hit* d_hits,
                            const float* d_target, float w_depth, float w_shade, int32_t accumulate,
                            float* d_grad_vertices, float* d_grad_minv, double* d_loss);
+
+/* ---- rigid-body physics driving scene objects ---------------------------- */
+
+/* RigidBody (physics/rigid_body.rs:66-86), every field, binary32.  Matrices
+ * column-major: inertia tensors m[c*3+r], transform m[c*4+r]; orientation is
+ * the quaternion (s, x, y, z).  RigidBody::default(): inverse_mass FLT_MAX,
+ * identity inverse inertia tensors, drags 1, orientation (0, 0, 0, 1)
+ * (Quaternion::unit_z), asleep, can_sleep, identity transform, the rest 0. */
+typedef struct {
+    float inverse_mass;
+    float inverse_inertia_tensor[9];
+    float linear_drag;
+    float angular_drag;
+    float position[3];
+    float orientation[4];
+    float velocity[3];
+    float rotation[3];
+    float inverse_inertia_tensor_world[9];
+    float motion;
+    uint32_t is_awake;
+    uint32_t can_sleep;
+    float transform[16];
+    float force_accumulator[3];
+    float torque_accumulator[3];
+    float acceleration[3];
+    float last_frame_acceleration[3];
+    float pad[3];
+} bvht_body;
+
+#define BVHT_FORCE_NULL 0    /* NullForce (force_generator.rs:9-31) */
+#define BVHT_FORCE_GRAVITY 1 /* Gravity (force_generator.rs:34-56): v0 = g */
+#define BVHT_FORCE_SPRING 2  /* Spring (force_generator.rs:58-121): v0 = connection point on `body`,
+                                v1 = on `other` (body coordinates), spring_constant, rest_length */
+/* One ForceRegistry entry (force_registry.rs:31-78): a generator acting on `body`. */
+typedef struct {
+    uint32_t kind;
+    uint32_t body;
+    uint32_t other;
+    uint32_t pad;
+    float v0[3];
+    float v1[3];
+    float spring_constant;
+    float rest_length;
+} bvht_force;
+
+typedef struct bvht_world_s* bvht_world;
+
+/* World (physics/world.rs:14-61) on the device: bodies and the force
+ * registry in registration order.  Forces acting on one body are applied in
+ * registry order, as ForceRegistry::apply_forces does. */
+int32_t bvht_world_create(bvht_ctx ctx, const bvht_body* bodies, uint32_t n_bodies, const bvht_force* forces,
+                          uint32_t n_forces, bvht_world* out);
+int32_t bvht_world_destroy(bvht_ctx ctx, bvht_world world);
+/* Replace bodies [first, first+count) / copy them back. */
+int32_t bvht_world_set_bodies(bvht_ctx ctx, bvht_world world, const bvht_body* bodies, uint32_t first,
+                              uint32_t count);
+int32_t bvht_world_get_bodies(bvht_ctx ctx, bvht_world world, bvht_body* out, uint32_t first, uint32_t count);
+/* World::start_frame (world.rs:44-49): clear accumulators, derived data. */
+int32_t bvht_world_start_frame(bvht_ctx ctx, bvht_world world);
+/* World::run_physics (world.rs:57-60): every force of the registry, then
+ * RigidBody::integrate (rigid_body.rs:138-203) of every body.  powf is
+ * evaluated in binary64 (ln by the atanh series, exp by range reduction and
+ * Taylor series) and rounded: exact for powf(1, y), powf(x, 0) and powers of
+ * two of integer exponent, within one binary32 ulp otherwise. */
+int32_t bvht_world_run_physics(bvht_ctx ctx, bvht_world world, float duration);
+/* Attach a world to a TLAS: object i follows body object_body[i]
+ * (BVHT_MISS: no body) with its builder transform transform_init[16 i]
+ * (SceneObjectBuilder::with_transform, column-major). */
+int32_t bvht_tlas_bind_world(bvht_ctx ctx, bvht_tlas tlas, bvht_world world, const uint32_t* object_body,
+                             const float* transform_init);
+/* Scene::run_physics (scene/scene.rs:40-45): World::run_physics, then
+ * SceneObject::update_transform (scene_object.rs:52-58) of every object with
+ * a body -- M = body transform x transform_init, M^-1 (binary64 adjugate,
+ * rounded), world bounds -- all enqueued on the device.  Scene::run is this
+ * followed by bvht_tlas_rebuild or bvht_tlas_rebuild_async. */
+int32_t bvht_scene_run_physics(bvht_ctx ctx, bvht_tlas tlas, float elapsed);
 
 /* ---- hipGraph capture of enqueued work ---------------------------------- */
 

@@ -24,7 +24,17 @@ INSTANCE_DTYPE = np.dtype([("minv", "<f4", 16), ("blas", "<u4"), ("pad", "<u4", 
 STATS_DTYPE = np.dtype([("pairs", "<u4"), ("tests", "<u4"), ("tlas_pairs", "<u4"), ("inst_visits", "<u4")])
 CAMERA_DTYPE = np.dtype([("tl", "<f4", 3), ("tr", "<f4", 3), ("bl", "<f4", 3), ("view_inv", "<f4", 16)])
 
+BODY_DTYPE = np.dtype([
+    ("inverse_mass", "<f4"), ("inverse_inertia_tensor", "<f4", 9), ("linear_drag", "<f4"), ("angular_drag", "<f4"),
+    ("position", "<f4", 3), ("orientation", "<f4", 4), ("velocity", "<f4", 3), ("rotation", "<f4", 3),
+    ("inverse_inertia_tensor_world", "<f4", 9), ("motion", "<f4"), ("is_awake", "<u4"), ("can_sleep", "<u4"),
+    ("transform", "<f4", 16), ("force_accumulator", "<f4", 3), ("torque_accumulator", "<f4", 3),
+    ("acceleration", "<f4", 3), ("last_frame_acceleration", "<f4", 3), ("pad", "<f4", 3)])
+FORCE_DTYPE = np.dtype([("kind", "<u4"), ("body", "<u4"), ("other", "<u4"), ("pad", "<u4"), ("v0", "<f4", 3),
+                        ("v1", "<f4", 3), ("spring_constant", "<f4"), ("rest_length", "<f4")])
+
 assert NODE_DTYPE.itemsize == 32 and TNODE_DTYPE.itemsize == 32
+assert BODY_DTYPE.itemsize == 272 and FORCE_DTYPE.itemsize == 48
 assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 24 and INSTANCE_DTYPE.itemsize == 80
 
 _lib = None
@@ -72,6 +82,10 @@ def lib():
             "orc_diff_forward": (i32, [P, P, P, P, P, P, P, u64, P, P, i32]),
             "orc_diff_grad64": (i32, [P, u32, P, u32, P, P, P, P, P, u64, P, ctypes.c_double, ctypes.c_double,
                                       P, P, P, P]),
+            "orc_powf": (ctypes.c_float, [ctypes.c_float, ctypes.c_float]),
+            "orc_world_start_frame": (None, [P, u32]),
+            "orc_world_run_physics": (None, [P, u32, P, u32, ctypes.c_float]),
+            "orc_objects_from_bodies": (None, [P, P, P, u32, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -312,3 +326,36 @@ def diff_grad64(V, vid, N, minv, light, rays, prims, target, w_depth=1.0, w_shad
     if rc != 0:
         raise ValueError(f"orc_diff_grad64 failed rc={rc}")
     return float(loss[0]), gv, gm, image
+
+
+def powf(x, y):
+    """f32::powf as the integrator evaluates it (binary64 restatement)."""
+    return np.float32(lib().orc_powf(float(np.float32(x)), float(np.float32(y))))
+
+
+def world_start_frame(bodies):
+    """World::start_frame (physics/world.rs:44-49), in place."""
+    assert bodies.dtype == BODY_DTYPE and bodies.flags.c_contiguous
+    lib().orc_world_start_frame(_p(bodies), bodies.shape[0])
+    return bodies
+
+
+def world_run_physics(bodies, forces, duration):
+    """World::run_physics (physics/world.rs:57-60), in place."""
+    assert bodies.dtype == BODY_DTYPE and bodies.flags.c_contiguous
+    forces = np.ascontiguousarray(forces, FORCE_DTYPE)
+    lib().orc_world_run_physics(_p(bodies), bodies.shape[0], _p(forces) if forces.shape[0] else None,
+                                forces.shape[0], float(np.float32(duration)))
+    return bodies
+
+
+def objects_from_bodies(bodies, object_body, init):
+    """SceneObject::update_transform for each object: (M, M^-1) as (n, 16)
+    column-major float32; rows of objects without a body stay zero."""
+    object_body = np.ascontiguousarray(object_body, np.uint32)
+    init = np.ascontiguousarray(init, np.float32).reshape(-1, 16)
+    n = object_body.shape[0]
+    m = np.zeros((n, 16), np.float32)
+    minv = np.zeros((n, 16), np.float32)
+    lib().orc_objects_from_bodies(_p(bodies), _p(object_body), _p(init), n, _p(m), _p(minv))
+    return m, minv

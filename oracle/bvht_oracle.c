@@ -1312,3 +1312,289 @@ int orc_diff_grad64(const double* V, uint32_t nv, const uint32_t* vid, uint32_t 
     *loss = L;
     return ORC_OK;
 }
+
+/* ========================================================================
+ * Physics: World / ForceRegistry / RigidBody (bvhtracer/src/physics/) and
+ * SceneObject::update_transform.  Binary32 in the reference's statement
+ * order; cglinalg's quaternion product, quaternion norm s*s + dot(v, v),
+ * 3x3 and 4x4 products are written left to right (cglinalg is not vendored).
+ * ======================================================================== */
+
+static const double ORC_LN2_HI = 6.93147180369123816490e-01;
+static const double ORC_LN2_LO = 1.90821492927058770002e-10;
+
+static uint64_t d2u(double x) { uint64_t u; memcpy(&u, &x, 8); return u; }
+static double u2d(uint64_t u) { double x; memcpy(&x, &u, 8); return x; }
+
+/* ln x for x > 0: x = m 2^e, m in [sqrt(1/2), sqrt(2)),
+ * ln m = 2 atanh(s) = 2 (s + s^3/3 + ...), s = (m-1)/(m+1). */
+static double orc_ln_pos(double x)
+{
+    const double ln2 = ORC_LN2_HI + ORC_LN2_LO;
+    uint64_t b = d2u(x);
+    int e = (int)((b >> 52) & 0x7FF);
+    if (e == 0) {
+        x *= 18014398509481984.0; /* 2^54 */
+        b = d2u(x);
+        e = (int)((b >> 52) & 0x7FF) - 54;
+    }
+    e -= 1022;
+    double m = u2d((b & 0x000FFFFFFFFFFFFFull) | 0x3FE0000000000000ull);
+    if (m < 0.70710678118654752440) {
+        m = m * 2.0;
+        e -= 1;
+    }
+    double s = (m - 1.0) / (m + 1.0);
+    double s2 = s * s, term = s, sum = s;
+    for (int k = 3; k <= 41; k += 2) {
+        term = term * s2;
+        sum = sum + term / (double)k;
+    }
+    return (double)e * ln2 + 2.0 * sum;
+}
+
+/* exp z, |z| < 700: z = k ln2 + r (Cody-Waite), Taylor series of exp r,
+ * scaled by 2^k through the exponent field. */
+static double orc_exp_small(double z)
+{
+    const double ln2 = ORC_LN2_HI + ORC_LN2_LO;
+    double kf = z / ln2;
+    double k = kf < 0.0 ? -(double)(long long)(0.5 - kf) : (double)(long long)(kf + 0.5);
+    double r = (z - k * ORC_LN2_HI) - k * ORC_LN2_LO;
+    double p = 1.0;
+    for (int i = 24; i >= 1; --i)
+        p = 1.0 + (r / (double)i) * p;
+    long long ki = (long long)k, k1 = ki / 2, k2 = ki - k1;
+    return (p * u2d((uint64_t)(k1 + 1023) << 52)) * u2d((uint64_t)(k2 + 1023) << 52);
+}
+
+/* f32::powf (rigid_body.rs:160-161,190): x >= 0; x < 0 gives NaN. */
+float orc_powf(float x, float y)
+{
+    if (y == 0.0f || x == 1.0f)
+        return 1.0f;
+    if (isnan(x) || isnan(y) || x < 0.0f)
+        return NAN;
+    if (x == 0.0f)
+        return y > 0.0f ? 0.0f : INFINITY;
+    if (isinf(x))
+        return y > 0.0f ? x : 0.0f;
+    if (isinf(y))
+        return ((x > 1.0f) == (y > 0.0f)) ? INFINITY : 0.0f;
+    double z = (double)y * orc_ln_pos((double)x);
+    if (z > 700.0)
+        return INFINITY;
+    if (z < -800.0)
+        return 0.0f;
+    return (float)orc_exp_small(z);
+}
+
+/* Transform3::transform_point (transform.rs:219-223) */
+static void orc_body_point(const float m[16], const float p[3], float out[3])
+{
+    for (int r = 0; r < 3; ++r)
+        out[r] = m[0 + r] * p[0] + m[4 + r] * p[1] + m[8 + r] * p[2] + m[12 + r] * 1.0f;
+}
+
+/* RigidBody::calculate_derived_data (rigid_body.rs:120-135) with
+ * Transform3::from_translation_rotation (transform.rs:190-217) and
+ * _transform_inertia_tensor (rigid_body.rs:28-64). */
+static void orc_derived_data(orc_body* b)
+{
+    float* q = b->orientation;
+    float norm = sqrtf(q[0] * q[0] + ((q[1] * q[1] + q[2] * q[2]) + q[3] * q[3]));
+    for (int k = 0; k < 4; ++k)
+        q[k] = q[k] / norm;
+    float s = q[0], x = q[1], y = q[2], z = q[3];
+    float one = 1.0f, two = one + one;
+    float* t = b->transform; /* t[c*4+r] = matrix[c][r] */
+    t[0] = one - two * y * y - two * z * z;
+    t[1] = two * x * y + two * s * z;
+    t[2] = two * x * z - two * s * y;
+    t[3] = 0.0f;
+    t[4] = two * x * y - two * s * z;
+    t[5] = one - two * x * x - two * z * z;
+    t[6] = two * y * z + two * s * x;
+    t[7] = 0.0f;
+    t[8] = two * x * z + two * s * y;
+    t[9] = two * y * z - two * s * x;
+    t[10] = one - two * x * x - two * y * y;
+    t[11] = 0.0f;
+    t[12] = b->position[0];
+    t[13] = b->position[1];
+    t[14] = b->position[2];
+    t[15] = one;
+    const float* ib = b->inverse_inertia_tensor; /* ib[c*3+r] = iit_body[c][r] */
+    float mm[3][3]; /* mm[c][r] = m{r}{c} of rigid_body.rs:34-48 ("m" + row + col) */
+    for (int c = 0; c < 3; ++c)     /* rot_mat row c */
+        for (int r = 0; r < 3; ++r) /* iit_body column r */
+            mm[c][r] = t[0 * 4 + c] * ib[r * 3 + 0] + t[1 * 4 + c] * ib[r * 3 + 1] + t[2 * 4 + c] * ib[r * 3 + 2];
+    /* iit_world[j][i] = m0i*rot[0][j] + m1i*rot[1][j] + m2i*rot[2][j] (rigid_body.rs:50-63) */
+    float* w = b->inverse_inertia_tensor_world;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            w[j * 3 + i] = mm[i][0] * t[0 * 4 + j] + mm[i][1] * t[1 * 4 + j] + mm[i][2] * t[2 * 4 + j];
+}
+
+/* RigidBody::integrate (rigid_body.rs:138-203). */
+static void orc_integrate(orc_body* b, float duration)
+{
+    if (!b->is_awake)
+        return;
+    for (int k = 0; k < 3; ++k) {
+        b->last_frame_acceleration[k] = b->acceleration[k];
+        b->last_frame_acceleration[k] += b->force_accumulator[k] * b->inverse_mass;
+    }
+    const float* w = b->inverse_inertia_tensor_world;
+    float ang[3];
+    for (int r = 0; r < 3; ++r)
+        ang[r] = w[0 + r] * b->torque_accumulator[0] + w[3 + r] * b->torque_accumulator[1] +
+                 w[6 + r] * b->torque_accumulator[2];
+    for (int k = 0; k < 3; ++k)
+        b->velocity[k] += b->last_frame_acceleration[k] * duration;
+    for (int k = 0; k < 3; ++k)
+        b->rotation[k] += ang[k] * duration;
+    float ld = orc_powf(b->linear_drag, duration), ad = orc_powf(b->angular_drag, duration);
+    for (int k = 0; k < 3; ++k)
+        b->velocity[k] *= ld;
+    for (int k = 0; k < 3; ++k)
+        b->rotation[k] *= ad;
+    for (int k = 0; k < 3; ++k)
+        b->position[k] += b->velocity[k] * duration;
+    /* q = Quaternion::from_pure(rotation * duration) * orientation */
+    float a[4] = {0.0f, b->rotation[0] * duration, b->rotation[1] * duration, b->rotation[2] * duration};
+    float* o = b->orientation;
+    float q[4];
+    q[0] = a[0] * o[0] - a[1] * o[1] - a[2] * o[2] - a[3] * o[3];
+    q[1] = a[0] * o[1] + a[1] * o[0] + a[2] * o[3] - a[3] * o[2];
+    q[2] = a[0] * o[2] + a[2] * o[0] + a[3] * o[1] - a[1] * o[3];
+    q[3] = a[0] * o[3] + a[3] * o[0] + a[1] * o[2] - a[2] * o[1];
+    float half = 0.5f;
+    for (int k = 0; k < 4; ++k)
+        o[k] += q[k] * half;
+    orc_derived_data(b);
+    for (int k = 0; k < 3; ++k)
+        b->force_accumulator[k] = b->torque_accumulator[k] = 0.0f;
+    if (b->can_sleep) {
+        const float *v = b->velocity, *r = b->rotation;
+        float motion_now = ((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]) + ((r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]);
+        float bias = orc_powf(half, duration);
+        b->motion = bias * b->motion + (1.0f - bias) * motion_now;
+        float threshold = 0.3f, ten = 10.0f;
+        if (b->motion < threshold) {
+            b->is_awake = 0;
+            for (int k = 0; k < 3; ++k)
+                b->velocity[k] = b->rotation[k] = 0.0f;
+        } else if (b->motion > ten * threshold) {
+            b->motion = ten * threshold;
+        }
+    }
+}
+
+void orc_world_start_frame(orc_body* bodies, uint32_t n)
+{
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k)
+            bodies[i].force_accumulator[k] = bodies[i].torque_accumulator[k] = 0.0f;
+        orc_derived_data(&bodies[i]);
+    }
+}
+
+void orc_world_run_physics(orc_body* bodies, uint32_t n, const orc_force* forces, uint32_t nf, float duration)
+{
+    /* ForceRegistry::apply_forces (force_registry.rs:72-77) */
+    for (uint32_t j = 0; j < nf; ++j) {
+        const orc_force* f = &forces[j];
+        orc_body* b = &bodies[f->body];
+        if (f->kind == 1) { /* Gravity::apply_force (force_generator.rs:50-55) */
+            if (b->inverse_mass >= 0.0f) {
+                float mass = b->inverse_mass == 0.0f ? FLT_MAX : 1.0f / b->inverse_mass;
+                for (int k = 0; k < 3; ++k)
+                    b->force_accumulator[k] += f->v0[k] * mass;
+                b->is_awake = 1;
+            }
+        } else if (f->kind == 2) { /* Spring::apply_force (force_generator.rs:101-120) */
+            float lws[3], ows[3], d[3], force[3], pcm[3];
+            orc_body_point(b->transform, f->v0, lws);
+            orc_body_point(bodies[f->other].transform, f->v1, ows);
+            for (int k = 0; k < 3; ++k)
+                d[k] = lws[k] - ows[k];
+            float len = sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+            float magnitude = f->spring_constant * fabsf(len - f->rest_length);
+            for (int k = 0; k < 3; ++k)
+                force[k] = (d[k] / len) * (-magnitude);
+            /* RigidBody::apply_force_at_point (rigid_body.rs:434-439) */
+            for (int k = 0; k < 3; ++k)
+                pcm[k] = lws[k] - b->position[k];
+            for (int k = 0; k < 3; ++k)
+                b->force_accumulator[k] += force[k];
+            b->torque_accumulator[0] += pcm[1] * force[2] - pcm[2] * force[1];
+            b->torque_accumulator[1] += pcm[2] * force[0] - pcm[0] * force[2];
+            b->torque_accumulator[2] += pcm[0] * force[1] - pcm[1] * force[0];
+            b->is_awake = 1;
+        }
+    }
+    for (uint32_t i = 0; i < n; ++i) /* World::integrate (world.rs:51-55) */
+        orc_integrate(&bodies[i], duration);
+}
+
+/* 4x4 inverse by the adjugate (cofactor expansion), binary64. */
+static void orc_inverse4(const float* mf, float* out)
+{
+    double m[16], inv[16];
+    for (int k = 0; k < 16; ++k)
+        m[k] = (double)mf[k];
+    inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] +
+             m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+    inv[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] -
+             m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+    inv[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] +
+             m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+    inv[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] -
+              m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+    inv[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] -
+             m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+    inv[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] +
+             m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+    inv[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] -
+             m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+    inv[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] +
+              m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+    inv[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] +
+             m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+    inv[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] -
+             m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+    inv[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] +
+              m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+    inv[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] -
+              m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+    inv[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] -
+             m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+    inv[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] +
+             m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+    inv[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] -
+              m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+    inv[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] +
+              m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+    double det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+    for (int k = 0; k < 16; ++k)
+        out[k] = (float)(inv[k] / det);
+}
+
+void orc_objects_from_bodies(const orc_body* bodies, const uint32_t* object_body, const float* init, uint32_t n,
+                             float* m_out, float* minv_out)
+{
+    for (uint32_t i = 0; i < n; ++i) {
+        if (object_body[i] == 0xFFFFFFFFu)
+            continue;
+        const float* A = bodies[object_body[i]].transform;
+        const float* B = init + 16 * (size_t)i;
+        float* M = m_out + 16 * (size_t)i;
+        /* Matrix4x4 product: (A B)[c][r] = sum_k A[k][r] B[c][k] */
+        for (int c = 0; c < 4; ++c)
+            for (int r = 0; r < 4; ++r)
+                M[c * 4 + r] = A[0 + r] * B[c * 4 + 0] + A[4 + r] * B[c * 4 + 1] + A[8 + r] * B[c * 4 + 2] +
+                               A[12 + r] * B[c * 4 + 3];
+        orc_inverse4(M, minv_out + 16 * (size_t)i);
+    }
+}

@@ -147,6 +147,50 @@ int orc_diff_grad64(const double* V, uint32_t nv, const uint32_t* vid, uint32_t 
                     uint64_t n, const float* target, double w_depth, double w_shade, double* loss, double* grad_v,
                     double* grad_minv, double* image);
 
+/* ---- physics (bvhtracer/src/physics/) ------------------------------------ */
+/* RigidBody (rigid_body.rs:66-86): field for field, binary32; matrices
+ * column-major, orientation (s, x, y, z).  Byte-identical to bvht_body. */
+typedef struct {
+    float inverse_mass;
+    float inverse_inertia_tensor[9];
+    float linear_drag, angular_drag;
+    float position[3];
+    float orientation[4];
+    float velocity[3];
+    float rotation[3];
+    float inverse_inertia_tensor_world[9];
+    float motion;
+    uint32_t is_awake, can_sleep;
+    float transform[16];
+    float force_accumulator[3];
+    float torque_accumulator[3];
+    float acceleration[3];
+    float last_frame_acceleration[3];
+    float pad[3];
+} orc_body;
+
+/* ForceRegistry entry (force_registry.rs:31-42): kind 0 NullForce, 1 Gravity
+ * (v0 = g), 2 Spring (v0 / v1 connection points on body / other, constant,
+ * rest length).  Byte-identical to bvht_force. */
+typedef struct {
+    uint32_t kind, body, other, pad;
+    float v0[3], v1[3];
+    float spring_constant, rest_length;
+} orc_force;
+
+/* f32::powf restated in binary64 (ln by the atanh series, exp by range
+ * reduction and Taylor series), rounded to binary32. */
+float orc_powf(float x, float y);
+/* World::start_frame (world.rs:44-49). */
+void orc_world_start_frame(orc_body* bodies, uint32_t n);
+/* World::run_physics (world.rs:57-60): the registry in order, then every
+ * body's integrate. */
+void orc_world_run_physics(orc_body* bodies, uint32_t n, const orc_force* forces, uint32_t nf, float duration);
+/* SceneObject::update_transform (scene_object.rs:52-58) for n objects:
+ * M = body transform x init, M^-1 by the binary64 adjugate, rounded. */
+void orc_objects_from_bodies(const orc_body* bodies, const uint32_t* object_body, const float* init, uint32_t n,
+                             float* m_out, float* minv_out);
+
 #ifdef __cplusplus
 }
 #endif

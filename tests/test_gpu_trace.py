@@ -217,3 +217,36 @@ def test_sorted_bounce_is_a_grouping_of_the_ray_order_bounce(ctx, orc, tlas):
     if not tlas:
         assert_hits_equal(bvh.intersect(srt), orc.bvh_trace(nodes, perm, srt.view(orc.RAY_DTYPE), nthreads=8),
                           "sorted bounce")
+
+
+@pytest.mark.parametrize("left_count", [63, 64, 65])
+def test_stack_entry_packing_boundary(ctx, orc, left_count):
+    """Two-leaf trees around the packed-stack limit (a pushed leaf of 63
+    triangles packs into its stack entry, 64 and more do not): the same hits
+    as the oracle, with finite and infinite ray lengths."""
+    rng = np.random.default_rng(left_count)
+    n = left_count + 40
+    tris = np.zeros((n, 9), f32)
+    # two overlapping clusters along z so that rays hit both leaf boxes
+    for lo, hi, z in ((0, left_count, 0.0), (left_count, n, 0.5)):
+        c = rng.uniform(-1, 1, size=(hi - lo, 1, 3)).astype(f32)
+        c[..., 2] = z + rng.uniform(-0.1, 0.1, size=(hi - lo, 1)).astype(f32)
+        tris[lo:hi] = (c + rng.uniform(-0.3, 0.3, size=(hi - lo, 3, 3)).astype(f32)).reshape(-1, 9)
+    nodes = np.zeros(4, orc.NODE_DTYPE)
+    nodes["left_first"][0], nodes["count"][0] = 2, 0
+    for k, (lo, hi) in ((2, (0, left_count)), (3, (left_count, n))):
+        v = tris[lo:hi].reshape(-1, 3)
+        nodes["bmin"][k], nodes["bmax"][k] = v.min(0), v.max(0)
+        nodes["left_first"][k], nodes["count"][k] = lo, hi - lo
+    v = tris.reshape(-1, 3)
+    nodes["bmin"][0], nodes["bmax"][0] = v.min(0), v.max(0)
+    bvh = upload(ctx, nodes, 4, tris)
+    m = 4096
+    r = np.zeros(m, B.RAY_DTYPE)
+    r["o"] = np.stack([rng.uniform(-1.2, 1.2, m), rng.uniform(-1.2, 1.2, m), np.full(m, -5.0)], 1).astype(f32)
+    r["d"] = np.stack([rng.uniform(-0.05, 0.05, m), rng.uniform(-0.05, 0.05, m), np.ones(m)], 1).astype(f32)
+    r["tmax"] = np.where(np.arange(m) % 3 == 0, f32(np.inf), np.finfo(np.float32).max)
+    ref = orc.bvh_trace(nodes, tris, r.view(orc.RAY_DTYPE))
+    got = bvh.intersect(r)
+    assert_hits_equal(got, ref, f"two-leaf tree, left leaf {left_count}")
+    assert (got["prim"] != B.MISS).sum() > m // 4
