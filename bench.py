@@ -64,7 +64,7 @@ def parse():
     p.add_argument("--batch-scenes", type=int, default=64)
     p.add_argument("--instanced-frames", type=int, default=64)
     p.add_argument("--adjoint-steps", type=int, default=100)
-    p.add_argument("--only", choices=["adjoint", "instanced", "batched", "meshes"], default=None,
+    p.add_argument("--only", choices=["adjoint", "instanced", "batched", "meshes", "physics"], default=None,
                    help="run one extra workload alone and print its JSON")
     p.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
                    help="only run the stream-copy kernel 5 times on BYTES (profiler counter calibration)")
@@ -114,7 +114,7 @@ def main():
         return
     if args.only:
         fn = {"adjoint": bench_adjoint, "instanced": bench_instanced, "batched": bench_batched,
-              "meshes": bench_meshes}[args.only]
+              "meshes": bench_meshes, "physics": bench_physics}[args.only]
         print(json.dumps({args.only: fn(args, B.Context(0))}), flush=True)
         return
 
@@ -293,7 +293,7 @@ def main():
                                               args.cpu_seconds)
     if not args.no_extras:
         for name, fn in (("instanced", bench_instanced), ("batched", bench_batched), ("adjoint", bench_adjoint),
-                         ("meshes", bench_meshes)):
+                         ("meshes", bench_meshes), ("physics", bench_physics)):
             try:
                 result[name] = fn(args, ctx)
             except Exception as e:  # a failing extra must not hide the main measurement
@@ -626,6 +626,75 @@ def bench_meshes(args, ctx):
             "bvh_rebuild_ms": {"gpu": round(rebuild_gpu, 3), "cpu_oracle_1_thread": round(rebuild_cpu, 3)},
             "bvh_refit_ms": {"gpu": round(refit_gpu, 4), "cpu_oracle_1_thread": round(refit_cpu, 4)},
             "build_identical_to_oracle": bool(same_build), "refit_identical_to_oracle": bool(same_refit)}
+
+
+def bench_physics(args, ctx, n=1 << 20, steps=50):
+    """World::run_physics (physics/world.rs:57-60) on n rigid bodies: every
+    body under Gravity and a Spring to its neighbour, awake, half of them
+    allowed to sleep; dt = 1/60.  GPU time from HIP events over `steps`
+    steps; the oracle (one host thread) on the first 65,536 bodies and their
+    forces beside it.  Bytes per body per step: 272 B record read twice
+    (forces, integrate) and written once, 2 x 48 B force entries, the
+    neighbour's 64 B transform, 28 B of accumulators written."""
+    import ctypes
+
+    import oracle
+
+    from bvhtracer_amd import _lib as L
+
+    rng = np.random.default_rng(0xF00D)
+    bodies = np.zeros(n, L.BODY_DTYPE)
+    bodies["inverse_mass"] = (1.0 / rng.uniform(1, 5, n)).astype(np.float32)
+    eye3 = np.eye(3, dtype=np.float32).reshape(9)
+    bodies["inverse_inertia_tensor"] = eye3
+    bodies["inverse_inertia_tensor_world"] = eye3
+    bodies["linear_drag"] = rng.uniform(0.95, 1.0, n).astype(np.float32)
+    bodies["angular_drag"] = 1.0
+    bodies["position"] = rng.uniform(-100, 100, (n, 3)).astype(np.float32)
+    q = rng.normal(size=(n, 4)).astype(np.float32)
+    bodies["orientation"] = q / np.linalg.norm(q, axis=1, keepdims=True).astype(np.float32)
+    bodies["rotation"] = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    bodies["is_awake"] = 1
+    bodies["motion"] = np.float32(0.6)
+    bodies["can_sleep"] = np.arange(n) % 2
+    bodies["transform"] = np.eye(4, dtype=np.float32).reshape(16)
+    forces = np.zeros(2 * n, L.FORCE_DTYPE)
+    forces["kind"][0::2], forces["body"][0::2] = L.FORCE_GRAVITY, np.arange(n)
+    forces["v0"][0::2] = (0, -9.81, 0)
+    forces["kind"][1::2], forces["body"][1::2], forces["other"][1::2] = L.FORCE_SPRING, np.arange(n), (np.arange(n) + 1) % n
+    forces["v0"][1::2], forces["v1"][1::2] = (0.1, 0, 0), (0, 0.1, 0)
+    forces["spring_constant"][1::2], forces["rest_length"][1::2] = 0.2, 1.0
+    h = ctypes.c_void_p()
+    L.check(L.lib().bvht_world_create(ctx.handle, L.ptr(bodies), n, L.ptr(forces), 2 * n, ctypes.byref(h)))
+    try:
+        L.check(L.lib().bvht_world_start_frame(ctx.handle, h))
+        dt = np.float32(1 / 60)
+        for _ in range(3):
+            L.check(L.lib().bvht_world_run_physics(ctx.handle, h, dt))
+        ctx.sync()
+        e0, e1 = ctx.event(), ctx.event()
+        ctx.record(e0)
+        for _ in range(steps):
+            L.check(L.lib().bvht_world_run_physics(ctx.handle, h, dt))
+        ctx.record(e1)
+        ms = ctx.elapsed_ms(e0, e1) / steps
+    finally:
+        L.lib().bvht_world_destroy(ctx.handle, h)
+    m = 1 << 16
+    sub = bodies[:m].view(oracle.BODY_DTYPE).copy()
+    fsub = forces[: 2 * m].copy()
+    fsub["other"] = np.where(fsub["kind"] == L.FORCE_SPRING, fsub["other"] % m, fsub["other"])
+    oracle.world_start_frame(sub)
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        oracle.world_run_physics(sub, fsub, dt)
+    cpu_s = (time.perf_counter() - t0) / reps
+    bytes_per_body = 3 * 272 + 2 * 48 + 64 + 28
+    return {"workload": f"{n} rigid bodies, Gravity + Spring each, dt 1/60 (World::run_physics)",
+            "ms_per_step": round(ms, 4), "bodies_per_s": round(n / (ms * 1e-3), 1),
+            "achieved_gbs": round(n * bytes_per_body / (ms * 1e-3) / 1e9, 1), "bytes_per_body": bytes_per_body,
+            "cpu_oracle_1_thread_bodies_per_s": round(m / cpu_s, 1)}
 
 
 def cpu_baseline(ctx, bvh, d_brays, d_bhits, n_bounce, d_rays, d_hits, n_primary, budget_s):

@@ -248,48 +248,54 @@ __device__ void integrate(bvht_body& b, float duration)
     }
 }
 
+// The part of a body the force generators read and write.
+struct ForceState {
+    float inverse_mass;
+    float position[3];
+    float transform[16];
+    float force[3];
+    float torque[3];
+    uint32_t awake;
+};
+
 // Gravity::apply_force (force_generator.rs:50-55) and RigidBody::apply_force
 // (rigid_body.rs:427-431).
-__device__ void apply_gravity(bvht_body& b, const bvht_force& f)
+__device__ void apply_gravity(ForceState& b, const bvht_force& f)
 {
     if (!(b.inverse_mass >= 0.0f))
         return;
     float mass = b.inverse_mass == 0.0f ? FLT_MAX : 1.0f / b.inverse_mass;
     for (int k = 0; k < 3; ++k)
-        b.force_accumulator[k] = b.force_accumulator[k] + f.v0[k] * mass;
-    b.is_awake = 1u;
+        b.force[k] = b.force[k] + f.v0[k] * mass;
+    b.awake = 1u;
 }
 
 // Spring::apply_force (force_generator.rs:101-120) with
-// RigidBody::apply_force_at_point (rigid_body.rs:434-439).
-// The other body's transform is read as the previous integration left it (a
-// spring from a body to itself makes the reference's RefCell borrow panic;
-// here it reads the body's own transform).
-__device__ void apply_spring(bvht_body& b, const float* __restrict__ other_transform, const bvht_force& f)
+// RigidBody::apply_force_at_point (rigid_body.rs:434-439).  The other body's
+// transform is read as the previous integration left it (a spring from a
+// body to itself makes the reference's RefCell borrow panic; here it reads
+// the body's own transform).
+__device__ void apply_spring(ForceState& b, const float* ot, const bvht_force& f)
 {
     float lws[3], ows[3], d[3];
     transform_point(b.transform, f.v0, lws);
-    float ot[16];
-    for (int k = 0; k < 16; ++k)
-        ot[k] = other_transform[k];
     transform_point(ot, f.v1, ows);
     for (int k = 0; k < 3; ++k)
         d[k] = lws[k] - ows[k];
     float len = sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
     float force_magnitude = f.spring_constant * fabsf(len - f.rest_length);
-    float nd = sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
     float force[3];
     for (int k = 0; k < 3; ++k)
-        force[k] = (d[k] / nd) * (-force_magnitude);
+        force[k] = (d[k] / len) * (-force_magnitude);
     float pcm[3];
     for (int k = 0; k < 3; ++k)
         pcm[k] = lws[k] - b.position[k];
     for (int k = 0; k < 3; ++k)
-        b.force_accumulator[k] = b.force_accumulator[k] + force[k];
-    b.torque_accumulator[0] = b.torque_accumulator[0] + (pcm[1] * force[2] - pcm[2] * force[1]);
-    b.torque_accumulator[1] = b.torque_accumulator[1] + (pcm[2] * force[0] - pcm[0] * force[2]);
-    b.torque_accumulator[2] = b.torque_accumulator[2] + (pcm[0] * force[1] - pcm[1] * force[0]);
-    b.is_awake = 1u;
+        b.force[k] = b.force[k] + force[k];
+    b.torque[0] = b.torque[0] + (pcm[1] * force[2] - pcm[2] * force[1]);
+    b.torque[1] = b.torque[1] + (pcm[2] * force[0] - pcm[0] * force[2]);
+    b.torque[2] = b.torque[2] + (pcm[0] * force[1] - pcm[1] * force[0]);
+    b.awake = 1u;
 }
 
 __global__ void k_apply_forces(bvht_body* __restrict__ bodies, const bvht_force* __restrict__ forces,
@@ -301,20 +307,35 @@ __global__ void k_apply_forces(bvht_body* __restrict__ bodies, const bvht_force*
     uint32_t a = off[i], e = off[i + 1];
     if (a == e)
         return;
-    bvht_body b = bodies[i];
+    bvht_body* bi = bodies + i;
+    ForceState b;
+    b.inverse_mass = bi->inverse_mass;
+    for (int k = 0; k < 3; ++k) {
+        b.position[k] = bi->position[k];
+        b.force[k] = bi->force_accumulator[k];
+        b.torque[k] = bi->torque_accumulator[k];
+    }
+    for (int k = 0; k < 16; ++k)
+        b.transform[k] = bi->transform[k];
+    b.awake = bi->is_awake;
     for (uint32_t j = a; j < e; ++j) {
         const bvht_force f = forces[j];
-        if (f.kind == BVHT_FORCE_GRAVITY)
+        if (f.kind == BVHT_FORCE_GRAVITY) {
             apply_gravity(b, f);
-        else if (f.kind == BVHT_FORCE_SPRING)
-            apply_spring(b, f.other == i ? b.transform : bodies[f.other].transform, f);
+        } else if (f.kind == BVHT_FORCE_SPRING) {
+            float ot[16];
+            const float* src = f.other == i ? b.transform : bodies[f.other].transform;
+            for (int k = 0; k < 16; ++k)
+                ot[k] = src[k];
+            apply_spring(b, ot, f);
+        }
     }
     // only the accumulators and the awake flag change
     for (int k = 0; k < 3; ++k) {
-        bodies[i].force_accumulator[k] = b.force_accumulator[k];
-        bodies[i].torque_accumulator[k] = b.torque_accumulator[k];
+        bi->force_accumulator[k] = b.force[k];
+        bi->torque_accumulator[k] = b.torque[k];
     }
-    bodies[i].is_awake = b.is_awake;
+    bi->is_awake = b.awake;
 }
 
 __global__ void k_integrate(bvht_body* __restrict__ bodies, uint32_t n_bodies, float duration)
