@@ -222,7 +222,10 @@ __global__ __launch_bounds__(WG) void k_trace_blas(const float4* __restrict__ no
     Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap};
     DRay ray = load_ray(rays, i);
     uint32_t pairs = 0, tests = 0;
-    BlasResult r = traverse_blas<STATS, PACKED>(nodes, tris, ray, st, 0u, pairs, tests);
+    // a wave whose rays all have t <= FLT_MAX (or NaN) takes the reduced child choice
+    BlasResult r = __builtin_amdgcn_ballot_w64(ray.t > kFltMax) == 0
+                       ? traverse_blas<STATS, PACKED, true>(nodes, tris, ray, st, 0u, pairs, tests)
+                       : traverse_blas<STATS, PACKED, false>(nodes, tris, ray, st, 0u, pairs, tests);
     store_hit(hits, i, r.found, r.t, r.u, r.v, r.prim, 0);
     if (STATS) {
         bvht_stats s;

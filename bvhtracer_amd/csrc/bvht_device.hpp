@@ -78,6 +78,28 @@ __device__ __forceinline__ bool slab(float4 lo, float4 hi, const DRay& ray, floa
     return (tmax >= tmin) && (tmin < t_closest) && (tmax > 0.0f);
 }
 
+// The slab test's distance key for a ray whose t is not above FLT_MAX: tmin
+// when the box is hit regardless of t (tmax >= tmin, tmax > 0), else
+// FLT_MAX.  With t_closest <= FLT_MAX (or NaN) the box passes slab() exactly
+// when its key is below t_closest, and between two children the key orders
+// a hit child before a missed one as slab()'s tmin-or-FLT_MAX does.
+__device__ __forceinline__ float slab_key(float4 lo, float4 hi, const DRay& ray)
+{
+    float tx1 = (lo.x - ray.o.x) * ray.r.x;
+    float tx2 = (hi.x - ray.o.x) * ray.r.x;
+    float tmin = fminf(tx1, tx2);
+    float tmax = fmaxf(tx1, tx2);
+    float ty1 = (lo.y - ray.o.y) * ray.r.y;
+    float ty2 = (hi.y - ray.o.y) * ray.r.y;
+    tmin = fmaxf(tmin, fminf(ty1, ty2));
+    tmax = fminf(tmax, fmaxf(ty1, ty2));
+    float tz1 = (lo.z - ray.o.z) * ray.r.z;
+    float tz2 = (hi.z - ray.o.z) * ray.r.z;
+    tmin = fmaxf(tmin, fminf(tz1, tz2));
+    tmax = fminf(tmax, fmaxf(tz1, tz2));
+    return (tmax >= tmin && tmax > 0.0f) ? tmin : kFltMax;
+}
+
 // Triangle::intersect, Moeller-Trumbore with the absolute 1e-4 thresholds
 // (geometry/triangle.rs:41-72).  Returns t = min(ray.t, t).
 __device__ __forceinline__ bool tri_test(const float* __restrict__ tri, const DRay& ray, float& t_out,

@@ -86,7 +86,10 @@ constexpr uint32_t kPopNext = 0xFFFFFFFFu;
 // slots at or above sp_base and leaves the stack as it found it.  PACKED:
 // every node of the tree fits a packed stack entry (no leaf of 64 or more
 // triangles, left_first below 2^25), so entries are packed without a check.
-template <bool STATS, bool PACKED, class StackT>
+// FINITE: the ray's t is not above FLT_MAX, so every distance below the
+// current nearest hit is below FLT_MAX and the child choice reduces to
+// comparisons of slab_key values (same decisions as the general form).
+template <bool STATS, bool PACKED, bool FINITE = false, class StackT>
 __device__ __forceinline__ BlasResult traverse_blas(const float4* __restrict__ nodes,
                                                     const float* __restrict__ tris, const DRay& ray,
                                                     StackT& st, uint32_t sp_base, uint32_t& pairs,
@@ -110,15 +113,24 @@ __device__ __forceinline__ BlasResult traverse_blas(const float4* __restrict__ n
                 ++pairs;
             const float4* c = nodes + 2 * (size_t)lf;
             float4 a0 = c[0], a1 = c[1], b0 = c[2], b1 = c[3];
-            float dl, dr;
-            bool hl = slab(a0, a1, ray, closest, dl);
-            bool hr = slab(b0, b1, ray, closest, dr);
-            float kl = hl ? dl : kFltMax;
-            float kr = hr ? dr : kFltMax;
-            bool left_near = kl < kr;
-            bool near_hit = left_near ? hl : hr;
+            bool left_near, near_hit, far_hit;
+            if (FINITE) {
+                float kl = slab_key(a0, a1, ray);
+                float kr = slab_key(b0, b1, ray);
+                left_near = kl < kr;
+                near_hit = fminf(kl, kr) < closest;
+                far_hit = fmaxf(kl, kr) < closest;
+            } else {
+                float dl, dr;
+                bool hl = slab(a0, a1, ray, closest, dl);
+                bool hr = slab(b0, b1, ray, closest, dr);
+                float kl = hl ? dl : kFltMax;
+                float kr = hr ? dr : kFltMax;
+                left_near = kl < kr;
+                near_hit = left_near ? hl : hr;
+                far_hit = left_near ? hr : hl;
+            }
             if (near_hit) {
-                bool far_hit = left_near ? hr : hl;
                 if (far_hit) {
                     uint32_t flf = __float_as_uint(left_near ? b0.w : a0.w);
                     uint32_t fcnt = __float_as_uint(left_near ? b1.w : a1.w);
