@@ -133,7 +133,12 @@ __global__ __launch_bounds__(WG) void k_batch_trace(const BatchScene* __restrict
         for (uint32_t r = base + lane; r < end; r += 64) {
             DRay ray = load_ray(rays, r);
             uint32_t pairs = 0, tests = 0, tpairs = 0, visits = 0;
-            TlasResult res = traverse_tlas<false>(sc.tnodes, sc.rows, sc.blases, ray, st, pairs, tests, tpairs, visits);
+            TlasResult res =
+                __builtin_amdgcn_ballot_w64(ray.t > kFltMax) == 0
+                    ? traverse_tlas<false, false, true>(sc.tnodes, sc.rows, sc.blases, ray, st, pairs, tests, tpairs,
+                                                        visits)
+                    : traverse_tlas<false, false, false>(sc.tnodes, sc.rows, sc.blases, ray, st, pairs, tests, tpairs,
+                                                         visits);
             store_hit(hits, r, res.found, res.t, res.u, res.v, res.prim, res.inst);
         }
     }

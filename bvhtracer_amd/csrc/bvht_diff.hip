@@ -160,7 +160,7 @@ __global__ void k_sgd(float* __restrict__ x, const float* __restrict__ g, const 
 
 // ------------------------------------------------------------------ forward
 
-template <int WG, int LDS_DEPTH>
+template <int WG, int LDS_DEPTH, bool PACKED>
 __global__ __launch_bounds__(WG) void k_trace_forward(const float4* __restrict__ nodes, const float* __restrict__ tris,
                                                       const uint32_t* __restrict__ vid,
                                                       const float* __restrict__ normals, Rows rows, V3 light,
@@ -179,7 +179,9 @@ __global__ __launch_bounds__(WG) void k_trace_forward(const float4* __restrict__
     V3 d = rows_mul(rows.r0, rows.r1, rows.r2, mk3(b.x, b.y, b.z), 0.0f);
     DRay ray = make_ray(o, d, a.w);
     uint32_t pairs = 0, tests = 0;
-    BlasResult r = traverse_blas<false, false>(nodes, tris, ray, st, 0u, pairs, tests);
+    BlasResult r = __builtin_amdgcn_ballot_w64(ray.t > kFltMax) == 0
+                       ? traverse_blas<false, PACKED, true>(nodes, tris, ray, st, 0u, pairs, tests)
+                       : traverse_blas<false, PACKED, false>(nodes, tris, ray, st, 0u, pairs, tests);
     store_hit(hits, i, r.found, r.t, r.u, r.v, r.prim, 0);
     float depth = kFltMax, shade = 0.0f;
     if (r.found) {
@@ -192,11 +194,11 @@ __global__ __launch_bounds__(WG) void k_trace_forward(const float4* __restrict__
 typedef void (*forward_fn)(const float4*, const float*, const uint32_t*, const float*, Rows, V3, const bvht_ray*,
                            uint32_t, bvht_hit*, float2*, uint32_t*, uint32_t);
 
-forward_fn pick_forward(int wg, int lds)
+forward_fn pick_forward(int wg, int lds, bool packed)
 {
-#define X(W, D)                      \
-    if (wg == W && lds == D)         \
-        return k_trace_forward<W, D>;
+#define X(W, D)                                                           \
+    if (wg == W && lds == D)                                              \
+        return packed ? k_trace_forward<W, D, true> : k_trace_forward<W, D, false>;
     BVHT_TRACE_CONFIGS(X)
 #undef X
     return nullptr;
@@ -577,7 +579,7 @@ extern "C" int32_t bvht_trace_forward(bvht_ctx ctx, bvht_mesh mesh, const float*
             return rc;
         spill = b->d_spill;
     }
-    forward_fn fn = pick_forward(wg, lds);
+    forward_fn fn = pick_forward(wg, lds, mesh->blas->packed_stack);
     if (!fn)
         return set_error(BVHT_EINVAL, "no forward kernel for workgroup %d, LDS depth %d", wg, lds);
     hipLaunchKernelGGL(fn, dim3(nblk(n, wg)), dim3(wg), 0, ctx->stream, reinterpret_cast<const float4*>(b->d_nodes),

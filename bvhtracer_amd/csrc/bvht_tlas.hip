@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kRebuildWG) void k_tlas_rebuild(const float* __rest
     tlas_rebuild_block(bounds, n, nodes, list, meta);
 }
 
-template <int WG, int LDS_DEPTH, bool STATS>
+template <int WG, int LDS_DEPTH, bool STATS, bool PACKED>
 __global__ __launch_bounds__(WG) void k_trace_tlas(const float4* __restrict__ tnodes, const ObjRows* __restrict__ rows,
                                                    const BlasRef* __restrict__ blases,
                                                    const bvht_ray* __restrict__ rays, bvht_hit* __restrict__ hits,
@@ -55,7 +55,11 @@ __global__ __launch_bounds__(WG) void k_trace_tlas(const float4* __restrict__ tn
     Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap};
     DRay ray = load_ray(rays, i);
     uint32_t pairs = 0, tests = 0, tpairs = 0, visits = 0;
-    TlasResult r = traverse_tlas<STATS>(tnodes, rows, blases, ray, st, pairs, tests, tpairs, visits);
+    // a wave whose rays all have t <= FLT_MAX (or NaN) takes the reduced child choice
+    TlasResult r = __builtin_amdgcn_ballot_w64(ray.t > kFltMax) == 0
+                       ? traverse_tlas<STATS, PACKED, true>(tnodes, rows, blases, ray, st, pairs, tests, tpairs, visits)
+                       : traverse_tlas<STATS, PACKED, false>(tnodes, rows, blases, ray, st, pairs, tests, tpairs,
+                                                             visits);
     store_hit(hits, i, r.found, r.t, r.u, r.v, r.prim, r.inst);
     if (STATS) {
         bvht_stats s;
@@ -70,11 +74,14 @@ __global__ __launch_bounds__(WG) void k_trace_tlas(const float4* __restrict__ tn
 typedef void (*trace_tlas_fn)(const float4*, const ObjRows*, const BlasRef*, const bvht_ray*, bvht_hit*, uint32_t,
                               const uint32_t*, bvht_stats*, uint32_t*, uint32_t);
 
-trace_tlas_fn pick_trace_tlas(int wg, int lds, bool stats)
+trace_tlas_fn pick_trace_tlas(int wg, int lds, bool stats, bool packed)
 {
-#define X(W, D)                                      \
-    if (wg == W && lds == D)                         \
-        return stats ? k_trace_tlas<W, D, true> : k_trace_tlas<W, D, false>;
+#define X(W, D)                                                                             \
+    if (wg == W && lds == D) {                                                              \
+        if (packed)                                                                         \
+            return stats ? k_trace_tlas<W, D, true, true> : k_trace_tlas<W, D, false, true>;  \
+        return stats ? k_trace_tlas<W, D, true, false> : k_trace_tlas<W, D, false, false>;    \
+    }
     BVHT_TRACE_CONFIGS(X)
 #undef X
     return nullptr;
@@ -188,7 +195,11 @@ int32_t launch_trace_tlas(bvht_tlas t, const bvht_ray* d_rays, uint32_t n, const
             return rc;
         spill = t->d_spill;
     }
-    trace_tlas_fn fn = pick_trace_tlas(wg, lds, d_stats != nullptr);
+    // packed stack entries when every BLAS the objects can enter fits them
+    bool packed = true;
+    for (bvht_blas b : t->blases)
+        packed = packed && b->packed_stack;
+    trace_tlas_fn fn = pick_trace_tlas(wg, lds, d_stats != nullptr, packed);
     if (!fn)
         return set_error(BVHT_EINVAL, "no TLAS trace kernel for workgroup %d, LDS depth %d", wg, lds);
     hipLaunchKernelGGL(fn, dim3(nblk(n, wg)), dim3(wg), 0, ctx->stream, reinterpret_cast<const float4*>(t->d_nodes),

@@ -233,7 +233,7 @@ struct TlasResult {
 // traversal; a visit's nearest hit is committed when its BLAS stack empties,
 // under the same conditions as the nested form (found, t < FLT_MAX, t below
 // the committed distance).
-template <bool STATS, class StackT>
+template <bool STATS, bool PACKED = false, bool FINITE = false, class StackT>
 __device__ __forceinline__ TlasResult traverse_tlas(const float4* __restrict__ tnodes, const ObjRows* __restrict__ objs,
                                                     const BlasRef* __restrict__ blases, const DRay& ray,
                                                     StackT& st, uint32_t& pairs, uint32_t& tests,
@@ -275,21 +275,32 @@ __device__ __forceinline__ TlasResult traverse_tlas(const float4* __restrict__ t
                 pr = tnodes + 2 * (size_t)(x & 0x0000FFFFu);
             }
             float4 a0 = pl[0], a1 = pl[1], b0 = pr[0], b1 = pr[1];
-            float dl, dr;
-            bool hl = slab(a0, a1, cr, cclosest, dl);
-            bool hr = slab(b0, b1, cr, cclosest, dr);
-            float kl = hl ? dl : kFltMax;
-            float kr = hr ? dr : kFltMax;
-            bool left_near = kl < kr;
-            bool near_hit = left_near ? hl : hr;
+            bool left_near, near_hit, far_hit;
+            if (FINITE) {
+                float kl = slab_key(a0, a1, cr);
+                float kr = slab_key(b0, b1, cr);
+                left_near = kl < kr;
+                near_hit = fminf(kl, kr) < cclosest;
+                far_hit = fmaxf(kl, kr) < cclosest;
+            } else {
+                float dl, dr;
+                bool hl = slab(a0, a1, cr, cclosest, dl);
+                bool hr = slab(b0, b1, cr, cclosest, dr);
+                float kl = hl ? dl : kFltMax;
+                float kr = hr ? dr : kFltMax;
+                left_near = kl < kr;
+                near_hit = left_near ? hl : hr;
+                far_hit = left_near ? hr : hl;
+            }
             if (near_hit) {
-                bool far_hit = left_near ? hr : hl;
                 if (far_hit) {
                     uint32_t e;
-                    if (in_blas)
-                        e = left_near ? pack_entry(x + 1, __float_as_uint(b0.w), __float_as_uint(b1.w))
-                                      : pack_entry(x, __float_as_uint(a0.w), __float_as_uint(a1.w));
-                    else
+                    if (in_blas) {
+                        uint32_t flf = __float_as_uint(left_near ? b0.w : a0.w);
+                        uint32_t fcnt = __float_as_uint(left_near ? b1.w : a1.w);
+                        e = PACKED ? (kPackFlag | (fcnt << kPackShift) | flf)
+                                   : pack_entry(left_near ? x + 1 : x, flf, fcnt);
+                    } else
                         e = left_near ? (x & 0x0000FFFFu) : ((x & 0xFFFF0000u) >> 16);
                     st.push(sp, e);
                 }
@@ -361,7 +372,7 @@ __device__ __forceinline__ TlasResult traverse_tlas(const float4* __restrict__ t
             if (!in_blas) {
                 x = __float_as_uint(tnodes[2 * (size_t)e].w);
                 y = __float_as_uint(tnodes[2 * (size_t)e + 1].w);
-            } else if (e & kPackFlag) {
+            } else if (PACKED || (e & kPackFlag)) {
                 x = e & kPackMask;
                 y = (e >> kPackShift) & 63u;
             } else {
