@@ -135,3 +135,35 @@ def test_cube_and_quad_known_answers_on_gpu(ctx, orc):
         h = tlas.intersect(r)[0]
         assert h["prim"] == prim and h["inst"] == 0 and h["packed"] == prim
         assert abs(float(h["t"]) - float(np.sqrt(np.float32(19) / np.float32(2)))) <= 3.7e-7
+
+
+def test_tlas_with_wide_leaves_and_infinite_rays(ctx, orc):
+    """A TLAS over a reference-built BLAS (leaves of hundreds of triangles:
+    unpacked stack entries) and an SAH BLAS, traced with rays of t = FLT_MAX,
+    t = +inf and short t in mixed waves: the same hits and counters as the
+    oracle."""
+    soup = scenes.triangle_soup(4000, seed=23)
+    ref_nodes, ref_used, ref_perm = orc.bvh_build(soup, True, 0)
+    assert ref_nodes["count"][:ref_used].max() >= 64
+    sph = scenes.icosphere(10, radius=2.0, seed=7)
+    s_nodes, s_used, s_perm = orc.bvh_build(sph, False, 2)
+    models = [B.Bvh.from_arrays(ref_nodes, ref_used, ref_perm, ctx=ctx),
+              B.Bvh.from_arrays(s_nodes, s_used, s_perm, ctx=ctx)]
+    blases = [(ref_nodes, ref_perm), (s_nodes, s_perm)]
+    M, Minv = scenes.random_affines(12, 77, trans_range=8.0)
+    objs = [B.SceneObject(k % 2, M[k].reshape(4, 4), Minv[k]) for k in range(12)]
+    tlas = B.TlasBuilder(ctx=ctx).build_for(objs, models)
+    tnodes, used = oracle_tlas(orc, blases, objs)
+    cam = scenes.camera(eye=(0, 0, -40), forward=(0, 0, 1), right=(1, 0, 0), up=(0, 1, 0), fovy_deg=50)
+    rays = orc.raygen(cam, 160, 160)
+    k = np.arange(rays.shape[0])
+    rays["tmax"] = np.where(k % 3 == 0, np.float32(np.inf), np.where(k % 7 == 0, np.float32(30.0), rays["tmax"]))
+    rays["tmax"][: 64 * 40] = np.finfo(np.float32).max  # whole waves of finite t
+    inst = oracle_instances(orc, objs)
+    ref, st_ref = orc.tlas_trace(tnodes, inst, blases, rays, nthreads=8, stats=True)
+    prays = np.zeros(rays.shape[0], B.RAY_DTYPE)
+    prays.view(np.uint8)[:] = rays.view(np.uint8)
+    got, st = tlas.intersect(prays, stats=True)
+    assert_hits_equal(got, ref, "wide-leaf TLAS")
+    assert np.array_equal(st.view(np.uint32), st_ref.view(np.uint32))
+    assert len(np.unique(got["inst"][got["prim"] != B.MISS])) > 4
