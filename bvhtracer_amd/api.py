@@ -511,6 +511,21 @@ def bounce_sorted(ctx, bvh, d_rays, d_hits, n, seed, d_out=None, d_src=None, d_c
     return d_out, d_src, d_count
 
 
+def trace_camera_bounce(ctx, bvh, cam, width, height, order, seed, d_hits=None, d_out=None, d_src=None,
+                        d_count=None):
+    """raygen + Bvh trace + bounce_sorted of one camera frame in one pass
+    (bvht_trace_camera_bounce): returns (d_hits, d_bounce_rays, d_src, d_count)."""
+    n = width * height
+    cam = np.ascontiguousarray(cam, CAMERA_DTYPE).reshape(1)
+    d_hits = d_hits or ctx.alloc(max(n, 1) * HIT_DTYPE.itemsize)
+    d_out = d_out or ctx.alloc(max(n, 1) * RAY_DTYPE.itemsize)
+    d_src = d_src or ctx.alloc(max(n, 1) * 4)
+    d_count = d_count or ctx.alloc(4)
+    check(L.lib().bvht_trace_camera_bounce(ctx.handle, bvh.handle, L.ptr(cam), width, height, order, seed,
+                                           d_hits.ptr, d_out.ptr, d_src.ptr, d_count.ptr))
+    return d_hits, d_out, d_src, d_count
+
+
 def bounce_tlas_sorted(ctx, tlas, d_rays, d_hits, n, seed, d_out=None, d_src=None, d_count=None):
     d_out = d_out or ctx.alloc(max(n, 1) * RAY_DTYPE.itemsize)
     d_src = d_src or ctx.alloc(max(n, 1) * 4)

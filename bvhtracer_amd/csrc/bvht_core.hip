@@ -254,13 +254,11 @@ static trace_blas_fn pick_trace_blas(int wg, int lds, bool stats, bool packed)
 }
 
 // Camera::get_ray_eye + get_ray_world (camera.rs:994-1010) with the
-// per-pixel mapping of PathTracer::evaluate (renderer.rs:353-362).
-__global__ void k_raygen(bvht_camera cam, uint32_t width, uint32_t height, int32_t order,
-                         bvht_ray* __restrict__ rays)
+// per-pixel mapping of PathTracer::evaluate (renderer.rs:353-362): ray j of a
+// width x height frame in row or 8x8-tile order, world origin and direction.
+__device__ __forceinline__ void camera_ray(const bvht_camera& cam, uint32_t width, uint32_t height, int32_t order,
+                                           uint32_t j, V3& ow, V3& dw)
 {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= width * height)
-        return;
     uint32_t x, y;
     if (order == 1) {
         uint32_t tile = j >> 6, w = j & 63u, tiles_x = width >> 3;
@@ -280,8 +278,18 @@ __global__ void k_raygen(bvht_camera cam, uint32_t width, uint32_t height, int32
     V3 p = sub3(pixel, origin);
     float nrm = sqrtf(dot3(p, p));
     V3 dir = mk3(p.x / nrm, p.y / nrm, p.z / nrm);
-    V3 ow = mat_mul4(cam.view_inv, origin, 1.0f);
-    V3 dw = mat_mul4(cam.view_inv, dir, 0.0f);
+    ow = mat_mul4(cam.view_inv, origin, 1.0f);
+    dw = mat_mul4(cam.view_inv, dir, 0.0f);
+}
+
+__global__ void k_raygen(bvht_camera cam, uint32_t width, uint32_t height, int32_t order,
+                         bvht_ray* __restrict__ rays)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= width * height)
+        return;
+    V3 ow, dw;
+    camera_ray(cam, width, height, order, j, ow, dw);
     float4* out = reinterpret_cast<float4*>(rays + j);
     out[0] = make_float4(ow.x, ow.y, ow.z, kFltMax);
     out[1] = make_float4(dw.x, dw.y, dw.z, 0.0f);
@@ -532,6 +540,109 @@ __global__ __launch_bounds__(256) void k_bounce_bin_write(BounceSrc src, const b
         src_index[pos] = i;
 }
 
+// ---- camera rays, trace and direction-grouped bounce rays in one pass ------
+// Lane j makes camera ray j (camera_ray), traces it, writes its hit and, if
+// it hit, its bounce ray (make_bounce, the same ray bounce_for makes from the
+// stored ray and hit) into a staging buffer in ray order, with key[j] =
+// bin << 8 | rank of the ray among its wave's rays of that bin (0xFFFFFFFF:
+// no bounce).  Lane k < 24 of each wave writes the wave's count of bin k to
+// counts[k * n_waves + wave].  A per-bin scan over the waves and a scatter
+// then give exactly the output of bvht_bounce_sorted: bins in order, ray
+// order inside a bin.
+template <int WG, int LDS_DEPTH, bool PACKED>
+__global__ __launch_bounds__(WG) void k_trace_camera_bounce(const float4* __restrict__ nodes,
+                                                            const float* __restrict__ tris, uint32_t ntris,
+                                                            bvht_camera cam, uint32_t width, uint32_t height,
+                                                            int32_t order, uint64_t seed, bvht_hit* __restrict__ hits,
+                                                            float4* __restrict__ staged, uint32_t* __restrict__ key,
+                                                            uint32_t* __restrict__ counts, uint32_t n_waves,
+                                                            uint32_t* __restrict__ spill, uint32_t cap)
+{
+    __shared__ uint32_t lds[LDS_DEPTH * WG];
+    uint32_t n = width * height;
+    uint32_t j = blockIdx.x * WG + threadIdx.x;
+    bool valid = false;
+    uint32_t bin = 0;
+    if (j < n) {
+        Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + j, (size_t)n, cap};
+        V3 ow, dw;
+        camera_ray(cam, width, height, order, j, ow, dw);
+        DRay ray = make_ray(ow, dw, kFltMax);
+        uint32_t pairs = 0, tests = 0;
+        BlasResult r = traverse_blas<false, PACKED, true>(nodes, tris, ray, st, 0u, pairs, tests);
+        store_hit(hits, j, r.found, r.t, r.u, r.v, r.prim, 0);
+        if (r.found && r.prim < ntris) {
+            float4 a, b;
+            make_bounce(tris + 9 * (size_t)r.prim, ray.o, ray.d, r.t, seed, j, a, b);
+            staged[2 * (size_t)j] = a;
+            staged[2 * (size_t)j + 1] = b;
+            bin = dir_bin(b.x, b.y, b.z);
+            valid = true;
+        }
+    }
+    uint32_t lane = threadIdx.x & 63u;
+    unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t rank = 0, wave_count = 0;
+    for (uint32_t k = 0; k < kDirBins; ++k) {
+        unsigned long long m = __ballot(valid && bin == k);
+        if (valid && bin == k)
+            rank = (uint32_t)__popcll(m & below);
+        if (lane == k)
+            wave_count = (uint32_t)__popcll(m);
+    }
+    uint32_t w = j >> 6;
+    if (w < n_waves && lane < kDirBins)
+        counts[(size_t)lane * n_waves + w] = wave_count;
+    if (j < n)
+        key[j] = valid ? (bin << 8 | rank) : 0xFFFFFFFFu;
+}
+
+__global__ __launch_bounds__(256) void k_bounce_scatter(const float4* __restrict__ staged,
+                                                        const uint32_t* __restrict__ key, uint32_t n,
+                                                        uint32_t n_waves, const uint32_t* __restrict__ offsets,
+                                                        const uint32_t* __restrict__ totals,
+                                                        bvht_ray* __restrict__ out, uint32_t* __restrict__ src_index,
+                                                        uint32_t* __restrict__ d_count)
+{
+    __shared__ uint32_t bin_base[kDirBins];
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t k = 0; k < kDirBins; ++k) {
+            bin_base[k] = acc;
+            acc += totals[k];
+        }
+        if (blockIdx.x == 0)
+            *d_count = acc;
+    }
+    __syncthreads();
+    uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n)
+        return;
+    uint32_t k = key[j];
+    if (k == 0xFFFFFFFFu)
+        return;
+    uint32_t bin = k >> 8;
+    uint32_t pos = bin_base[bin] + offsets[(size_t)bin * n_waves + (j >> 6)] + (k & 255u);
+    float4* o = reinterpret_cast<float4*>(out + pos);
+    o[0] = staged[2 * (size_t)j];
+    o[1] = staged[2 * (size_t)j + 1];
+    if (src_index)
+        src_index[pos] = j;
+}
+
+typedef void (*camera_bounce_fn)(const float4*, const float*, uint32_t, bvht_camera, uint32_t, uint32_t, int32_t,
+                                 uint64_t, bvht_hit*, float4*, uint32_t*, uint32_t*, uint32_t, uint32_t*, uint32_t);
+
+static camera_bounce_fn pick_camera_bounce(int wg, int lds, bool packed)
+{
+#define X(W, D)                                                                              \
+    if (wg == W && lds == D)                                                                 \
+        return packed ? k_trace_camera_bounce<W, D, true> : k_trace_camera_bounce<W, D, false>;
+    BVHT_TRACE_CONFIGS(X)
+#undef X
+    return nullptr;
+}
+
 __global__ __launch_bounds__(256) void k_stream_copy(float4* __restrict__ dst, const float4* __restrict__ src,
                                                      size_t n)
 {
@@ -766,6 +877,50 @@ int32_t hit_block_offsets(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, uint
 }
 
 // Bounce rays grouped by direction bin, ray order inside a bin: counts per
+// Camera rays + trace + direction-grouped bounce rays: the fused kernel, the
+// per-bin scan over wave counts, the scatter.  Staging buffers in ctx scratch.
+int32_t launch_trace_camera_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_camera* cam, uint32_t width,
+                                   uint32_t height, int32_t order, uint64_t seed, bvht_hit* d_hits, bvht_ray* d_out,
+                                   uint32_t* d_src, uint32_t* d_count)
+{
+    uint32_t n = width * height;
+    if (n == 0)
+        return fill_u32(ctx, d_count, 0, 1);
+    uint32_t need = blas->depth > 0 ? blas->depth - 1 : 0;
+    int wg, lds;
+    trace_config(ctx, need, &wg, &lds);
+    uint32_t* spill = nullptr;
+    if (need > (uint32_t)lds) {
+        int32_t rc = ensure_spill(ctx, &blas->d_spill, &blas->spill_words, (size_t)(need - lds) * n);
+        if (rc)
+            return rc;
+        spill = blas->d_spill;
+    }
+    camera_bounce_fn fn = pick_camera_bounce(wg, lds, blas->packed_stack);
+    if (!fn)
+        return set_error(BVHT_EINVAL, "no trace kernel for workgroup %d, LDS depth %d", wg, lds);
+    uint32_t n_waves = (n + 63) / 64;
+    size_t staged_bytes = sizeof(float4) * 2 * (size_t)n;
+    size_t key_bytes = sizeof(uint32_t) * (size_t)n;
+    size_t count_entries = (size_t)kDirBins * n_waves;
+    int32_t rc = ensure_scratch(ctx, staged_bytes + key_bytes + sizeof(uint32_t) * (count_entries + kDirBins));
+    if (rc)
+        return rc;
+    char* base = static_cast<char*>(ctx->scratch);
+    float4* staged = reinterpret_cast<float4*>(base);
+    uint32_t* key = reinterpret_cast<uint32_t*>(base + staged_bytes);
+    uint32_t* counts = key + n;
+    uint32_t* totals = counts + count_entries;
+    hipLaunchKernelGGL(fn, dim3(blocks_for(n, wg)), dim3(wg), 0, ctx->stream,
+                       reinterpret_cast<const float4*>(blas->d_nodes), blas->d_tris, blas->n_tris, *cam, width, height,
+                       order, seed, d_hits, staged, key, counts, n_waves, spill, need);
+    hipLaunchKernelGGL(k_bin_scan, dim3(kDirBins), dim3(1024), 0, ctx->stream, counts, n_waves, totals);
+    hipLaunchKernelGGL(k_bounce_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, staged, key, n, n_waves,
+                       counts, totals, d_out, d_src, d_count);
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
 // (bin, block), one scan (bin-major), write.  rows/blases select the TLAS form.
 int32_t launch_bounce_sorted(bvht_ctx ctx, const float* d_tris, uint32_t ntris, const void* rows, const void* blases,
                              uint32_t n_obj, const bvht_ray* d_rays, const bvht_hit* d_hits, uint32_t n,
@@ -1111,6 +1266,18 @@ int32_t bvht_bounce_sorted(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays,
     BVHT_REQUIRE(n == 0 || (d_rays && d_hits && d_out), "NULL buffer");
     return bvht::launch_bounce_sorted(ctx, blas->d_tris, blas->n_tris, nullptr, nullptr, 0, d_rays, d_hits, n, seed,
                                       d_out, d_src, d_count);
+}
+
+int32_t bvht_trace_camera_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_camera* cam, uint32_t width,
+                                 uint32_t height, int32_t order, uint64_t seed, bvht_hit* d_hits, bvht_ray* d_out,
+                                 uint32_t* d_src, uint32_t* d_count)
+{
+    BVHT_REQUIRE(ctx && blas && cam && d_count, "NULL argument");
+    BVHT_REQUIRE(order == BVHT_ORDER_ROWS || (order == BVHT_ORDER_TILES && width % 8 == 0 && height % 8 == 0),
+                 "order %d invalid for %ux%u (tile order needs multiples of 8)", order, width, height);
+    BVHT_REQUIRE((uint64_t)width * height < (1ull << 31), "frame %ux%u too large", width, height);
+    BVHT_REQUIRE(width * height == 0 || (d_hits && d_out), "NULL buffer");
+    return bvht::launch_trace_camera_bounce(ctx, blas, cam, width, height, order, seed, d_hits, d_out, d_src, d_count);
 }
 
 int32_t bvht_depth(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, float* d_depth)

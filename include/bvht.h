@@ -242,6 +242,17 @@ int32_t bvht_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, const 
 int32_t bvht_bounce_sorted(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, const bvht_hit* d_hits,
                            uint32_t n, uint64_t seed, bvht_ray* d_out, uint32_t* d_src, uint32_t* d_count);
 
+/* bvht_raygen + bvht_trace_blas + bvht_bounce_sorted for the width x height
+ * camera frame in one pass, without storing the camera rays: d_hits[width *
+ * height] receives the hits of the frame's rays (in `order`), d_out / d_src /
+ * *d_count the bounce rays exactly as bvht_bounce_sorted gives them for
+ * those rays and `seed`.  (PathTracer::evaluate's primary pass,
+ * renderer.rs:345-368, with the frame's first diffuse bounce generated in
+ * the same kernel.)  Staging buffers come from the context's scratch. */
+int32_t bvht_trace_camera_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_camera* cam, uint32_t width,
+                                 uint32_t height, int32_t order, uint64_t seed, bvht_hit* d_hits, bvht_ray* d_out,
+                                 uint32_t* d_src, uint32_t* d_count);
+
 /* Depth accumulator (renderer.rs:176-194): d_depth[i] = hit t or f32::MAX. */
 int32_t bvht_depth(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, float* d_depth);
 

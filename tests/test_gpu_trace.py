@@ -250,3 +250,32 @@ def test_stack_entry_packing_boundary(ctx, orc, left_count):
     got = bvh.intersect(r)
     assert_hits_equal(got, ref, f"two-leaf tree, left leaf {left_count}")
     assert (got["prim"] != B.MISS).sum() > m // 4
+
+
+@pytest.mark.parametrize("faithful,width,height,order", [(False, 640, 640, B.ORDER_TILES), (True, 200, 150, B.ORDER_ROWS),
+                                                          (False, 328, 120, B.ORDER_TILES)])
+def test_camera_bounce_pass_equals_separate_calls(ctx, orc, faithful, width, height, order):
+    """bvht_trace_camera_bounce = bvht_raygen + bvht_trace_blas +
+    bvht_bounce_sorted: hits, bounce rays, source indices and count bit for
+    bit, for packed and unpacked trees and a frame that is not a multiple of
+    64 rays."""
+    mesh = scenes.triangle_soup()
+    nodes, used, perm = orc.bvh_build(mesh, faithful, 0 if faithful else 2)
+    bvh = upload(ctx, nodes, used, perm)
+    cam = scenes.soup_camera()
+    n = width * height
+    d_rays = B.raygen(ctx, cam, width, height, order)
+    d_hits = ctx.alloc(n * B.HIT_DTYPE.itemsize)
+    bvh.intersect_device(d_rays, n, d_hits)
+    d_out, d_src, d_cnt = B.bounce_sorted(ctx, bvh, d_rays, d_hits, n, 0x5EED)
+    f_hits, f_out, f_src, f_cnt = B.trace_camera_bounce(ctx, bvh, cam, width, height, order, 0x5EED)
+    ctx.sync()
+    nb = int(d_cnt.download(np.uint32, 1)[0])
+    assert int(f_cnt.download(np.uint32, 1)[0]) == nb and nb > 1000
+    assert f_hits.download(B.HIT_DTYPE, n).tobytes() == d_hits.download(B.HIT_DTYPE, n).tobytes()
+    assert f_out.download(B.RAY_DTYPE, nb).tobytes() == d_out.download(B.RAY_DTYPE, nb).tobytes()
+    assert f_src.download(np.uint32, nb).tobytes() == d_src.download(np.uint32, nb).tobytes()
+    # and the hits are the oracle's
+    ref = orc.bvh_trace(nodes, perm, orc.raygen(cam, width, height) if order == B.ORDER_ROWS else
+                        d_rays.download(B.RAY_DTYPE, n).view(orc.RAY_DTYPE), nthreads=8)
+    assert_hits_equal(f_hits.download(B.HIT_DTYPE, n), ref, "camera pass")
