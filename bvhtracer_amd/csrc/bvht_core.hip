@@ -328,32 +328,104 @@ __global__ __launch_bounds__(256) void k_hit_counts(const bvht_hit* __restrict__
         block_counts[blockIdx.x] = wave_counts[0] + wave_counts[1] + wave_counts[2] + wave_counts[3];
 }
 
-__global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* __restrict__ counts, uint32_t nblocks,
-                                                      uint32_t* __restrict__ total)
+// Exclusive scan of a flat u32 array in place, in two launches: per-tile
+// sums, then each tile rescans itself on top of the sum of the tiles before
+// it.  A tile is 4096 entries, 16 consecutive ones per thread of a 256-thread
+// block.  The total goes to *total (written by the last tile).
+constexpr uint32_t kScanTile = 4096;
+
+__device__ __forceinline__ void load16(const uint32_t* __restrict__ a, size_t i, uint32_t m, uint32_t (&v)[16])
 {
-    // exclusive scan in place; each thread owns a contiguous run
-    __shared__ uint32_t part[1024];
-    uint32_t per = (nblocks + 1023) / 1024;
-    uint32_t begin = threadIdx.x * per, end = min(begin + per, nblocks);
-    uint32_t s = 0;
-    for (uint32_t k = begin; k < end; ++k)
-        s += counts[k];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        size_t k = i + 4 * q;
+        if (k + 4 <= m) {
+            uint4 x = *reinterpret_cast<const uint4*>(a + k);
+            v[4 * q] = x.x, v[4 * q + 1] = x.y, v[4 * q + 2] = x.z, v[4 * q + 3] = x.w;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                v[4 * q + e] = k + e < m ? a[k + e] : 0u;
+        }
     }
-    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
-    for (uint32_t k = begin; k < end; ++k) {
-        uint32_t c = counts[k];
-        counts[k] = run;
+}
+
+__device__ __forceinline__ uint32_t block_sum_256(uint32_t v, uint32_t* red)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t s = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+    return s;
+}
+
+__global__ __launch_bounds__(256) void k_tile_sums(const uint32_t* __restrict__ a, uint32_t m,
+                                                   uint32_t* __restrict__ sums)
+{
+    __shared__ uint32_t red[4];
+    uint32_t v[16];
+    load16(a, (size_t)blockIdx.x * kScanTile + 16 * threadIdx.x, m, v);
+    uint32_t s = 0;
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+        s += v[e];
+    s = block_sum_256(s, red);
+    if (threadIdx.x == 0)
+        sums[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_tile_scan(uint32_t* __restrict__ a, uint32_t m,
+                                                   const uint32_t* __restrict__ sums, uint32_t* __restrict__ total)
+{
+    __shared__ uint32_t red[4];
+    __shared__ uint32_t wsum[4];
+    uint32_t carry = 0;
+    for (uint32_t k = threadIdx.x; k < blockIdx.x; k += 256)
+        carry += sums[k];
+    carry = block_sum_256(carry, red);
+    size_t i = (size_t)blockIdx.x * kScanTile + 16 * threadIdx.x;
+    uint32_t v[16];
+    load16(a, i, m, v);
+    uint32_t own = 0;
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+        own += v[e];
+    uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = own;  // inclusive scan over the wave
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= (uint32_t)off)
+            x += y;
+    }
+    if (lane == 63)
+        wsum[w] = x;
+    __syncthreads();
+    uint32_t run = carry + x - own;
+    for (uint32_t k = 0; k < w; ++k)
+        run += wsum[k];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        uint32_t c = v[e];
+        v[e] = run;
         run += c;
     }
-    if (threadIdx.x == 1023)
-        *total = part[1023];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        size_t k = i + 4 * q;
+        if (k + 4 <= m) {
+            *reinterpret_cast<uint4*>(a + k) = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (k + e < m)
+                    a[k + e] = v[4 * q + e];
+        }
+    }
+    if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+        *total = carry + wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
 __global__ __launch_bounds__(256) void k_bounce_write(const float* __restrict__ tris, uint32_t ntris,
@@ -472,56 +544,13 @@ __global__ __launch_bounds__(256) void k_bounce_bin_counts(BounceSrc src, const 
             wc[0][threadIdx.x] + wc[1][threadIdx.x] + wc[2][threadIdx.x] + wc[3][threadIdx.x];
 }
 
-// Exclusive scan of one bin's per-block counts (one workgroup per bin, each
-// thread a contiguous run); totals[bin] = the bin's size.
-__global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* __restrict__ counts, uint32_t nb,
-                                                   uint32_t* __restrict__ totals)
-{
-    __shared__ uint32_t part[1024];
-    uint32_t* c = counts + (size_t)blockIdx.x * nb;
-    uint32_t per = (nb + 1023) / 1024;
-    uint32_t begin = min(threadIdx.x * per, nb), end = min(begin + per, nb);
-    uint32_t s = 0;
-    for (uint32_t k = begin; k < end; ++k)
-        s += c[k];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
-    for (uint32_t k = begin; k < end; ++k) {
-        uint32_t v = c[k];
-        c[k] = run;
-        run += v;
-    }
-    if (threadIdx.x == 1023)
-        totals[blockIdx.x] = part[1023];
-}
-
 __global__ __launch_bounds__(256) void k_bounce_bin_write(BounceSrc src, const bvht_ray* __restrict__ rays,
                                                           const bvht_hit* __restrict__ hits, uint32_t n,
                                                           uint64_t seed, uint32_t nb,
                                                           const uint32_t* __restrict__ offsets,
-                                                          const uint32_t* __restrict__ totals,
-                                                          bvht_ray* __restrict__ out, uint32_t* __restrict__ src_index,
-                                                          uint32_t* __restrict__ d_count)
+                                                          bvht_ray* __restrict__ out, uint32_t* __restrict__ src_index)
 {
     __shared__ uint32_t wc[4][kDirBins];
-    __shared__ uint32_t bin_base[kDirBins + 1];
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (uint32_t k = 0; k < kDirBins; ++k) {
-            bin_base[k] = acc;
-            acc += totals[k];
-        }
-        bin_base[kDirBins] = acc;
-        if (blockIdx.x == 0)
-            *d_count = acc;
-    }
     uint32_t i = blockIdx.x * 256 + threadIdx.x;
     float4 a, b;
     bool valid = bounce_for(src, rays, n, hits, i, seed, a, b);
@@ -530,7 +559,7 @@ __global__ __launch_bounds__(256) void k_bounce_bin_write(BounceSrc src, const b
     __syncthreads();
     if (!valid)
         return;
-    uint32_t pos = bin_base[bin] + offsets[(size_t)bin * nb + blockIdx.x] + rank;
+    uint32_t pos = offsets[(size_t)bin * nb + blockIdx.x] + rank;
     for (uint32_t w = 0; w < (threadIdx.x >> 6); ++w)
         pos += wc[w][bin];
     float4* o = reinterpret_cast<float4*>(out + pos);
@@ -546,7 +575,7 @@ __global__ __launch_bounds__(256) void k_bounce_bin_write(BounceSrc src, const b
 // stored ray and hit) into a staging buffer in ray order, with key[j] =
 // bin << 8 | rank of the ray among its wave's rays of that bin (0xFFFFFFFF:
 // no bounce).  Lane k < 24 of each wave writes the wave's count of bin k to
-// counts[k * n_waves + wave].  A per-bin scan over the waves and a scatter
+// counts[k * n_waves + wave].  A scan of those bin-major counts and a scatter
 // then give exactly the output of bvht_bounce_sorted: bins in order, ray
 // order inside a bin.
 template <int WG, int LDS_DEPTH, bool PACKED>
@@ -600,21 +629,8 @@ __global__ __launch_bounds__(WG) void k_trace_camera_bounce(const float4* __rest
 __global__ __launch_bounds__(256) void k_bounce_scatter(const float4* __restrict__ staged,
                                                         const uint32_t* __restrict__ key, uint32_t n,
                                                         uint32_t n_waves, const uint32_t* __restrict__ offsets,
-                                                        const uint32_t* __restrict__ totals,
-                                                        bvht_ray* __restrict__ out, uint32_t* __restrict__ src_index,
-                                                        uint32_t* __restrict__ d_count)
+                                                        bvht_ray* __restrict__ out, uint32_t* __restrict__ src_index)
 {
-    __shared__ uint32_t bin_base[kDirBins];
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (uint32_t k = 0; k < kDirBins; ++k) {
-            bin_base[k] = acc;
-            acc += totals[k];
-        }
-        if (blockIdx.x == 0)
-            *d_count = acc;
-    }
-    __syncthreads();
     uint32_t j = blockIdx.x * 256 + threadIdx.x;
     if (j >= n)
         return;
@@ -622,7 +638,7 @@ __global__ __launch_bounds__(256) void k_bounce_scatter(const float4* __restrict
     if (k == 0xFFFFFFFFu)
         return;
     uint32_t bin = k >> 8;
-    uint32_t pos = bin_base[bin] + offsets[(size_t)bin * n_waves + (j >> 6)] + (k & 255u);
+    uint32_t pos = offsets[(size_t)bin * n_waves + (j >> 6)] + (k & 255u);
     float4* o = reinterpret_cast<float4*>(out + pos);
     o[0] = staged[2 * (size_t)j];
     o[1] = staged[2 * (size_t)j + 1];
@@ -663,6 +679,16 @@ __global__ void k_depth_acc(const bvht_hit* __restrict__ hits, uint32_t n, float
 namespace bvht {
 
 static inline uint32_t blocks_for(size_t n, uint32_t wg) { return (uint32_t)((n + wg - 1) / wg); }
+static inline size_t align256(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+
+// In-place exclusive scan of a[0, m) (m > 0, a 16-byte aligned), total to
+// *total; sums holds blocks_for(m, kScanTile) entries.
+static void launch_scan(bvht_ctx ctx, uint32_t* a, uint32_t m, uint32_t* sums, uint32_t* total)
+{
+    uint32_t tiles = blocks_for(m, kScanTile);
+    hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(256), 0, ctx->stream, a, m, sums);
+    hipLaunchKernelGGL(k_tile_scan, dim3(tiles), dim3(256), 0, ctx->stream, a, m, sums, total);
+}
 
 int32_t fill_u32(bvht_ctx ctx, uint32_t* p, uint32_t v, size_t n)
 {
@@ -862,7 +888,8 @@ int32_t hit_block_offsets(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, uint
                           const void* rows, const void* blases, uint32_t* d_count, uint32_t** offsets)
 {
     uint32_t nb = blocks_for(n, 256);
-    int32_t rc = ensure_scratch(ctx, sizeof(uint32_t) * std::max<uint32_t>(nb, 1));
+    size_t count_bytes = align256(sizeof(uint32_t) * std::max<uint32_t>(nb, 1));
+    int32_t rc = ensure_scratch(ctx, count_bytes + sizeof(uint32_t) * blocks_for(nb, kScanTile));
     if (rc)
         return rc;
     uint32_t* bc = static_cast<uint32_t*>(ctx->scratch);
@@ -871,14 +898,13 @@ int32_t hit_block_offsets(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, uint
         return fill_u32(ctx, d_count, 0, 1);
     hipLaunchKernelGGL(k_hit_counts, dim3(nb), dim3(256), 0, ctx->stream, d_hits, n, prim_limit, inst_limit,
                        static_cast<const ObjRows*>(rows), static_cast<const BlasRef*>(blases), bc);
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, ctx->stream, bc, nb, d_count);
+    launch_scan(ctx, bc, nb, reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->scratch) + count_bytes), d_count);
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }
 
-// Bounce rays grouped by direction bin, ray order inside a bin: counts per
 // Camera rays + trace + direction-grouped bounce rays: the fused kernel, the
-// per-bin scan over wave counts, the scatter.  Staging buffers in ctx scratch.
+// scan of the bin-major wave counts, the scatter.  Staging buffers in ctx scratch.
 int32_t launch_trace_camera_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_camera* cam, uint32_t width,
                                    uint32_t height, int32_t order, uint64_t seed, bvht_hit* d_hits, bvht_ray* d_out,
                                    uint32_t* d_src, uint32_t* d_count)
@@ -903,24 +929,27 @@ int32_t launch_trace_camera_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_came
     size_t staged_bytes = sizeof(float4) * 2 * (size_t)n;
     size_t key_bytes = sizeof(uint32_t) * (size_t)n;
     size_t count_entries = (size_t)kDirBins * n_waves;
-    int32_t rc = ensure_scratch(ctx, staged_bytes + key_bytes + sizeof(uint32_t) * (count_entries + kDirBins));
+    size_t count_at = align256(staged_bytes + key_bytes);
+    size_t sums_at = align256(count_at + sizeof(uint32_t) * count_entries);
+    int32_t rc = ensure_scratch(ctx, sums_at + sizeof(uint32_t) * blocks_for(count_entries, kScanTile));
     if (rc)
         return rc;
     char* base = static_cast<char*>(ctx->scratch);
     float4* staged = reinterpret_cast<float4*>(base);
     uint32_t* key = reinterpret_cast<uint32_t*>(base + staged_bytes);
-    uint32_t* counts = key + n;
-    uint32_t* totals = counts + count_entries;
+    uint32_t* counts = reinterpret_cast<uint32_t*>(base + count_at);
+    uint32_t* sums = reinterpret_cast<uint32_t*>(base + sums_at);
     hipLaunchKernelGGL(fn, dim3(blocks_for(n, wg)), dim3(wg), 0, ctx->stream,
                        reinterpret_cast<const float4*>(blas->d_nodes), blas->d_tris, blas->n_tris, *cam, width, height,
                        order, seed, d_hits, staged, key, counts, n_waves, spill, need);
-    hipLaunchKernelGGL(k_bin_scan, dim3(kDirBins), dim3(1024), 0, ctx->stream, counts, n_waves, totals);
+    launch_scan(ctx, counts, (uint32_t)count_entries, sums, d_count);
     hipLaunchKernelGGL(k_bounce_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, staged, key, n, n_waves,
-                       counts, totals, d_out, d_src, d_count);
+                       counts, d_out, d_src);
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }
 
+// Bounce rays grouped by direction bin, ray order inside a bin: counts per
 // (bin, block), one scan (bin-major), write.  rows/blases select the TLAS form.
 int32_t launch_bounce_sorted(bvht_ctx ctx, const float* d_tris, uint32_t ntris, const void* rows, const void* blases,
                              uint32_t n_obj, const bvht_ray* d_rays, const bvht_hit* d_hits, uint32_t n,
@@ -930,17 +959,18 @@ int32_t launch_bounce_sorted(bvht_ctx ctx, const float* d_tris, uint32_t ntris, 
         return fill_u32(ctx, d_count, 0, 1);
     uint32_t nb = blocks_for(n, 256);
     size_t entries = (size_t)kDirBins * nb;
-    int32_t rc = ensure_scratch(ctx, sizeof(uint32_t) * (entries + kDirBins));
+    size_t sums_at = align256(sizeof(uint32_t) * entries);
+    int32_t rc = ensure_scratch(ctx, sums_at + sizeof(uint32_t) * blocks_for(entries, kScanTile));
     if (rc)
         return rc;
     uint32_t* counts = static_cast<uint32_t*>(ctx->scratch);
-    uint32_t* totals = counts + entries;
+    uint32_t* sums = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->scratch) + sums_at);
     BounceSrc src{d_tris, ntris, static_cast<const ObjRows*>(rows), static_cast<const BlasRef*>(blases), n_obj};
     hipLaunchKernelGGL(k_bounce_bin_counts, dim3(nb), dim3(256), 0, ctx->stream, src, d_rays, d_hits, n, seed, nb,
                        counts);
-    hipLaunchKernelGGL(k_bin_scan, dim3(kDirBins), dim3(1024), 0, ctx->stream, counts, nb, totals);
+    launch_scan(ctx, counts, (uint32_t)entries, sums, d_count);
     hipLaunchKernelGGL(k_bounce_bin_write, dim3(nb), dim3(256), 0, ctx->stream, src, d_rays, d_hits, n, seed, nb,
-                       counts, totals, d_out, d_src, d_count);
+                       counts, d_out, d_src);
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }
