@@ -519,6 +519,7 @@ __device__ __forceinline__ void bin_ranks(uint32_t bin, bool valid, uint32_t (*w
     uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     unsigned long long below = (1ull << lane) - 1ull;
     rank = 0;
+#pragma unroll 1
     for (uint32_t k = 0; k < kDirBins; ++k) {
         unsigned long long m = __ballot(valid && bin == k);
         if (lane == 0)
@@ -612,6 +613,8 @@ __global__ __launch_bounds__(WG) void k_trace_camera_bounce(const float4* __rest
     uint32_t lane = threadIdx.x & 63u;
     unsigned long long below = (1ull << lane) - 1ull;
     uint32_t rank = 0, wave_count = 0;
+    // not unrolled: 24 live ballot masks would cost a wave slot in SGPRs
+#pragma unroll 1
     for (uint32_t k = 0; k < kDirBins; ++k) {
         unsigned long long m = __ballot(valid && bin == k);
         if (valid && bin == k)
