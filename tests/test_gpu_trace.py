@@ -279,3 +279,81 @@ def test_camera_bounce_pass_equals_separate_calls(ctx, orc, faithful, width, hei
     ref = orc.bvh_trace(nodes, perm, orc.raygen(cam, width, height) if order == B.ORDER_ROWS else
                         d_rays.download(B.RAY_DTYPE, n).view(orc.RAY_DTYPE), nthreads=8)
     assert_hits_equal(f_hits.download(B.HIT_DTYPE, n), ref, "camera pass")
+
+
+def test_camera_bounce_pass_full_frame_configs_and_no_src(ctx):
+    """1920x1080 on an 81,920-triangle sphere: the wave counts span 190 scan
+    tiles.  With 64- and 256-lane workgroups, an 8-entry LDS stack (the tree
+    is deeper, so the stack spills) and without source indices, the fused
+    pass equals bvht_raygen + bvht_trace_blas + bvht_bounce_sorted bit for
+    bit."""
+    from bvhtracer_amd import _lib as L
+    from bvhtracer_amd import autotune
+
+    W, H, seed = 1920, 1080, 0xB0B
+    n = W * H
+    bvh = B.BvhBuilder(B.BUILD_SAH, 2, ctx=ctx).build_for(scenes.icosphere(64, radius=32.0))
+    assert bvh.depth() - 1 > 8
+    cam = scenes.sphere_camera(32.0, W, H)
+    d_rays = B.raygen(ctx, cam, W, H, B.ORDER_TILES)
+    d_hits = ctx.alloc(n * B.HIT_DTYPE.itemsize)
+    bvh.intersect_device(d_rays, n, d_hits)
+    d_out, d_src, d_cnt = B.bounce_sorted(ctx, bvh, d_rays, d_hits, n, seed)
+    ctx.sync()
+    nb = int(d_cnt.download(np.uint32, 1)[0])
+    assert nb > n // 2
+    hits, out, src = (d_hits.download(B.HIT_DTYPE, n).tobytes(), d_out.download(B.RAY_DTYPE, nb).tobytes(),
+                      d_src.download(np.uint32, nb).tobytes())
+    try:
+        for wg, depth in ((64, 16), (256, 8)):
+            autotune.set_config(ctx, wg, depth)
+            f_hits, f_out, f_src, f_cnt = B.trace_camera_bounce(ctx, bvh, cam, W, H, B.ORDER_TILES, seed)
+            ctx.sync()
+            assert int(f_cnt.download(np.uint32, 1)[0]) == nb, (wg, depth)
+            assert f_hits.download(B.HIT_DTYPE, n).tobytes() == hits, (wg, depth)
+            assert f_out.download(B.RAY_DTYPE, nb).tobytes() == out, (wg, depth)
+            assert f_src.download(np.uint32, nb).tobytes() == src, (wg, depth)
+        cam_rec = np.ascontiguousarray(cam, B.CAMERA_DTYPE).reshape(1)
+        g_hits, g_out, g_cnt = ctx.alloc(n * B.HIT_DTYPE.itemsize), ctx.alloc(n * B.RAY_DTYPE.itemsize), ctx.alloc(4)
+        L.check(L.lib().bvht_trace_camera_bounce(ctx.handle, bvh.handle, L.ptr(cam_rec), W, H, B.ORDER_TILES, seed,
+                                                 g_hits.ptr, g_out.ptr, None, g_cnt.ptr))
+        ctx.sync()
+        assert int(g_cnt.download(np.uint32, 1)[0]) == nb
+        assert g_out.download(B.RAY_DTYPE, nb).tobytes() == out
+    finally:
+        autotune.set_config(ctx, 0, 0)
+
+
+def test_camera_bounce_pass_empty_frame(ctx):
+    bvh = B.BvhBuilder(B.BUILD_SAH, 2, ctx=ctx).build_for(scenes.triangle_soup(500, seed=3))
+    d_cnt = ctx.alloc(4)
+    d_cnt.upload(np.array([77], np.uint32))
+    B.trace_camera_bounce(ctx, bvh, scenes.soup_camera(), 0, 0, B.ORDER_TILES, 1, d_count=d_cnt)
+    ctx.sync()
+    assert int(d_cnt.download(np.uint32, 1)[0]) == 0
+
+
+def test_bounce_orders_agree_beyond_one_scan_tile(ctx):
+    """1280x960 rays give 4800 blocks of 256 rays: the hit compaction of
+    bvht_bounce scans more than one 4096-entry tile.  Its ray-order output
+    equals bvht_bounce_sorted's put back in source order, and both count
+    exactly the rays that hit."""
+    W, H, seed = 1280, 960, 0xF00D
+    n = W * H
+    mesh = scenes.icosphere(64, radius=32.0)
+    bvh = B.BvhBuilder(B.BUILD_SAH, 2, ctx=ctx).build_for(mesh.copy())
+    d_rays = B.raygen(ctx, scenes.sphere_camera(32.0, W, H), W, H, B.ORDER_TILES)
+    d_hits = ctx.alloc(n * B.HIT_DTYPE.itemsize)
+    bvh.intersect_device(d_rays, n, d_hits)
+    d_plain, d_pcnt = B.bounce(ctx, bvh, d_rays, d_hits, n, seed)
+    d_sorted, d_src, d_scnt = B.bounce_sorted(ctx, bvh, d_rays, d_hits, n, seed)
+    ctx.sync()
+    hits = d_hits.download(B.HIT_DTYPE, n)
+    expect = int((hits["prim"] < mesh.shape[0]).sum())
+    assert expect > n // 4
+    assert int(d_pcnt.download(np.uint32, 1)[0]) == expect
+    assert int(d_scnt.download(np.uint32, 1)[0]) == expect
+    src = d_src.download(np.uint32, expect)
+    assert np.array_equal(np.sort(src), np.flatnonzero(hits["prim"] < mesh.shape[0]).astype(np.uint32))
+    back = d_sorted.download(B.RAY_DTYPE, expect)[np.argsort(src)]
+    assert back.tobytes() == d_plain.download(B.RAY_DTYPE, expect).tobytes()
