@@ -870,10 +870,10 @@ int32_t launch_trace_blas(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, 
     trace_config(ctx, need, &wg, &lds);
     uint32_t* spill = nullptr;
     if (need > (uint32_t)lds) {
-        int32_t rc = ensure_spill(ctx, &blas->d_spill, &blas->spill_words, (size_t)(need - lds) * n);
+        int32_t rc = ensure_spill(ctx, &ctx->spill, &ctx->spill_words, (size_t)(need - lds) * n);
         if (rc)
             return rc;
-        spill = blas->d_spill;
+        spill = ctx->spill;
     }
     trace_blas_fn fn = pick_trace_blas(wg, lds, d_stats != nullptr, blas->packed_stack);
     if (!fn)
@@ -920,10 +920,10 @@ int32_t launch_trace_camera_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_came
     trace_config(ctx, need, &wg, &lds);
     uint32_t* spill = nullptr;
     if (need > (uint32_t)lds) {
-        int32_t rc = ensure_spill(ctx, &blas->d_spill, &blas->spill_words, (size_t)(need - lds) * n);
+        int32_t rc = ensure_spill(ctx, &ctx->spill, &ctx->spill_words, (size_t)(need - lds) * n);
         if (rc)
             return rc;
-        spill = blas->d_spill;
+        spill = ctx->spill;
     }
     camera_bounce_fn fn = pick_camera_bounce(wg, lds, blas->packed_stack);
     if (!fn)
@@ -1026,6 +1026,8 @@ int32_t bvht_destroy(bvht_ctx ctx)
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->scratch)
         (void)hipFree(ctx->scratch);
+    if (ctx->spill)
+        (void)hipFree(ctx->spill);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BVHT_OK;
@@ -1225,7 +1227,6 @@ int32_t bvht_blas_destroy(bvht_blas b)
     (void)hipFree(b->d_attr_tex);
     (void)hipFree(b->d_texture);
     (void)hipFree(b->d_attr_ref);
-    (void)hipFree(b->d_spill);
     delete b;
     return BVHT_OK;
 }

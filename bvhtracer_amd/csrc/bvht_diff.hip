@@ -574,10 +574,10 @@ extern "C" int32_t bvht_trace_forward(bvht_ctx ctx, bvht_mesh mesh, const float*
     trace_config(ctx, need, &wg, &lds);
     uint32_t* spill = nullptr;
     if (need > (uint32_t)lds) {
-        int32_t rc = ensure_spill(ctx, &b->d_spill, &b->spill_words, (size_t)(need - lds) * n);
+        int32_t rc = ensure_spill(ctx, &ctx->spill, &ctx->spill_words, (size_t)(need - lds) * n);
         if (rc)
             return rc;
-        spill = b->d_spill;
+        spill = ctx->spill;
     }
     forward_fn fn = pick_forward(wg, lds, mesh->blas->packed_stack);
     if (!fn)

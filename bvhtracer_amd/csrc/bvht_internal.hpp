@@ -16,6 +16,11 @@ struct bvht_ctx_s {
     // Reusable device scratch (grown on demand; freed with the context).
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
+    // Traversal stack spill of this context's trace launches (grown on
+    // demand).  Per context, not per tree: a tree may be traced from several
+    // contexts' streams at once.
+    uint32_t* spill = nullptr;
+    size_t spill_words = 0;
     // trace kernel configuration (0 = choose from the tree depth)
     int trace_wg = 0;
     int trace_lds = 0;
@@ -54,8 +59,6 @@ struct bvht_blas_s {
     uint32_t tex_w = 0, tex_h = 0;
     uint64_t attr_version = 0;
     AttrRef* d_attr_ref = nullptr;      // this BLAS's AttrRef on the device
-    uint32_t* d_spill = nullptr;    // traversal stack spill (grown on demand)
-    size_t spill_words = 0;
 };
 
 struct bvht_tlas_s {
@@ -71,8 +74,6 @@ struct bvht_tlas_s {
     uint32_t* d_meta = nullptr;     // {nodes_used, depth, status}
     uint32_t depth = 0;             // TLAS levels after the last rebuild
     uint32_t max_blas_depth = 0;
-    uint32_t* d_spill = nullptr;
-    size_t spill_words = 0;
     AttrRef* d_attr_table = nullptr;        // per-BLAS attributes for the attribute accumulators
     std::vector<uint64_t> attr_versions;    // BLAS attribute versions the table was built from
     // physics binding (bvht_tlas_bind_world)

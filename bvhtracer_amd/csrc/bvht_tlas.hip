@@ -141,7 +141,6 @@ void tlas_free(bvht_tlas t)
     (void)hipFree(t->d_nodes);
     (void)hipFree(t->d_list);
     (void)hipFree(t->d_meta);
-    (void)hipFree(t->d_spill);
     (void)hipFree(t->d_object_body);
     (void)hipFree(t->d_transform_init);
     delete t;
@@ -190,10 +189,10 @@ int32_t launch_trace_tlas(bvht_tlas t, const bvht_ray* d_rays, uint32_t n, const
     trace_config(ctx, need, &wg, &lds);
     uint32_t* spill = nullptr;
     if (need > (uint32_t)lds) {
-        int32_t rc = ensure_spill(ctx, &t->d_spill, &t->spill_words, (size_t)(need - lds) * n);
+        int32_t rc = ensure_spill(ctx, &ctx->spill, &ctx->spill_words, (size_t)(need - lds) * n);
         if (rc)
             return rc;
-        spill = t->d_spill;
+        spill = ctx->spill;
     }
     // packed stack entries when every BLAS the objects can enter fits them
     bool packed = true;

@@ -14,7 +14,11 @@
  *     trace entry points are DEVICE pointers (HBM-resident); triangle and
  *     node arrays passed to build/upload/download are HOST pointers.
  *   - All work is enqueued on the context's HIP stream; bvht_sync() or a
- *     read-back is the fence.  One context per host thread.  Destroy every
+ *     read-back is the fence.  One context per host thread.  A tree (BLAS,
+ *     TLAS) is only read by traces, so several contexts may trace it at the
+ *     same time; each context has its own scratch and traversal-stack spill
+ *     buffers, grown on demand (a graph captured on a context keeps their
+ *     addresses: size them with one eager call first).  Destroy every
  *     object made with a context (BLAS, TLAS, batch, mesh, world, graph,
  *     device allocation) before the context itself, and a world after the
  *     TLASes bound to it.
