@@ -357,3 +357,48 @@ def test_bounce_orders_agree_beyond_one_scan_tile(ctx):
     assert np.array_equal(np.sort(src), np.flatnonzero(hits["prim"] < mesh.shape[0]).astype(np.uint32))
     back = d_sorted.download(B.RAY_DTYPE, expect)[np.argsort(src)]
     assert back.tobytes() == d_plain.download(B.RAY_DTYPE, expect).tobytes()
+
+
+def test_two_contexts_trace_one_tree_at_once(ctx):
+    """Frames on two contexts (two streams) tracing the same BVH at the same
+    time, with an 8-entry LDS stack so both spill: each context has its own
+    spill buffer, and every frame equals the frame traced alone."""
+    from bvhtracer_amd import _lib as L
+    from bvhtracer_amd import autotune
+
+    lib = L.lib()
+    W, H, seed = 640, 480, 0x2C7
+    n = W * H
+    bvh = B.BvhBuilder(B.BUILD_SAH, 2, ctx=ctx).build_for(scenes.icosphere(64, radius=32.0))
+    assert bvh.depth() - 1 > 8
+    cam = np.ascontiguousarray(scenes.sphere_camera(32.0, W, H), B.CAMERA_DTYPE).reshape(1)
+    other = B.Context(0)
+    ctxs = [ctx, other]
+    try:
+        bufs = [dict(h=c.alloc(n * 24), b=c.alloc(n * 32), bh=c.alloc(n * 24), src=c.alloc(n * 4), cnt=c.alloc(4))
+                for c in ctxs]
+
+        def frame(k):
+            c, d = ctxs[k], bufs[k]
+            L.check(lib.bvht_trace_camera_bounce(c.handle, bvh.handle, L.ptr(cam), W, H, B.ORDER_TILES, seed,
+                                                 d["h"].ptr, d["b"].ptr, d["src"].ptr, d["cnt"].ptr))
+            L.check(lib.bvht_trace_blas_count(c.handle, bvh.handle, d["b"].ptr, d["cnt"].ptr, n, d["bh"].ptr, None))
+
+        for c in ctxs:
+            autotune.set_config(c, 64, 8)
+        frame(0)
+        ctx.sync()
+        nb = int(bufs[0]["cnt"].download(np.uint32, 1)[0])
+        ref = (bufs[0]["h"].download(B.HIT_DTYPE, n).tobytes(), bufs[0]["bh"].download(B.HIT_DTYPE, nb).tobytes())
+        for _ in range(6):
+            frame(0)
+            frame(1)
+        for c in ctxs:
+            c.sync()
+        for d in bufs:
+            assert int(d["cnt"].download(np.uint32, 1)[0]) == nb
+            assert d["h"].download(B.HIT_DTYPE, n).tobytes() == ref[0]
+            assert d["bh"].download(B.HIT_DTYPE, nb).tobytes() == ref[1]
+    finally:
+        autotune.set_config(ctx, 0, 0)
+        other.close()
