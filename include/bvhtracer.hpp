@@ -1,0 +1,720 @@
+/*
+ * bvhtracer.hpp -- C++17 host API over the C ABI (bvht.h), mirroring the
+ * reference crate's Rust API for the traversal path (bvhtracer/src, paths
+ * relative to the reference repository root): the same type and method
+ * names, argument meaning and results, so a caller -- or a test written like
+ * the reference's own tests -- reads the same.  Header-only; link
+ * libbvht_hip.so.
+ *
+ * Every query runs on the GPU through libbvht_hip.so; these classes hold
+ * host bookkeeping (meshes, transforms, object lists) and device handles.
+ * There is no CPU path.
+ *
+ * Mapping notes
+ *   - Rust `T::new(..)` constructors are C++ constructors; `Option<T>` is
+ *     std::optional<T>; `Rc<RefCell<Model>>` is std::shared_ptr<Model>.
+ *   - Where the reference panics (empty mesh, bvh.rs:530-535; empty TLAS,
+ *     tlas.rs:248) or the library reports an error, bvhtracer::Error is
+ *     thrown with the library's message and status code.
+ *   - Vector arithmetic is binary32, left to right (dot = x*x' + y*y' +
+ *     z*z', normalize = v / sqrt(dot(v, v))), the convention the GPU path
+ *     and its tests use.  Compile with -ffp-contract=off.
+ */
+#ifndef BVHTRACER_HPP
+#define BVHTRACER_HPP
+
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "bvht.h"
+
+namespace bvhtracer {
+
+/* ---- errors ------------------------------------------------------------ */
+
+class Error : public std::runtime_error {
+public:
+    Error(int32_t code, const std::string& what) : std::runtime_error(what), code_(code) {}
+    int32_t code() const { return code_; }
+
+private:
+    int32_t code_;
+};
+
+inline void check(int32_t rc)
+{
+    if (rc != BVHT_OK)
+        throw Error(rc, bvht_last_error());
+}
+
+/* ---- device context and buffers ---------------------------------------- */
+
+/* One HIP device and stream (bvht_ctx). */
+class Context {
+public:
+    explicit Context(int32_t device = 0) { check(bvht_create(device, &ctx_)); }
+    ~Context()
+    {
+        if (ctx_)
+            bvht_destroy(ctx_);
+    }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    bvht_ctx handle() const { return ctx_; }
+    void sync() const { check(bvht_sync(ctx_)); }
+
+private:
+    bvht_ctx ctx_ = nullptr;
+};
+
+/* The context the value types below use when none is given (device 0),
+ * created on first use.  It is never destroyed: the HIP runtime may be torn
+ * down before static destructors run, so the process exit releases it. */
+inline std::shared_ptr<Context> default_context()
+{
+    static auto* ctx = new std::shared_ptr<Context>(std::make_shared<Context>(0));
+    return *ctx;
+}
+
+/* Device array of n T (bvht_device_alloc), freed with the object. */
+template <class T>
+class DeviceBuffer {
+public:
+    DeviceBuffer(std::shared_ptr<Context> ctx, size_t n) : ctx_(std::move(ctx)), n_(n)
+    {
+        check(bvht_device_alloc(ctx_->handle(), sizeof(T) * (n ? n : 1), &ptr_));
+    }
+    DeviceBuffer(std::shared_ptr<Context> ctx, const std::vector<T>& host) : DeviceBuffer(std::move(ctx), host.size())
+    {
+        upload(host);
+    }
+    ~DeviceBuffer()
+    {
+        if (ptr_)
+            bvht_device_free(ctx_->handle(), ptr_);
+    }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    T* data() const { return static_cast<T*>(ptr_); }
+    size_t size() const { return n_; }
+    void upload(const std::vector<T>& host)
+    {
+        if (host.size() > n_)
+            throw Error(BVHT_EINVAL, "upload larger than the device buffer");
+        if (!host.empty())
+            check(bvht_copy_to_device(ctx_->handle(), ptr_, host.data(), sizeof(T) * host.size()));
+    }
+    std::vector<T> download(size_t count) const
+    {
+        std::vector<T> out(count);
+        if (count)
+            check(bvht_copy_to_host(ctx_->handle(), out.data(), ptr_, sizeof(T) * count));
+        return out;
+    }
+
+private:
+    std::shared_ptr<Context> ctx_;
+    size_t n_;
+    void* ptr_ = nullptr;
+};
+
+/* ---- vectors and matrices (cglinalg's Vector3 / Matrix4x4 subset) ------ */
+
+struct Vector3 {
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+
+    constexpr Vector3() = default;
+    constexpr Vector3(float x_, float y_, float z_) : x(x_), y(y_), z(z_) {}
+    static constexpr Vector3 zero() { return Vector3(); }
+
+    float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    Vector3 operator+(const Vector3& o) const { return {x + o.x, y + o.y, z + o.z}; }
+    Vector3 operator-(const Vector3& o) const { return {x - o.x, y - o.y, z - o.z}; }
+    Vector3 operator*(float s) const { return {x * s, y * s, z * s}; }
+    Vector3 operator/(float s) const { return {x / s, y / s, z / s}; }
+    Vector3& operator+=(const Vector3& o) { return *this = *this + o; }
+    Vector3& operator-=(const Vector3& o) { return *this = *this - o; }
+    bool operator==(const Vector3& o) const { return x == o.x && y == o.y && z == o.z; }
+    bool operator!=(const Vector3& o) const { return !(*this == o); }
+
+    float dot(const Vector3& o) const { return x * o.x + y * o.y + z * o.z; }
+    Vector3 cross(const Vector3& o) const { return {y * o.z - z * o.y, z * o.x - x * o.z, x * o.y - y * o.x}; }
+    float magnitude() const { return std::sqrt(dot(*this)); }
+    Vector3 normalize() const
+    {
+        float n = magnitude();
+        return {x / n, y / n, z / n};
+    }
+};
+inline Vector3 operator*(float s, const Vector3& v) { return v * s; }
+
+/* Column-major 4x4 (m[col * 4 + row]), the layout of bvht_instance. */
+struct Matrix4x4 {
+    std::array<float, 16> m{};
+
+    static Matrix4x4 identity()
+    {
+        Matrix4x4 r;
+        r.m[0] = r.m[5] = r.m[10] = r.m[15] = 1.0f;
+        return r;
+    }
+    static Matrix4x4 from_affine_translation(const Vector3& t)
+    {
+        Matrix4x4 r = identity();
+        r.m[12] = t.x, r.m[13] = t.y, r.m[14] = t.z;
+        return r;
+    }
+    static Matrix4x4 from_affine_scale(float s)
+    {
+        Matrix4x4 r = identity();
+        r.m[0] = r.m[5] = r.m[10] = s;
+        return r;
+    }
+    float at(int row, int col) const { return m[col * 4 + row]; }
+    /* Inverse in binary64, rounded to binary32 (the cached M^-1 of a scene
+     * object; transform_component.rs:7-27).  Throws on a singular matrix. */
+    Matrix4x4 inverse() const
+    {
+        double a[4][8];
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 8; ++c)
+                a[r][c] = c < 4 ? (double)at(r, c) : (c - 4 == r ? 1.0 : 0.0);
+        for (int c = 0; c < 4; ++c) {
+            int p = c;
+            for (int r = c + 1; r < 4; ++r)
+                if (std::fabs(a[r][c]) > std::fabs(a[p][c]))
+                    p = r;
+            if (a[p][c] == 0.0)
+                throw Error(BVHT_EINVAL, "singular transform");
+            for (int k = 0; k < 8; ++k)
+                std::swap(a[c][k], a[p][k]);
+            double inv = 1.0 / a[c][c];
+            for (int k = 0; k < 8; ++k)
+                a[c][k] *= inv;
+            for (int r = 0; r < 4; ++r)
+                if (r != c) {
+                    double f = a[r][c];
+                    for (int k = 0; k < 8; ++k)
+                        a[r][k] -= f * a[c][k];
+                }
+        }
+        Matrix4x4 out;
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c)
+                out.m[c * 4 + r] = (float)a[r][c + 4];
+        return out;
+    }
+};
+
+/* ---- query types (query/ray.rs, query/intersection.rs) ------------------ */
+
+struct Ray {
+    Vector3 origin;
+    Vector3 direction;
+    float t = FLT_MAX;
+
+    Ray() = default;
+    Ray(const Vector3& origin_, const Vector3& direction_, float t_) : origin(origin_), direction(direction_), t(t_) {}
+    static Ray from_origin_dir(const Vector3& origin, const Vector3& direction)
+    {
+        return Ray(origin, direction, FLT_MAX);
+    }
+    Vector3 interpolate(float s) const { return origin + direction * s; }
+    bool operator==(const Ray& o) const { return origin == o.origin && direction == o.direction && t == o.t; }
+
+    bvht_ray record() const
+    {
+        bvht_ray r{};
+        r.o[0] = origin.x, r.o[1] = origin.y, r.o[2] = origin.z;
+        r.d[0] = direction.x, r.d[1] = direction.y, r.d[2] = direction.z;
+        r.tmax = t;
+        return r;
+    }
+};
+
+struct SurfaceInteraction {
+    float t = 0.0f, u = 0.0f, v = 0.0f;
+
+    SurfaceInteraction() = default;
+    SurfaceInteraction(float t_, float u_, float v_) : t(t_), u(u_), v(v_) {}
+    bool operator==(const SurfaceInteraction& o) const { return t == o.t && u == o.u && v == o.v; }
+};
+
+/* Instance and primitive index packed into one word: 12 instance bits, 20
+ * primitive bits (intersection.rs:33-67). */
+class InstancePrimitiveIndex {
+public:
+    constexpr InstancePrimitiveIndex() = default;
+    constexpr InstancePrimitiveIndex(uint32_t instance_index, uint32_t primitive_index)
+        : data_(((instance_index & 0x00000FFFu) << 20) | (primitive_index & 0x000FFFFFu))
+    {
+    }
+    static constexpr InstancePrimitiveIndex from_primitive(uint32_t primitive_index)
+    {
+        return InstancePrimitiveIndex(0, primitive_index);
+    }
+    static constexpr InstancePrimitiveIndex from_word(uint32_t word)
+    {
+        InstancePrimitiveIndex r;
+        r.data_ = word;
+        return r;
+    }
+    constexpr uint32_t instance_index() const { return (data_ & 0xFFF00000u) >> 20; }
+    constexpr uint32_t primitive_index() const { return data_ & 0x000FFFFFu; }
+    constexpr bool operator==(const InstancePrimitiveIndex& o) const { return data_ == o.data_; }
+
+private:
+    uint32_t data_ = 0;
+};
+
+struct Intersection {
+    Ray ray;
+    SurfaceInteraction interaction;
+    InstancePrimitiveIndex instance_primitive;
+    /* The hit's true primitive and object indices (the packed word above
+     * keeps the reference's instance bits, always 0). */
+    uint32_t primitive = BVHT_MISS;
+    uint32_t object = BVHT_MISS;
+
+    /* The query ray with t = the hit's t, as the reference's intersect_mut
+     * leaves it; nullopt on a miss. */
+    static std::optional<Intersection> from_hit(const Ray& query, const bvht_hit& h)
+    {
+        if (h.prim == BVHT_MISS)
+            return std::nullopt;
+        Intersection r;
+        r.ray = Ray(query.origin, query.direction, h.t);
+        r.interaction = SurfaceInteraction(h.t, h.u, h.v);
+        r.instance_primitive = InstancePrimitiveIndex::from_word(h.packed);
+        r.primitive = h.prim;
+        r.object = h.inst;
+        return r;
+    }
+};
+
+/* ---- geometry (geometry/aabb.rs, geometry/triangle.rs) ------------------ */
+
+struct Aabb {
+    Vector3 bounds_min;
+    Vector3 bounds_max;
+
+    Aabb() = default;
+    Aabb(const Vector3& bmin, const Vector3& bmax) : bounds_min(bmin), bounds_max(bmax) {}
+    Vector3 extent() const { return bounds_max - bounds_min; }
+    bool operator==(const Aabb& o) const { return bounds_min == o.bounds_min && bounds_max == o.bounds_max; }
+};
+
+struct Triangle;
+std::vector<bvht_hit> intersect_pairs(const std::vector<Triangle>& tris, const std::vector<Ray>& rays,
+                                      const std::shared_ptr<Context>& ctx = default_context());
+
+struct Triangle {
+    std::array<Vector3, 3> vertices;
+
+    Triangle() = default;
+    Triangle(const Vector3& v0, const Vector3& v1, const Vector3& v2) : vertices{v0, v1, v2} {}
+    Vector3 centroid() const
+    {
+        float one_third = 1.0f / 3.0f;
+        return (vertices[0] + vertices[1] + vertices[2]) * one_third;
+    }
+    bool operator==(const Triangle& o) const { return vertices == o.vertices; }
+
+    /* Triangle::intersect (triangle.rs:41-72): t = min(ray.t, t), u, v. */
+    std::optional<SurfaceInteraction> intersect(const Ray& ray) const
+    {
+        bvht_hit h = intersect_pairs({*this}, {ray})[0];
+        if (h.prim == BVHT_MISS)
+            return std::nullopt;
+        return SurfaceInteraction(h.t, h.u, h.v);
+    }
+    /* Triangle::intersect_mut (triangle.rs:74-108). */
+    bool intersect_mut(Intersection& intersection) const
+    {
+        auto s = intersect(intersection.ray);
+        if (!s)
+            return false;
+        intersection.ray.t = s->t;
+        intersection.interaction = *s;
+        return true;
+    }
+};
+static_assert(sizeof(Triangle) == 36, "Triangle must be nine packed floats");
+
+/* Triangle::intersect for n independent (triangle, ray) pairs in one GPU
+ * call (bvht_intersect_pairs). */
+inline std::vector<bvht_hit> intersect_pairs(const std::vector<Triangle>& tris, const std::vector<Ray>& rays,
+                                             const std::shared_ptr<Context>& ctx)
+{
+    if (tris.size() != rays.size())
+        throw Error(BVHT_EINVAL, "intersect_pairs: one triangle per ray");
+    if (tris.empty())
+        return {};
+    std::vector<bvht_ray> recs;
+    recs.reserve(rays.size());
+    for (const Ray& r : rays)
+        recs.push_back(r.record());
+    DeviceBuffer<Triangle> d_tris(ctx, tris);
+    DeviceBuffer<bvht_ray> d_rays(ctx, recs);
+    DeviceBuffer<bvht_hit> d_hits(ctx, tris.size());
+    check(bvht_intersect_pairs(ctx->handle(), reinterpret_cast<const float*>(d_tris.data()), d_rays.data(),
+                               (uint32_t)tris.size(), d_hits.data()));
+    return d_hits.download(tris.size());
+}
+
+/* ---- BVH (model/bvh.rs) ------------------------------------------------- */
+
+class Bvh {
+public:
+    Bvh(std::shared_ptr<Context> ctx, bvht_blas blas) : ctx_(std::move(ctx)), blas_(blas) {}
+    ~Bvh()
+    {
+        if (blas_)
+            bvht_blas_destroy(blas_);
+    }
+    Bvh(const Bvh&) = delete;
+    Bvh& operator=(const Bvh&) = delete;
+    Bvh(Bvh&& o) noexcept : ctx_(std::move(o.ctx_)), blas_(std::exchange(o.blas_, nullptr)) {}
+
+    bvht_blas handle() const { return blas_; }
+    const std::shared_ptr<Context>& context() const { return ctx_; }
+
+    /* Bvh::intersect (bvh.rs:307-309).  `mesh` is the build-ordered slice the
+     * BVH was built over (kept for the reference's signature; the GPU holds
+     * its own copy). */
+    std::optional<Intersection> intersect(const std::vector<Triangle>& /*mesh*/, const Ray& ray) const
+    {
+        return Intersection::from_hit(ray, intersect_batch({ray})[0]);
+    }
+    /* Bvh::intersect for a batch of rays in one launch. */
+    std::vector<bvht_hit> intersect_batch(const std::vector<Ray>& rays) const
+    {
+        if (rays.empty())
+            return {};
+        std::vector<bvht_ray> recs;
+        recs.reserve(rays.size());
+        for (const Ray& r : rays)
+            recs.push_back(r.record());
+        DeviceBuffer<bvht_ray> d_rays(ctx_, recs);
+        DeviceBuffer<bvht_hit> d_hits(ctx_, rays.size());
+        check(bvht_trace_blas(ctx_->handle(), blas_, d_rays.data(), (uint32_t)rays.size(), d_hits.data(), nullptr));
+        return d_hits.download(rays.size());
+    }
+    /* Bvh::refit (bvh.rs:469-493) against the mesh's current vertices. */
+    void refit(const std::vector<Triangle>& mesh)
+    {
+        check(bvht_blas_set_vertices(ctx_->handle(), blas_, reinterpret_cast<const float*>(mesh.data()),
+                                     (uint32_t)mesh.size()));
+        check(bvht_blas_refit(ctx_->handle(), blas_));
+    }
+    /* Bvh::bounds (bvh.rs:499-501). */
+    Aabb bounds() const
+    {
+        float b[6];
+        check(bvht_blas_bounds(ctx_->handle(), blas_, b));
+        return Aabb(Vector3(b[0], b[1], b[2]), Vector3(b[3], b[4], b[5]));
+    }
+    uint32_t nodes_used() const { return info()[2]; }
+    uint32_t depth() const { return info()[3]; }
+    /* The node array (bvh.rs:228-233) as the device holds it. */
+    std::vector<bvht_node> nodes() const
+    {
+        std::vector<bvht_node> out(info()[1]);
+        check(bvht_blas_download(ctx_->handle(), blas_, out.data(), nullptr));
+        return out;
+    }
+    /* Bvh's derived PartialEq: the same node array and node count. */
+    bool operator==(const Bvh& o) const
+    {
+        auto a = nodes(), b = o.nodes();
+        return nodes_used() == o.nodes_used() && a.size() == b.size() &&
+               std::memcmp(a.data(), b.data(), a.size() * sizeof(bvht_node)) == 0;
+    }
+
+private:
+    std::array<uint32_t, 4> info() const
+    {
+        std::array<uint32_t, 4> v{};
+        check(bvht_blas_info(blas_, &v[0], &v[1], &v[2], &v[3]));
+        return v;
+    }
+
+    std::shared_ptr<Context> ctx_;
+    bvht_blas blas_ = nullptr;
+};
+
+/* BvhBuilder (bvh.rs:504-541).  build_for permutes `mesh` in place into the
+ * build order, as the reference reorders its slice.  mode BVHT_BUILD_SAH with
+ * a leaf cap is this library's binned-SAH variant. */
+class BvhBuilder {
+public:
+    explicit BvhBuilder(std::shared_ptr<Context> ctx = default_context(), int32_t mode = BVHT_BUILD_REFERENCE,
+                        uint32_t max_leaf = 0)
+        : ctx_(std::move(ctx)), mode_(mode), max_leaf_(max_leaf)
+    {
+    }
+    Bvh build_for(std::vector<Triangle>& mesh) const
+    {
+        bvht_blas b = nullptr;
+        check(bvht_blas_build(ctx_->handle(), reinterpret_cast<float*>(mesh.data()), (uint32_t)mesh.size(), mode_,
+                              max_leaf_, &b));
+        return Bvh(ctx_, b);
+    }
+
+private:
+    std::shared_ptr<Context> ctx_;
+    int32_t mode_;
+    uint32_t max_leaf_;
+};
+
+/* ---- mesh and model (mesh/mesh.rs, model/model.rs) ---------------------- */
+
+struct TextureCoordinates {
+    std::array<std::array<float, 2>, 3> uv{};
+};
+struct Normals {
+    std::array<Vector3, 3> normals{};
+};
+
+class Mesh {
+public:
+    const std::vector<Triangle>& primitives() const { return primitives_; }
+    std::vector<Triangle>& primitives_mut() { return primitives_; }
+    const std::vector<TextureCoordinates>& tex_coords() const { return tex_coords_; }
+    const std::vector<Normals>& normals() const { return normals_; }
+    size_t len() const { return primitives_.size(); }
+
+private:
+    friend class MeshBuilder;
+    std::vector<Triangle> primitives_;
+    std::vector<TextureCoordinates> tex_coords_;
+    std::vector<Normals> normals_;
+};
+
+class MeshBuilder {
+public:
+    MeshBuilder& with_primitive(const Triangle& primitive, const TextureCoordinates& tex_coords,
+                                const Normals& normals)
+    {
+        mesh_.primitives_.push_back(primitive);
+        mesh_.tex_coords_.push_back(tex_coords);
+        mesh_.normals_.push_back(normals);
+        return *this;
+    }
+    Mesh build() const { return mesh_; }
+
+private:
+    Mesh mesh_;
+};
+
+/* Model (model.rs:55-110): a mesh (triangles in build order) and its BVH on
+ * the GPU. */
+class Model {
+public:
+    Model(Mesh mesh, Bvh bvh) : mesh_(std::move(mesh)), bvh_(std::move(bvh)) {}
+    std::optional<Intersection> intersect(const Ray& ray) const { return bvh_.intersect(mesh_.primitives(), ray); }
+    /* Model::refit (model.rs:69-71) against the current primitives. */
+    void refit() { bvh_.refit(mesh_.primitives()); }
+    Aabb bounds() const { return bvh_.bounds(); }
+    const Mesh& mesh() const { return mesh_; }
+    const std::vector<Triangle>& primitives() const { return mesh_.primitives(); }
+    std::vector<Triangle>& primitives_mut() { return mesh_.primitives_mut(); }
+    size_t len() const { return mesh_.len(); }
+    size_t len_primitives() const { return mesh_.len(); }
+    const Bvh& bvh() const { return bvh_; }
+
+private:
+    Mesh mesh_;
+    Bvh bvh_;
+};
+
+/* ModelInstance (model.rs:15-53): shared ownership of one Model. */
+class ModelInstance {
+public:
+    explicit ModelInstance(std::shared_ptr<Model> model) : model_(std::move(model)) {}
+    std::optional<Intersection> intersect(const Ray& ray) const { return model_->intersect(ray); }
+    void refit() { model_->refit(); }
+    Aabb bounds() const { return model_->bounds(); }
+    std::shared_ptr<Model> model() const { return model_; }
+    size_t len() const { return model_->len(); }
+    size_t len_primitives() const { return model_->len_primitives(); }
+
+private:
+    std::shared_ptr<Model> model_;
+};
+
+/* ModelBuilder (model.rs:113-150): build() builds the BVH on the GPU over
+ * the mesh's primitives, which end up in build order. */
+class ModelBuilder {
+public:
+    explicit ModelBuilder(std::shared_ptr<Context> ctx = default_context()) : ctx_(std::move(ctx)) {}
+    ModelBuilder& with_mesh(Mesh mesh)
+    {
+        mesh_ = std::move(mesh);
+        return *this;
+    }
+    ModelBuilder& with_build(int32_t mode, uint32_t max_leaf = 0)
+    {
+        mode_ = mode, max_leaf_ = max_leaf;
+        return *this;
+    }
+    ModelInstance build()
+    {
+        Bvh bvh = BvhBuilder(ctx_, mode_, max_leaf_).build_for(mesh_.primitives_mut());
+        return ModelInstance(std::make_shared<Model>(std::move(mesh_), std::move(bvh)));
+    }
+
+private:
+    std::shared_ptr<Context> ctx_;
+    Mesh mesh_;
+    int32_t mode_ = BVHT_BUILD_REFERENCE;
+    uint32_t max_leaf_ = 0;
+};
+
+/* ---- scene (scene/scene_object.rs, scene/tlas.rs, scene/scene.rs) ------- */
+
+/* SceneObject (scene_object.rs:11-89): a model instance placed by M, with
+ * its cached inverse. */
+class SceneObject {
+public:
+    SceneObject(ModelInstance model, const Matrix4x4& transform)
+        : model_(std::move(model)), m_(transform), minv_(transform.inverse())
+    {
+    }
+    const ModelInstance& model() const { return model_; }
+    const Matrix4x4& transform() const { return m_; }
+    const Matrix4x4& transform_inv() const { return minv_; }
+    /* SceneObject::set_transform (scene_object.rs:60-75); takes effect in
+     * the scene at the next Scene::rebuild. */
+    void set_transform(const Matrix4x4& transform)
+    {
+        m_ = transform;
+        minv_ = transform.inverse();
+    }
+
+private:
+    ModelInstance model_;
+    Matrix4x4 m_, minv_;
+};
+
+class SceneObjectBuilder {
+public:
+    explicit SceneObjectBuilder(ModelInstance model) : model_(std::move(model)) {}
+    SceneObjectBuilder& with_transform(const Matrix4x4& transform)
+    {
+        m_ = transform;
+        return *this;
+    }
+    SceneObject build() const { return SceneObject(model_, m_); }
+
+private:
+    ModelInstance model_;
+    Matrix4x4 m_ = Matrix4x4::identity();
+};
+
+/* Scene (scene.rs:7-58): objects + TLAS on the GPU.  Objects made from one
+ * ModelInstance share one BLAS. */
+class Scene {
+public:
+    Scene(std::shared_ptr<Context> ctx, std::vector<SceneObject> objects)
+        : ctx_(std::move(ctx)), objects_(std::move(objects))
+    {
+        std::vector<bvht_blas> blases;
+        std::vector<bvht_instance> recs = records(&blases);
+        check(bvht_tlas_create(ctx_->handle(), blases.data(), (uint32_t)blases.size(), recs.data(),
+                               (uint32_t)recs.size(), &tlas_));
+        check(bvht_tlas_rebuild(ctx_->handle(), tlas_));
+    }
+    ~Scene()
+    {
+        if (tlas_)
+            bvht_tlas_destroy(tlas_);
+    }
+    Scene(const Scene&) = delete;
+    Scene& operator=(const Scene&) = delete;
+
+    /* Scene::intersect (scene.rs:32-34) for a world-space ray. */
+    std::optional<Intersection> intersect(const Ray& ray) const
+    {
+        return Intersection::from_hit(ray, intersect_batch({ray})[0]);
+    }
+    std::vector<bvht_hit> intersect_batch(const std::vector<Ray>& rays) const
+    {
+        if (rays.empty())
+            return {};
+        std::vector<bvht_ray> recs;
+        recs.reserve(rays.size());
+        for (const Ray& r : rays)
+            recs.push_back(r.record());
+        DeviceBuffer<bvht_ray> d_rays(ctx_, recs);
+        DeviceBuffer<bvht_hit> d_hits(ctx_, rays.size());
+        check(bvht_trace_tlas(ctx_->handle(), tlas_, d_rays.data(), (uint32_t)rays.size(), d_hits.data(), nullptr));
+        return d_hits.download(rays.size());
+    }
+    /* Scene::rebuild (scene.rs:36-38): the objects' transforms and world
+     * bounds from the models' current bounds, then Tlas::rebuild. */
+    void rebuild()
+    {
+        std::vector<bvht_instance> recs = records(nullptr);
+        check(bvht_tlas_set_objects(ctx_->handle(), tlas_, recs.data(), 0, (uint32_t)recs.size()));
+        check(bvht_tlas_rebuild(ctx_->handle(), tlas_));
+    }
+    SceneObject& get_mut_unchecked(size_t index) { return objects_[index]; }
+    const SceneObject& get_unchecked(size_t index) const { return objects_[index]; }
+    size_t len() const { return objects_.size(); }
+
+private:
+    std::vector<bvht_instance> records(std::vector<bvht_blas>* blases) const
+    {
+        std::vector<const Model*> models;
+        std::vector<bvht_instance> recs(objects_.size());
+        for (size_t i = 0; i < objects_.size(); ++i) {
+            const Model* m = objects_[i].model().model().get();
+            uint32_t k = 0;
+            while (k < models.size() && models[k] != m)
+                ++k;
+            if (k == models.size()) {
+                models.push_back(m);
+                if (blases)
+                    blases->push_back(m->bvh().handle());
+            }
+            std::memcpy(recs[i].m, objects_[i].transform().m.data(), sizeof(recs[i].m));
+            std::memcpy(recs[i].minv, objects_[i].transform_inv().m.data(), sizeof(recs[i].minv));
+            recs[i].blas = k;
+        }
+        return recs;
+    }
+
+    std::shared_ptr<Context> ctx_;
+    std::vector<SceneObject> objects_;
+    bvht_tlas tlas_ = nullptr;
+};
+
+/* SceneBuilder (scene.rs:61-96). */
+class SceneBuilder {
+public:
+    explicit SceneBuilder(std::shared_ptr<Context> ctx = default_context()) : ctx_(std::move(ctx)) {}
+    SceneBuilder& with_object(SceneObject object)
+    {
+        objects_.push_back(std::move(object));
+        return *this;
+    }
+    std::unique_ptr<Scene> build() { return std::make_unique<Scene>(ctx_, std::move(objects_)); }
+
+private:
+    std::shared_ptr<Context> ctx_;
+    std::vector<SceneObject> objects_;
+};
+
+}  // namespace bvhtracer
+
+#endif  // BVHTRACER_HPP

@@ -1,0 +1,340 @@
+// The reference crate's own tests for the traversal path, written against the
+// C++ host API (include/bvhtracer.hpp) the way the reference writes them in
+// Rust, and run on the GPU.  Each test names the reference test it restates
+// (bvhtracer/tests/*.rs, relative to the reference repository root).
+// Prints "PASS <name>" / "FAIL <name>: <why>" per test; exit status = number
+// of failures.
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "bvhtracer.hpp"
+
+using namespace bvhtracer;
+
+namespace {
+
+struct Case {
+    const char* name;
+    std::function<void()> fn;
+};
+std::vector<Case>& cases()
+{
+    static std::vector<Case> c;
+    return c;
+}
+struct Register {
+    Register(const char* name, std::function<void()> fn) { cases().push_back({name, std::move(fn)}); }
+};
+struct Failure {
+    std::string what;
+};
+#define TEST(name)                                 \
+    static void name();                            \
+    static Register reg_##name(#name, name);       \
+    static void name()
+#define CHECK(cond)                                                                      \
+    do {                                                                                 \
+        if (!(cond))                                                                     \
+            throw Failure{std::string(__FILE__ ":") + std::to_string(__LINE__) + ": " #cond}; \
+    } while (0)
+
+const float kSqrt3 = std::sqrt(3.0f);
+
+// bvhtracer/tests/test_triangle_intersection.rs:11-17 (and the meshes built from it)
+Triangle unit_triangle()
+{
+    return Triangle(Vector3(0.0f, 1.0f / 2.0f, 0.0f), Vector3(-1.0f / kSqrt3, -1.0f / 2.0f, 0.0f),
+                    Vector3(1.0f / kSqrt3, -1.0f / 2.0f, 0.0f));
+}
+
+Triangle displaced(const Triangle& t, const Vector3& d)
+{
+    return Triangle(t.vertices[0] + d, t.vertices[1] + d, t.vertices[2] + d);
+}
+
+ModelInstance model_of(const std::vector<Triangle>& tris)
+{
+    MeshBuilder builder;
+    for (const Triangle& t : tris)
+        builder.with_primitive(t, TextureCoordinates{}, Normals{});
+    return ModelBuilder().with_mesh(builder.build()).build();
+}
+
+Ray ray_towards(const Vector3& origin, const Vector3& target, float t = FLT_MAX)
+{
+    return Ray(origin, (target - origin).normalize(), t);
+}
+
+}  // namespace
+
+// ---- test_triangle_intersection.rs ----------------------------------------
+
+TEST(test_triangle_intersection_hits)  // :19-29
+{
+    CHECK(unit_triangle().intersect(ray_towards({0, 0, 5}, Vector3::zero())).has_value());
+}
+
+TEST(test_triangle_intersection_vertex_hits)  // :31-65
+{
+    Triangle tri = unit_triangle();
+    for (const Vector3& v : tri.vertices)
+        CHECK(tri.intersect(ray_towards({0, 0, 5}, v)).has_value());
+}
+
+TEST(test_triangle_intersection_center)  // :67-80
+{
+    Ray ray = ray_towards({0, 0, 5}, Vector3::zero());
+    float t = unit_triangle().intersect(ray)->t;
+    CHECK(Ray(ray.origin, ray.direction, t) == Ray(ray.origin, ray.direction, 5.0f));
+}
+
+TEST(test_triangle_intersection_vertex1)  // :82-95
+{
+    Triangle tri = unit_triangle();
+    CHECK(tri.intersect(ray_towards({0, 0, 5}, tri.vertices[0]))->t == std::sqrt(101.0f / 4.0f));
+}
+
+TEST(test_triangle_intersection_vertex2_and_3)  // :97-125
+{
+    Triangle tri = unit_triangle();
+    CHECK(tri.intersect(ray_towards({0, 0, 5}, tri.vertices[1]))->t == std::sqrt(307.0f / 12.0f));
+    CHECK(tri.intersect(ray_towards({0, 0, 5}, tri.vertices[2]))->t == std::sqrt(307.0f / 12.0f));
+}
+
+TEST(test_triangle_intersection_vertex_misses)  // :127-167
+{
+    Triangle tri = unit_triangle();
+    CHECK(!tri.intersect(ray_towards({0, 0, 5}, tri.vertices[0] + Vector3(0, 0.5f, 0))).has_value());
+    CHECK(!tri.intersect(ray_towards({0, 0, 5}, tri.vertices[1] + Vector3(0, -0.5f, 0))).has_value());
+    CHECK(!tri.intersect(ray_towards({0, 0, 5}, tri.vertices[2] + Vector3(0, -0.5f, 0))).has_value());
+}
+
+TEST(test_triangle_closest_t)  // :169-207
+{
+    for (float ray_t : {FLT_MAX, 100.0f, 0.01f}) {
+        auto r = unit_triangle().intersect(ray_towards({0, 0, 5}, Vector3::zero(), ray_t));
+        CHECK(r.has_value() && r->t <= ray_t);
+    }
+}
+
+TEST(test_triangle_intersect_mut_updates_ray)  // triangle.rs:74-108
+{
+    Intersection it;
+    it.ray = ray_towards({0, 0, 5}, Vector3::zero());
+    CHECK(unit_triangle().intersect_mut(it));
+    CHECK(it.ray.t == 5.0f && it.interaction.t == 5.0f);
+}
+
+// ---- test_bvh_one_triangle.rs ----------------------------------------------
+
+TEST(test_one_triangle_intersection_hits)  // :31-41
+{
+    CHECK(model_of({unit_triangle()}).intersect(ray_towards({0, 0, 5}, Vector3::zero())).has_value());
+}
+
+TEST(test_one_triangle_intersection_vertex_hits)  // :43-79
+{
+    ModelInstance scene = model_of({unit_triangle()});
+    for (const Vector3& v : unit_triangle().vertices)
+        CHECK(scene.intersect(ray_towards({0, 0, 5}, v)).has_value());
+}
+
+TEST(test_one_triangle_exact_t)  // :81-139
+{
+    ModelInstance scene = model_of({unit_triangle()});
+    Triangle tri = unit_triangle();
+    CHECK(scene.intersect(ray_towards({0, 0, 5}, Vector3::zero()))->interaction.t == 5.0f);
+    CHECK(scene.intersect(ray_towards({0, 0, 5}, tri.vertices[0]))->interaction.t == std::sqrt(101.0f / 4.0f));
+    CHECK(scene.intersect(ray_towards({0, 0, 5}, tri.vertices[1]))->interaction.t == std::sqrt(307.0f / 12.0f));
+}
+
+// ---- test_bvh_two_triangles.rs / test_bvh_three_triangles.rs ----------------
+
+TEST(test_two_triangles_intersection_hits_and_miss)  // two_triangles.rs:16-82
+{
+    Vector3 d(5, 0, 0);
+    ModelInstance scene = model_of({displaced(unit_triangle(), Vector3() - d), displaced(unit_triangle(), d)});
+    for (const Triangle& t : scene.model()->primitives())
+        CHECK(scene.intersect(ray_towards({0, 0, 5}, t.centroid())).has_value());
+    CHECK(!scene.intersect(ray_towards({0, 0, 5}, Vector3::zero())).has_value());
+}
+
+TEST(test_three_triangles_centroids_hit)  // three_triangles.rs:16-60
+{
+    Vector3 d(0, 10, 0);
+    ModelInstance scene = model_of({unit_triangle(), displaced(unit_triangle(), Vector3() - d),
+                                    displaced(unit_triangle(), d)});
+    for (const Triangle& t : scene.model()->primitives()) {
+        Ray ray = ray_towards({0, 0, 10}, t.centroid());
+        CHECK(t.intersect(ray).has_value());
+        CHECK(scene.intersect(ray).has_value());
+    }
+}
+
+// ---- test_bvh_closest_intersection.rs ----------------------------------------
+
+static ModelInstance stack_scene()  // :25-43: 100 triangles stacked along -z
+{
+    std::vector<Triangle> tris;
+    for (int i = 0; i < 100; ++i)
+        tris.push_back(displaced(unit_triangle(), Vector3() - Vector3(0, 0, (float)i)));
+    return model_of(tris);
+}
+
+TEST(test_scene_stack_xy)  // :45-56
+{
+    ModelInstance scene = stack_scene();
+    Triangle top = unit_triangle();
+    for (const Triangle& t : scene.model()->primitives())
+        for (int k = 0; k < 3; ++k)
+            CHECK(t.vertices[k].x == top.vertices[k].x && t.vertices[k].y == top.vertices[k].y);
+}
+
+TEST(test_intersection_should_return_closest_point_triangle)  // :58-77
+{
+    ModelInstance scene = stack_scene();
+    Triangle closest = unit_triangle();
+    Ray ray = ray_towards({0, 0, 5}, closest.centroid());
+    auto result = scene.intersect(ray);
+    auto expected = closest.intersect(ray);
+    CHECK(result.has_value() && expected.has_value());
+    CHECK(result->interaction.t == expected->t);
+}
+
+TEST(test_intersection_all_primitives_along_ray_should_hit)  // :79-93
+{
+    ModelInstance scene = stack_scene();
+    Vector3 c = unit_triangle().centroid();
+    Ray ray = ray_towards({c.x, c.y, 5}, c);
+    const auto& prims = scene.model()->primitives();
+    auto hits = intersect_pairs(prims, std::vector<Ray>(prims.size(), ray));
+    for (const bvht_hit& h : hits)
+        CHECK(h.prim != BVHT_MISS);
+}
+
+TEST(test_intersection_closest_point_should_have_lowest_t)  // :95-119
+{
+    ModelInstance scene = stack_scene();
+    Triangle closest = unit_triangle();
+    Vector3 c = closest.centroid();
+    Ray ray = ray_towards({c.x, c.y, 5}, c);
+    SurfaceInteraction expected = *closest.intersect(ray);
+    const auto& prims = scene.model()->primitives();
+    auto hits = intersect_pairs(prims, std::vector<Ray>(prims.size(), ray));
+    SurfaceInteraction result(FLT_MAX, FLT_MAX, FLT_MAX);
+    for (const bvht_hit& h : hits) {
+        SurfaceInteraction s(h.t, h.u, h.v);
+        if (!(s.t < result.t))
+            continue;
+        result = s;
+    }
+    CHECK(result == expected);
+    CHECK(scene.intersect(ray)->interaction == expected);
+}
+
+// ---- test_bvh_refit.rs -------------------------------------------------------
+
+static std::vector<Triangle> refit_mesh()  // :11-27
+{
+    Vector3 dx(5, 0, 0), dy(0, 5, 0);
+    std::vector<Triangle> mesh;
+    for (int i = -100, j = -100; i < 100; ++i, ++j)
+        mesh.push_back(displaced(unit_triangle(), dx * (float)i + dy * (float)j));
+    return mesh;
+}
+
+static void animate(std::vector<Triangle>& mesh)  // :29-40
+{
+    for (Triangle& t : mesh) {
+        t.vertices[0] += Vector3(0.3f, 0, 0);
+        t.vertices[1] += Vector3(0, 0.3f, 0);
+        t.vertices[2] += Vector3(0, 0, 0.3f);
+    }
+}
+
+TEST(test_bvh_refit_unchanged_on_same_mesh)  // :42-49
+{
+    std::vector<Triangle> mesh = refit_mesh();
+    std::vector<Triangle> copy = mesh;
+    Bvh result = BvhBuilder().build_for(mesh);
+    Bvh expected = BvhBuilder().build_for(copy);  // the reference clones the built Bvh
+    CHECK(mesh == copy);
+    result.refit(mesh);
+    CHECK(result == expected);
+}
+
+TEST(test_bvh_refit_idemotent)  // :51-61
+{
+    std::vector<Triangle> mesh = refit_mesh();
+    Bvh bvh = BvhBuilder().build_for(mesh);
+    animate(mesh);
+    bvh.refit(mesh);
+    std::vector<bvht_node> expected = bvh.nodes();
+    bvh.refit(mesh);
+    std::vector<bvht_node> result = bvh.nodes();
+    CHECK(result.size() == expected.size() &&
+          std::memcmp(result.data(), expected.data(), result.size() * sizeof(bvht_node)) == 0);
+    // refit moved the root bounds by the animation's displacement
+    Aabb b = bvh.bounds();
+    CHECK(b.bounds_max.z == 0.3f);
+}
+
+// ---- the reference's panics become errors -----------------------------------
+
+TEST(test_empty_mesh_is_an_error)  // bvh.rs:530-535 panics on an empty mesh
+{
+    std::vector<Triangle> empty;
+    bool threw = false;
+    try {
+        BvhBuilder().build_for(empty);
+    } catch (const Error& e) {
+        threw = e.code() == BVHT_EINVAL;
+    }
+    CHECK(threw);
+}
+
+// ---- scene: two instances of one model (scene.rs, tlas.rs) ------------------
+
+TEST(test_scene_two_instances_nearest_and_object_index)
+{
+    ModelInstance model = model_of({unit_triangle()});
+    SceneBuilder builder;
+    builder.with_object(SceneObjectBuilder(model).with_transform(Matrix4x4::from_affine_translation({0, 0, -2})).build());
+    builder.with_object(SceneObjectBuilder(model).with_transform(Matrix4x4::from_affine_translation({0, 0, 1})).build());
+    auto scene = builder.build();
+    // translations are exact, so the object-space rays are (0, 0, 4) and
+    // (0, 0, 7) looking down -z, and the scene's t is the triangle's own
+    Vector3 down(0, 0, -1);
+    auto hit = scene->intersect(Ray::from_origin_dir({0, 0, 5}, down));
+    CHECK(hit.has_value());
+    CHECK(hit->object == 1 && hit->interaction.t == unit_triangle().intersect(Ray::from_origin_dir({0, 0, 4}, down))->t);
+    // move the near object out of the way: the far one is hit after a rebuild
+    scene->get_mut_unchecked(1).set_transform(Matrix4x4::from_affine_translation({10, 0, 1}));
+    scene->rebuild();
+    hit = scene->intersect(Ray::from_origin_dir({0, 0, 5}, down));
+    CHECK(hit.has_value() && hit->object == 0 &&
+          hit->interaction.t == unit_triangle().intersect(Ray::from_origin_dir({0, 0, 7}, down))->t);
+}
+
+int main()
+{
+    int failed = 0;
+    for (const Case& c : cases()) {
+        try {
+            c.fn();
+            std::printf("PASS %s\n", c.name);
+        } catch (const Failure& f) {
+            std::printf("FAIL %s: %s\n", c.name, f.what.c_str());
+            ++failed;
+        } catch (const std::exception& e) {
+            std::printf("FAIL %s: exception %s\n", c.name, e.what());
+            ++failed;
+        }
+        std::fflush(stdout);
+    }
+    std::printf("%zu tests, %d failed\n", cases().size(), failed);
+    return failed;
+}
