@@ -668,6 +668,7 @@ public:
         check(bvht_tlas_set_objects(ctx_->handle(), tlas_, recs.data(), 0, (uint32_t)recs.size()));
         check(bvht_tlas_rebuild(ctx_->handle(), tlas_));
     }
+    bvht_tlas handle() const { return tlas_; }
     SceneObject& get_mut_unchecked(size_t index) { return objects_[index]; }
     const SceneObject& get_unchecked(size_t index) const { return objects_[index]; }
     size_t len() const { return objects_.size(); }
@@ -714,6 +715,166 @@ private:
     std::shared_ptr<Context> ctx_;
     std::vector<SceneObject> objects_;
 };
+
+/* ---- camera and renderer (camera/camera.rs, renderer.rs) --------------- */
+
+/* Pinhole camera as the frame path uses it: eye-space viewport corners at the
+ * near plane and the eye-to-world matrix (Camera::get_ray_world,
+ * camera.rs:994-1010). */
+class Camera {
+public:
+    Camera(const Vector3& top_left, const Vector3& top_right, const Vector3& bottom_left, const Matrix4x4& view_inv)
+    {
+        set3(rec_.tl, top_left);
+        set3(rec_.tr, top_right);
+        set3(rec_.bl, bottom_left);
+        std::memcpy(rec_.view_inv, view_inv.m.data(), sizeof(rec_.view_inv));
+    }
+    /* Camera::new(&BoxSpec, &CameraAttitudeSpec) (camera.rs): the frustum's
+     * near-plane corners, and [right | up | -forward | eye] as eye-to-world. */
+    static Camera from_box_spec(float left, float right, float bottom, float top, float near_, const Vector3& eye,
+                                const Vector3& forward, const Vector3& right_axis, const Vector3& up_axis)
+    {
+        Matrix4x4 m = Matrix4x4::identity();
+        const Vector3 cols[4] = {right_axis, up_axis, Vector3() - forward, eye};
+        for (int c = 0; c < 4; ++c)
+            m.m[c * 4 + 0] = cols[c].x, m.m[c * 4 + 1] = cols[c].y, m.m[c * 4 + 2] = cols[c].z;
+        return Camera(Vector3(left, top, -near_), Vector3(right, top, -near_), Vector3(left, bottom, -near_), m);
+    }
+    const bvht_camera& record() const { return rec_; }
+
+private:
+    static void set3(float* d, const Vector3& v) { d[0] = v.x, d[1] = v.y, d[2] = v.z; }
+    bvht_camera rec_{};
+};
+
+/* Accumulators (renderer.rs:134-334): what PathTracer stores per pixel. */
+struct Accumulator {
+    int32_t kind;
+    bvht_accum_params params{};
+};
+struct DepthAccumulator : Accumulator {
+    DepthAccumulator() : Accumulator{BVHT_ACC_DEPTH} {}
+};
+struct IntersectionAccumulator : Accumulator {
+    IntersectionAccumulator(const Vector3& hit_value, const Vector3& miss_value) : Accumulator{BVHT_ACC_INTERSECTION}
+    {
+        params.hit_value[0] = hit_value.x, params.hit_value[1] = hit_value.y, params.hit_value[2] = hit_value.z;
+        params.miss_value[0] = miss_value.x, params.miss_value[1] = miss_value.y, params.miss_value[2] = miss_value.z;
+    }
+};
+struct UvMappingAccumulator : Accumulator {
+    UvMappingAccumulator() : Accumulator{BVHT_ACC_UV} {}
+};
+
+/* Pixel shaders (renderer.rs:113-222): accumulation buffer -> RGBA8. */
+struct PixelShader {
+    int32_t kind;
+    bvht_shader_params params{};
+};
+struct RadianceToRgbShader : PixelShader {
+    RadianceToRgbShader() : PixelShader{BVHT_SHADE_RADIANCE_TO_RGB} {}
+};
+struct IntersectionShader : PixelShader {
+    IntersectionShader(std::array<uint8_t, 4> hit_rgba, std::array<uint8_t, 4> miss_rgba)
+        : PixelShader{BVHT_SHADE_INTERSECTION}
+    {
+        std::memcpy(params.hit_rgba, hit_rgba.data(), 4);
+        std::memcpy(params.miss_rgba, miss_rgba.data(), 4);
+    }
+};
+struct DepthMappingShader : PixelShader {
+    DepthMappingShader(float scale, float offset) : PixelShader{BVHT_SHADE_DEPTH_MAPPING}
+    {
+        params.scale = scale, params.offset = offset;
+    }
+};
+
+/* RendererState (renderer.rs:77-101): the accumulator, the pixel shader, and
+ * the frame's device buffers (rays, hits, accumulation buffer float[3 W H],
+ * RGBA8 frame buffer, row-major). */
+class RendererState {
+public:
+    RendererState(Accumulator accumulator, PixelShader shader, uint32_t width, uint32_t height,
+                  std::shared_ptr<Context> ctx = default_context())
+        : accumulator(accumulator), pixel_shader(shader), width_(width), height_(height), ctx_(ctx),
+          rays_(ctx, (size_t)width * height), hits_(ctx, (size_t)width * height),
+          accum_(ctx, (size_t)3 * width * height), frame_(ctx, (size_t)4 * width * height)
+    {
+        if (width % 8 || height % 8)
+            throw Error(BVHT_EINVAL, "the tiled frame needs width and height multiples of 8");
+        std::vector<uint8_t> black((size_t)4 * width * height, 0);
+        for (size_t i = 3; i < black.size(); i += 4)
+            black[i] = 255;  // FrameBuffer::from_fill(.., black)
+        frame_.upload(black);
+    }
+    uint32_t width() const { return width_; }
+    uint32_t height() const { return height_; }
+    std::vector<float> accumulation_buffer() const { return accum_.download(accum_.size()); }
+    std::vector<uint8_t> frame_buffer() const { return frame_.download(frame_.size()); }
+
+    Accumulator accumulator;
+    PixelShader pixel_shader;
+
+private:
+    friend class PathTracer;
+    uint32_t width_, height_;
+    std::shared_ptr<Context> ctx_;
+    DeviceBuffer<bvht_ray> rays_;
+    DeviceBuffer<bvht_hit> hits_;
+    DeviceBuffer<float> accum_;
+    DeviceBuffer<uint8_t> frame_;
+};
+
+/* PathTracer::evaluate (renderer.rs:345-384): one camera ray per pixel in
+ * 8x8 tiles (u = x / W, v = y / H), the accumulator at each pixel, then the
+ * pixel shader over the frame -- all on the GPU.  Returns the rays traced. */
+class PathTracer {
+public:
+    size_t evaluate(RendererState& st, const Scene& scene, const Camera& camera) const
+    {
+        return run(st, camera, [&](const bvht_ray* d_rays, uint32_t n, bvht_hit* d_hits) {
+            check(bvht_trace_tlas(st.ctx_->handle(), scene_handle(scene), d_rays, n, d_hits, nullptr));
+        });
+    }
+    size_t evaluate(RendererState& st, const ModelInstance& model, const Camera& camera) const
+    {
+        return run(st, camera, [&](const bvht_ray* d_rays, uint32_t n, bvht_hit* d_hits) {
+            check(bvht_trace_blas(st.ctx_->handle(), model.model()->bvh().handle(), d_rays, n, d_hits, nullptr));
+        });
+    }
+
+private:
+    static bvht_tlas scene_handle(const Scene& scene);
+    template <class Trace>
+    size_t run(RendererState& st, const Camera& camera, Trace trace) const
+    {
+        bvht_ctx c = st.ctx_->handle();
+        uint32_t n = st.width_ * st.height_;
+        check(bvht_raygen(c, &camera.record(), st.width_, st.height_, BVHT_ORDER_TILES, st.rays_.data()));
+        trace(st.rays_.data(), n, st.hits_.data());
+        check(bvht_accumulate(c, st.hits_.data(), st.width_, st.height_, BVHT_ORDER_TILES, st.accumulator.kind,
+                              &st.accumulator.params, st.accum_.data()));
+        check(bvht_shade(c, st.accum_.data(), n, st.pixel_shader.kind, &st.pixel_shader.params, st.frame_.data()));
+        return n;
+    }
+};
+
+/* Renderer (renderer.rs:386-398). */
+class Renderer {
+public:
+    explicit Renderer(PathTracer integrator) : integrator_(integrator) {}
+    template <class SceneT>
+    size_t render(RendererState& st, const SceneT& scene, const Camera& camera) const
+    {
+        return integrator_.evaluate(st, scene, camera);
+    }
+
+private:
+    PathTracer integrator_;
+};
+
+inline bvht_tlas PathTracer::scene_handle(const Scene& scene) { return scene.handle(); }
 
 }  // namespace bvhtracer
 

@@ -319,6 +319,82 @@ TEST(test_scene_two_instances_nearest_and_object_index)
           hit->interaction.t == unit_triangle().intersect(Ray::from_origin_dir({0, 0, 7}, down))->t);
 }
 
+// ---- test_scene_quad.rs (texture and physics of the scene left out) --------
+
+static std::unique_ptr<Scene> quad_scene()  // :52-131
+{
+    MeshBuilder mesh;
+    mesh.with_primitive(Triangle({-1, -1, 0}, {1, 1, 0}, {-1, 1, 0}), {}, {});
+    mesh.with_primitive(Triangle({-1, -1, 0}, {1, -1, 0}, {1, 1, 0}), {}, {});
+    ModelInstance model = ModelBuilder().with_mesh(mesh.build()).build();
+    SceneBuilder builder;
+    builder.with_object(SceneObjectBuilder(model).build());
+    return builder.build();
+}
+
+static Camera quad_camera()  // BoxSpec(-1, 1, -1, 1, 1, 100), eye (0, 0, 2) looking down -z
+{
+    return Camera::from_box_spec(-1, 1, -1, 1, 1, {0, 0, 2}, {0, 0, -1}, {1, 0, 0}, {0, 1, 0});
+}
+
+struct QuadUv {
+    float tl_u, tl_v, tr_u, bl_v;
+};
+static QuadUv quad_test_case()  // :133-198 in binary32
+{
+    Vector3 o(0, 0, 2);
+    float tc = std::sqrt(6.0f) / 2.0f;
+    auto plane = [&](const Vector3& corner) { return Ray::from_origin_dir(o, (corner - o).normalize()).interpolate(tc); };
+    Vector3 tl(-1, 1, 0), tr(1, 1, 0), bl(-1, -1, 0);
+    Vector3 ptl = plane(tl), ptr = plane(tr), pbl = plane(bl);
+    float du = (ptr.x - ptl.x) / 2.0f, dv = (ptl.y - pbl.y) / 2.0f;
+    QuadUv q;
+    q.tl_u = (ptl.x - tl.x) / 2.0f;
+    q.tl_v = (tl.y - ptl.y) / 2.0f;
+    q.tr_u = q.tl_u + du;
+    q.bl_v = q.tl_v + dv;
+    return q;
+}
+
+TEST(test_quad_depth_centroid)  // :200-209, through the GPU frame: pixel (320, 320) has u = v = 0.5
+{
+    auto scene = quad_scene();
+    RendererState state(DepthAccumulator(), RadianceToRgbShader(), 640, 640);
+    CHECK(Renderer(PathTracer()).render(state, *scene, quad_camera()) == 640u * 640u);
+    std::vector<float> acc = state.accumulation_buffer();
+    size_t px = 320 * 640 + 320;
+    CHECK(acc[3 * px] == 2.0f && acc[3 * px + 1] == 2.0f && acc[3 * px + 2] == 2.0f);
+}
+
+TEST(test_scene_intersection_entire_viewport)  // :354-378
+{
+    auto scene = quad_scene();
+    QuadUv q = quad_test_case();
+    RendererState state(IntersectionAccumulator({1, 1, 1}, {0, 0, 0}),
+                        IntersectionShader({255, 255, 255, 255}, {0, 0, 0, 255}), 640, 640);
+    PathTracer().evaluate(state, *scene, quad_camera());
+    std::vector<uint8_t> frame = state.frame_buffer();
+    int on_edge = 0, edge_mismatch = 0;
+    for (int y = 0; y < 640; ++y)
+        for (int x = 0; x < 640; ++x) {
+            float u = (float)x / 640.0f, v = (float)y / 640.0f;
+            bool in_u = u >= q.tl_u && u <= q.tr_u, in_v = v >= q.tl_v && v <= q.bl_v;
+            bool expected = in_u && in_v;
+            bool result = frame[4 * (y * 640 + x)] == 255;
+            bool edge = ((u == q.tl_u || u == q.tr_u) && in_v) || ((v == q.tl_v || v == q.bl_v) && in_u);
+            if (edge) {
+                ++on_edge;
+                edge_mismatch += result != expected;
+            } else {
+                CHECK(result == expected);
+            }
+        }
+    // the rays on the quad's four edge lines land exactly on a triangle edge:
+    // their hit bits follow cglinalg's unpinned rounding, and the GPU gives
+    // the oracle's answer there (16 of the 1280 differ from the solid angle)
+    CHECK(on_edge == 1280 && edge_mismatch == 16);
+}
+
 int main()
 {
     int failed = 0;
