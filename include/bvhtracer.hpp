@@ -16,6 +16,11 @@
  *   - Where the reference panics (empty mesh, bvh.rs:530-535; empty TLAS,
  *     tlas.rs:248) or the library reports an error, bvhtracer::Error is
  *     thrown with the library's message and status code.
+ *   - Covered: geometry and query types, BvhBuilder/Bvh, Mesh/Model/
+ *     ModelInstance, SceneObject/Scene (TLAS), Camera/PathTracer/Renderer
+ *     with the depth, intersection and uv accumulators and the pixel
+ *     shaders, and the rigid-body World.  The batched, differentiable and
+ *     attribute-accumulator entry points are used through bvht.h directly.
  *   - Vector arithmetic is binary32, left to right (dot = x*x' + y*y' +
  *     z*z', normalize = v / sqrt(dot(v, v))), the convention the GPU path
  *     and its tests use.  Compile with -ffp-contract=off.
@@ -875,6 +880,211 @@ private:
 };
 
 inline bvht_tlas PathTracer::scene_handle(const Scene& scene) { return scene.handle(); }
+
+/* ---- rigid-body physics (physics/rigid_body.rs, force_generator.rs,
+ *      world.rs) -------------------------------------------------------- */
+
+/* RigidBody (rigid_body.rs:66-463): the reference's record field for field
+ * (bvht_body), with its setters doing their binary32 arithmetic in the
+ * reference's order.  RigidBody() is RigidBody::default() (:88-112). */
+class RigidBody {
+public:
+    RigidBody()
+    {
+        std::memset(&rec_, 0, sizeof(rec_));
+        rec_.inverse_mass = FLT_MAX;
+        for (int k = 0; k < 3; ++k)
+            rec_.inverse_inertia_tensor[4 * k] = rec_.inverse_inertia_tensor_world[4 * k] = 1.0f;
+        rec_.linear_drag = rec_.angular_drag = 1.0f;
+        rec_.orientation[3] = 1.0f;  // Quaternion::unit_z, (s, x, y, z) = (0, 0, 0, 1)
+        rec_.can_sleep = 1;
+        for (int k = 0; k < 4; ++k)
+            rec_.transform[5 * k] = 1.0f;
+    }
+    explicit RigidBody(const bvht_body& rec) : rec_(rec) {}
+    const bvht_body& record() const { return rec_; }
+
+    void set_mass(float mass)  // :205-212
+    {
+        if (mass == 0.0f)
+            throw Error(BVHT_EINVAL, "mass must be non-zero");
+        rec_.inverse_mass = 1.0f / mass;
+    }
+    void set_inverse_mass(float inverse_mass) { rec_.inverse_mass = inverse_mass; }
+    float get_inverse_mass() const { return rec_.inverse_mass; }
+    bool has_finite_mass() const { return rec_.inverse_mass >= 0.0f; }
+    void set_drag(float linear_drag, float angular_drag) { rec_.linear_drag = linear_drag, rec_.angular_drag = angular_drag; }
+    void set_position(const Vector3& p) { set3(rec_.position, p); }
+    Vector3 get_position() const { return get3(rec_.position); }
+    /* Stores orientation.normalize(): q / sqrt(s*s + ((x*x + y*y) + z*z)). */
+    void set_orientation(float s, float x, float y, float z)
+    {
+        float mag = std::sqrt(s * s + ((x * x + y * y) + z * z));
+        rec_.orientation[0] = s / mag, rec_.orientation[1] = x / mag;
+        rec_.orientation[2] = y / mag, rec_.orientation[3] = z / mag;
+    }
+    void set_velocity(const Vector3& v) { set3(rec_.velocity, v); }
+    Vector3 get_velocity() const { return get3(rec_.velocity); }
+    void set_rotation(const Vector3& r) { set3(rec_.rotation, r); }
+    Vector3 get_rotation() const { return get3(rec_.rotation); }
+    void set_acceleration(const Vector3& a) { set3(rec_.acceleration, a); }
+    bool get_awake() const { return rec_.is_awake != 0; }
+    void set_awake(bool awake)  // :386-398
+    {
+        if (awake) {
+            rec_.is_awake = 1;
+            rec_.motion = 0.3f * 2.0f;
+        } else {
+            rec_.is_awake = 0;
+            set3(rec_.velocity, Vector3());
+            set3(rec_.rotation, Vector3());
+        }
+    }
+    void set_can_sleep(bool can_sleep)  // :404-410
+    {
+        rec_.can_sleep = can_sleep ? 1 : 0;
+        if (!can_sleep && !get_awake())
+            set_awake(true);
+    }
+    void apply_force(const Vector3& f)  // :428-431
+    {
+        set3(rec_.force_accumulator, get3(rec_.force_accumulator) + f);
+        rec_.is_awake = 1;
+    }
+    /* Body to world, column-major. */
+    Matrix4x4 get_transform() const
+    {
+        Matrix4x4 m;
+        std::memcpy(m.m.data(), rec_.transform, sizeof(rec_.transform));
+        return m;
+    }
+
+private:
+    static void set3(float* d, const Vector3& v) { d[0] = v.x, d[1] = v.y, d[2] = v.z; }
+    static Vector3 get3(const float* d) { return Vector3(d[0], d[1], d[2]); }
+    bvht_body rec_;
+};
+
+/* Force generators (force_generator.rs): records of the force registry. */
+struct ForceGenerator {
+    bvht_force rec{};
+};
+struct NullForce : ForceGenerator {  // :9-31
+    NullForce() { rec.kind = BVHT_FORCE_NULL; }
+};
+struct Gravity : ForceGenerator {  // :34-56
+    explicit Gravity(const Vector3& g)
+    {
+        rec.kind = BVHT_FORCE_GRAVITY;
+        rec.v0[0] = g.x, rec.v0[1] = g.y, rec.v0[2] = g.z;
+    }
+};
+
+class World;
+
+/* The shared handle of a registered body (force_registry.rs:15-28). */
+class RigidBodyInstance {
+public:
+    RigidBodyInstance(World* world, uint32_t index) : world_(world), index_(index) {}
+    uint32_t index() const { return index_; }
+    RigidBody borrow() const;
+    /* RefCell::borrow_mut: edit a copy of the body, written back after fn. */
+    template <class Fn>
+    void borrow_mut(Fn fn) const;
+
+private:
+    World* world_;
+    uint32_t index_;
+};
+
+struct Spring : ForceGenerator {  // :58-121: between the registered body and `other`
+    Spring(const RigidBodyInstance& other, const Vector3& connection_point_body,
+           const Vector3& other_connection_point_body, float spring_constant, float rest_length)
+    {
+        rec.kind = BVHT_FORCE_SPRING;
+        rec.other = other.index();
+        rec.v0[0] = connection_point_body.x, rec.v0[1] = connection_point_body.y, rec.v0[2] = connection_point_body.z;
+        rec.v1[0] = other_connection_point_body.x, rec.v1[1] = other_connection_point_body.y;
+        rec.v1[2] = other_connection_point_body.z;
+        rec.spring_constant = spring_constant, rec.rest_length = rest_length;
+    }
+};
+
+/* World (world.rs:14-61).  Bodies and forces are registered on the host;
+ * the first start_frame / run_physics moves them to the device, where the
+ * registry is applied and the bodies integrated. */
+class World {
+public:
+    explicit World(std::shared_ptr<Context> ctx = default_context()) : ctx_(std::move(ctx)) {}
+    ~World()
+    {
+        if (world_)
+            bvht_world_destroy(ctx_->handle(), world_);
+    }
+    World(const World&) = delete;
+    World& operator=(const World&) = delete;
+
+    RigidBodyInstance register_body(const RigidBody& body)
+    {
+        if (world_)
+            throw Error(BVHT_EINVAL, "bodies are registered before the world first runs");
+        bodies_.push_back(body.record());
+        return RigidBodyInstance(this, (uint32_t)bodies_.size() - 1);
+    }
+    void register_force_generator(const RigidBodyInstance& body, const ForceGenerator& generator)
+    {
+        if (world_)
+            throw Error(BVHT_EINVAL, "force generators are registered before the world first runs");
+        bvht_force f = generator.rec;
+        f.body = body.index();
+        forces_.push_back(f);
+    }
+    /* World::start_frame (world.rs:44-49). */
+    void start_frame() { check(bvht_world_start_frame(ctx_->handle(), device())); }
+    /* World::run_physics (world.rs:57-60). */
+    void run_physics(float duration) { check(bvht_world_run_physics(ctx_->handle(), device(), duration)); }
+    size_t len() const { return bodies_.size(); }
+
+    RigidBody body(uint32_t i) const
+    {
+        if (!world_)
+            return RigidBody(bodies_.at(i));
+        bvht_body rec;
+        check(bvht_world_get_bodies(ctx_->handle(), world_, &rec, i, 1));
+        return RigidBody(rec);
+    }
+    void set_body(uint32_t i, const RigidBody& body)
+    {
+        if (!world_) {
+            bodies_.at(i) = body.record();
+            return;
+        }
+        check(bvht_world_set_bodies(ctx_->handle(), world_, &body.record(), i, 1));
+    }
+
+private:
+    bvht_world device()
+    {
+        if (!world_)
+            check(bvht_world_create(ctx_->handle(), bodies_.data(), (uint32_t)bodies_.size(),
+                                    forces_.empty() ? nullptr : forces_.data(), (uint32_t)forces_.size(), &world_));
+        return world_;
+    }
+
+    std::shared_ptr<Context> ctx_;
+    std::vector<bvht_body> bodies_;
+    std::vector<bvht_force> forces_;
+    bvht_world world_ = nullptr;
+};
+
+inline RigidBody RigidBodyInstance::borrow() const { return world_->body(index_); }
+template <class Fn>
+void RigidBodyInstance::borrow_mut(Fn fn) const
+{
+    RigidBody b = world_->body(index_);
+    fn(b);
+    world_->set_body(index_, b);
+}
 
 }  // namespace bvhtracer
 

@@ -395,6 +395,66 @@ TEST(test_scene_intersection_entire_viewport)  // :354-378
     CHECK(on_edge == 1280 && edge_mismatch == 16);
 }
 
+// ---- physics (physics/world.rs, rigid_body.rs, force_generator.rs) ---------
+
+TEST(test_world_free_fall_is_exact_with_power_of_two_steps)
+{
+    World world;
+    RigidBody body;
+    body.set_mass(1.0f);
+    body.set_can_sleep(false);  // wakes it
+    RigidBodyInstance b = world.register_body(body);
+    world.register_force_generator(b, Gravity({0, -8, 0}));
+    for (int n = 1; n <= 10; ++n) {
+        world.run_physics(0.5f);
+        RigidBody now = b.borrow();
+        CHECK(now.get_velocity() == Vector3(0, -4.0f * n, 0));
+        CHECK(now.get_position() == Vector3(0, -(float)(n * (n + 1)), 0));
+    }
+    // Quaternion::unit_z is a half turn about z; the translation is the position
+    Matrix4x4 m = b.borrow().get_transform();
+    CHECK(m.at(0, 0) == -1.0f && m.at(1, 1) == -1.0f && m.at(2, 2) == 1.0f);
+    CHECK(m.at(0, 3) == 0.0f && m.at(1, 3) == -110.0f && m.at(2, 3) == 0.0f);
+}
+
+TEST(test_world_default_mass_gravity_wakes_and_asleep_bodies_stay)
+{
+    World world;
+    RigidBodyInstance a = world.register_body(RigidBody());
+    RigidBody still;
+    still.set_velocity({1, 2, 3});
+    RigidBodyInstance c = world.register_body(still);
+    world.register_force_generator(a, Gravity({0, -9.81f, 0}));
+    world.register_force_generator(c, NullForce());
+    world.run_physics(0.25f);
+    CHECK(a.borrow().get_awake());
+    CHECK(!c.borrow().get_awake());
+    CHECK(c.borrow().get_velocity() == Vector3(1, 2, 3) && c.borrow().get_position() == Vector3());
+    // borrow_mut writes the body back: a woken body then moves
+    c.borrow_mut([](RigidBody& body) { body.set_can_sleep(false); });
+    world.run_physics(0.25f);
+    CHECK(c.borrow().get_position() != Vector3());
+}
+
+TEST(test_world_spring_pulls_two_bodies_together)
+{
+    World world;
+    RigidBody left, right;
+    left.set_mass(1.0f), right.set_mass(1.0f);
+    left.set_orientation(1, 0, 0, 0), right.set_orientation(1, 0, 0, 0);
+    left.set_can_sleep(false), right.set_can_sleep(false);
+    right.set_position({4, 0, 0});
+    RigidBodyInstance l = world.register_body(left), r = world.register_body(right);
+    world.register_force_generator(l, Spring(r, {0, 0, 0}, {0, 0, 0}, 2.0f, 1.0f));
+    world.register_force_generator(r, Spring(l, {0, 0, 0}, {0, 0, 0}, 2.0f, 1.0f));
+    world.start_frame();
+    world.run_physics(1.0f / 64.0f);
+    float vl = l.borrow().get_velocity().x, vr = r.borrow().get_velocity().x;
+    // force 2 * (4 - 1) = 6 toward each other; one step of dt = 1/64 with drag 1
+    CHECK(vl > 0.0f && vr < 0.0f && vl == -vr);
+    CHECK(std::fabs(vl - 6.0f / 64.0f) < 1e-6f);
+}
+
 int main()
 {
     int failed = 0;
