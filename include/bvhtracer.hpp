@@ -587,8 +587,11 @@ private:
 
 /* ---- scene (scene/scene_object.rs, scene/tlas.rs, scene/scene.rs) ------- */
 
+class Tlas;
+
 /* SceneObject (scene_object.rs:11-89): a model instance placed by M, with
- * its cached inverse. */
+ * its cached inverse.  intersect() and bounds() run on the GPU through a
+ * one-object TLAS made on first use (and remade after set_transform). */
 class SceneObject {
 public:
     SceneObject(ModelInstance model, const Matrix4x4& transform)
@@ -598,17 +601,26 @@ public:
     const ModelInstance& model() const { return model_; }
     const Matrix4x4& transform() const { return m_; }
     const Matrix4x4& transform_inv() const { return minv_; }
-    /* SceneObject::set_transform (scene_object.rs:60-75); takes effect in
-     * the scene at the next Scene::rebuild. */
+    /* SceneObject::set_transform (scene_object.rs:60-75); a scene sees it at
+     * its next rebuild. */
     void set_transform(const Matrix4x4& transform)
     {
         m_ = transform;
         minv_ = transform.inverse();
+        alone_.reset();
     }
+    /* SceneObject::intersect (scene_object.rs:77-89): the world-space ray
+     * carried into model space by M^-1, traced against the model. */
+    std::optional<Intersection> intersect(const Ray& ray) const;
+    /* SceneObject::bounds (scene_object.rs:77-79): world-space bounds of the
+     * model's bounds under M. */
+    Aabb bounds() const;
 
 private:
+    const Tlas& alone() const;
     ModelInstance model_;
     Matrix4x4 m_, minv_;
+    mutable std::shared_ptr<Tlas> alone_;
 };
 
 class SceneObjectBuilder {
@@ -626,29 +638,31 @@ private:
     Matrix4x4 m_ = Matrix4x4::identity();
 };
 
-/* Scene (scene.rs:7-58): objects + TLAS on the GPU.  Objects made from one
- * ModelInstance share one BLAS. */
-class Scene {
+/* Tlas (scene/tlas.rs): the top-level BVH over scene objects, on the GPU.
+ * Objects made from one ModelInstance share one BLAS; rebuild() takes the
+ * same objects' current transforms (the set of models is fixed at build). */
+class Tlas {
 public:
-    Scene(std::shared_ptr<Context> ctx, std::vector<SceneObject> objects)
-        : ctx_(std::move(ctx)), objects_(std::move(objects))
+    Tlas(std::shared_ptr<Context> ctx, const std::vector<SceneObject>& objects) : ctx_(std::move(ctx))
     {
         std::vector<bvht_blas> blases;
-        std::vector<bvht_instance> recs = records(&blases);
+        std::vector<bvht_instance> recs = records(objects, &blases);
         check(bvht_tlas_create(ctx_->handle(), blases.data(), (uint32_t)blases.size(), recs.data(),
                                (uint32_t)recs.size(), &tlas_));
         check(bvht_tlas_rebuild(ctx_->handle(), tlas_));
     }
-    ~Scene()
+    ~Tlas()
     {
         if (tlas_)
             bvht_tlas_destroy(tlas_);
     }
-    Scene(const Scene&) = delete;
-    Scene& operator=(const Scene&) = delete;
+    Tlas(const Tlas&) = delete;
+    Tlas& operator=(const Tlas&) = delete;
+    bvht_tlas handle() const { return tlas_; }
 
-    /* Scene::intersect (scene.rs:32-34) for a world-space ray. */
-    std::optional<Intersection> intersect(const Ray& ray) const
+    /* Tlas::intersect (tlas.rs:123-177).  `objects` are the ones the TLAS
+     * was built or rebuilt from (the GPU holds their transforms). */
+    std::optional<Intersection> intersect(const std::vector<SceneObject>& /*objects*/, const Ray& ray) const
     {
         return Intersection::from_hit(ray, intersect_batch({ray})[0]);
     }
@@ -665,44 +679,103 @@ public:
         check(bvht_trace_tlas(ctx_->handle(), tlas_, d_rays.data(), (uint32_t)rays.size(), d_hits.data(), nullptr));
         return d_hits.download(rays.size());
     }
-    /* Scene::rebuild (scene.rs:36-38): the objects' transforms and world
-     * bounds from the models' current bounds, then Tlas::rebuild. */
-    void rebuild()
+    /* Tlas::rebuild (tlas.rs:204-250): the objects' transforms and world
+     * bounds from the models' current bounds, then the tree. */
+    void rebuild(const std::vector<SceneObject>& objects)
     {
-        std::vector<bvht_instance> recs = records(nullptr);
+        std::vector<bvht_instance> recs = records(objects, nullptr);
         check(bvht_tlas_set_objects(ctx_->handle(), tlas_, recs.data(), 0, (uint32_t)recs.size()));
         check(bvht_tlas_rebuild(ctx_->handle(), tlas_));
     }
-    bvht_tlas handle() const { return tlas_; }
-    SceneObject& get_mut_unchecked(size_t index) { return objects_[index]; }
-    const SceneObject& get_unchecked(size_t index) const { return objects_[index]; }
-    size_t len() const { return objects_.size(); }
+    /* The node array (tlas.rs:41-46) and its used count. */
+    std::vector<bvht_tnode> nodes(uint32_t* nodes_used = nullptr) const
+    {
+        uint32_t n = 0;
+        std::vector<bvht_tnode> out(2 * count_);
+        check(bvht_tlas_download(ctx_->handle(), tlas_, out.data(), nodes_used ? nodes_used : &n));
+        return out;
+    }
 
 private:
-    std::vector<bvht_instance> records(std::vector<bvht_blas>* blases) const
+    std::vector<bvht_instance> records(const std::vector<SceneObject>& objects, std::vector<bvht_blas>* blases)
     {
-        std::vector<const Model*> models;
-        std::vector<bvht_instance> recs(objects_.size());
-        for (size_t i = 0; i < objects_.size(); ++i) {
-            const Model* m = objects_[i].model().model().get();
+        count_ = (uint32_t)objects.size();
+        std::vector<bvht_instance> recs(objects.size());
+        for (size_t i = 0; i < objects.size(); ++i) {
+            const Model* m = objects[i].model().model().get();
             uint32_t k = 0;
-            while (k < models.size() && models[k] != m)
+            while (k < models_.size() && models_[k] != m)
                 ++k;
-            if (k == models.size()) {
-                models.push_back(m);
-                if (blases)
-                    blases->push_back(m->bvh().handle());
+            if (k == models_.size()) {
+                if (!blases)
+                    throw Error(BVHT_EINVAL, "rebuild: an object names a model the TLAS was not built with");
+                models_.push_back(m);
+                blases->push_back(m->bvh().handle());
             }
-            std::memcpy(recs[i].m, objects_[i].transform().m.data(), sizeof(recs[i].m));
-            std::memcpy(recs[i].minv, objects_[i].transform_inv().m.data(), sizeof(recs[i].minv));
+            std::memcpy(recs[i].m, objects[i].transform().m.data(), sizeof(recs[i].m));
+            std::memcpy(recs[i].minv, objects[i].transform_inv().m.data(), sizeof(recs[i].minv));
             recs[i].blas = k;
         }
         return recs;
     }
 
     std::shared_ptr<Context> ctx_;
-    std::vector<SceneObject> objects_;
     bvht_tlas tlas_ = nullptr;
+    std::vector<const Model*> models_;
+    uint32_t count_ = 0;
+};
+
+/* TlasBuilder (tlas.rs:258-279). */
+class TlasBuilder {
+public:
+    explicit TlasBuilder(std::shared_ptr<Context> ctx = default_context()) : ctx_(std::move(ctx)) {}
+    std::unique_ptr<Tlas> build_for(const std::vector<SceneObject>& objects) const
+    {
+        return std::make_unique<Tlas>(ctx_, objects);
+    }
+
+private:
+    std::shared_ptr<Context> ctx_;
+};
+
+inline const Tlas& SceneObject::alone() const
+{
+    if (!alone_)
+        alone_ = std::make_shared<Tlas>(model_.model()->bvh().context(), std::vector<SceneObject>{*this});
+    return *alone_;
+}
+inline std::optional<Intersection> SceneObject::intersect(const Ray& ray) const
+{
+    return alone().intersect({}, ray);
+}
+inline Aabb SceneObject::bounds() const
+{
+    uint32_t used = 0;
+    bvht_tnode root = alone().nodes(&used)[0];
+    return Aabb(Vector3(root.bmin[0], root.bmin[1], root.bmin[2]), Vector3(root.bmax[0], root.bmax[1], root.bmax[2]));
+}
+
+/* Scene (scene.rs:7-58): scene objects and their TLAS on the GPU. */
+class Scene {
+public:
+    Scene(std::shared_ptr<Context> ctx, std::vector<SceneObject> objects)
+        : objects_(std::move(objects)), tlas_(std::make_unique<Tlas>(std::move(ctx), objects_))
+    {
+    }
+    /* Scene::intersect (scene.rs:32-34) for a world-space ray. */
+    std::optional<Intersection> intersect(const Ray& ray) const { return tlas_->intersect(objects_, ray); }
+    std::vector<bvht_hit> intersect_batch(const std::vector<Ray>& rays) const { return tlas_->intersect_batch(rays); }
+    /* Scene::rebuild (scene.rs:36-38). */
+    void rebuild() { tlas_->rebuild(objects_); }
+    bvht_tlas handle() const { return tlas_->handle(); }
+    const Tlas& tlas() const { return *tlas_; }
+    SceneObject& get_mut_unchecked(size_t index) { return objects_[index]; }
+    const SceneObject& get_unchecked(size_t index) const { return objects_[index]; }
+    size_t len() const { return objects_.size(); }
+
+private:
+    std::vector<SceneObject> objects_;
+    std::unique_ptr<Tlas> tlas_;
 };
 
 /* SceneBuilder (scene.rs:61-96). */

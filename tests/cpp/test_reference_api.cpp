@@ -319,6 +319,32 @@ TEST(test_scene_two_instances_nearest_and_object_index)
           hit->interaction.t == unit_triangle().intersect(Ray::from_origin_dir({0, 0, 7}, down))->t);
 }
 
+TEST(test_scene_object_intersect_and_bounds)  // scene_object.rs:77-89, tlas.rs:258-279
+{
+    ModelInstance model = model_of({unit_triangle()});
+    SceneObject near = SceneObjectBuilder(model).with_transform(Matrix4x4::from_affine_translation({0, 0, 1})).build();
+    SceneObject far = SceneObjectBuilder(model).with_transform(Matrix4x4::from_affine_translation({0, 0, -2})).build();
+    Vector3 down(0, 0, -1);
+    // each object alone: its own hit, in world space
+    CHECK(near.intersect(Ray::from_origin_dir({0, 0, 5}, down))->interaction.t ==
+          unit_triangle().intersect(Ray::from_origin_dir({0, 0, 4}, down))->t);
+    CHECK(far.intersect(Ray::from_origin_dir({0, 0, 5}, down))->interaction.t ==
+          unit_triangle().intersect(Ray::from_origin_dir({0, 0, 7}, down))->t);
+    // world bounds = the model's bounds translated
+    Aabb mb = model.bounds(), b = near.bounds();
+    CHECK(b.bounds_min == mb.bounds_min + Vector3(0, 0, 1) && b.bounds_max == mb.bounds_max + Vector3(0, 0, 1));
+    near.set_transform(Matrix4x4::from_affine_translation({0, 0, 3}));
+    CHECK(near.bounds().bounds_min.z == 3.0f);
+    // TlasBuilder over both; the nearest of the two is hit
+    std::vector<SceneObject> objects{near, far};
+    auto tlas = TlasBuilder().build_for(objects);
+    uint32_t used = 0;
+    tlas->nodes(&used);
+    CHECK(used == 4);  // two leaves, their merged node, and the root that copies it (tlas.rs:204-250)
+    auto hit = tlas->intersect(objects, Ray::from_origin_dir({0, 0, 5}, down));
+    CHECK(hit.has_value() && hit->object == 0);
+}
+
 // ---- test_scene_quad.rs (texture and physics of the scene left out) --------
 
 static std::unique_ptr<Scene> quad_scene()  // :52-131
