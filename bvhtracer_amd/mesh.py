@@ -36,12 +36,19 @@ def parse_f32(tokens):
         if "_" in t:
             raise ValueError(f"invalid float literal {t!r}")
     x = np.array([float(t) for t in toks], np.float64)
-    y = x.astype(F32)
+    with np.errstate(over="ignore"):
+        y = x.astype(F32)
     # double rounding can only go wrong where x is exactly a binary32 midpoint
+    # (the overflow threshold FLT_MAX + 2^103 included)
+    overflow_mid = float(np.finfo(F32).max) + 2.0 ** 103
+    for i in np.nonzero(np.isinf(y) & np.isfinite(x) & (np.abs(x) == overflow_mid))[0]:
+        if abs(Fraction(toks[i])) < Fraction(overflow_mid):
+            y[i] = np.copysign(np.finfo(F32).max, x[i])
     off = (y.astype(np.float64) != x) & np.isfinite(x) & np.isfinite(y)
     for i in np.nonzero(off)[0]:
         toward = F32(np.inf) if x[i] > float(y[i]) else F32(-np.inf)
-        nb = np.nextafter(y[i], toward)
+        with np.errstate(over="ignore"):
+            nb = np.nextafter(y[i], toward)
         mid = (float(y[i]) + float(nb)) / 2.0
         if mid != x[i]:
             continue

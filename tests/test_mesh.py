@@ -116,3 +116,19 @@ def test_parse_f32_is_correctly_rounded():
     assert got[3] == np.float32(0.1) and got[4] == np.float32(-2.5e-3)
     assert np.isinf(got[5]) and np.isnan(got[6])
     assert float(np.float64(float(dec(above)))) == float(mid)  # the binary64 intermediate is the midpoint
+
+
+def test_parse_f32_overflow_threshold():
+    """Rust's parse rounds correctly at the overflow threshold FLT_MAX + 2^103
+    (the midpoint between FLT_MAX and 2^128): below it FLT_MAX, at it or
+    above infinity (the tie goes to the even significand, 2^128)."""
+    from fractions import Fraction
+
+    fmax = np.finfo(np.float32).max
+    mid = Fraction(float(fmax)) + Fraction(2) ** 103
+    assert str(mid) == "340282356779733661637539395458142568448"
+    below = "3.4028235677973366e38"  # within a binary64 rounding of the midpoint
+    assert Fraction(below) < mid and float(below) == float(mid)
+    r = M.parse_f32([below, "-" + below, str(mid), "3.4028235677973367e38", "1e39"])
+    assert r[0] == fmax and r[1] == -fmax
+    assert np.isinf(r[2]) and np.isinf(r[3]) and np.isinf(r[4])
