@@ -97,7 +97,7 @@ __device__ __forceinline__ float slab_key(float4 lo, float4 hi, const DRay& ray)
     float tz2 = (hi.z - ray.o.z) * ray.r.z;
     tmin = fmaxf(tmin, fminf(tz1, tz2));
     tmax = fminf(tmax, fmaxf(tz1, tz2));
-    return (tmax >= tmin && tmax > 0.0f) ? tmin : kFltMax;
+    return (tmax >= tmin && tmax > 0.0f) ? tmin : kFltMax;  // a NaN tmin or tmax gives kFltMax: never NaN
 }
 
 // Triangle::intersect, Moeller-Trumbore with the absolute 1e-4 thresholds

@@ -117,9 +117,11 @@ __device__ __forceinline__ BlasResult traverse_blas(const float4* __restrict__ n
             if (FINITE) {
                 float kl = slab_key(a0, a1, ray);
                 float kr = slab_key(b0, b1, ray);
+                // keys are never NaN, so the nearer and farther key are
+                // selects on left_near (fmaxf would add two canonicalizes)
                 left_near = kl < kr;
-                near_hit = fminf(kl, kr) < closest;
-                far_hit = fmaxf(kl, kr) < closest;
+                near_hit = (left_near ? kl : kr) < closest;
+                far_hit = (left_near ? kr : kl) < closest;
             } else {
                 float dl, dr;
                 bool hl = slab(a0, a1, ray, closest, dl);
@@ -280,8 +282,8 @@ __device__ __forceinline__ TlasResult traverse_tlas(const float4* __restrict__ t
                 float kl = slab_key(a0, a1, cr);
                 float kr = slab_key(b0, b1, cr);
                 left_near = kl < kr;
-                near_hit = fminf(kl, kr) < cclosest;
-                far_hit = fmaxf(kl, kr) < cclosest;
+                near_hit = (left_near ? kl : kr) < cclosest;
+                far_hit = (left_near ? kr : kl) < cclosest;
             } else {
                 float dl, dr;
                 bool hl = slab(a0, a1, cr, cclosest, dl);
