@@ -19,8 +19,9 @@
  *   - Covered: geometry and query types, BvhBuilder/Bvh, Mesh/Model/
  *     ModelInstance, SceneObject/Scene (TLAS), Camera/PathTracer/Renderer
  *     with the depth, intersection and uv accumulators and the pixel
- *     shaders, and the rigid-body World.  The batched, differentiable and
- *     attribute-accumulator entry points are used through bvht.h directly.
+ *     shaders, the rigid-body World, and this library's own batched trace
+ *     (Batch) and forward/adjoint pair (DiffMesh).  The attribute
+ *     accumulators are used through bvht.h directly.
  *   - Vector arithmetic is binary32, left to right (dot = x*x' + y*y' +
  *     z*z', normalize = v / sqrt(dot(v, v))), the convention the GPU path
  *     and its tests use.  Compile with -ffp-contract=off.
@@ -792,6 +793,126 @@ public:
 private:
     std::shared_ptr<Context> ctx_;
     std::vector<SceneObject> objects_;
+};
+
+/* ---- batched scenes (no reference counterpart) -------------------------- */
+
+/* Many scenes updated and traced with one launch per stage (bvht_batch_*):
+ * refit of every distinct BLAS, rebuild of every TLAS, and one persistent
+ * trace of all scenes' rays.  The scenes must live on ctx and outlive the
+ * batch. */
+class Batch {
+public:
+    Batch(std::shared_ptr<Context> ctx, const std::vector<const Scene*>& scenes) : ctx_(std::move(ctx))
+    {
+        std::vector<bvht_tlas> handles;
+        for (const Scene* sc : scenes)
+            handles.push_back(sc->handle());
+        check(bvht_batch_create(ctx_->handle(), handles.data(), (uint32_t)handles.size(), &batch_));
+        n_scenes_ = (uint32_t)handles.size();
+    }
+    ~Batch()
+    {
+        if (batch_)
+            bvht_batch_destroy(batch_);
+    }
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+
+    void refit() { check(bvht_batch_refit(ctx_->handle(), batch_)); }
+    void rebuild() { check(bvht_batch_rebuild(ctx_->handle(), batch_)); }
+    /* Scene s owns rays [offsets[s], offsets[s+1]) of the ray buffer. */
+    void set_rays(const std::vector<uint32_t>& offsets, uint32_t rays_per_item = 128)
+    {
+        if (offsets.size() != n_scenes_ + 1)
+            throw Error(BVHT_EINVAL, "set_rays: one offset per scene plus the end");
+        check(bvht_batch_set_rays(ctx_->handle(), batch_, offsets.data(), rays_per_item));
+        n_rays_ = offsets.back();
+    }
+    void config(int32_t wg, int32_t lds_depth, int32_t rays_per_item)
+    {
+        check(bvht_batch_config(ctx_->handle(), batch_, wg, lds_depth, rays_per_item));
+    }
+    /* Every scene's rays in one persistent launch (device buffers). */
+    void trace(const DeviceBuffer<bvht_ray>& rays, DeviceBuffer<bvht_hit>& hits)
+    {
+        check(bvht_batch_trace(ctx_->handle(), batch_, rays.data(), hits.data()));
+    }
+    /* Host convenience: upload, trace, download. */
+    std::vector<bvht_hit> trace(const std::vector<Ray>& rays)
+    {
+        if (rays.size() != n_rays_)
+            throw Error(BVHT_EINVAL, "trace: the ray count differs from set_rays");
+        std::vector<bvht_ray> recs;
+        recs.reserve(rays.size());
+        for (const Ray& r : rays)
+            recs.push_back(r.record());
+        DeviceBuffer<bvht_ray> d_rays(ctx_, recs);
+        DeviceBuffer<bvht_hit> d_hits(ctx_, rays.size());
+        trace(d_rays, d_hits);
+        return d_hits.download(rays.size());
+    }
+
+private:
+    std::shared_ptr<Context> ctx_;
+    bvht_batch batch_ = nullptr;
+    uint32_t n_scenes_ = 0, n_rays_ = 0;
+};
+
+/* ---- differentiable depth/shading (no reference counterpart) ------------ */
+
+/* An indexed triangle mesh (vertices float[3 nv], faces uint32[3 nf]) bound
+ * to the BLAS built from its face soup (bvht_mesh_*): forward trace + shade,
+ * the adjoint of the depth/shading L2 loss, a gradient step and the refit. */
+class DiffMesh {
+public:
+    DiffMesh(const Bvh& bvh, const std::vector<float>& vertices, const std::vector<uint32_t>& faces)
+        : ctx_(bvh.context())
+    {
+        check(bvht_mesh_create(ctx_->handle(), bvh.handle(), vertices.data(), (uint32_t)(vertices.size() / 3),
+                               faces.data(), (uint32_t)(faces.size() / 3), nullptr, &mesh_));
+        n_vertices_ = (uint32_t)(vertices.size() / 3);
+    }
+    ~DiffMesh()
+    {
+        if (mesh_)
+            bvht_mesh_destroy(mesh_);
+    }
+    DiffMesh(const DiffMesh&) = delete;
+    DiffMesh& operator=(const DiffMesh&) = delete;
+    uint32_t n_vertices() const { return n_vertices_; }
+
+    /* hits[n] as a one-object TLAS trace; image[2n] = {depth, shade}. */
+    void forward(const Matrix4x4& minv, const Vector3& light, const DeviceBuffer<bvht_ray>& rays, uint32_t n,
+                 DeviceBuffer<bvht_hit>& hits, DeviceBuffer<float>& image) const
+    {
+        float l[3] = {light.x, light.y, light.z};
+        check(bvht_trace_forward(ctx_->handle(), mesh_, minv.m.data(), l, rays.data(), n, hits.data(), image.data()));
+    }
+    /* dLoss/dvertex (float[3 nv]), dLoss/dM^-1 (float[16]) and the loss
+     * (one double) against target[2n]. */
+    void adjoint(const Matrix4x4& minv, const Vector3& light, const DeviceBuffer<bvht_ray>& rays, uint32_t n,
+                 const DeviceBuffer<bvht_hit>& hits, const DeviceBuffer<float>& target, float w_depth, float w_shade,
+                 bool accumulate, DeviceBuffer<float>& grad_vertices, DeviceBuffer<float>& grad_minv,
+                 DeviceBuffer<double>& loss) const
+    {
+        float l[3] = {light.x, light.y, light.z};
+        check(bvht_trace_adjoint(ctx_->handle(), mesh_, minv.m.data(), l, rays.data(), n, hits.data(), target.data(),
+                                 w_depth, w_shade, accumulate ? 1 : 0, grad_vertices.data(), grad_minv.data(),
+                                 loss.data()));
+    }
+    /* v -= lr g_v (each step clipped to max_step when max_step > 0). */
+    void sgd(const DeviceBuffer<float>& grad_vertices, float lr, float max_step, int32_t flags = 0)
+    {
+        check(bvht_mesh_sgd(ctx_->handle(), mesh_, grad_vertices.data(), lr, max_step, flags));
+    }
+    /* Gather the vertices into the BLAS, refit it, and recompute normals. */
+    void update(bool normals = true) { check(bvht_mesh_update(ctx_->handle(), mesh_, normals ? 1 : 0)); }
+
+private:
+    std::shared_ptr<Context> ctx_;
+    bvht_mesh mesh_ = nullptr;
+    uint32_t n_vertices_ = 0;
 };
 
 /* ---- camera and renderer (camera/camera.rs, renderer.rs) --------------- */

@@ -345,6 +345,107 @@ TEST(test_scene_object_intersect_and_bounds)  // scene_object.rs:77-89, tlas.rs:
     CHECK(hit.has_value() && hit->object == 0);
 }
 
+// ---- this library's batched trace and forward/adjoint pair -----------------
+
+static std::vector<Ray> fan_rays(size_t n, const Vector3& origin, float spread, uint32_t seed)
+{
+    std::vector<Ray> rays;
+    uint32_t s = seed;
+    auto next = [&]() {
+        s = s * 1664525u + 1013904223u;
+        return (float)(s >> 8) * (1.0f / 16777216.0f);
+    };
+    for (size_t i = 0; i < n; ++i) {
+        Vector3 target((next() - 0.5f) * spread, (next() - 0.5f) * spread, 0.0f);
+        rays.push_back(Ray::from_origin_dir(origin, (target - origin).normalize()));
+    }
+    return rays;
+}
+
+TEST(test_batch_trace_equals_each_scene)
+{
+    std::vector<Triangle> grid;
+    for (int i = -3; i <= 3; ++i)
+        for (int j = -3; j <= 3; ++j)
+            grid.push_back(displaced(unit_triangle(), Vector3((float)i, (float)j, 0.1f * (float)(i * j))));
+    ModelInstance a = model_of(grid), b = model_of({unit_triangle()});
+    SceneBuilder sa, sb;
+    sa.with_object(SceneObjectBuilder(a).with_transform(Matrix4x4::from_affine_translation({0, 0, -1})).build());
+    sb.with_object(SceneObjectBuilder(b).with_transform(Matrix4x4::from_affine_scale(2.0f)).build());
+    sb.with_object(SceneObjectBuilder(a).with_transform(Matrix4x4::from_affine_translation({0.5f, 0, -3})).build());
+    auto scene_a = sa.build(), scene_b = sb.build();
+    std::vector<Ray> ra = fan_rays(300, {0, 0, 8}, 9.0f, 1), rb = fan_rays(500, {0, 0, 8}, 9.0f, 2);
+    std::vector<Ray> all = ra;
+    all.insert(all.end(), rb.begin(), rb.end());
+    Batch batch(default_context(), {scene_a.get(), scene_b.get()});
+    batch.set_rays({0, 300, 800}, 64);
+    std::vector<bvht_hit> got = batch.trace(all);
+    std::vector<bvht_hit> want = scene_a->intersect_batch(ra), wb = scene_b->intersect_batch(rb);
+    want.insert(want.end(), wb.begin(), wb.end());
+    CHECK(got.size() == want.size() && std::memcmp(got.data(), want.data(), got.size() * sizeof(bvht_hit)) == 0);
+    size_t hits = 0;
+    for (const bvht_hit& h : got)
+        hits += h.prim != BVHT_MISS;
+    CHECK(hits > 100);
+}
+
+TEST(test_diff_mesh_forward_matches_scene_and_adjoint_loss)
+{
+    // an octahedron of radius 2, faces in order
+    std::vector<float> v = {2, 0, 0, -2, 0, 0, 0, 2, 0, 0, -2, 0, 0, 0, 2, 0, 0, -2};
+    std::vector<uint32_t> f = {0, 2, 4, 2, 1, 4, 1, 3, 4, 3, 0, 4, 2, 0, 5, 1, 2, 5, 3, 1, 5, 0, 3, 5};
+    std::vector<Triangle> soup;
+    for (size_t k = 0; k < f.size(); k += 3)
+        soup.emplace_back(Vector3(v[3 * f[k]], v[3 * f[k] + 1], v[3 * f[k] + 2]),
+                          Vector3(v[3 * f[k + 1]], v[3 * f[k + 1] + 1], v[3 * f[k + 1] + 2]),
+                          Vector3(v[3 * f[k + 2]], v[3 * f[k + 2] + 1], v[3 * f[k + 2] + 2]));
+    std::vector<Triangle> soup2 = soup;
+    Bvh bvh = BvhBuilder().build_for(soup);
+    DiffMesh mesh(bvh, v, f);
+    std::vector<Ray> rays = fan_rays(256, {0.3f, 0.2f, 9}, 5.0f, 3);
+    std::vector<bvht_ray> recs;
+    for (const Ray& r : rays)
+        recs.push_back(r.record());
+    auto ctx = default_context();
+    uint32_t n = (uint32_t)rays.size();
+    DeviceBuffer<bvht_ray> d_rays(ctx, recs);
+    DeviceBuffer<bvht_hit> d_hits(ctx, n);
+    DeviceBuffer<float> d_img(ctx, 2 * n), d_tgt(ctx, 2 * n), g_v(ctx, 3 * mesh.n_vertices()), g_m(ctx, 16);
+    DeviceBuffer<double> d_loss(ctx, 1);
+    Matrix4x4 eye = Matrix4x4::identity();
+    mesh.forward(eye, {0, 0, 1}, d_rays, n, d_hits, d_img);
+    std::vector<bvht_hit> hits = d_hits.download(n);
+    // the forward's hits are a one-object scene trace of the same soup
+    SceneBuilder sb;
+    sb.with_object(SceneObjectBuilder(model_of(soup2)).build());
+    std::vector<bvht_hit> want = sb.build()->intersect_batch(rays);
+    CHECK(std::memcmp(hits.data(), want.data(), n * sizeof(bvht_hit)) == 0);
+    // target = image: the adjoint re-solves t in binary64, so what remains is
+    // the binary32 rounding of the image (|t - t32| below half an ulp of 8)
+    std::vector<float> img = d_img.download(2 * n);
+    d_tgt.upload(img);
+    mesh.adjoint(eye, {0, 0, 1}, d_rays, n, d_hits, d_tgt, 1.0f, 1.0f, false, g_v, g_m, d_loss);
+    CHECK(d_loss.download(1)[0] < 1e-9);
+    for (float g : g_v.download(3 * mesh.n_vertices()))
+        CHECK(std::fabs(g) < 1e-3f);
+    // target depth + 0.5 on every hit pixel: loss = sum of the squared depth gaps
+    std::vector<float> tgt = img;
+    double expected = 0.0;
+    size_t hit = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        if (hits[i].prim != BVHT_MISS) {
+            tgt[2 * i] = img[2 * i] + 0.5f;
+            double d = (double)img[2 * i] - (double)tgt[2 * i];
+            expected += d * d;
+            ++hit;
+        }
+    CHECK(hit > 50);
+    d_tgt.upload(tgt);
+    mesh.adjoint(eye, {0, 0, 1}, d_rays, n, d_hits, d_tgt, 1.0f, 0.0f, false, g_v, g_m, d_loss);
+    double loss = d_loss.download(1)[0];
+    CHECK(std::fabs(loss - expected) <= 1e-4 * expected);
+}
+
 // ---- test_scene_quad.rs (texture and physics of the scene left out) --------
 
 static std::unique_ptr<Scene> quad_scene()  // :52-131
