@@ -19,9 +19,9 @@
  *   - Covered: geometry and query types, BvhBuilder/Bvh, Mesh/Model/
  *     ModelInstance, SceneObject/Scene (TLAS), Camera/PathTracer/Renderer
  *     with the depth, intersection and uv accumulators and the pixel
- *     shaders, the rigid-body World, and this library's own batched trace
- *     (Batch) and forward/adjoint pair (DiffMesh).  The attribute
- *     accumulators are used through bvht.h directly.
+ *     shaders and the normal-mapping and texture accumulators, the
+ *     rigid-body World, and this library's own batched trace (Batch) and
+ *     forward/adjoint pair (DiffMesh).
  *   - Vector arithmetic is binary32, left to right (dot = x*x' + y*y' +
  *     z*z', normalize = v / sqrt(dot(v, v))), the convention the GPU path
  *     and its tests use.  Compile with -ffp-contract=off.
@@ -498,6 +498,18 @@ public:
     const std::vector<TextureCoordinates>& tex_coords() const { return tex_coords_; }
     const std::vector<Normals>& normals() const { return normals_; }
     size_t len() const { return primitives_.size(); }
+    /* Reorder the per-triangle attributes like the triangles: new[k] =
+     * old[perm[k]] (the BVH build's permutation; the reference permutes only
+     * the triangles, bvh.rs:420-429, leaving attributes misaligned). */
+    void permute(const std::vector<uint32_t>& perm)
+    {
+        std::vector<TextureCoordinates> t(perm.size());
+        std::vector<Normals> nr(perm.size());
+        for (size_t k = 0; k < perm.size(); ++k)
+            t[k] = tex_coords_.at(perm[k]), nr[k] = normals_.at(perm[k]);
+        tex_coords_.swap(t);
+        normals_.swap(nr);
+    }
 
 private:
     friend class MeshBuilder;
@@ -573,14 +585,43 @@ public:
         mode_ = mode, max_leaf_ = max_leaf;
         return *this;
     }
+    /* ModelBuilder::with_texture (model.rs:134-139): an RGB8 image,
+     * row-major, width x height. */
+    ModelBuilder& with_texture(uint32_t width, uint32_t height, std::vector<uint8_t> rgb)
+    {
+        if (rgb.size() != (size_t)3 * width * height)
+            throw Error(BVHT_EINVAL, "with_texture: rgb must hold 3 * width * height bytes");
+        tex_w_ = width, tex_h_ = height, texture_ = std::move(rgb);
+        return *this;
+    }
+    /* Builds the BVH on the GPU (the mesh's triangles end up in build order,
+     * its attributes permuted with them) and uploads the normals, texture
+     * coordinates and texture for the attribute accumulators. */
     ModelInstance build()
     {
         Bvh bvh = BvhBuilder(ctx_, mode_, max_leaf_).build_for(mesh_.primitives_mut());
+        size_t n = mesh_.len();
+        std::vector<uint32_t> perm(n);
+        if (mode_ == BVHT_BUILD_REFERENCE || mode_ == BVHT_BUILD_SAH)
+            check(bvht_blas_permutation(ctx_->handle(), bvh.handle(), perm.data()));
+        mesh_.permute(perm);
+        std::vector<float> nrm(9 * n), tc(6 * n);
+        for (size_t k = 0; k < n; ++k)
+            for (int c = 0; c < 3; ++c) {
+                const Vector3& v = mesh_.normals()[k].normals[c];
+                nrm[9 * k + 3 * c] = v.x, nrm[9 * k + 3 * c + 1] = v.y, nrm[9 * k + 3 * c + 2] = v.z;
+                tc[6 * k + 2 * c] = mesh_.tex_coords()[k].uv[c][0];
+                tc[6 * k + 2 * c + 1] = mesh_.tex_coords()[k].uv[c][1];
+            }
+        check(bvht_blas_set_attributes(ctx_->handle(), bvh.handle(), nrm.data(), tc.data(),
+                                       texture_.empty() ? nullptr : texture_.data(), tex_w_, tex_h_));
         return ModelInstance(std::make_shared<Model>(std::move(mesh_), std::move(bvh)));
     }
 
 private:
     std::shared_ptr<Context> ctx_;
+    std::vector<uint8_t> texture_;
+    uint32_t tex_w_ = 0, tex_h_ = 0;
     Mesh mesh_;
     int32_t mode_ = BVHT_BUILD_REFERENCE;
     uint32_t max_leaf_ = 0;
@@ -965,6 +1006,17 @@ struct IntersectionAccumulator : Accumulator {
 struct UvMappingAccumulator : Accumulator {
     UvMappingAccumulator() : Accumulator{BVHT_ACC_UV} {}
 };
+/* renderer.rs:248-290: the hit triangle's interpolated corner normal carried
+ * to world space (needs the models' normals). */
+struct NormalMappingAccumulator : Accumulator {
+    NormalMappingAccumulator() : Accumulator{BVHT_ACC_NORMAL} {}
+};
+/* renderer.rs:293-334: the texel at the interpolated texture coordinates
+ * (needs the models' texture coordinates and texture). */
+struct TextureMaterialAccumulator : Accumulator {
+    TextureMaterialAccumulator() : Accumulator{BVHT_ACC_TEXTURE} {}
+};
+inline bool is_attribute_accumulator(int32_t kind) { return kind == BVHT_ACC_NORMAL || kind == BVHT_ACC_TEXTURE; }
 
 /* Pixel shaders (renderer.rs:113-222): accumulation buffer -> RGBA8. */
 struct PixelShader {
@@ -1032,32 +1084,67 @@ class PathTracer {
 public:
     size_t evaluate(RendererState& st, const Scene& scene, const Camera& camera) const
     {
-        return run(st, camera, [&](const bvht_ray* d_rays, uint32_t n, bvht_hit* d_hits) {
-            check(bvht_trace_tlas(st.ctx_->handle(), scene_handle(scene), d_rays, n, d_hits, nullptr));
-        });
+        bvht_ctx c = st.ctx_->handle();
+        bvht_tlas t = scene_handle(scene);
+        return run(
+            st, camera,
+            [&](const bvht_ray* d_rays, uint32_t n, bvht_hit* d_hits) {
+                check(bvht_trace_tlas(c, t, d_rays, n, d_hits, nullptr));
+            },
+            [&](const bvht_hit* d_hits, float* d_accum) {
+                check(bvht_accumulate_tlas(c, t, d_hits, st.width_, st.height_, BVHT_ORDER_TILES, st.accumulator.kind,
+                                           d_accum));
+            });
     }
     size_t evaluate(RendererState& st, const ModelInstance& model, const Camera& camera) const
     {
-        return run(st, camera, [&](const bvht_ray* d_rays, uint32_t n, bvht_hit* d_hits) {
-            check(bvht_trace_blas(st.ctx_->handle(), model.model()->bvh().handle(), d_rays, n, d_hits, nullptr));
-        });
+        bvht_ctx c = st.ctx_->handle();
+        bvht_blas b = model.model()->bvh().handle();
+        return run(
+            st, camera,
+            [&](const bvht_ray* d_rays, uint32_t n, bvht_hit* d_hits) {
+                check(bvht_trace_blas(c, b, d_rays, n, d_hits, nullptr));
+            },
+            [&](const bvht_hit* d_hits, float* d_accum) {
+                check(bvht_accumulate_blas(c, b, d_hits, st.width_, st.height_, BVHT_ORDER_TILES, st.accumulator.kind,
+                                           d_accum));
+            });
     }
 
 private:
     static bvht_tlas scene_handle(const Scene& scene);
-    template <class Trace>
-    size_t run(RendererState& st, const Camera& camera, Trace trace) const
+    template <class Trace, class AccumulateAttributes>
+    size_t run(RendererState& st, const Camera& camera, Trace trace, AccumulateAttributes accumulate_attributes) const
     {
         bvht_ctx c = st.ctx_->handle();
         uint32_t n = st.width_ * st.height_;
         check(bvht_raygen(c, &camera.record(), st.width_, st.height_, BVHT_ORDER_TILES, st.rays_.data()));
         trace(st.rays_.data(), n, st.hits_.data());
-        check(bvht_accumulate(c, st.hits_.data(), st.width_, st.height_, BVHT_ORDER_TILES, st.accumulator.kind,
-                              &st.accumulator.params, st.accum_.data()));
+        if (is_attribute_accumulator(st.accumulator.kind))
+            accumulate_attributes(st.hits_.data(), st.accum_.data());
+        else
+            check(bvht_accumulate(c, st.hits_.data(), st.width_, st.height_, BVHT_ORDER_TILES, st.accumulator.kind,
+                                  &st.accumulator.params, st.accum_.data()));
         check(bvht_shade(c, st.accum_.data(), n, st.pixel_shader.kind, &st.pixel_shader.params, st.frame_.data()));
         return n;
     }
 };
+
+/* An attribute accumulator (normal mapping or texture material) over hits of
+ * the scene given in ray order (order BVHT_ORDER_ROWS: hit i is pixel i of a
+ * width x height image); returns the accumulation buffer float[3 W H]. */
+inline std::vector<float> accumulate_attributes(const Scene& scene, const std::vector<bvht_hit>& hits,
+                                                uint32_t width, uint32_t height, const Accumulator& acc,
+                                                std::shared_ptr<Context> ctx = default_context())
+{
+    if (!is_attribute_accumulator(acc.kind) || hits.size() != (size_t)width * height)
+        throw Error(BVHT_EINVAL, "accumulate_attributes: an attribute accumulator and width * height hits");
+    DeviceBuffer<bvht_hit> d_hits(ctx, hits);
+    DeviceBuffer<float> d_acc(ctx, (size_t)3 * width * height);
+    check(bvht_accumulate_tlas(ctx->handle(), scene.handle(), d_hits.data(), width, height, BVHT_ORDER_ROWS, acc.kind,
+                               d_acc.data()));
+    return d_acc.download(d_acc.size());
+}
 
 /* Renderer (renderer.rs:386-398). */
 class Renderer {

@@ -10,9 +10,14 @@
 #include <string>
 #include <vector>
 
+#include <fstream>
+
 #include "bvhtracer.hpp"
+#include "bvhtracer_mesh.hpp"
 
 using namespace bvhtracer;
+
+static std::string g_golden = "tests/golden";  // the reference's assets (argv[1] overrides)
 
 namespace {
 
@@ -446,6 +451,29 @@ TEST(test_diff_mesh_forward_matches_scene_and_adjoint_loss)
     CHECK(std::fabs(loss - expected) <= 1e-4 * expected);
 }
 
+// ---- test_scene_cube.rs ------------------------------------------------------
+
+TEST(test_cube_scene_normal_mapping_known_answer)  // :139-175, 230-265
+{
+    std::ifstream in(g_golden + "/cube.obj");
+    CHECK(in.good());
+    ModelInstance model = ModelBuilder().with_mesh(ObjMeshDecoder(in).read_mesh()).build();
+    Matrix4x4 t = Matrix4x4::from_affine_scale(2.0f);
+    t.m[12] = t.m[13] = t.m[14] = -1.0f;  // scaled by 2 and moved by (-1, -1, -1)
+    SceneBuilder builder;
+    builder.with_object(SceneObjectBuilder(model).with_transform(t).build());
+    auto scene = builder.build();
+    Vector3 o(0, 4, 0);
+    std::vector<Ray> rays = {ray_towards(o, {0.5f, 1, -0.5f}), ray_towards(o, {-0.5f, 1, 0.5f})};
+    std::vector<bvht_hit> hits = scene->intersect_batch(rays);
+    CHECK(hits[0].prim == 6 && hits[1].prim == 9);  // :124-135, 215-226
+    // the top face's world normal is +y at every corner: (0.5, 1, 0.5) for both
+    // (with the attributes permuted like the triangles; the reference reads
+    // the unpermuted normal array there)
+    std::vector<float> acc = accumulate_attributes(*scene, hits, 2, 1, NormalMappingAccumulator());
+    CHECK(acc == std::vector<float>({0.5f, 1.0f, 0.5f, 0.5f, 1.0f, 0.5f}));
+}
+
 // ---- test_scene_quad.rs (texture and physics of the scene left out) --------
 
 static std::unique_ptr<Scene> quad_scene()  // :52-131
@@ -582,8 +610,10 @@ TEST(test_world_spring_pulls_two_bodies_together)
     CHECK(std::fabs(vl - 6.0f / 64.0f) < 1e-6f);
 }
 
-int main()
+int main(int argc, char** argv)
 {
+    if (argc > 1)
+        g_golden = argv[1];
     int failed = 0;
     for (const Case& c : cases()) {
         try {

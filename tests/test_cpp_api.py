@@ -20,7 +20,7 @@ def test_header_compiles_standalone():
 @pytest.mark.gpu
 def test_cpp_reference_tests_pass_on_gpu():
     assert os.path.exists(EXE), "tests/cpp/test_reference_api is built by __graft_entry__.build()"
-    r = subprocess.run([EXE], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([EXE, os.path.join(HERE, "golden")], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("PASS ") >= 29 and "FAIL" not in r.stdout
+    assert r.stdout.count("PASS ") >= 30 and "FAIL" not in r.stdout
