@@ -34,6 +34,14 @@ struct bvht_mesh_s {
     uint32_t* d_vid = nullptr;     // 3 per BLAS triangle (build order): vertex ids of its corners
     uint32_t* d_vf_off = nullptr;  // n_vertices + 1: incident faces of vertex i are d_vf_idx[off[i], off[i+1])
     uint32_t* d_vf_idx = nullptr;  // 3 n_faces, ascending face index per vertex
+    // Forward wave schedule: each forward records every 64-ray group's
+    // duration and orders the groups longest first for the next forward
+    // with the same ray count (the mesh changes little between steps).
+    uint32_t* d_sched = nullptr;   // groups: group run by wave slot k
+    uint32_t* d_cost = nullptr;    // groups: duration of each group's wave (10 ns ticks)
+    uint32_t* d_bins = nullptr;    // kSchedBins counts, then kSchedBins cursors
+    uint32_t sched_cap = 0;        // groups the buffers hold
+    uint32_t sched_n = 0;          // ray count d_sched was made for (0: none)
 };
 
 namespace bvht {
@@ -160,16 +168,29 @@ __global__ void k_sgd(float* __restrict__ x, const float* __restrict__ g, const 
 
 // ------------------------------------------------------------------ forward
 
+// Each wave runs one 64-ray group: group sched[slot] (slot = the wave's
+// position in the grid), or group slot without a schedule.  Lane 0 records
+// the wave's duration in cost[group] (s_memrealtime, 10 ns ticks).  Every
+// ray is traced exactly as without a schedule; only the order of the groups
+// over the launch changes.
 template <int WG, int LDS_DEPTH, bool PACKED>
 __global__ __launch_bounds__(WG) void k_trace_forward(const float4* __restrict__ nodes, const float* __restrict__ tris,
                                                       const uint32_t* __restrict__ vid,
                                                       const float* __restrict__ normals, Rows rows, V3 light,
                                                       const bvht_ray* __restrict__ rays, uint32_t n,
                                                       bvht_hit* __restrict__ hits, float2* __restrict__ image,
-                                                      uint32_t* __restrict__ spill, uint32_t cap)
+                                                      uint32_t* __restrict__ spill, uint32_t cap,
+                                                      const uint32_t* __restrict__ sched,
+                                                      uint32_t* __restrict__ cost)
 {
     __shared__ uint32_t lds[LDS_DEPTH * WG];
-    uint32_t i = blockIdx.x * WG + threadIdx.x;
+    uint64_t t0 = (uint64_t)wall_clock64();
+    uint32_t groups = (n + 63) / 64;
+    uint32_t slot = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+    if (slot >= groups)
+        return;
+    uint32_t group = sched ? sched[slot] : slot;
+    uint32_t i = group * 64 + (threadIdx.x & 63);
     if (i >= n)
         return;
     Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap};
@@ -189,10 +210,72 @@ __global__ __launch_bounds__(WG) void k_trace_forward(const float4* __restrict__
         shade = shade_at(normals, vid, r.prim, r.u, r.v, light);
     }
     image[i] = make_float2(depth, shade);
+    // lane 0 is always a ray of the group (group * 64 < n)
+    if ((threadIdx.x & 63) == 0) {
+        uint64_t dt = (uint64_t)wall_clock64() - t0;
+        cost[group] = dt > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)dt;
+    }
 }
 
 typedef void (*forward_fn)(const float4*, const float*, const uint32_t*, const float*, Rows, V3, const bvht_ray*,
-                           uint32_t, bvht_hit*, float2*, uint32_t*, uint32_t);
+                           uint32_t, bvht_hit*, float2*, uint32_t*, uint32_t, const uint32_t*, uint32_t*);
+
+// Longest-first order of the groups from their recorded durations: a
+// counting sort on a log scale (16 bins per octave, longest bin first).
+// The order inside a bin comes from atomics and is not fixed; the schedule
+// is always a permutation of the groups.
+constexpr uint32_t kSchedBins = 512;
+
+__device__ __forceinline__ uint32_t sched_bin(uint32_t ticks)
+{
+    // floor(16 log2(ticks + 1)) from the leading bit and the next four bits,
+    // reversed so that long waves come first
+    uint64_t v = (uint64_t)ticks + 1u;
+    uint32_t msb = 63u - (uint32_t)__builtin_clzll(v);
+    uint32_t frac = msb >= 4 ? (uint32_t)(v >> (msb - 4)) & 15u : (uint32_t)(v << (4 - msb)) & 15u;
+    uint32_t b = msb * 16u + frac;
+    return kSchedBins - 1 - (b < kSchedBins ? b : kSchedBins - 1);
+}
+
+__global__ __launch_bounds__(256) void k_sched_count(const uint32_t* __restrict__ cost, uint32_t groups,
+                                                     uint32_t* __restrict__ bins)
+{
+    __shared__ uint32_t h[kSchedBins];
+    for (uint32_t k = threadIdx.x; k < kSchedBins; k += 256)
+        h[k] = 0;
+    __syncthreads();
+    for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < groups; g += gridDim.x * 256)
+        atomicAdd(&h[sched_bin(cost[g])], 1u);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < kSchedBins; k += 256)
+        if (h[k])
+            atomicAdd(&bins[k], h[k]);
+}
+
+__global__ __launch_bounds__(kSchedBins) void k_sched_offsets(uint32_t* __restrict__ bins)
+{
+    __shared__ uint32_t part[kSchedBins];
+    uint32_t t = threadIdx.x;
+    part[t] = bins[t];
+    __syncthreads();
+    for (uint32_t off = 1; off < kSchedBins; off <<= 1) {
+        uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    bins[kSchedBins + t] = t ? part[t - 1] : 0u;  // cursor: exclusive prefix
+}
+
+__global__ __launch_bounds__(256) void k_sched_place(const uint32_t* __restrict__ cost, uint32_t groups,
+                                                     uint32_t* __restrict__ bins, uint32_t* __restrict__ sched)
+{
+    uint32_t g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= groups)
+        return;
+    uint32_t pos = atomicAdd(&bins[kSchedBins + sched_bin(cost[g])], 1u);
+    sched[pos] = g;
+}
 
 forward_fn pick_forward(int wg, int lds, bool packed)
 {
@@ -425,6 +508,9 @@ extern "C" int32_t bvht_mesh_destroy(bvht_mesh m)
     (void)hipFree(m->d_vid);
     (void)hipFree(m->d_vf_off);
     (void)hipFree(m->d_vf_idx);
+    (void)hipFree(m->d_sched);
+    (void)hipFree(m->d_cost);
+    (void)hipFree(m->d_bins);
     delete m;
     return BVHT_OK;
 }
@@ -582,9 +668,35 @@ extern "C" int32_t bvht_trace_forward(bvht_ctx ctx, bvht_mesh mesh, const float*
     forward_fn fn = pick_forward(wg, lds, mesh->blas->packed_stack);
     if (!fn)
         return set_error(BVHT_EINVAL, "no forward kernel for workgroup %d, LDS depth %d", wg, lds);
-    hipLaunchKernelGGL(fn, dim3(nblk(n, wg)), dim3(wg), 0, ctx->stream, reinterpret_cast<const float4*>(b->d_nodes),
-                       b->d_tris, mesh->d_vid, mesh->d_normals, rows_of(minv), V3{light[0], light[1], light[2]},
-                       d_rays, n, d_hits, reinterpret_cast<float2*>(d_image), spill, need);
+    uint32_t groups = nblk(n, 64);
+    if (groups > mesh->sched_cap) {
+        BVHT_HIP(hipStreamSynchronize(ctx->stream));
+        (void)hipFree(mesh->d_sched);
+        (void)hipFree(mesh->d_cost);
+        mesh->d_sched = mesh->d_cost = nullptr;
+        mesh->sched_cap = mesh->sched_n = 0;
+        if (hipMalloc(&mesh->d_sched, sizeof(uint32_t) * groups) != hipSuccess ||
+            hipMalloc(&mesh->d_cost, sizeof(uint32_t) * groups) != hipSuccess ||
+            (!mesh->d_bins && hipMalloc(&mesh->d_bins, sizeof(uint32_t) * 2 * kSchedBins) != hipSuccess))
+            return set_error(BVHT_ENOMEM, "forward schedule for %u groups", groups);
+        mesh->sched_cap = groups;
+    }
+    // the schedule made by the last forward over the same number of rays
+    const uint32_t* sched = mesh->sched_n == n ? mesh->d_sched : nullptr;
+    hipLaunchKernelGGL(fn, dim3(nblk(groups, wg / 64)), dim3(wg), 0, ctx->stream,
+                       reinterpret_cast<const float4*>(b->d_nodes), b->d_tris, mesh->d_vid, mesh->d_normals,
+                       rows_of(minv), V3{light[0], light[1], light[2]}, d_rays, n, d_hits,
+                       reinterpret_cast<float2*>(d_image), spill, need, sched, mesh->d_cost);
+    // the schedule for the next forward: longest groups first
+    int32_t rc = fill_u32(ctx, mesh->d_bins, 0u, kSchedBins);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_sched_count, dim3(std::min<uint32_t>(nblk(groups, 256), 64u)), dim3(256), 0, ctx->stream,
+                       mesh->d_cost, groups, mesh->d_bins);
+    hipLaunchKernelGGL(k_sched_offsets, dim3(1), dim3(kSchedBins), 0, ctx->stream, mesh->d_bins);
+    hipLaunchKernelGGL(k_sched_place, dim3(nblk(groups, 256)), dim3(256), 0, ctx->stream, mesh->d_cost, groups,
+                       mesh->d_bins, mesh->d_sched);
+    mesh->sched_n = n;
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }

@@ -102,6 +102,36 @@ def test_forward_bitwise_equals_oracle(ctx, orc, scene, which):
     assert (hits["prim"] != B.MISS).sum() > NRAY // 3
 
 
+def test_forward_wave_schedule_keeps_results(ctx, scene):
+    """Each forward orders its 64-ray groups longest first from the durations
+    the previous forward over the same ray count recorded.  The results are
+    the unscheduled forward's bit for bit, on repeated calls, after a call
+    with another ray count, and after the mesh moved (SGD step + refit)."""
+    w = scene["w"]
+    mesh, _ = make_mesh(ctx, w["V"], w["F"])
+    first = [a.download(np.uint8).tobytes() for a in forward(ctx, scene, mesh)]
+    for _ in range(2):
+        again = [a.download(np.uint8).tobytes() for a in forward(ctx, scene, mesh)]
+        assert again == first
+    half = NRAY // 2 + 37
+    d_h, d_i = ctx.alloc(half * B.HIT_DTYPE.itemsize), ctx.alloc(half * 8)
+    B.trace_forward(mesh, w["Minv"], w["light"], scene["d_rays"], half, d_h, d_i)
+    B.trace_forward(mesh, w["Minv"], w["light"], scene["d_rays"], half, d_h, d_i)  # scheduled, odd count
+    assert d_h.download(np.uint8).tobytes() == first[0][:half * B.HIT_DTYPE.itemsize]
+    assert d_i.download(np.uint8).tobytes() == first[1][:half * 8]
+    assert [a.download(np.uint8).tobytes() for a in forward(ctx, scene, mesh)] == first
+    # one gradient step applied to this mesh and to a fresh twin without a
+    # schedule: the scheduled forward of the moved mesh is the twin's
+    d_hits, _ = forward(ctx, scene, mesh)
+    d_gv, _, _ = adjoint(ctx, scene, scene["d_rays"], NRAY, d_hits, scene["d_target"], mesh=mesh)
+    twin, _ = make_mesh(ctx, w["V"], w["F"])
+    for m in (mesh, twin):
+        m.sgd(d_gv, scenes.DIFF_LR, scenes.DIFF_MAX_STEP)
+        m.update(normals=True)
+    assert [a.download(np.uint8).tobytes() for a in forward(ctx, scene, mesh)] == \
+        [a.download(np.uint8).tobytes() for a in forward(ctx, scene, twin)]
+
+
 @pytest.mark.parametrize("w_depth,w_shade", [(1.0, 1.0), (0.0, 1.0)])
 def test_adjoint_matches_fp64_gradient(ctx, orc, scene, w_depth, w_shade):
     mesh, w = scene["mesh"], scene["w"]
