@@ -123,7 +123,11 @@ __global__ __launch_bounds__(WG) void k_batch_trace(const BatchScene* __restrict
         uint32_t item = 0;
         if (lane == 0)
             item = atomicAdd(counter, 1u);
-        item = __shfl(item, 0);
+        // readfirstlane makes the item (and so the loop exit) wave-uniform to
+        // the compiler; with a divergent exit it may thread lane-0 code at
+        // the end of the body into this fetch and leave lanes 1-63 in a loop
+        // of their own
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
         if (item >= total)
             break;
         uint32_t s = upper_index(item_prefix, n_scenes, item);

@@ -681,7 +681,90 @@ __global__ void k_depth_acc(const bvht_hit* __restrict__ hits, uint32_t n, float
 
 namespace bvht {
 
+// Longest-first order of work units (64-ray groups, batch items) from the
+// durations their last run recorded: a counting sort on a log scale (16 bins
+// per octave, longest bin first).  The order inside a bin comes from atomics
+// and is not fixed; the schedule is always a permutation of the units.
+__device__ __forceinline__ uint32_t sched_bin(uint32_t ticks)
+{
+    // floor(16 log2(ticks + 1)) from the leading bit and the next four bits,
+    // reversed so that long waves come first
+    uint64_t v = (uint64_t)ticks + 1u;
+    uint32_t msb = 63u - (uint32_t)__builtin_clzll(v);
+    uint32_t frac = msb >= 4 ? (uint32_t)(v >> (msb - 4)) & 15u : (uint32_t)(v << (4 - msb)) & 15u;
+    uint32_t b = msb * 16u + frac;
+    return kSchedBins - 1 - (b < kSchedBins ? b : kSchedBins - 1);
+}
+
+__global__ __launch_bounds__(256) void k_sched_count(const uint32_t* __restrict__ cost, uint32_t units,
+                                                     uint32_t* __restrict__ bins)
+{
+    __shared__ uint32_t h[kSchedBins];
+    for (uint32_t k = threadIdx.x; k < kSchedBins; k += 256)
+        h[k] = 0;
+    __syncthreads();
+    for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < units; g += gridDim.x * 256)
+        atomicAdd(&h[sched_bin(cost[g])], 1u);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < kSchedBins; k += 256)
+        if (h[k])
+            atomicAdd(&bins[k], h[k]);
+}
+
+__global__ __launch_bounds__(kSchedBins) void k_sched_offsets(uint32_t* __restrict__ bins)
+{
+    __shared__ uint32_t part[kSchedBins];
+    uint32_t t = threadIdx.x;
+    part[t] = bins[t];
+    __syncthreads();
+    for (uint32_t off = 1; off < kSchedBins; off <<= 1) {
+        uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    bins[kSchedBins + t] = t ? part[t - 1] : 0u;  // cursor: exclusive prefix
+}
+
+// Each block ranks its units per bin in LDS and reserves one range per
+// non-empty bin with a single global atomic (most units fall in a few bins,
+// so one global atomic per unit would serialise on those counters).
+__global__ __launch_bounds__(256) void k_sched_place(const uint32_t* __restrict__ cost, uint32_t units,
+                                                     uint32_t* __restrict__ bins, uint32_t* __restrict__ sched)
+{
+    __shared__ uint32_t h[kSchedBins];
+    for (uint32_t k = threadIdx.x; k < kSchedBins; k += 256)
+        h[k] = 0;
+    __syncthreads();
+    uint32_t g = blockIdx.x * 256 + threadIdx.x;
+    bool valid = g < units;
+    uint32_t b = valid ? sched_bin(cost[g]) : 0u;
+    uint32_t rank = valid ? atomicAdd(&h[b], 1u) : 0u;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < kSchedBins; k += 256)
+        if (h[k])
+            h[k] = atomicAdd(&bins[kSchedBins + k], h[k]);
+    __syncthreads();
+    if (valid)
+        sched[h[b] + rank] = g;
+}
+
 static inline uint32_t blocks_for(size_t n, uint32_t wg) { return (uint32_t)((n + wg - 1) / wg); }
+
+int32_t launch_longest_first(bvht_ctx ctx, const uint32_t* d_cost, uint32_t n, uint32_t* d_bins, uint32_t* d_sched)
+{
+    if (n == 0)
+        return BVHT_OK;
+    int32_t rc = fill_u32(ctx, d_bins, 0u, kSchedBins);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_sched_count, dim3(std::min<uint32_t>(blocks_for(n, 256), 64u)), dim3(256), 0, ctx->stream,
+                       d_cost, n, d_bins);
+    hipLaunchKernelGGL(k_sched_offsets, dim3(1), dim3(kSchedBins), 0, ctx->stream, d_bins);
+    hipLaunchKernelGGL(k_sched_place, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, d_cost, n, d_bins, d_sched);
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
 static inline size_t align256(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
 
 // In-place exclusive scan of a[0, m) (m > 0, a 16-byte aligned), total to

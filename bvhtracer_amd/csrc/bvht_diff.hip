@@ -220,63 +220,6 @@ __global__ __launch_bounds__(WG) void k_trace_forward(const float4* __restrict__
 typedef void (*forward_fn)(const float4*, const float*, const uint32_t*, const float*, Rows, V3, const bvht_ray*,
                            uint32_t, bvht_hit*, float2*, uint32_t*, uint32_t, const uint32_t*, uint32_t*);
 
-// Longest-first order of the groups from their recorded durations: a
-// counting sort on a log scale (16 bins per octave, longest bin first).
-// The order inside a bin comes from atomics and is not fixed; the schedule
-// is always a permutation of the groups.
-constexpr uint32_t kSchedBins = 512;
-
-__device__ __forceinline__ uint32_t sched_bin(uint32_t ticks)
-{
-    // floor(16 log2(ticks + 1)) from the leading bit and the next four bits,
-    // reversed so that long waves come first
-    uint64_t v = (uint64_t)ticks + 1u;
-    uint32_t msb = 63u - (uint32_t)__builtin_clzll(v);
-    uint32_t frac = msb >= 4 ? (uint32_t)(v >> (msb - 4)) & 15u : (uint32_t)(v << (4 - msb)) & 15u;
-    uint32_t b = msb * 16u + frac;
-    return kSchedBins - 1 - (b < kSchedBins ? b : kSchedBins - 1);
-}
-
-__global__ __launch_bounds__(256) void k_sched_count(const uint32_t* __restrict__ cost, uint32_t groups,
-                                                     uint32_t* __restrict__ bins)
-{
-    __shared__ uint32_t h[kSchedBins];
-    for (uint32_t k = threadIdx.x; k < kSchedBins; k += 256)
-        h[k] = 0;
-    __syncthreads();
-    for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < groups; g += gridDim.x * 256)
-        atomicAdd(&h[sched_bin(cost[g])], 1u);
-    __syncthreads();
-    for (uint32_t k = threadIdx.x; k < kSchedBins; k += 256)
-        if (h[k])
-            atomicAdd(&bins[k], h[k]);
-}
-
-__global__ __launch_bounds__(kSchedBins) void k_sched_offsets(uint32_t* __restrict__ bins)
-{
-    __shared__ uint32_t part[kSchedBins];
-    uint32_t t = threadIdx.x;
-    part[t] = bins[t];
-    __syncthreads();
-    for (uint32_t off = 1; off < kSchedBins; off <<= 1) {
-        uint32_t v = t >= off ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    bins[kSchedBins + t] = t ? part[t - 1] : 0u;  // cursor: exclusive prefix
-}
-
-__global__ __launch_bounds__(256) void k_sched_place(const uint32_t* __restrict__ cost, uint32_t groups,
-                                                     uint32_t* __restrict__ bins, uint32_t* __restrict__ sched)
-{
-    uint32_t g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= groups)
-        return;
-    uint32_t pos = atomicAdd(&bins[kSchedBins + sched_bin(cost[g])], 1u);
-    sched[pos] = g;
-}
-
 forward_fn pick_forward(int wg, int lds, bool packed)
 {
 #define X(W, D)                                                           \
@@ -688,14 +631,9 @@ extern "C" int32_t bvht_trace_forward(bvht_ctx ctx, bvht_mesh mesh, const float*
                        rows_of(minv), V3{light[0], light[1], light[2]}, d_rays, n, d_hits,
                        reinterpret_cast<float2*>(d_image), spill, need, sched, mesh->d_cost);
     // the schedule for the next forward: longest groups first
-    int32_t rc = fill_u32(ctx, mesh->d_bins, 0u, kSchedBins);
+    int32_t rc = launch_longest_first(ctx, mesh->d_cost, groups, mesh->d_bins, mesh->d_sched);
     if (rc)
         return rc;
-    hipLaunchKernelGGL(k_sched_count, dim3(std::min<uint32_t>(nblk(groups, 256), 64u)), dim3(256), 0, ctx->stream,
-                       mesh->d_cost, groups, mesh->d_bins);
-    hipLaunchKernelGGL(k_sched_offsets, dim3(1), dim3(kSchedBins), 0, ctx->stream, mesh->d_bins);
-    hipLaunchKernelGGL(k_sched_place, dim3(nblk(groups, 256)), dim3(256), 0, ctx->stream, mesh->d_cost, groups,
-                       mesh->d_bins, mesh->d_sched);
     mesh->sched_n = n;
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;

@@ -90,6 +90,10 @@ int32_t compute_blas_topology(bvht_blas b, const bvht_node* h_nodes);
 int32_t ensure_spill(bvht_ctx ctx, uint32_t** buf, size_t* words, size_t need);
 int32_t blas_alloc(bvht_ctx ctx, uint32_t n, uint32_t node_count, bvht_blas* out);
 int32_t fill_u32(bvht_ctx ctx, uint32_t* p, uint32_t v, size_t n);
+// Longest first: d_sched[0, n) = the units ordered by descending d_cost (a
+// permutation of 0..n-1); d_bins holds 2 * kSchedBins counters.
+constexpr uint32_t kSchedBins = 512;
+int32_t launch_longest_first(bvht_ctx ctx, const uint32_t* d_cost, uint32_t n, uint32_t* d_bins, uint32_t* d_sched);
 void trace_config(bvht_ctx ctx, uint32_t need, int* wg, int* lds);
 int32_t launch_bounce_sorted(bvht_ctx ctx, const float* d_tris, uint32_t ntris, const void* rows, const void* blases,
                              uint32_t n_obj, const bvht_ray* d_rays, const bvht_hit* d_hits, uint32_t n,

@@ -54,6 +54,11 @@ def test_batched_trace_equals_plain_and_oracle(ctx, orc):
         ref = oracle_scene_hits(orc, models[s], tlases[s], work[s]["Minv"], rays[sl])
         assert_hits_equal(got[sl], ref, f"batched scene {s} vs oracle")
     assert (got["prim"] != B.MISS).sum() > 1000
+    # the queue order differs from trace to trace: every ray's hit does not
+    for _ in range(2):
+        d_again = ctx.alloc(rays.shape[0] * B.HIT_DTYPE.itemsize)
+        batch.trace(d_rays, d_again)
+        assert d_again.download(B.HIT_DTYPE, rays.shape[0]).tobytes() == got.tobytes()
 
 
 def test_every_batch_config_is_bit_identical(ctx):
