@@ -232,7 +232,10 @@ forward_fn pick_forward(int wg, int lds, bool packed)
 
 // ------------------------------------------------------------------ adjoint
 
-constexpr int kAdjWG = 256;
+constexpr int kAdjWG = 512;  // 256 and 1024 measured slower
+constexpr uint32_t kAdjSlotBits = 11;  // LDS vertex table of the scatter: 2048 > 3 * kAdjWG slots
+constexpr uint32_t kAdjSlots = 1u << kAdjSlotBits;
+static_assert(3 * kAdjWG < kAdjSlots, "the scatter's vertex table must keep an empty slot");
 constexpr int kRed = 13;  // 12 entries of rows 0..2 of dLoss/dM^-1 (row-major r*4+c) + loss
 
 // Binary64 vector helpers for the adjoint's per-pixel solve.
@@ -342,33 +345,43 @@ __global__ __launch_bounds__(kAdjWG) void k_trace_adjoint(
         }
         acc[12] = (double)w_depth * rt * rt + (double)w_shade * rs * rs;
     }
-    // Vertex scatter.  Neighbouring pixels mostly hit the same triangle: the
-    // lanes of a wave that share one add their contributions first, and one
-    // leader per distinct triangle issues the nine atomics.
-    uint32_t lane_id = threadIdx.x & 63u;
-    unsigned long long todo = __ballot(key != BVHT_MISS);
-    while (todo) {
-        int leader = __ffsll(todo) - 1;
-        uint32_t k = __shfl(key, leader, 64);
-        bool mine = key == k;
-        unsigned long long members = __ballot(mine);
-        float sum[9];
+    // Vertex scatter.  The block's pixels hit a few dozen neighbouring
+    // triangles that share their vertices: each corner's contribution is
+    // added into an LDS table keyed by vertex id (linear probing; at most
+    // 3 * kAdjWG keys in kAdjSlots slots, so every probe ends), and the block
+    // then issues one global atomic triple per distinct vertex.
+    __shared__ uint32_t tkey[kAdjSlots];
+    __shared__ float tval[3][kAdjSlots];
+    for (uint32_t k = threadIdx.x; k < kAdjSlots; k += kAdjWG) {
+        tkey[k] = BVHT_MISS;
+        tval[0][k] = tval[1][k] = tval[2][k] = 0.0f;
+    }
+    __syncthreads();
+    if (key != BVHT_MISS) {
 #pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            sum[c] = mine ? cv[c] : 0.0f;
-            for (int m = 32; m >= 1; m >>= 1)
-                sum[c] += __shfl_xor(sum[c], m, 64);
-        }
-        if (lane_id == (uint32_t)leader) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float* g = grad_v + 3 * (size_t)vid3[c];
-                atomicAdd(g + 0, sum[3 * c + 0]);
-                atomicAdd(g + 1, sum[3 * c + 1]);
-                atomicAdd(g + 2, sum[3 * c + 2]);
+        for (int c = 0; c < 3; ++c) {
+            uint32_t vtx = vid3[c];
+            uint32_t h = (vtx * 2654435761u) >> (32 - kAdjSlotBits);
+            while (true) {
+                uint32_t old = atomicCAS(&tkey[h], BVHT_MISS, vtx);
+                if (old == BVHT_MISS || old == vtx)
+                    break;
+                h = (h + 1) & (kAdjSlots - 1);
             }
+            atomicAdd(&tval[0][h], cv[3 * c + 0]);
+            atomicAdd(&tval[1][h], cv[3 * c + 1]);
+            atomicAdd(&tval[2][h], cv[3 * c + 2]);
         }
-        todo &= ~members;
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < kAdjSlots; k += kAdjWG) {
+        uint32_t vtx = tkey[k];
+        if (vtx != BVHT_MISS) {
+            float* g = grad_v + 3 * (size_t)vtx;
+            atomicAdd(g + 0, tval[0][k]);
+            atomicAdd(g + 1, tval[1][k]);
+            atomicAdd(g + 2, tval[2][k]);
+        }
     }
     // fixed-order reduction: butterfly within the wave, then the waves in order
 #pragma unroll
