@@ -77,6 +77,29 @@ __global__ void k_batch_refit_level(const BatchBlas* __restrict__ blases, const 
     refit_node(bb.nodes, bb.tris, e.y);
 }
 
+// Consecutive small levels in one workgroup, deepest first, a barrier
+// between levels (one workgroup shares one CU, so its own global writes are
+// visible to it after the barrier): level j is entries[begin[j], end[j]).
+constexpr uint32_t kRunWG = 1024;
+constexpr uint32_t kRunLevels = 32;
+struct EntryRun {
+    uint32_t levels;
+    uint32_t begin[kRunLevels], end[kRunLevels];
+};
+
+__global__ __launch_bounds__(kRunWG) void k_batch_refit_levels(const BatchBlas* __restrict__ blases,
+                                                               const uint2* __restrict__ entries, EntryRun run)
+{
+    for (uint32_t j = 0; j < run.levels; ++j) {
+        for (uint32_t i = run.begin[j] + threadIdx.x; i < run.end[j]; i += kRunWG) {
+            uint2 e = entries[i];
+            BatchBlas bb = blases[e.x];
+            refit_node(bb.nodes, bb.tris, e.y);
+        }
+        __syncthreads();
+    }
+}
+
 // Big leaves of every BLAS of the batch, one wave per (BLAS, leaf) entry.
 __global__ __launch_bounds__(256) void k_batch_refit_big(const BatchBlas* __restrict__ blases,
                                                          const uint2* __restrict__ entries, uint32_t count)
@@ -379,12 +402,31 @@ int32_t bvht_batch_refit(bvht_ctx ctx, bvht_batch b)
     if (b->n_refit_big)
         hipLaunchKernelGGL(k_batch_refit_big, dim3(nblk(b->n_refit_big, 4)), dim3(256), 0, ctx->stream, b->d_blases,
                            b->d_refit_big, b->n_refit_big);
+    // a level of more than kRunWG entries gets a launch of its own; runs of
+    // consecutive smaller levels share one single-workgroup launch
+    EntryRun run{};
+    auto flush = [&]() {
+        if (run.levels)
+            hipLaunchKernelGGL(k_batch_refit_levels, dim3(1), dim3(kRunWG), 0, ctx->stream, b->d_blases, b->d_refit,
+                               run);
+        run.levels = 0;
+    };
     for (size_t L = b->refit_off.size() - 1; L-- > 0;) {
         uint32_t off = b->refit_off[L], cnt = b->refit_off[L + 1] - off;
-        if (cnt)
+        if (cnt == 0)
+            continue;
+        if (cnt > kRunWG) {
+            flush();
             hipLaunchKernelGGL(k_batch_refit_level, dim3(nblk(cnt, 256)), dim3(256), 0, ctx->stream, b->d_blases,
                                b->d_refit + off, cnt);
+            continue;
+        }
+        run.begin[run.levels] = off;
+        run.end[run.levels] = off + cnt;
+        if (++run.levels == kRunLevels)
+            flush();
     }
+    flush();
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }

@@ -207,6 +207,27 @@ __global__ void k_refit_level(bvht_node* __restrict__ nodes, const float* __rest
         refit_node(nodes, tris, order[i]);
 }
 
+// Consecutive small levels of Bvh::refit in one workgroup: level j of the
+// run is order[begin[j], end[j]), the run goes deepest first, and a barrier
+// separates the levels (all threads of a workgroup share one CU, so its own
+// global writes are visible to it after the barrier).
+constexpr uint32_t kRunWG = 1024;     // threads of the one-workgroup pass
+constexpr uint32_t kRunLevels = 32;   // levels per run
+struct LevelRun {
+    uint32_t levels;
+    uint32_t begin[kRunLevels], end[kRunLevels];
+};
+
+__global__ __launch_bounds__(kRunWG) void k_refit_levels(bvht_node* __restrict__ nodes, const float* __restrict__ tris,
+                                                         const uint32_t* __restrict__ order, LevelRun run)
+{
+    for (uint32_t j = 0; j < run.levels; ++j) {
+        for (uint32_t i = run.begin[j] + threadIdx.x; i < run.end[j]; i += kRunWG)
+            refit_node(nodes, tris, order[i]);
+        __syncthreads();
+    }
+}
+
 template <int WG, int LDS_DEPTH, bool STATS, bool PACKED>
 __global__ __launch_bounds__(WG) void k_trace_blas(const float4* __restrict__ nodes, const float* __restrict__ tris,
                                                    const bvht_ray* __restrict__ rays, bvht_hit* __restrict__ hits,
@@ -860,19 +881,37 @@ int32_t compute_blas_topology(bvht_blas b, const bvht_node*)
     return BVHT_OK;
 }
 
-// Bvh::refit: one launch per tree level, deepest first.
+// Bvh::refit, deepest level first: a level of more than kRunWG nodes gets a
+// launch of its own; runs of consecutive smaller levels (the top of the tree
+// and sparse deep levels) share one single-workgroup launch.
 int32_t launch_refit(bvht_ctx ctx, bvht_blas blas)
 {
     if (blas->n_big_leaves)
         hipLaunchKernelGGL(k_refit_big_leaves, dim3(blocks_for(blas->n_big_leaves, 4)), dim3(256), 0, ctx->stream,
                            blas->d_nodes, blas->d_tris, blas->d_big_leaves, blas->n_big_leaves);
+    LevelRun run{};
+    auto flush = [&]() {
+        if (run.levels)
+            hipLaunchKernelGGL(k_refit_levels, dim3(1), dim3(kRunWG), 0, ctx->stream, blas->d_nodes, blas->d_tris,
+                               blas->d_level_order, run);
+        run.levels = 0;
+    };
     for (size_t L = blas->depth; L-- > 0;) {
         uint32_t off = blas->level_off[L], cnt = blas->level_off[L + 1] - off;
         if (cnt == 0)
             continue;
-        hipLaunchKernelGGL(k_refit_level, dim3(blocks_for(cnt, 256)), dim3(256), 0, ctx->stream, blas->d_nodes,
-                           blas->d_tris, blas->d_level_order + off, cnt);
+        if (cnt > kRunWG) {
+            flush();
+            hipLaunchKernelGGL(k_refit_level, dim3(blocks_for(cnt, 256)), dim3(256), 0, ctx->stream, blas->d_nodes,
+                               blas->d_tris, blas->d_level_order + off, cnt);
+            continue;
+        }
+        run.begin[run.levels] = off;
+        run.end[run.levels] = off + cnt;
+        if (++run.levels == kRunLevels)
+            flush();
     }
+    flush();
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }
