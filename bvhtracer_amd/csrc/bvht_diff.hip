@@ -404,37 +404,57 @@ __global__ __launch_bounds__(kAdjWG) void k_trace_adjoint(
 }
 
 // Sum the per-block partials in a fixed order; write (or add) dLoss/dM^-1
-// (column-major, bottom row 0) and the loss.
-__global__ __launch_bounds__(256) void k_adjoint_finish(const double* __restrict__ partials, uint32_t nblocks,
-                                                        int accumulate, float* __restrict__ grad_minv,
-                                                        double* __restrict__ loss)
+// (column-major, bottom row 0) and the loss.  Thread t sums blocks t,
+// t + kFinWG, ... for all components at once (each block's partials are one
+// 128-byte row), then a butterfly within each wave and the waves in order.
+constexpr int kFinWG = 1024;
+
+__global__ __launch_bounds__(kFinWG) void k_adjoint_finish(const double* __restrict__ partials, uint32_t nblocks,
+                                                           int accumulate, float* __restrict__ grad_minv,
+                                                           double* __restrict__ loss)
 {
-    __shared__ double red[256];
-    for (int c = 0; c < kRed; ++c) {
-        double s = 0.0;
-        for (uint32_t b = threadIdx.x; b < nblocks; b += 256)
-            s += partials[(size_t)b * 16 + c];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (uint32_t h = 128; h > 0; h >>= 1) {
-            if (threadIdx.x < h)
-                red[threadIdx.x] += red[threadIdx.x + h];
-            __syncthreads();
+    __shared__ double red[kFinWG / 64][kRed];
+    double acc[kRed];
+#pragma unroll
+    for (int c = 0; c < kRed; ++c)
+        acc[c] = 0.0;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += kFinWG) {
+        const double2* row = reinterpret_cast<const double2*>(partials + (size_t)b * 16);
+#pragma unroll
+        for (int c = 0; c + 1 < kRed; c += 2) {
+            double2 v = row[c / 2];
+            acc[c] += v.x;
+            acc[c + 1] += v.y;
         }
-        if (threadIdx.x == 0) {
-            if (c < 12) {
-                int r = c / 4, col = c % 4;
-                float g = (float)red[0];
-                float* p = grad_minv + col * 4 + r;
-                *p = accumulate ? *p + g : g;
-            } else {
-                *loss = accumulate ? *loss + red[0] : red[0];
-            }
-        }
-        __syncthreads();
+        acc[kRed - 1] += partials[(size_t)b * 16 + kRed - 1];
     }
-    if (threadIdx.x < 4 && !accumulate)
-        grad_minv[threadIdx.x * 4 + 3] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < kRed; ++c)
+        for (int m = 32; m >= 1; m >>= 1)
+            acc[c] += __shfl_xor(acc[c], m, 64);
+    uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < kRed; ++c)
+            red[wave][c] = acc[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)kRed) {
+        int c = (int)threadIdx.x;
+        double sum = 0.0;
+        for (int w = 0; w < kFinWG / 64; ++w)
+            sum += red[w][c];
+        if (c < 12) {
+            int r = c / 4, col = c % 4;
+            float g = (float)sum;
+            float* p = grad_minv + col * 4 + r;
+            *p = accumulate ? *p + g : g;
+        } else {
+            *loss = accumulate ? *loss + sum : sum;
+        }
+    } else if (threadIdx.x >= 64 && threadIdx.x < 68 && !accumulate) {
+        grad_minv[(threadIdx.x - 64) * 4 + 3] = 0.0f;
+    }
 }
 
 }  // namespace
@@ -676,7 +696,7 @@ extern "C" int32_t bvht_trace_adjoint(bvht_ctx ctx, bvht_mesh mesh, const float*
                        mesh->d_normals, rows_of(minv), V3{light[0], light[1], light[2]}, d_rays, n, d_hits,
                        reinterpret_cast<const float2*>(d_target), w_depth, w_shade, d_grad_vertices, partials);
     BVHT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_adjoint_finish, dim3(1), dim3(256), 0, ctx->stream, partials, nb, (int)accumulate,
+    hipLaunchKernelGGL(k_adjoint_finish, dim3(1), dim3(kFinWG), 0, ctx->stream, partials, nb, (int)accumulate,
                        d_grad_minv, d_loss);
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
