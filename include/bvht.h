@@ -438,7 +438,9 @@ int32_t bvht_mesh_sgd(bvht_ctx ctx, bvht_mesh mesh, const float* d_grad, float l
                       int32_t flags);
 /* Forward: trace d_rays[n] through the object and shade.  d_hits[n] as
  * bvht_trace_tlas with one object (hit.inst = 0); d_image[2n] = {depth,
- * shade} per ray. */
+ * shade} per ray.  Each call records how long every 64-ray group took and
+ * the next call with the same n starts the groups longest first; only the
+ * order of the work changes, never a result. */
 int32_t bvht_trace_forward(bvht_ctx ctx, bvht_mesh mesh, const float* minv, const float* light,
                            const bvht_ray* d_rays, uint32_t n, bvht_hit* d_hits, float* d_image);
 /* Adjoint of the forward for the loss above against d_target[2n]: for each
