@@ -151,6 +151,104 @@ __device__ __forceinline__ int32_t block_best_match(const bvht_tnode* nodes, con
     return r;
 }
 
+// The merge loop of Tlas::rebuild (tlas.rs:204-250) for 1 <= n <= 64 objects
+// by one wave, without barriers: lane i holds list[i] and the box of node
+// list[i] in registers (entries past count keep their last values, as the
+// reference's list does), and find_best_match is a shuffle argmin.  The
+// candidates are the areas < f32::MAX ordered by (area, index): a total
+// order, so the butterfly picks the sequential scan's first minimum in every
+// lane.  Lane 0 writes the merged nodes and the results; the loop is capped
+// (it ends after at most n merges and chain steps, far below the cap).
+__device__ void wave_rebuild_loop(bvht_tnode* nodes, int32_t* list, int32_t n, int32_t* out_a, int32_t* out_used,
+                                  int32_t* out_fail)
+{
+    const int32_t lane = (int32_t)(threadIdx.x & 63u);
+    int32_t li = 0;
+    V3 lo = mk3(0.0f, 0.0f, 0.0f), hi = lo;
+    if (lane < n) {
+        li = list[lane];
+        const bvht_tnode& nd = nodes[li];
+        lo = mk3(nd.bmin[0], nd.bmin[1], nd.bmin[2]);
+        hi = mk3(nd.bmax[0], nd.bmax[1], nd.bmax[2]);
+    }
+    auto best_match = [&](int32_t count, int32_t x) -> int32_t {
+        V3 xl = mk3(__shfl(lo.x, x), __shfl(lo.y, x), __shfl(lo.z, x));
+        V3 xh = mk3(__shfl(hi.x, x), __shfl(hi.y, x), __shfl(hi.z, x));
+        float v = 0.0f;
+        int32_t idx = -1;
+        if (lane < count && lane != x) {
+            V3 e = sub3(max3(xh, hi), min3(xl, lo));
+            float sa = e.x * e.y + e.y * e.z + e.z * e.x;
+            if (sa < kFltMax) {
+                v = sa;
+                idx = lane;
+            }
+        }
+        for (int m = 32; m >= 1; m >>= 1) {
+            float v2 = __shfl_xor(v, m);
+            int32_t i2 = __shfl_xor(idx, m);
+            if (i2 >= 0 && (idx < 0 || v2 < v || (!(v < v2) && i2 < idx))) {
+                v = v2;
+                idx = i2;
+            }
+        }
+        return __builtin_amdgcn_readfirstlane(idx);
+    };
+    int32_t count = n, used = n + 1, a = 0, fail = 0;
+    int32_t b = best_match(count, a);
+    for (int32_t guard = 0; count > 1; ++guard) {
+        if (b < 0 || guard > 8 * 64 * 64) {  // every area >= f32::MAX (the reference panics)
+            fail = 1;
+            break;
+        }
+        int32_t c = best_match(count, b);
+        if (a == c) {
+            int32_t ia = __shfl(li, a), ib = __shfl(li, b);
+            V3 al = mk3(__shfl(lo.x, a), __shfl(lo.y, a), __shfl(lo.z, a));
+            V3 ah = mk3(__shfl(hi.x, a), __shfl(hi.y, a), __shfl(hi.z, a));
+            V3 bl = mk3(__shfl(lo.x, b), __shfl(lo.y, b), __shfl(lo.z, b));
+            V3 bh = mk3(__shfl(hi.x, b), __shfl(hi.y, b), __shfl(hi.z, b));
+            V3 mn = min3(al, bl), mx = max3(ah, bh);
+            if (lane == 0) {
+                bvht_tnode nn;
+                nn.left_right = (uint32_t)ia + ((uint32_t)ib << 16);
+                nn.bmin[0] = mn.x; nn.bmin[1] = mn.y; nn.bmin[2] = mn.z;
+                nn.bmax[0] = mx.x; nn.bmax[1] = mx.y; nn.bmax[2] = mx.z;
+                nn.blas = 0;
+                nodes[used] = nn;
+            }
+            // list[a] = used, then list[b] = list[count - 1] (which is the new
+            // entry when a == count - 1)
+            if (lane == a) {
+                li = used;
+                lo = mn;
+                hi = mx;
+            }
+            int32_t lt = __shfl(li, count - 1);
+            V3 tl = mk3(__shfl(lo.x, count - 1), __shfl(lo.y, count - 1), __shfl(lo.z, count - 1));
+            V3 th = mk3(__shfl(hi.x, count - 1), __shfl(hi.y, count - 1), __shfl(hi.z, count - 1));
+            if (lane == b) {
+                li = lt;
+                lo = tl;
+                hi = th;
+            }
+            used += 1;
+            count -= 1;
+            b = best_match(count, a);
+        } else {
+            a = b;
+            b = c;
+        }
+    }
+    if (lane < n)
+        list[lane] = li;
+    if (lane == 0) {
+        *out_a = a;
+        *out_used = used;
+        *out_fail = fail;
+    }
+}
+
 // Tlas::rebuild (tlas.rs:204-250) + TlasBuilder::build_for (tlas.rs:273-279):
 // 2n default nodes (empty box), leaves 1..n in object order, mutual-best-match
 // merges with LeftRightIndex::new(a, b) = a + (b << 16), root = copy of the
@@ -187,6 +285,11 @@ __device__ void tlas_rebuild_block(const float* __restrict__ bounds, uint32_t n,
         sh_fail = 0;
     }
     __syncthreads();
+    if (n >= 1 && n <= 64) {
+        if (threadIdx.x < 64)
+            wave_rebuild_loop(nodes, list, (int32_t)n, &sh_a, &sh_used, &sh_fail);
+        __syncthreads();
+    } else {
     int32_t b0 = block_best_match(nodes, list, sh_count, sh_a, s_val, s_idx);
     if (threadIdx.x == 0)
         sh_b = b0;
@@ -231,6 +334,7 @@ __device__ void tlas_rebuild_block(const float* __restrict__ bounds, uint32_t n,
             __syncthreads();
         }
     }
+    }  // more than 64 objects
     if (threadIdx.x == 0) {
         if (!sh_fail) {
             nodes[0] = nodes[list[sh_a]];
