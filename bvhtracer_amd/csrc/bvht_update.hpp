@@ -152,28 +152,37 @@ __device__ __forceinline__ int32_t block_best_match(const bvht_tnode* nodes, con
 }
 
 // The merge loop of Tlas::rebuild (tlas.rs:204-250) for 1 <= n <= 64 objects
-// by one wave, without barriers: lane i holds list[i] and the box of node
-// list[i] in registers (entries past count keep their last values, as the
-// reference's list does), and find_best_match is a shuffle argmin.  The
-// candidates are the areas < f32::MAX ordered by (area, index): a total
-// order, so the butterfly picks the sequential scan's first minimum in every
-// lane.  Lane 0 writes the merged nodes and the results; the loop is capped
-// (it ends after at most n merges and chain steps, far below the cap).
-__device__ void wave_rebuild_loop(bvht_tnode* nodes, int32_t* list, int32_t n, int32_t* out_a, int32_t* out_used,
-                                  int32_t* out_fail)
+// by one wave, without barriers.  Lane i holds list entry i in registers: the
+// node index, its box, link field, object index and subtree height (entries
+// past count keep their last values, as the reference's list does).
+// find_best_match is a shuffle argmin: the candidates are the areas <
+// f32::MAX ordered by (area, index), a total order, so the butterfly picks the
+// sequential scan's first minimum in every lane.  Lane 0 writes the merged
+// nodes, the root copy in node 0 and the results; the height of the root is
+// the depth that walking the tree from node 0 gives.  The loop is capped (it
+// ends after at most n merges and chain steps, far below the cap).
+__device__ __forceinline__ float lane_f(float v, int32_t l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ int32_t lane_i(int32_t v, int32_t l) { return __builtin_amdgcn_readlane(v, l); }
+
+__device__ void wave_rebuild_loop(const float* __restrict__ bounds, bvht_tnode* nodes, int32_t* list, int32_t n,
+                                  int32_t* out_a, int32_t* out_used, int32_t* out_fail, uint32_t* out_depth)
 {
     const int32_t lane = (int32_t)(threadIdx.x & 63u);
-    int32_t li = 0;
+    int32_t li = 0, lr = 0, obj = 0, h = 1;
     V3 lo = mk3(0.0f, 0.0f, 0.0f), hi = lo;
-    if (lane < n) {
-        li = list[lane];
-        const bvht_tnode& nd = nodes[li];
-        lo = mk3(nd.bmin[0], nd.bmin[1], nd.bmin[2]);
-        hi = mk3(nd.bmax[0], nd.bmax[1], nd.bmax[2]);
+    if (lane < n) {  // leaves 1..n in object order
+        const float* bb = bounds + 6 * (size_t)lane;
+        li = lane + 1;
+        obj = lane;
+        lo = mk3(bb[0], bb[1], bb[2]);
+        hi = mk3(bb[3], bb[4], bb[5]);
     }
     auto best_match = [&](int32_t count, int32_t x) -> int32_t {
-        V3 xl = mk3(__shfl(lo.x, x), __shfl(lo.y, x), __shfl(lo.z, x));
-        V3 xh = mk3(__shfl(hi.x, x), __shfl(hi.y, x), __shfl(hi.z, x));
+        V3 xl = mk3(lane_f(lo.x, x), lane_f(lo.y, x), lane_f(lo.z, x));
+        V3 xh = mk3(lane_f(hi.x, x), lane_f(hi.y, x), lane_f(hi.z, x));
         float v = 0.0f;
         int32_t idx = -1;
         if (lane < count && lane != x) {
@@ -203,15 +212,16 @@ __device__ void wave_rebuild_loop(bvht_tnode* nodes, int32_t* list, int32_t n, i
         }
         int32_t c = best_match(count, b);
         if (a == c) {
-            int32_t ia = __shfl(li, a), ib = __shfl(li, b);
-            V3 al = mk3(__shfl(lo.x, a), __shfl(lo.y, a), __shfl(lo.z, a));
-            V3 ah = mk3(__shfl(hi.x, a), __shfl(hi.y, a), __shfl(hi.z, a));
-            V3 bl = mk3(__shfl(lo.x, b), __shfl(lo.y, b), __shfl(lo.z, b));
-            V3 bh = mk3(__shfl(hi.x, b), __shfl(hi.y, b), __shfl(hi.z, b));
-            V3 mn = min3(al, bl), mx = max3(ah, bh);
+            int32_t ia = lane_i(li, a), ib = lane_i(li, b);
+            V3 mn = min3(mk3(lane_f(lo.x, a), lane_f(lo.y, a), lane_f(lo.z, a)),
+                         mk3(lane_f(lo.x, b), lane_f(lo.y, b), lane_f(lo.z, b)));
+            V3 mx = max3(mk3(lane_f(hi.x, a), lane_f(hi.y, a), lane_f(hi.z, a)),
+                         mk3(lane_f(hi.x, b), lane_f(hi.y, b), lane_f(hi.z, b)));
+            int32_t hn = 1 + max(lane_i(h, a), lane_i(h, b));
+            int32_t lrn = (int32_t)((uint32_t)ia + ((uint32_t)ib << 16));
             if (lane == 0) {
                 bvht_tnode nn;
-                nn.left_right = (uint32_t)ia + ((uint32_t)ib << 16);
+                nn.left_right = (uint32_t)lrn;
                 nn.bmin[0] = mn.x; nn.bmin[1] = mn.y; nn.bmin[2] = mn.z;
                 nn.bmax[0] = mx.x; nn.bmax[1] = mx.y; nn.bmax[2] = mx.z;
                 nn.blas = 0;
@@ -223,14 +233,21 @@ __device__ void wave_rebuild_loop(bvht_tnode* nodes, int32_t* list, int32_t n, i
                 li = used;
                 lo = mn;
                 hi = mx;
+                lr = lrn;
+                obj = 0;
+                h = hn;
             }
-            int32_t lt = __shfl(li, count - 1);
-            V3 tl = mk3(__shfl(lo.x, count - 1), __shfl(lo.y, count - 1), __shfl(lo.z, count - 1));
-            V3 th = mk3(__shfl(hi.x, count - 1), __shfl(hi.y, count - 1), __shfl(hi.z, count - 1));
+            int32_t t = count - 1;
+            int32_t tli = lane_i(li, t), tlr = lane_i(lr, t), tobj = lane_i(obj, t), th_ = lane_i(h, t);
+            V3 tlo = mk3(lane_f(lo.x, t), lane_f(lo.y, t), lane_f(lo.z, t));
+            V3 thi = mk3(lane_f(hi.x, t), lane_f(hi.y, t), lane_f(hi.z, t));
             if (lane == b) {
-                li = lt;
-                lo = tl;
-                hi = th;
+                li = tli;
+                lr = tlr;
+                obj = tobj;
+                h = th_;
+                lo = tlo;
+                hi = thi;
             }
             used += 1;
             count -= 1;
@@ -242,11 +259,31 @@ __device__ void wave_rebuild_loop(bvht_tnode* nodes, int32_t* list, int32_t n, i
     }
     if (lane < n)
         list[lane] = li;
+    // the root (entry a) copied to node 0, and its height
+    V3 rl = mk3(lane_f(lo.x, a), lane_f(lo.y, a), lane_f(lo.z, a));
+    V3 rh = mk3(lane_f(hi.x, a), lane_f(hi.y, a), lane_f(hi.z, a));
+    int32_t rlr = lane_i(lr, a), robj = lane_i(obj, a), rh_ = lane_i(h, a);
     if (lane == 0) {
+        if (!fail) {
+            bvht_tnode r;
+            r.bmin[0] = rl.x; r.bmin[1] = rl.y; r.bmin[2] = rl.z;
+            r.bmax[0] = rh.x; r.bmax[1] = rh.y; r.bmax[2] = rh.z;
+            r.left_right = (uint32_t)rlr;
+            r.blas = (uint32_t)robj;
+            nodes[0] = r;
+        }
         *out_a = a;
         *out_used = used;
         *out_fail = fail;
+        *out_depth = (uint32_t)rh_;
     }
+}
+
+__device__ __forceinline__ void meta_out(uint32_t* meta, uint32_t used, uint32_t depth, uint32_t status)
+{
+    meta[0] = used;
+    meta[1] = depth;
+    meta[2] = status;
 }
 
 // Tlas::rebuild (tlas.rs:204-250) + TlasBuilder::build_for (tlas.rs:273-279):
@@ -259,6 +296,7 @@ __device__ void tlas_rebuild_block(const float* __restrict__ bounds, uint32_t n,
     __shared__ float s_val[kRebuildWG];
     __shared__ int32_t s_idx[kRebuildWG];
     __shared__ int32_t sh_a, sh_b, sh_count, sh_used, sh_merged, sh_fail;
+    __shared__ uint32_t sh_depth;
     for (uint32_t i = threadIdx.x; i < 2 * n; i += kRebuildWG) {
         bvht_tnode d;
         d.bmin[0] = d.bmin[1] = d.bmin[2] = kFltMax;
@@ -287,8 +325,11 @@ __device__ void tlas_rebuild_block(const float* __restrict__ bounds, uint32_t n,
     __syncthreads();
     if (n >= 1 && n <= 64) {
         if (threadIdx.x < 64)
-            wave_rebuild_loop(nodes, list, (int32_t)n, &sh_a, &sh_used, &sh_fail);
+            wave_rebuild_loop(bounds, nodes, list, (int32_t)n, &sh_a, &sh_used, &sh_fail, &sh_depth);
         __syncthreads();
+        if (threadIdx.x == 0)
+            meta_out(meta, (uint32_t)sh_used, sh_fail ? 0u : sh_depth, sh_fail ? 1u : 0u);
+        return;
     } else {
     int32_t b0 = block_best_match(nodes, list, sh_count, sh_a, s_val, s_idx);
     if (threadIdx.x == 0)
