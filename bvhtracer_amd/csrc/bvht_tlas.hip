@@ -251,7 +251,7 @@ int32_t bvht_tlas_create(bvht_ctx ctx, const bvht_blas* blases, uint32_t blas_co
               hipMalloc(&t->d_rows, sizeof(ObjRows) * n) == hipSuccess &&
               hipMalloc(&t->d_bounds, sizeof(float) * 6 * n) == hipSuccess &&
               hipMalloc(&t->d_nodes, sizeof(bvht_tnode) * 2 * n) == hipSuccess &&
-              hipMalloc(&t->d_list, sizeof(int32_t) * std::max<uint32_t>(n, 4) * 2) == hipSuccess &&
+              hipMalloc(&t->d_list, sizeof(int32_t) * (std::max<uint32_t>(n, 4) * 2 + 4)) == hipSuccess &&
               hipMalloc(&t->d_meta, sizeof(uint32_t) * 4) == hipSuccess;
     if (!ok) {
         tlas_free(t);

@@ -388,7 +388,7 @@ __device__ void tlas_rebuild_block(const float* __restrict__ bounds, uint32_t n,
                 uint32_t lvl = st[--sp], ni = st[--sp];
                 best = max(best, lvl);
                 uint32_t lr = nodes[ni].left_right;
-                if (lr != 0 && sp + 4 <= 2 * max(n, 4u)) {
+                if (lr != 0 && sp + 4 <= 2 * max(n, 4u) + 4) {  // the list holds 2 max(n, 4) + 4 words
                     st[sp++] = lr >> 16;
                     st[sp++] = lvl + 1;
                     st[sp++] = lr & 0xFFFFu;

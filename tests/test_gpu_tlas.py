@@ -57,6 +57,56 @@ def test_tlas_build_and_trace(ctx, orc, n_obj, n_models):
     assert (got["prim"] != B.MISS).sum() > 100
 
 
+def translated(offsets):
+    """Column-major (M, Minv) pairs of pure translations."""
+    n = len(offsets)
+    M = np.tile(np.eye(4, dtype=np.float32).reshape(16), (n, 1))
+    Minv = M.copy()
+    for k, (x, y, z) in enumerate(offsets):
+        M[k, 12:15] = (x, y, z)
+        Minv[k, 12:15] = (-x, -y, -z)
+    return M, Minv
+
+
+@pytest.mark.parametrize("case", ["random64", "random65", "chain48", "duplicates20", "duplicates70"])
+def test_tlas_rebuild_at_the_wave_limit_chains_and_ties(ctx, orc, case):
+    """Up to 64 objects take the one-wave merge loop, more the block-wide one;
+    a chain with growing gaps makes a deep tree (the traversal's stack
+    follows its depth), and identical boxes tie on every area (first-index
+    rule) and chain all objects (height n).  Nodes bit for bit and hits
+    equal the oracle's."""
+    if case.startswith("random"):
+        n = int(case[6:])
+        M, Minv = scenes.random_affines(n, 0x5EED + n, trans_range=10.0)
+    elif case == "chain48":
+        gaps = 7.0 * 1.25 ** np.arange(48)
+        M, Minv = translated([(x, 0.0, 0.0) for x in np.cumsum(gaps) - gaps.sum() / 2])
+    else:
+        M, Minv = translated([(0.0, 0.0, 0.0)] * int(case[10:]))
+    mesh = scenes.icosphere(8, radius=3.0, seed=7)
+    nodes, used, perm = orc.bvh_build(mesh, False, 2)
+    blases = [(nodes, perm)]
+    model = B.Bvh.from_arrays(nodes, used, perm, ctx=ctx)
+    objs = [B.SceneObject(0, M[k].reshape(4, 4), Minv[k]) for k in range(M.shape[0])]
+    tlas = B.TlasBuilder(ctx=ctx).build_for(objs, [model])
+    tnodes, t_used = oracle_tlas(orc, blases, objs)
+    g_nodes, g_used = tlas.download()
+    assert g_used == t_used == 2 * len(objs)
+    assert g_nodes.tobytes() == tnodes.tobytes()
+    # rays along +x through the objects' rows, and back along -x
+    y, z = np.meshgrid(np.linspace(-9.0, 9.0, 48), np.linspace(-9.0, 9.0, 48))
+    rays = np.zeros(2 * y.size, B.RAY_DTYPE)
+    rays["o"][: y.size] = np.stack([np.full(y.size, -2.0e6), y.ravel(), z.ravel()], 1)
+    rays["d"][: y.size] = (1.0, 0.0, 0.0)
+    rays["o"][y.size:] = np.stack([np.full(y.size, 2.0e6), y.ravel(), z.ravel()], 1)
+    rays["d"][y.size:] = (-1.0, 0.0, 0.0)
+    rays["tmax"] = np.float32(3.4028235e38)
+    ref = orc.tlas_trace(tnodes, oracle_instances(orc, objs), blases, rays.view(orc.RAY_DTYPE), nthreads=8)
+    got = tlas.intersect(rays)
+    assert_hits_equal(got, ref, f"tlas {case}")
+    assert (got["prim"] != B.MISS).sum() > 100
+
+
 def test_tlas_frame_refit_refresh_rebuild_bounce(ctx, orc):
     """Per frame: deform the shared model, refit it, refresh the object
     bounds (set_transform with the same transform), rebuild the TLAS, trace
