@@ -155,9 +155,9 @@ __device__ __forceinline__ int32_t block_best_match(const bvht_tnode* nodes, con
 // by one wave, without barriers.  Lane i holds list entry i in registers: the
 // node index, its box, link field, object index and subtree height (entries
 // past count keep their last values, as the reference's list does).
-// find_best_match is a shuffle argmin: the candidates are the areas <
-// f32::MAX ordered by (area, index), a total order, so the butterfly picks the
-// sequential scan's first minimum in every lane.  Lane 0 writes the merged
+// find_best_match is a shuffle minimum over the areas < f32::MAX (as
+// order-preserving unsigned keys) followed by a ballot for the first lane
+// holding it: the sequential scan's first minimum.  Lane 0 writes the merged
 // nodes, the root copy in node 0 and the results; the height of the root is
 // the depth that walking the tree from node 0 gives.  The loop is capped (it
 // ends after at most n merges and chain steps, far below the cap).
@@ -183,25 +183,25 @@ __device__ void wave_rebuild_loop(const float* __restrict__ bounds, bvht_tnode* 
     auto best_match = [&](int32_t count, int32_t x) -> int32_t {
         V3 xl = mk3(lane_f(lo.x, x), lane_f(lo.y, x), lane_f(lo.z, x));
         V3 xh = mk3(lane_f(hi.x, x), lane_f(hi.y, x), lane_f(hi.z, x));
-        float v = 0.0f;
-        int32_t idx = -1;
+        // candidate areas as order-preserving unsigned keys (an area is
+        // never -0, so equal areas give equal keys); no candidate: ~0
+        uint32_t key = 0xFFFFFFFFu;
         if (lane < count && lane != x) {
             V3 e = sub3(max3(xh, hi), min3(xl, lo));
             float sa = e.x * e.y + e.y * e.z + e.z * e.x;
             if (sa < kFltMax) {
-                v = sa;
-                idx = lane;
+                uint32_t u = __float_as_uint(sa);
+                key = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
             }
         }
-        for (int m = 32; m >= 1; m >>= 1) {
-            float v2 = __shfl_xor(v, m);
-            int32_t i2 = __shfl_xor(idx, m);
-            if (i2 >= 0 && (idx < 0 || v2 < v || (!(v < v2) && i2 < idx))) {
-                v = v2;
-                idx = i2;
-            }
-        }
-        return __builtin_amdgcn_readfirstlane(idx);
+        uint32_t mk = key;
+        for (int m = 32; m >= 1; m >>= 1)
+            mk = min(mk, (uint32_t)__shfl_xor((int)mk, m));
+        mk = __builtin_amdgcn_readfirstlane(mk);
+        if (mk == 0xFFFFFFFFu)
+            return -1;
+        // the first lane holding the minimum: the sequential scan's choice
+        return (int32_t)__ffsll((long long)__ballot(key == mk)) - 1;
     };
     int32_t count = n, used = n + 1, a = 0, fail = 0;
     int32_t b = best_match(count, a);
