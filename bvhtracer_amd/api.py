@@ -107,7 +107,13 @@ class Context:
         return DeviceArray(self, max(a.nbytes, 1)).upload(a)
 
     def sync(self):
+        """Wait for the stream; raises BvhtError (BVHT_ESTACK) if a trace since
+        the last fence dropped stack entries at the stack limit."""
         check(L.lib().bvht_sync(self.handle))
+
+    def set_stack_limit(self, max_entries):
+        """Cap the traversal stack of this context's traces (0: tree depth)."""
+        check(L.lib().bvht_set_stack_limit(self.handle, int(max_entries)))
 
     def event(self):
         e = ctypes.c_void_p()
@@ -449,8 +455,11 @@ class Graph:
         self.ctx.adopt(self)
 
     @classmethod
-    def capture(cls, ctx, enqueue):
-        """Capture everything ``enqueue()`` launches on ctx's stream."""
+    def capture(cls, ctx, enqueue, keep=()):
+        """Capture everything ``enqueue()`` launches on ctx's stream.  The
+        graph holds device addresses of what it launched on: it keeps
+        ``enqueue`` (with everything its closure refers to) and the objects in
+        ``keep`` alive until it is closed."""
         check(L.lib().bvht_graph_begin(ctx.handle))
         try:
             enqueue()
@@ -458,7 +467,9 @@ class Graph:
             h = ctypes.c_void_p()
             rc = L.lib().bvht_graph_end(ctx.handle, ctypes.byref(h))
         check(rc)
-        return cls(ctx, h)
+        g = cls(ctx, h)
+        g._keep = (enqueue, tuple(keep))
+        return g
 
     def launch(self):
         check(L.lib().bvht_graph_launch(self.ctx.handle, self.handle))
@@ -467,6 +478,7 @@ class Graph:
         if self.handle:
             L.lib().bvht_graph_destroy(self.handle)
             self.handle = None
+            self._keep = ()
 
     def __del__(self):
         try:

@@ -135,13 +135,13 @@ __global__ __launch_bounds__(WG) void k_batch_trace(const BatchScene* __restrict
                                                     const uint32_t* __restrict__ ray_off, uint32_t n_scenes,
                                                     const bvht_ray* __restrict__ rays, bvht_hit* __restrict__ hits,
                                                     uint32_t* __restrict__ counter, uint32_t* __restrict__ spill,
-                                                    uint32_t cap)
+                                                    uint32_t cap, uint32_t* __restrict__ status)
 {
     __shared__ uint32_t lds[LDS_DEPTH * WG];
     uint32_t lane = threadIdx.x & 63;
     uint32_t total = item_prefix[n_scenes];
     uint32_t gtid = blockIdx.x * WG + threadIdx.x;
-    Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + gtid, (size_t)gridDim.x * WG, cap};
+    Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + gtid, (size_t)gridDim.x * WG, cap, status};
     while (true) {
         uint32_t item = 0;
         if (lane == 0)
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(WG) void k_batch_trace(const BatchScene* __restrict
     X(256, 16, 64) X(256, 16, 128) X(256, 16, 256) X(256, 24, 64) X(256, 24, 128) X(256, 24, 256)
 
 typedef void (*batch_trace_fn)(const BatchScene*, const uint32_t*, const uint32_t*, uint32_t, const bvht_ray*,
-                               bvht_hit*, uint32_t*, uint32_t*, uint32_t);
+                               bvht_hit*, uint32_t*, uint32_t*, uint32_t, uint32_t*);
 
 batch_trace_fn pick_batch(int wg, int lds, int rpi)
 {
@@ -232,6 +232,7 @@ struct bvht_batch_s {
 };
 
 struct bvht_graph_s {
+    bvht_ctx ctx = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -434,6 +435,13 @@ int32_t bvht_batch_refit(bvht_ctx ctx, bvht_batch b)
 int32_t bvht_batch_rebuild(bvht_ctx ctx, bvht_batch b)
 {
     BVHT_REQUIRE(ctx && b && b->ctx == ctx, "NULL argument or foreign context");
+    // the depth of a rebuilt TLAS is not read back: per-scene traces size
+    // their stack from the bound (a TLAS over n objects has at most n levels)
+    for (bvht_tlas t : b->scenes) {
+        t->depth = t->n;
+        for (bvht_blas bl : t->blases)
+            t->max_blas_depth = std::max(t->max_blas_depth, bl->depth);
+    }
     hipLaunchKernelGGL(k_batch_bounds, dim3(nblk(b->total_objs, 64)), dim3(64), 0, ctx->stream, b->d_scenes,
                        b->d_obj_prefix, (uint32_t)b->scenes.size());
     hipLaunchKernelGGL(k_batch_rebuild, dim3((uint32_t)b->scenes.size()), dim3(kRebuildWG), 0, ctx->stream,
@@ -490,7 +498,8 @@ int32_t bvht_batch_trace(bvht_ctx ctx, bvht_batch b, const bvht_ray* d_rays, bvh
         return rc;
     batch_trace_fn fn = pick_batch(b->wg, b->lds, b->rpi);
     hipLaunchKernelGGL(fn, dim3(b->grid), dim3(b->wg), 0, ctx->stream, b->d_scenes, b->d_items, b->d_ray_off,
-                       (uint32_t)b->scenes.size(), d_rays, d_hits, b->d_counter, b->d_spill, b->cap);
+                       (uint32_t)b->scenes.size(), d_rays, d_hits, b->d_counter, b->d_spill,
+                       stack_cap(ctx, b->cap), ctx->d_status);
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }
@@ -517,7 +526,9 @@ int32_t bvht_counter_add(bvht_ctx ctx, uint32_t* d_value, uint32_t v)
 int32_t bvht_graph_begin(bvht_ctx ctx)
 {
     BVHT_REQUIRE(ctx, "ctx is NULL");
+    BVHT_REQUIRE(!ctx->capturing, "a graph capture is already open on this context");
     BVHT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
     return BVHT_OK;
 }
 
@@ -527,6 +538,8 @@ int32_t bvht_graph_end(bvht_ctx ctx, bvht_graph* out)
     bvht_graph g = new (std::nothrow) bvht_graph_s();
     if (!g)
         return set_error(BVHT_ENOMEM, "host allocation failed");
+    g->ctx = ctx;
+    ctx->capturing = false;
     hipError_t e = hipStreamEndCapture(ctx->stream, &g->graph);
     if (e != hipSuccess) {
         delete g;
@@ -538,6 +551,7 @@ int32_t bvht_graph_end(bvht_ctx ctx, bvht_graph* out)
         delete g;
         return set_error(BVHT_EDEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
     }
+    ++ctx->live_graphs;
     *out = g;
     return BVHT_OK;
 }
@@ -555,6 +569,7 @@ int32_t bvht_graph_destroy(bvht_graph g)
         return BVHT_OK;
     (void)hipGraphExecDestroy(g->exec);
     (void)hipGraphDestroy(g->graph);
+    --g->ctx->live_graphs;
     delete g;
     return BVHT_OK;
 }

@@ -36,10 +36,22 @@ int32_t set_error(int32_t code, const char* fmt, ...)
     return code;
 }
 
+// A graph captured on ctx (or a capture in progress) holds the addresses of
+// the context's buffers as kernel arguments: they must not move then.
+static int32_t refuse_move(bvht_ctx ctx, const char* what, size_t bytes)
+{
+    return set_error(BVHT_EINVAL,
+                     "%s must grow to %zu bytes, but %s on this context holds its address: make one eager call of "
+                     "the largest size before capturing, or destroy the graphs first",
+                     what, bytes, ctx->capturing ? "the open graph capture" : "a captured graph");
+}
+
 int32_t ensure_scratch(bvht_ctx ctx, size_t bytes)
 {
     if (ctx->scratch_bytes >= bytes)
         return BVHT_OK;
+    if (ctx->live_graphs > 0 || ctx->capturing)
+        return refuse_move(ctx, "the context scratch", bytes);
     if (ctx->scratch) {
         BVHT_HIP(hipStreamSynchronize(ctx->stream));
         BVHT_HIP(hipFree(ctx->scratch));
@@ -57,6 +69,8 @@ int32_t ensure_spill(bvht_ctx ctx, uint32_t** buf, size_t* words, size_t need)
 {
     if (*words >= need)
         return BVHT_OK;
+    if (ctx->live_graphs > 0 || ctx->capturing)
+        return refuse_move(ctx, "a traversal stack spill buffer", need * sizeof(uint32_t));
     if (*buf) {
         BVHT_HIP(hipStreamSynchronize(ctx->stream));
         BVHT_HIP(hipFree(*buf));
@@ -66,6 +80,17 @@ int32_t ensure_spill(bvht_ctx ctx, uint32_t** buf, size_t* words, size_t need)
     if (hipMalloc(reinterpret_cast<void**>(buf), need * sizeof(uint32_t)) != hipSuccess)
         return set_error(BVHT_ENOMEM, "hipMalloc(%zu) for the traversal spill failed", need * 4);
     *words = need;
+    return BVHT_OK;
+}
+
+int32_t take_status(bvht_ctx ctx)
+{
+    // the stream is idle, so every kernel's store to the mapped word is visible
+    uint32_t s = __atomic_exchange_n(ctx->h_status, 0u, __ATOMIC_ACQ_REL);
+    if (s & kStatusStack)
+        return set_error(BVHT_ESTACK,
+                         "a trace dropped traversal stack entries at its stack bound: hits since the last fence may "
+                         "be wrong (raise the stack limit, bvht_set_stack_limit)");
     return BVHT_OK;
 }
 
@@ -233,14 +258,14 @@ __global__ __launch_bounds__(WG) void k_trace_blas(const float4* __restrict__ no
                                                    const bvht_ray* __restrict__ rays, bvht_hit* __restrict__ hits,
                                                    uint32_t n, const uint32_t* __restrict__ d_n,
                                                    bvht_stats* __restrict__ stats, uint32_t* __restrict__ spill,
-                                                   uint32_t cap)
+                                                   uint32_t cap, uint32_t* __restrict__ status)
 {
     __shared__ uint32_t lds[LDS_DEPTH * WG];
     uint32_t i = blockIdx.x * WG + threadIdx.x;
     uint32_t count = d_n ? min(*d_n, n) : n;
     if (i >= count)
         return;
-    Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap};
+    Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap, status};
     DRay ray = load_ray(rays, i);
     uint32_t pairs = 0, tests = 0;
     // a wave whose rays all have t <= FLT_MAX (or NaN) takes the reduced child choice
@@ -259,7 +284,7 @@ __global__ __launch_bounds__(WG) void k_trace_blas(const float4* __restrict__ no
 }
 
 typedef void (*trace_blas_fn)(const float4*, const float*, const bvht_ray*, bvht_hit*, uint32_t, const uint32_t*,
-                              bvht_stats*, uint32_t*, uint32_t);
+                              bvht_stats*, uint32_t*, uint32_t, uint32_t*);
 
 static trace_blas_fn pick_trace_blas(int wg, int lds, bool stats, bool packed)
 {
@@ -592,22 +617,28 @@ __global__ __launch_bounds__(256) void k_bounce_bin_write(BounceSrc src, const b
 }
 
 // ---- camera rays, trace and direction-grouped bounce rays in one pass ------
-// Lane j makes camera ray j (camera_ray), traces it, writes its hit and, if
-// it hit, its bounce ray (make_bounce, the same ray bounce_for makes from the
-// stored ray and hit) into a staging buffer in ray order, with key[j] =
-// bin << 8 | rank of the ray among its wave's rays of that bin (0xFFFFFFFF:
-// no bounce).  Lane k < 24 of each wave writes the wave's count of bin k to
-// counts[k * n_waves + wave].  A scan of those bin-major counts and a scatter
-// then give exactly the output of bvht_bounce_sorted: bins in order, ray
-// order inside a bin.
-template <int WG, int LDS_DEPTH, bool PACKED>
+// Lane j makes camera ray j (camera_ray), traces it and writes its hit.  A hit
+// spawns bounce ray j (make_bounce, the same ray bounce_for makes from the
+// stored ray and hit), in direction bin `bin`; lane k < 24 of each wave
+// writes the wave's count of bin k to counts[k * n_waves + wave].  A scan of
+// those bin-major counts gives every (bin, wave) its first output slot, so
+// the rays come out exactly as bvht_bounce_sorted gives them: bins in order,
+// ray order inside a bin.  Two ways to place them:
+// - STAGED: the kernel also writes the bounce ray to a staging buffer in ray
+//   order with key[j] = bin << 8 | rank among the wave's rays of that bin
+//   (0xFFFFFFFF: no bounce), and k_bounce_scatter moves it to its slot;
+// - not STAGED (no per-ray scratch): k_camera_bounce_regroup makes each
+//   bounce ray again from the stored hit and the regenerated camera ray, and
+//   ranks it with the same ballots.
+template <int WG, int LDS_DEPTH, bool PACKED, bool STAGED>
 __global__ __launch_bounds__(WG) void k_trace_camera_bounce(const float4* __restrict__ nodes,
                                                             const float* __restrict__ tris, uint32_t ntris,
                                                             bvht_camera cam, uint32_t width, uint32_t height,
                                                             int32_t order, uint64_t seed, bvht_hit* __restrict__ hits,
                                                             float4* __restrict__ staged, uint32_t* __restrict__ key,
                                                             uint32_t* __restrict__ counts, uint32_t n_waves,
-                                                            uint32_t* __restrict__ spill, uint32_t cap)
+                                                            uint32_t* __restrict__ spill, uint32_t cap,
+                                                            uint32_t* __restrict__ status)
 {
     __shared__ uint32_t lds[LDS_DEPTH * WG];
     uint32_t n = width * height;
@@ -615,7 +646,7 @@ __global__ __launch_bounds__(WG) void k_trace_camera_bounce(const float4* __rest
     bool valid = false;
     uint32_t bin = 0;
     if (j < n) {
-        Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + j, (size_t)n, cap};
+        Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + j, (size_t)n, cap, status};
         V3 ow, dw;
         camera_ray(cam, width, height, order, j, ow, dw);
         DRay ray = make_ray(ow, dw, kFltMax);
@@ -625,8 +656,10 @@ __global__ __launch_bounds__(WG) void k_trace_camera_bounce(const float4* __rest
         if (r.found && r.prim < ntris) {
             float4 a, b;
             make_bounce(tris + 9 * (size_t)r.prim, ray.o, ray.d, r.t, seed, j, a, b);
-            staged[2 * (size_t)j] = a;
-            staged[2 * (size_t)j + 1] = b;
+            if (STAGED) {
+                staged[2 * (size_t)j] = a;
+                staged[2 * (size_t)j + 1] = b;
+            }
             bin = dir_bin(b.x, b.y, b.z);
             valid = true;
         }
@@ -638,7 +671,7 @@ __global__ __launch_bounds__(WG) void k_trace_camera_bounce(const float4* __rest
 #pragma unroll 1
     for (uint32_t k = 0; k < kDirBins; ++k) {
         unsigned long long m = __ballot(valid && bin == k);
-        if (valid && bin == k)
+        if (STAGED && valid && bin == k)
             rank = (uint32_t)__popcll(m & below);
         if (lane == k)
             wave_count = (uint32_t)__popcll(m);
@@ -646,7 +679,7 @@ __global__ __launch_bounds__(WG) void k_trace_camera_bounce(const float4* __rest
     uint32_t w = j >> 6;
     if (w < n_waves && lane < kDirBins)
         counts[(size_t)lane * n_waves + w] = wave_count;
-    if (j < n)
+    if (STAGED && j < n)
         key[j] = valid ? (bin << 8 | rank) : 0xFFFFFFFFu;
 }
 
@@ -670,14 +703,66 @@ __global__ __launch_bounds__(256) void k_bounce_scatter(const float4* __restrict
         src_index[pos] = j;
 }
 
-typedef void (*camera_bounce_fn)(const float4*, const float*, uint32_t, bvht_camera, uint32_t, uint32_t, int32_t,
-                                 uint64_t, bvht_hit*, float4*, uint32_t*, uint32_t*, uint32_t, uint32_t*, uint32_t);
-
-static camera_bounce_fn pick_camera_bounce(int wg, int lds, bool packed)
+// Bounce ray j of a camera frame from its stored hit: the camera ray again
+// (camera_ray is exact arithmetic on (j, camera), so it is the traced ray),
+// the hit triangle and t, the same make_bounce as the camera pass; its slot
+// is the scanned (bin, wave) offset plus its rank among the wave's rays of
+// that bin.  Waves are the camera pass's (64 consecutive rays).
+__global__ __launch_bounds__(256) void k_camera_bounce_regroup(const float* __restrict__ tris, uint32_t ntris,
+                                                               bvht_camera cam, uint32_t width, uint32_t height,
+                                                               int32_t order, uint64_t seed,
+                                                               const bvht_hit* __restrict__ hits, uint32_t n_waves,
+                                                               const uint32_t* __restrict__ offsets,
+                                                               bvht_ray* __restrict__ out,
+                                                               uint32_t* __restrict__ src_index)
 {
-#define X(W, D)                                                                              \
-    if (wg == W && lds == D)                                                                 \
-        return packed ? k_trace_camera_bounce<W, D, true> : k_trace_camera_bounce<W, D, false>;
+    uint32_t n = width * height;
+    uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    bool valid = false;
+    uint32_t bin = 0;
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a;
+    if (j < n) {
+        const bvht_hit* h = hits + j;
+        uint32_t prim = h->prim;
+        if (prim != BVHT_MISS && prim < ntris) {
+            V3 ow, dw;
+            camera_ray(cam, width, height, order, j, ow, dw);
+            make_bounce(tris + 9 * (size_t)prim, ow, dw, h->t, seed, j, a, b);
+            bin = dir_bin(b.x, b.y, b.z);
+            valid = true;
+        }
+    }
+    uint32_t lane = threadIdx.x & 63u;
+    unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t rank = 0;
+#pragma unroll 1
+    for (uint32_t k = 0; k < kDirBins; ++k) {
+        unsigned long long m = __ballot(valid && bin == k);
+        if (valid && bin == k)
+            rank = (uint32_t)__popcll(m & below);
+    }
+    if (!valid)
+        return;
+    uint32_t pos = offsets[(size_t)bin * n_waves + (j >> 6)] + rank;
+    float4* o = reinterpret_cast<float4*>(out + pos);
+    o[0] = a;
+    o[1] = b;
+    if (src_index)
+        src_index[pos] = j;
+}
+
+typedef void (*camera_bounce_fn)(const float4*, const float*, uint32_t, bvht_camera, uint32_t, uint32_t, int32_t,
+                                 uint64_t, bvht_hit*, float4*, uint32_t*, uint32_t*, uint32_t, uint32_t*, uint32_t,
+                                 uint32_t*);
+
+static camera_bounce_fn pick_camera_bounce(int wg, int lds, bool packed, bool staged)
+{
+#define X(W, D)                                                                                          \
+    if (wg == W && lds == D) {                                                                           \
+        if (staged)                                                                                      \
+            return packed ? k_trace_camera_bounce<W, D, true, true> : k_trace_camera_bounce<W, D, false, true>; \
+        return packed ? k_trace_camera_bounce<W, D, true, false> : k_trace_camera_bounce<W, D, false, false>;   \
+    }
     BVHT_TRACE_CONFIGS(X)
 #undef X
     return nullptr;
@@ -987,7 +1072,7 @@ int32_t launch_trace_blas(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, 
     if (n == 0)
         return BVHT_OK;
     // stack entries needed: at most depth - 1
-    uint32_t need = blas->depth > 0 ? blas->depth - 1 : 0;
+    uint32_t need = stack_cap(ctx, blas->depth > 0 ? blas->depth - 1 : 0);
     int wg, lds;
     trace_config(ctx, need, &wg, &lds);
     uint32_t* spill = nullptr;
@@ -1002,7 +1087,7 @@ int32_t launch_trace_blas(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, 
         return set_error(BVHT_EINVAL, "no trace kernel for workgroup %d, LDS depth %d", wg, lds);
     hipLaunchKernelGGL(fn, dim3(blocks_for(n, wg)), dim3(wg), 0, ctx->stream,
                        reinterpret_cast<const float4*>(blas->d_nodes), blas->d_tris, d_rays, d_hits, n, d_n, d_stats,
-                       spill, need);
+                       spill, need, ctx->d_status);
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }
@@ -1028,8 +1113,53 @@ int32_t hit_block_offsets(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, uint
     return BVHT_OK;
 }
 
-// Camera rays + trace + direction-grouped bounce rays: the fused kernel, the
-// scan of the bin-major wave counts, the scatter.  Staging buffers in ctx scratch.
+// Scratch layout of the camera pass: [staged rays (32 B) and keys (4 B) per
+// pixel] (staged form only), per-(bin, wave) counts, scan tile sums.
+// Returns the bytes needed.
+static size_t camera_pass_layout(uint32_t n, bool staged, size_t* count_at, size_t* sums_at)
+{
+    size_t entries = (size_t)kDirBins * ((n + 63) / 64);
+    *count_at = staged ? align256((sizeof(float4) * 2 + sizeof(uint32_t)) * (size_t)n) : 0;
+    *sums_at = align256(*count_at + sizeof(uint32_t) * entries);
+    return *sums_at + sizeof(uint32_t) * blocks_for(entries, kScanTile);
+}
+
+// Camera rays + trace + direction-grouped bounce rays on a scratch area of
+// camera_pass_layout's size (n > 0): the fused kernel, the scan of the
+// bin-major wave counts, then the scatter of the staged rays or the regroup.
+static int32_t enqueue_camera_pass(bvht_ctx ctx, bvht_blas blas, const bvht_camera* cam, uint32_t width,
+                                   uint32_t height, int32_t order, uint64_t seed, bvht_hit* d_hits, bvht_ray* d_out,
+                                   uint32_t* d_src, uint32_t* d_count, int wg, int lds, bool staged, char* base,
+                                   uint32_t* spill, uint32_t need)
+{
+    camera_bounce_fn fn = pick_camera_bounce(wg, lds, blas->packed_stack, staged);
+    if (!fn)
+        return set_error(BVHT_EINVAL, "no trace kernel for workgroup %d, LDS depth %d", wg, lds);
+    uint32_t n = width * height;
+    uint32_t n_waves = (n + 63) / 64;
+    size_t count_at, sums_at;
+    camera_pass_layout(n, staged, &count_at, &sums_at);
+    size_t entries = (size_t)kDirBins * n_waves;
+    float4* stg = staged ? reinterpret_cast<float4*>(base) : nullptr;
+    uint32_t* key = staged ? reinterpret_cast<uint32_t*>(base + sizeof(float4) * 2 * (size_t)n) : nullptr;
+    uint32_t* counts = reinterpret_cast<uint32_t*>(base + count_at);
+    uint32_t* sums = reinterpret_cast<uint32_t*>(base + sums_at);
+    hipLaunchKernelGGL(fn, dim3(blocks_for(n, wg)), dim3(wg), 0, ctx->stream,
+                       reinterpret_cast<const float4*>(blas->d_nodes), blas->d_tris, blas->n_tris, *cam, width, height,
+                       order, seed, d_hits, stg, key, counts, n_waves, spill, need, ctx->d_status);
+    launch_scan(ctx, counts, (uint32_t)entries, sums, d_count);
+    if (staged)
+        hipLaunchKernelGGL(k_bounce_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, stg, key, n,
+                           n_waves, counts, d_out, d_src);
+    else
+        hipLaunchKernelGGL(k_camera_bounce_regroup, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream,
+                           blas->d_tris, blas->n_tris, *cam, width, height, order, seed, d_hits, n_waves, counts,
+                           d_out, d_src);
+    BVHT_HIP(hipGetLastError());
+    return BVHT_OK;
+}
+
+// The camera pass with its staging buffers in the context scratch.
 int32_t launch_trace_camera_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_camera* cam, uint32_t width,
                                    uint32_t height, int32_t order, uint64_t seed, bvht_hit* d_hits, bvht_ray* d_out,
                                    uint32_t* d_src, uint32_t* d_count)
@@ -1037,7 +1167,7 @@ int32_t launch_trace_camera_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_came
     uint32_t n = width * height;
     if (n == 0)
         return fill_u32(ctx, d_count, 0, 1);
-    uint32_t need = blas->depth > 0 ? blas->depth - 1 : 0;
+    uint32_t need = stack_cap(ctx, blas->depth > 0 ? blas->depth - 1 : 0);
     int wg, lds;
     trace_config(ctx, need, &wg, &lds);
     uint32_t* spill = nullptr;
@@ -1047,31 +1177,12 @@ int32_t launch_trace_camera_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_came
             return rc;
         spill = ctx->spill;
     }
-    camera_bounce_fn fn = pick_camera_bounce(wg, lds, blas->packed_stack);
-    if (!fn)
-        return set_error(BVHT_EINVAL, "no trace kernel for workgroup %d, LDS depth %d", wg, lds);
-    uint32_t n_waves = (n + 63) / 64;
-    size_t staged_bytes = sizeof(float4) * 2 * (size_t)n;
-    size_t key_bytes = sizeof(uint32_t) * (size_t)n;
-    size_t count_entries = (size_t)kDirBins * n_waves;
-    size_t count_at = align256(staged_bytes + key_bytes);
-    size_t sums_at = align256(count_at + sizeof(uint32_t) * count_entries);
-    int32_t rc = ensure_scratch(ctx, sums_at + sizeof(uint32_t) * blocks_for(count_entries, kScanTile));
+    size_t count_at, sums_at;
+    int32_t rc = ensure_scratch(ctx, camera_pass_layout(n, true, &count_at, &sums_at));
     if (rc)
         return rc;
-    char* base = static_cast<char*>(ctx->scratch);
-    float4* staged = reinterpret_cast<float4*>(base);
-    uint32_t* key = reinterpret_cast<uint32_t*>(base + staged_bytes);
-    uint32_t* counts = reinterpret_cast<uint32_t*>(base + count_at);
-    uint32_t* sums = reinterpret_cast<uint32_t*>(base + sums_at);
-    hipLaunchKernelGGL(fn, dim3(blocks_for(n, wg)), dim3(wg), 0, ctx->stream,
-                       reinterpret_cast<const float4*>(blas->d_nodes), blas->d_tris, blas->n_tris, *cam, width, height,
-                       order, seed, d_hits, staged, key, counts, n_waves, spill, need);
-    launch_scan(ctx, counts, (uint32_t)count_entries, sums, d_count);
-    hipLaunchKernelGGL(k_bounce_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, staged, key, n, n_waves,
-                       counts, d_out, d_src);
-    BVHT_HIP(hipGetLastError());
-    return BVHT_OK;
+    return enqueue_camera_pass(ctx, blas, cam, width, height, order, seed, d_hits, d_out, d_src, d_count, wg, lds,
+                               true, static_cast<char*>(ctx->scratch), spill, need);
 }
 
 // Bounce rays grouped by direction bin, ray order inside a bin: counts per
@@ -1137,6 +1248,15 @@ int32_t bvht_create(int32_t device, bvht_ctx* out)
         delete c;
         return set_error(BVHT_EDEVICE, "hipStreamCreate failed");
     }
+    if (hipHostMalloc(reinterpret_cast<void**>(&c->h_status), 64, hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_status), c->h_status, 0) != hipSuccess) {
+        if (c->h_status)
+            (void)hipHostFree(c->h_status);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return set_error(BVHT_ENOMEM, "pinned status word allocation failed");
+    }
+    *c->h_status = 0;
     *out = c;
     return BVHT_OK;
 }
@@ -1150,6 +1270,8 @@ int32_t bvht_destroy(bvht_ctx ctx)
         (void)hipFree(ctx->scratch);
     if (ctx->spill)
         (void)hipFree(ctx->spill);
+    if (ctx->h_status)
+        (void)hipHostFree(ctx->h_status);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BVHT_OK;
@@ -1159,6 +1281,13 @@ int32_t bvht_sync(bvht_ctx ctx)
 {
     BVHT_REQUIRE(ctx, "ctx is NULL");
     BVHT_HIP(hipStreamSynchronize(ctx->stream));
+    return take_status(ctx);
+}
+
+int32_t bvht_set_stack_limit(bvht_ctx ctx, uint32_t max_entries)
+{
+    BVHT_REQUIRE(ctx, "ctx is NULL");
+    ctx->stack_limit = max_entries;
     return BVHT_OK;
 }
 
@@ -1195,7 +1324,7 @@ int32_t bvht_copy_to_host(bvht_ctx ctx, void* dst, const void* src, size_t bytes
     BVHT_REQUIRE(ctx && (bytes == 0 || (dst && src)), "NULL argument");
     BVHT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     BVHT_HIP(hipStreamSynchronize(ctx->stream));
-    return BVHT_OK;
+    return take_status(ctx);
 }
 
 int32_t bvht_event_create(bvht_ctx ctx, void** out)

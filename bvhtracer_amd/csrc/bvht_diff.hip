@@ -180,6 +180,7 @@ __global__ __launch_bounds__(WG) void k_trace_forward(const float4* __restrict__
                                                       const bvht_ray* __restrict__ rays, uint32_t n,
                                                       bvht_hit* __restrict__ hits, float2* __restrict__ image,
                                                       uint32_t* __restrict__ spill, uint32_t cap,
+                                                      uint32_t* __restrict__ status,
                                                       const uint32_t* __restrict__ sched,
                                                       uint32_t* __restrict__ cost)
 {
@@ -193,7 +194,7 @@ __global__ __launch_bounds__(WG) void k_trace_forward(const float4* __restrict__
     uint32_t i = group * 64 + (threadIdx.x & 63);
     if (i >= n)
         return;
-    Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap};
+    Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap, status};
     const float4* rp = reinterpret_cast<const float4*>(rays + i);
     float4 a = rp[0], b = rp[1];
     V3 o = rows_mul(rows.r0, rows.r1, rows.r2, mk3(a.x, a.y, a.z), 1.0f);
@@ -218,7 +219,8 @@ __global__ __launch_bounds__(WG) void k_trace_forward(const float4* __restrict__
 }
 
 typedef void (*forward_fn)(const float4*, const float*, const uint32_t*, const float*, Rows, V3, const bvht_ray*,
-                           uint32_t, bvht_hit*, float2*, uint32_t*, uint32_t, const uint32_t*, uint32_t*);
+                           uint32_t, bvht_hit*, float2*, uint32_t*, uint32_t, uint32_t*, const uint32_t*,
+                           uint32_t*);
 
 forward_fn pick_forward(int wg, int lds, bool packed)
 {
@@ -631,7 +633,7 @@ extern "C" int32_t bvht_trace_forward(bvht_ctx ctx, bvht_mesh mesh, const float*
     BVHT_REQUIRE(d_rays && d_hits && d_image, "NULL device buffer");
     BVHT_REQUIRE(n <= 0xFFFFFF00u, "too many rays (%u)", n);
     bvht_blas b = mesh->blas;
-    uint32_t need = b->depth > 0 ? b->depth - 1 : 0;
+    uint32_t need = stack_cap(ctx, b->depth > 0 ? b->depth - 1 : 0);
     int wg, lds;
     trace_config(ctx, need, &wg, &lds);
     uint32_t* spill = nullptr;
@@ -646,6 +648,9 @@ extern "C" int32_t bvht_trace_forward(bvht_ctx ctx, bvht_mesh mesh, const float*
         return set_error(BVHT_EINVAL, "no forward kernel for workgroup %d, LDS depth %d", wg, lds);
     uint32_t groups = nblk(n, 64);
     if (groups > mesh->sched_cap) {
+        BVHT_REQUIRE(ctx->live_graphs == 0 && !ctx->capturing,
+                     "the forward schedule must grow to %u groups, but a graph captured on this context holds its "
+                     "address: make one eager call of the largest ray count first", groups);
         BVHT_HIP(hipStreamSynchronize(ctx->stream));
         (void)hipFree(mesh->d_sched);
         (void)hipFree(mesh->d_cost);
@@ -662,7 +667,7 @@ extern "C" int32_t bvht_trace_forward(bvht_ctx ctx, bvht_mesh mesh, const float*
     hipLaunchKernelGGL(fn, dim3(nblk(groups, wg / 64)), dim3(wg), 0, ctx->stream,
                        reinterpret_cast<const float4*>(b->d_nodes), b->d_tris, mesh->d_vid, mesh->d_normals,
                        rows_of(minv), V3{light[0], light[1], light[2]}, d_rays, n, d_hits,
-                       reinterpret_cast<float2*>(d_image), spill, need, sched, mesh->d_cost);
+                       reinterpret_cast<float2*>(d_image), spill, need, ctx->d_status, sched, mesh->d_cost);
     // the schedule for the next forward: longest groups first
     int32_t rc = launch_longest_first(ctx, mesh->d_cost, groups, mesh->d_bins, mesh->d_sched);
     if (rc)

@@ -24,7 +24,22 @@ struct bvht_ctx_s {
     // trace kernel configuration (0 = choose from the tree depth)
     int trace_wg = 0;
     int trace_lds = 0;
+    // Upper bound on a trace's stack entries (0: the tree's depth bound).
+    uint32_t stack_limit = 0;
+    // Status word the traversal kernels set when a push is dropped at the
+    // stack bound (kStatusStack).  Pinned host memory mapped into the
+    // device, so a fence (bvht_sync, a read-back) reads it without a copy.
+    uint32_t* h_status = nullptr;
+    uint32_t* d_status = nullptr;
+    // hipGraphs captured on this context and not yet destroyed, and whether
+    // a capture is open: while either is non-zero, scratch and spill buffers
+    // must keep their addresses (a graph holds them as kernel arguments).
+    int live_graphs = 0;
+    bool capturing = false;
 };
+
+// Bits of the context status word.
+constexpr uint32_t kStatusStack = 1u;
 
 // Per-model shading attributes as the attribute accumulators read them
 // (device pointers; NULL where absent), triangles in build order.
@@ -88,6 +103,15 @@ int32_t set_error(int32_t code, const char* fmt, ...);
 int32_t ensure_scratch(bvht_ctx ctx, size_t bytes);
 int32_t compute_blas_topology(bvht_blas b, const bvht_node* h_nodes);
 int32_t ensure_spill(bvht_ctx ctx, uint32_t** buf, size_t* words, size_t need);
+// After a fence on ctx's stream: BVHT_ESTACK (and the status cleared) if a
+// trace since the last check dropped a stack entry, else BVHT_OK.
+int32_t take_status(bvht_ctx ctx);
+// Stack entries a trace over a tree needing `need` may use: `need`, or the
+// context's stack limit when one is set and smaller.
+inline uint32_t stack_cap(bvht_ctx ctx, uint32_t need)
+{
+    return (ctx->stack_limit && ctx->stack_limit < need) ? ctx->stack_limit : need;
+}
 int32_t blas_alloc(bvht_ctx ctx, uint32_t n, uint32_t node_count, bvht_blas* out);
 int32_t fill_u32(bvht_ctx ctx, uint32_t* p, uint32_t v, size_t n);
 // Longest first: d_sched[0, n) = the units ordered by descending d_cost (a

@@ -45,14 +45,14 @@ __global__ __launch_bounds__(WG) void k_trace_tlas(const float4* __restrict__ tn
                                                    const bvht_ray* __restrict__ rays, bvht_hit* __restrict__ hits,
                                                    uint32_t n, const uint32_t* __restrict__ d_n,
                                                    bvht_stats* __restrict__ stats, uint32_t* __restrict__ spill,
-                                                   uint32_t cap)
+                                                   uint32_t cap, uint32_t* __restrict__ status)
 {
     __shared__ uint32_t lds[LDS_DEPTH * WG];
     uint32_t i = blockIdx.x * WG + threadIdx.x;
     uint32_t count = d_n ? min(*d_n, n) : n;
     if (i >= count)
         return;
-    Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap};
+    Stack<LDS_DEPTH, WG> st{(lds_u32*)(lds + threadIdx.x), spill + i, (size_t)n, cap, status};
     DRay ray = load_ray(rays, i);
     uint32_t pairs = 0, tests = 0, tpairs = 0, visits = 0;
     // a wave whose rays all have t <= FLT_MAX (or NaN) takes the reduced child choice
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(WG) void k_trace_tlas(const float4* __restrict__ tn
 }
 
 typedef void (*trace_tlas_fn)(const float4*, const ObjRows*, const BlasRef*, const bvht_ray*, bvht_hit*, uint32_t,
-                              const uint32_t*, bvht_stats*, uint32_t*, uint32_t);
+                              const uint32_t*, bvht_stats*, uint32_t*, uint32_t, uint32_t*);
 
 trace_tlas_fn pick_trace_tlas(int wg, int lds, bool stats, bool packed)
 {
@@ -184,7 +184,8 @@ int32_t launch_trace_tlas(bvht_tlas t, const bvht_ray* d_rays, uint32_t n, const
     if (n == 0)
         return BVHT_OK;
     bvht_ctx ctx = t->ctx;
-    uint32_t need = (t->depth > 0 ? t->depth - 1 : 0) + (t->max_blas_depth > 0 ? t->max_blas_depth - 1 : 0);
+    uint32_t need =
+        stack_cap(ctx, (t->depth > 0 ? t->depth - 1 : 0) + (t->max_blas_depth > 0 ? t->max_blas_depth - 1 : 0));
     int wg, lds;
     trace_config(ctx, need, &wg, &lds);
     uint32_t* spill = nullptr;
@@ -203,7 +204,7 @@ int32_t launch_trace_tlas(bvht_tlas t, const bvht_ray* d_rays, uint32_t n, const
         return set_error(BVHT_EINVAL, "no TLAS trace kernel for workgroup %d, LDS depth %d", wg, lds);
     hipLaunchKernelGGL(fn, dim3(nblk(n, wg)), dim3(wg), 0, ctx->stream, reinterpret_cast<const float4*>(t->d_nodes),
                        static_cast<const ObjRows*>(t->d_rows), static_cast<const BlasRef*>(t->d_blas_table), d_rays,
-                       d_hits, n, d_n, d_stats, spill, need);
+                       d_hits, n, d_n, d_stats, spill, need, ctx->d_status);
     BVHT_HIP(hipGetLastError());
     return BVHT_OK;
 }

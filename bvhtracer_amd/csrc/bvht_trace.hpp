@@ -13,22 +13,27 @@ typedef __attribute__((address_space(3))) uint32_t lds_u32;
 // lds[slot * WG] (lds already offset by the lane: the lanes of a wave hit
 // distinct banks), deeper entries in a per-ray column of a global buffer
 // (spill[(slot - LDS_DEPTH) * stride]).  `cap` bounds the total; a push beyond
-// it is dropped rather than written out of bounds (cap is sized from the tree
-// depth, so this never happens for a structurally valid tree).
+// it is dropped rather than written out of bounds, and sets kStatusStack in
+// *status, which the host reports as BVHT_ESTACK at the next fence.  cap comes
+// from the tree depth, so only a caller's smaller stack limit (or a wrong
+// depth bound) can drop a push.
 template <int LDS_DEPTH, int WG>
 struct Stack {
     lds_u32* lds;
     uint32_t* spill;
     size_t stride;
     uint32_t cap;
+    uint32_t* status;
 
     __device__ __forceinline__ void push(uint32_t& sp, uint32_t v)
     {
         if (sp < (uint32_t)LDS_DEPTH) {
             lds[sp * WG] = v;
         } else {
-            if (sp >= cap)
+            if (sp >= cap) {
+                *status = 1u;  // kStatusStack
                 return;
+            }
             spill[(size_t)(sp - LDS_DEPTH) * stride] = v;
         }
         ++sp;

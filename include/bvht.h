@@ -134,6 +134,10 @@ int32_t bvht_create(int32_t device, bvht_ctx* out);
 int32_t bvht_destroy(bvht_ctx ctx);
 /* Message for the last failing call on this thread ("" if none). */
 const char* bvht_last_error(void);
+/* Wait for the context's stream.  BVHT_ESTACK if a trace enqueued since the
+ * last fence dropped traversal stack entries at its stack bound (only
+ * possible under bvht_set_stack_limit; its hits may be wrong); the status is
+ * then cleared. */
 int32_t bvht_sync(bvht_ctx ctx);
 /* The context's hipStream_t, for callers that order their own work on it. */
 void* bvht_stream(bvht_ctx ctx);
@@ -141,6 +145,8 @@ void* bvht_stream(bvht_ctx ctx);
 int32_t bvht_device_alloc(bvht_ctx ctx, size_t bytes, void** out);
 int32_t bvht_device_free(bvht_ctx ctx, void* ptr);
 int32_t bvht_copy_to_device(bvht_ctx ctx, void* dst, const void* src, size_t bytes);
+/* Blocking read-back; a fence like bvht_sync, with the same BVHT_ESTACK
+ * report (the copy itself has completed when that is returned). */
 int32_t bvht_copy_to_host(bvht_ctx ctx, void* dst, const void* src, size_t bytes);
 
 /* HIP events on the context's stream, for timing enqueued work. */
@@ -218,6 +224,14 @@ int32_t bvht_trace_blas_count(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_ra
  * (bvhtracer_amd/autotune.py) picks the fastest. */
 int32_t bvht_set_trace_config(bvht_ctx ctx, int32_t wg, int32_t lds_depth);
 int32_t bvht_get_trace_config(bvht_ctx ctx, int32_t* wg, int32_t* lds_depth);
+
+/* Upper bound on the traversal stack entries of this context's traces (no
+ * reference counterpart; the reference's stack is a growable Vec,
+ * bvh.rs:244).  0, the default, sizes the stack from the tree depth, which is
+ * always enough.  A smaller bound caps the spill memory a trace allocates; a
+ * ray that needs more drops the pushes beyond it (every access stays in
+ * bounds) and the next fence returns BVHT_ESTACK. */
+int32_t bvht_set_stack_limit(bvht_ctx ctx, uint32_t max_entries);
 
 /* ---- frame loop ------------------------------------------------------- */
 
@@ -540,7 +554,11 @@ int32_t bvht_scene_run_physics(bvht_ctx ctx, bvht_tlas tlas, float elapsed);
 
 /* Capture the calls made on ctx between begin and end into a graph (calls
  * that synchronise or allocate are not capturable: build, upload, download,
- * bvht_tlas_rebuild; use the batch calls, which only enqueue kernels). */
+ * bvht_tlas_rebuild; use the batch calls, which only enqueue kernels).  A
+ * graph holds the addresses of the context's scratch and spill buffers: from
+ * bvht_graph_begin until every graph of the context is destroyed, a call that
+ * would have to grow one of them returns BVHT_EINVAL instead of moving it, so
+ * make one eager call of each captured kind at its largest size first. */
 int32_t bvht_graph_begin(bvht_ctx ctx);
 int32_t bvht_graph_end(bvht_ctx ctx, bvht_graph* out);
 int32_t bvht_graph_launch(bvht_ctx ctx, bvht_graph graph);

@@ -54,45 +54,64 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run `make -C oracle`)")
-        L = ctypes.CDLL(_LIB_PATH)
-        P = ctypes.c_void_p
-        u32, u64, i32, f32 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_float
-        sig = {
-            "orc_normalize": (None, [P, P]),
-            "orc_ray_recip": (None, [P, P]),
-            "orc_aabb_intersect": (i32, [P, P, P, P]),
-            "orc_triangle_intersect": (i32, [P, P, P, P, P]),
-            "orc_triangle_centroid": (None, [P, P]),
-            "orc_pack_index": (u32, [u32, u32]),
-            "orc_bvh_build": (i32, [P, u32, i32, u32, P, P]),
-            "orc_bvh_refit": (i32, [P, u32, P]),
-            "orc_bvh_depth": (u32, [P]),
-            "orc_bvh_intersect": (i32, [P, P, P, P, P]),
-            "orc_bvh_trace": (i32, [P, P, P, u64, P, P, i32]),
-            "orc_brute_trace": (i32, [P, u32, P, u64, P, P]),
-            "orc_transform_point": (None, [P, P, P]),
-            "orc_transform_vector": (None, [P, P, P]),
-            "orc_instance_bounds": (None, [P, P, P, P, P]),
-            "orc_tlas_build": (i32, [P, u32, P, P]),
-            "orc_tlas_intersect": (i32, [P, P, P, P, P, P, P]),
-            "orc_tlas_trace": (i32, [P, P, P, P, P, u64, P, P, i32]),
-            "orc_camera_ray": (None, [P, f32, f32, P]),
-            "orc_raygen": (i32, [P, u32, u32, P]),
-            "orc_mesh_normals": (None, [P, u32, P, u32, P]),
-            "orc_diff_forward": (i32, [P, P, P, P, P, P, P, u64, P, P, i32]),
-            "orc_diff_grad64": (i32, [P, u32, P, u32, P, P, P, P, P, u64, P, ctypes.c_double, ctypes.c_double,
-                                      P, P, P, P]),
-            "orc_powf": (ctypes.c_float, [ctypes.c_float, ctypes.c_float]),
-            "orc_world_start_frame": (None, [P, u32]),
-            "orc_world_run_physics": (None, [P, u32, P, u32, ctypes.c_float]),
-            "orc_objects_from_bodies": (None, [P, P, P, u32, P, P]),
-        }
-        for name, (res, args) in sig.items():
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = L
+        _lib = _load(_LIB_PATH)
     return _lib
+
+
+_native = {}
+
+
+def native_lib(out_dir):
+    """The same oracle compiled with -march=native for the running host (the
+    CPU baseline's timed build), made in ``out_dir`` on first use."""
+    import subprocess
+
+    path = os.path.join(out_dir, "liboracle_native.so")
+    if path not in _native:
+        subprocess.run(["make", "-s", "-B", "-C", _HERE, "native", f"NATIVE={path}"], check=True)
+        _native[path] = _load(path)
+    return _native[path]
+
+
+def _load(path):
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    u32, u64, i32, f32 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_float
+    sig = {
+        "orc_normalize": (None, [P, P]),
+        "orc_ray_recip": (None, [P, P]),
+        "orc_aabb_intersect": (i32, [P, P, P, P]),
+        "orc_triangle_intersect": (i32, [P, P, P, P, P]),
+        "orc_triangle_centroid": (None, [P, P]),
+        "orc_pack_index": (u32, [u32, u32]),
+        "orc_bvh_build": (i32, [P, u32, i32, u32, P, P]),
+        "orc_bvh_refit": (i32, [P, u32, P]),
+        "orc_bvh_depth": (u32, [P]),
+        "orc_bvh_intersect": (i32, [P, P, P, P, P]),
+        "orc_bvh_trace": (i32, [P, P, P, u64, P, P, i32]),
+        "orc_brute_trace": (i32, [P, u32, P, u64, P, P]),
+        "orc_transform_point": (None, [P, P, P]),
+        "orc_transform_vector": (None, [P, P, P]),
+        "orc_instance_bounds": (None, [P, P, P, P, P]),
+        "orc_tlas_build": (i32, [P, u32, P, P]),
+        "orc_tlas_intersect": (i32, [P, P, P, P, P, P, P]),
+        "orc_tlas_trace": (i32, [P, P, P, P, P, u64, P, P, i32]),
+        "orc_camera_ray": (None, [P, f32, f32, P]),
+        "orc_raygen": (i32, [P, u32, u32, P]),
+        "orc_mesh_normals": (None, [P, u32, P, u32, P]),
+        "orc_diff_forward": (i32, [P, P, P, P, P, P, P, u64, P, P, i32]),
+        "orc_diff_grad64": (i32, [P, u32, P, u32, P, P, P, P, P, u64, P, ctypes.c_double, ctypes.c_double,
+                                  P, P, P, P]),
+        "orc_powf": (ctypes.c_float, [ctypes.c_float, ctypes.c_float]),
+        "orc_world_start_frame": (None, [P, u32]),
+        "orc_world_run_physics": (None, [P, u32, P, u32, ctypes.c_float]),
+        "orc_objects_from_bodies": (None, [P, P, P, u32, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
 
 
 def _p(a):
@@ -171,13 +190,15 @@ def bvh_depth(nodes):
     return int(lib().orc_bvh_depth(_p(nodes)))
 
 
-def bvh_trace(nodes, tris, rays, nthreads=1, stats=False):
+def bvh_trace(nodes, tris, rays, nthreads=1, stats=False, using=None):
+    """Bvh::intersect for every ray on nthreads threads (contiguous ray
+    chunks); ``using``: a library from native_lib() instead of the default."""
     tris = np.ascontiguousarray(np.asarray(tris, np.float32).reshape(-1, 9))
     rays = np.ascontiguousarray(rays)
     assert rays.dtype == RAY_DTYPE
     hits = np.zeros(rays.shape[0], HIT_DTYPE)
     st = np.zeros(rays.shape[0], STATS_DTYPE) if stats else None
-    rc = lib().orc_bvh_trace(_p(nodes), _p(tris), _p(rays), rays.shape[0], _p(hits),
+    rc = (using or lib()).orc_bvh_trace(_p(nodes), _p(tris), _p(rays), rays.shape[0], _p(hits),
                              _p(st) if stats else None, nthreads)
     if rc != 0:
         raise ValueError(f"orc_bvh_trace failed rc={rc}")
