@@ -115,6 +115,7 @@ SIGNATURES = {
     "bvht_trace_forward": (I32, [P, P, P, P, P, U32, P, P]),
     "bvht_trace_adjoint": (I32, [P, P, P, P, P, U32, P, P, F32, F32, I32, P, P, P]),
     "bvht_trace_camera_bounce": (I32, [P, P, P, U32, U32, I32, U64, P, P, P, P]),
+    "bvht_trace_camera_bounce_noalloc": (I32, [P, P, P, U32, U32, I32, U64, P, P, P, P]),
     "bvht_world_create": (I32, [P, P, U32, P, U32, P]),
     "bvht_world_destroy": (I32, [P, P]),
     "bvht_world_set_bodies": (I32, [P, P, P, U32, U32]),

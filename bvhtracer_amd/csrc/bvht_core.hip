@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "bvht_device.hpp"
@@ -91,6 +92,42 @@ int32_t take_status(bvht_ctx ctx)
         return set_error(BVHT_ESTACK,
                          "a trace dropped traversal stack entries at its stack bound: hits since the last fence may "
                          "be wrong (raise the stack limit, bvht_set_stack_limit)");
+    return BVHT_OK;
+}
+
+}  // namespace bvht
+
+// ---- fixed scratch of bvht_trace_camera_bounce_noalloc --------------------
+// The allocation-free camera pass keeps its per-(bin, wave) counts and scan
+// tile sums here: 24 counts per 64 pixels for up to kNoAllocMaxPixels pixels,
+// plus one sum per 4096 counts.  Each GPU holds its own copy (6.0 MiB) for
+// the life of the process.  Host code never names the array: it uses the
+// address hipGetSymbolAddress gives on each device (noalloc_scratch).
+constexpr uint32_t kNoAllocMaxPixels = 1u << 22;
+constexpr size_t kNoAllocCounts = 24 * (size_t)(kNoAllocMaxPixels / 64);
+constexpr size_t kNoAllocWords = kNoAllocCounts + (kNoAllocCounts + 4095) / 4096;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+namespace bvht {
+
+static std::mutex g_noalloc_mu;
+static std::vector<void*> g_noalloc_ptr;  // device address of `scratch` on each device
+
+// Called with `device` current (context creation).
+static int32_t noalloc_scratch(int device, void** out)
+{
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if ((size_t)device >= g_noalloc_ptr.size())
+        g_noalloc_ptr.resize((size_t)device + 1, nullptr);
+    if (!g_noalloc_ptr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(BVHT_EDEVICE, "hipGetSymbolAddress of the fixed camera-pass scratch failed: %s",
+                             hipGetErrorString(e));
+        g_noalloc_ptr[device] = p;
+    }
+    *out = g_noalloc_ptr[device];
     return BVHT_OK;
 }
 
@@ -1257,6 +1294,13 @@ int32_t bvht_create(int32_t device, bvht_ctx* out)
         return set_error(BVHT_ENOMEM, "pinned status word allocation failed");
     }
     *c->h_status = 0;
+    int32_t rc = bvht::noalloc_scratch(device, &c->noalloc_scratch);
+    if (rc) {
+        (void)hipHostFree(c->h_status);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
     *out = c;
     return BVHT_OK;
 }
@@ -1563,6 +1607,36 @@ int32_t bvht_trace_camera_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_camera
     BVHT_REQUIRE((uint64_t)width * height < (1ull << 31), "frame %ux%u too large", width, height);
     BVHT_REQUIRE(width * height == 0 || (d_hits && d_out), "NULL buffer");
     return bvht::launch_trace_camera_bounce(ctx, blas, cam, width, height, order, seed, d_hits, d_out, d_src, d_count);
+}
+
+int32_t bvht_trace_camera_bounce_noalloc(bvht_ctx ctx, bvht_blas blas, const bvht_camera* cam, uint32_t width,
+                                         uint32_t height, int32_t order, uint64_t seed, bvht_hit* d_hits,
+                                         bvht_ray* d_out, uint32_t* d_src, uint32_t* d_count)
+{
+    BVHT_REQUIRE(ctx && blas && cam && d_count, "NULL argument");
+    BVHT_REQUIRE(order == BVHT_ORDER_ROWS || (order == BVHT_ORDER_TILES && width % 8 == 0 && height % 8 == 0),
+                 "order %d invalid for %ux%u (tile order needs multiples of 8)", order, width, height);
+    BVHT_REQUIRE((uint64_t)width * height <= kNoAllocMaxPixels,
+                 "frame %ux%u exceeds the %u pixels of the allocation-free camera pass's fixed scratch", width,
+                 height, kNoAllocMaxPixels);
+    uint32_t n = width * height;
+    BVHT_REQUIRE(n == 0 || (d_hits && d_out), "NULL buffer");
+    uint32_t need = stack_cap(ctx, blas->depth > 0 ? blas->depth - 1 : 0);
+    BVHT_REQUIRE(need <= 32,
+                 "a tree of depth %u needs %u traversal stack entries; the allocation-free camera pass keeps at most "
+                 "32 per ray, in LDS (bvht_set_stack_limit can cap it)", blas->depth, need);
+    BVHT_REQUIRE(ctx->noalloc_scratch, "the fixed camera-pass scratch is not set up on this context");
+    if (n == 0)
+        return bvht::fill_u32(ctx, d_count, 0, 1);
+    // the context's configuration if its LDS stack holds the tree, else 64
+    // lanes with the smallest LDS stack that does: nothing spills
+    int wg = 64, lds = need <= 8 ? 8 : (need <= 16 ? 16 : (need <= 24 ? 24 : 32));
+    if (ctx->trace_wg > 0 && (uint32_t)ctx->trace_lds >= need) {
+        wg = ctx->trace_wg;
+        lds = ctx->trace_lds;
+    }
+    return bvht::enqueue_camera_pass(ctx, blas, cam, width, height, order, seed, d_hits, d_out, d_src, d_count, wg,
+                                     lds, false, static_cast<char*>(ctx->noalloc_scratch), nullptr, need);
 }
 
 int32_t bvht_depth(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, float* d_depth)

@@ -36,6 +36,9 @@ struct bvht_ctx_s {
     // must keep their addresses (a graph holds them as kernel arguments).
     int live_graphs = 0;
     bool capturing = false;
+    // Device address, on this context's device, of the fixed scratch array
+    // of bvht_trace_camera_bounce_noalloc (shared by every context there).
+    void* noalloc_scratch = nullptr;
 };
 
 // Bits of the context status word.

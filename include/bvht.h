@@ -271,6 +271,23 @@ int32_t bvht_trace_camera_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_camera
                                  uint32_t height, int32_t order, uint64_t seed, bvht_hit* d_hits, bvht_ray* d_out,
                                  uint32_t* d_src, uint32_t* d_count);
 
+/* bvht_trace_camera_bounce without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory in
+ * the call (code under HIP graph capture, serving loops, hosts that forbid
+ * hipMalloc).  Same arguments and the same results bit for bit; the bounce
+ * rays are made again from the stored hits instead of staged.  Its scratch
+ * (per-wave direction-bin counts and scan partials) is a fixed array of the
+ * library, 6.0 MiB on each GPU, held until the process ends; nothing spills.
+ * Limits, else BVHT_EINVAL: width * height <= BVHT_NOALLOC_MAX_PIXELS, and a
+ * BLAS depth of at most 33 (or a stack limit of at most 32 entries, see
+ * bvht_set_stack_limit).  Every call on one GPU uses that GPU's one array:
+ * calls that overlap on one GPU (on several contexts' streams) race and give
+ * wrong bounce rays; calls in order on one stream do not. */
+#define BVHT_NOALLOC_MAX_PIXELS 4194304u
+int32_t bvht_trace_camera_bounce_noalloc(bvht_ctx ctx, bvht_blas blas, const bvht_camera* cam, uint32_t width,
+                                         uint32_t height, int32_t order, uint64_t seed, bvht_hit* d_hits,
+                                         bvht_ray* d_out, uint32_t* d_src, uint32_t* d_count);
+
 /* Depth accumulator (renderer.rs:176-194): d_depth[i] = hit t or f32::MAX. */
 int32_t bvht_depth(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, float* d_depth);
 
