@@ -912,7 +912,9 @@ static inline size_t align256(size_t bytes) { return (bytes + 255) & ~(size_t)25
 
 // In-place exclusive scan of a[0, m) (m > 0, a 16-byte aligned), total to
 // *total; sums holds blocks_for(m, kScanTile) entries.
-static void launch_scan(bvht_ctx ctx, uint32_t* a, uint32_t m, uint32_t* sums, uint32_t* total)
+uint32_t scan_sums_entries(uint32_t m) { return std::max<uint32_t>(blocks_for(m, kScanTile), 1); }
+
+void launch_scan(bvht_ctx ctx, uint32_t* a, uint32_t m, uint32_t* sums, uint32_t* total)
 {
     uint32_t tiles = blocks_for(m, kScanTile);
     hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(256), 0, ctx->stream, a, m, sums);

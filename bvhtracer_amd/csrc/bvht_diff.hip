@@ -42,12 +42,31 @@ struct bvht_mesh_s {
     uint32_t* d_bins = nullptr;    // kSchedBins counts, then kSchedBins cursors
     uint32_t sched_cap = 0;        // groups the buffers hold
     uint32_t sched_n = 0;          // ray count d_sched was made for (0: none)
+    // Adjoint gather.  Vertex -> (BLAS triangle, corner) entries 3k + c in
+    // ascending order; per-triangle corner gradients (9 floats); per-triangle
+    // record counts (scanned in place to bucket offsets) and cursors, both
+    // left zero by every adjoint for the next one; scan tile sums.
+    uint32_t* d_vt_off = nullptr;  // n_vertices + 1
+    uint32_t* d_vt_idx = nullptr;  // 3 n_tris
+    float* d_tgrad = nullptr;      // 9 n_tris
+    uint32_t* d_poff = nullptr;    // n_tris
+    uint32_t* d_pcur = nullptr;    // n_tris
+    uint32_t* d_psums = nullptr;   // scan tile sums of d_poff
+    uint32_t* d_big = nullptr;     // n_tris: triangles whose records k_adj_big sums
+    uint32_t* d_nbig = nullptr;    // their count
 };
 
 namespace bvht {
 namespace {
 
 inline uint32_t nblk(size_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+inline size_t align256(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, uint32_t m)
+{
+    uint32_t lo = __shfl_xor((uint32_t)v, (int)m, 64), hi = __shfl_xor((uint32_t)(v >> 32), (int)m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
 
 __device__ __forceinline__ V3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
 
@@ -234,10 +253,7 @@ forward_fn pick_forward(int wg, int lds, bool packed)
 
 // ------------------------------------------------------------------ adjoint
 
-constexpr int kAdjWG = 512;  // 256 and 1024 measured slower
-constexpr uint32_t kAdjSlotBits = 11;  // LDS vertex table of the scatter: 2048 > 3 * kAdjWG slots
-constexpr uint32_t kAdjSlots = 1u << kAdjSlotBits;
-static_assert(3 * kAdjWG < kAdjSlots, "the scatter's vertex table must keep an empty slot");
+constexpr int kAdjWG = 512;
 constexpr int kRed = 13;  // 12 entries of rows 0..2 of dLoss/dM^-1 (row-major r*4+c) + loss
 
 // Binary64 vector helpers for the adjoint's per-pixel solve.
@@ -270,7 +286,8 @@ __global__ __launch_bounds__(kAdjWG) void k_trace_adjoint(
     const float* __restrict__ tris, uint32_t ntris, const uint32_t* __restrict__ vid,
     const float* __restrict__ normals, Rows rows, V3 light, const bvht_ray* __restrict__ rays, uint32_t n,
     const bvht_hit* __restrict__ hits, const float2* __restrict__ target, float w_depth, float w_shade,
-    float* __restrict__ grad_v, double* __restrict__ partials)
+    uint32_t* __restrict__ rec_key, float* __restrict__ rec_val, uint32_t* __restrict__ tri_count,
+    double* __restrict__ partials)
 {
     __shared__ double red[kAdjWG / 64][kRed];
     uint32_t i = blockIdx.x * kAdjWG + threadIdx.x;
@@ -281,7 +298,6 @@ __global__ __launch_bounds__(kAdjWG) void k_trace_adjoint(
     uint32_t prim = i < n ? hits[i].prim : BVHT_MISS;
     float2 tg = i < n ? target[i] : make_float2(kFltMax, 0.0f);
     uint32_t key = BVHT_MISS;  // the hit triangle, for contributing rays
-    uint32_t vid3[3] = {0, 0, 0};
     float cv[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c)
@@ -333,7 +349,6 @@ __global__ __launch_bounds__(kAdjWG) void k_trace_adjoint(
             cv[3 * k + 0] = (float)(lam.x * bw[k]);
             cv[3 * k + 1] = (float)(lam.y * bw[k]);
             cv[3 * k + 2] = (float)(lam.z * bw[k]);
-            vid3[k] = vv[k];
         }
         key = prim;
         double lr[3] = {lam.x, lam.y, lam.z};
@@ -347,44 +362,60 @@ __global__ __launch_bounds__(kAdjWG) void k_trace_adjoint(
         }
         acc[12] = (double)w_depth * rt * rt + (double)w_shade * rs * rs;
     }
-    // Vertex scatter.  The block's pixels hit a few dozen neighbouring
-    // triangles that share their vertices: each corner's contribution is
-    // added into an LDS table keyed by vertex id (linear probing; at most
-    // 3 * kAdjWG keys in kAdjSlots slots, so every probe ends), and the block
-    // then issues one global atomic triple per distinct vertex.
-    __shared__ uint32_t tkey[kAdjSlots];
-    __shared__ float tval[3][kAdjSlots];
-    for (uint32_t k = threadIdx.x; k < kAdjSlots; k += kAdjWG) {
-        tkey[k] = BVHT_MISS;
-        tval[0][k] = tval[1][k] = tval[2][k] = 0.0f;
-    }
-    __syncthreads();
-    if (key != BVHT_MISS) {
+    // Per-triangle sums within the wave, in a fixed order, without floating-
+    // point atomics.  The 64 lanes are sorted by (hit triangle, lane) with a
+    // bitonic network of shuffles (lanes without a contribution sort last);
+    // sorted position p takes the contributions of the lane it came from;
+    // a segmented inclusive scan over the sorted positions sums each
+    // triangle's run, and the run's last position writes the record
+    // (triangle, 9 sums) in its own slot and counts it for the triangle.
+    // Every other slot is marked empty.
+    uint32_t lane64 = threadIdx.x & 63u;
+    uint32_t rkey = BVHT_MISS;
+    if (__ballot(key != BVHT_MISS)) {
+        uint64_t sk = ((uint64_t)key << 6) | lane64;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            uint32_t vtx = vid3[c];
-            uint32_t h = (vtx * 2654435761u) >> (32 - kAdjSlotBits);
-            while (true) {
-                uint32_t old = atomicCAS(&tkey[h], BVHT_MISS, vtx);
-                if (old == BVHT_MISS || old == vtx)
-                    break;
-                h = (h + 1) & (kAdjSlots - 1);
+        for (uint32_t k = 2; k <= 64; k <<= 1)
+#pragma unroll
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                uint64_t other = shfl_xor_u64(sk, j);
+                bool keep_min = ((lane64 & j) == 0) == ((lane64 & k) == 0);
+                sk = keep_min ? (other < sk ? other : sk) : (other > sk ? other : sk);
             }
-            atomicAdd(&tval[0][h], cv[3 * c + 0]);
-            atomicAdd(&tval[1][h], cv[3 * c + 1]);
-            atomicAdd(&tval[2][h], cv[3 * c + 2]);
+        uint32_t src = (uint32_t)(sk & 63u);
+        uint32_t skey = (uint32_t)(sk >> 6);
+        float sv[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+            sv[c] = __shfl(cv[c], (int)src, 64);
+        uint32_t prev = __shfl_up(skey, 1, 64), next = __shfl_down(skey, 1, 64);
+        // first sorted position of this position's run
+        uint32_t start = (lane64 == 0 || prev != skey) ? lane64 : 0u;
+#pragma unroll
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            uint32_t t = __shfl_up(start, off, 64);
+            if (lane64 >= off && t > start)
+                start = t;
+        }
+#pragma unroll
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                float t = __shfl_up(sv[c], off, 64);
+                if (lane64 >= off && lane64 - off >= start)
+                    sv[c] = sv[c] + t;
+            }
+        }
+        if (skey != BVHT_MISS && (lane64 == 63u || next != skey)) {
+            rkey = skey;
+            float* out = rec_val + 9 * (size_t)i;
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+                out[c] = sv[c];
+            atomicAdd(tri_count + skey, 1u);
         }
     }
-    __syncthreads();
-    for (uint32_t k = threadIdx.x; k < kAdjSlots; k += kAdjWG) {
-        uint32_t vtx = tkey[k];
-        if (vtx != BVHT_MISS) {
-            float* g = grad_v + 3 * (size_t)vtx;
-            atomicAdd(g + 0, tval[0][k]);
-            atomicAdd(g + 1, tval[1][k]);
-            atomicAdd(g + 2, tval[2][k]);
-        }
-    }
+    rec_key[i] = rkey;
     // fixed-order reduction: butterfly within the wave, then the waves in order
 #pragma unroll
     for (int c = 0; c < kRed; ++c)
@@ -459,6 +490,178 @@ __global__ __launch_bounds__(kFinWG) void k_adjoint_finish(const double* __restr
     }
 }
 
+// Bucket the wave records by triangle: tri_off[t] is the exclusive prefix of
+// the per-triangle record counts, tri_cur[t] starts at 0.  The order inside
+// a bucket comes from the atomics; k_adj_triangles sorts it.
+__global__ __launch_bounds__(256) void k_adj_place(const uint32_t* __restrict__ rec_key, uint32_t slots,
+                                                    const uint32_t* __restrict__ tri_off,
+                                                    uint32_t* __restrict__ tri_cur, uint32_t* __restrict__ bucket)
+{
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= slots)
+        return;
+    uint32_t t = rec_key[i];
+    if (t != BVHT_MISS)
+        bucket[tri_off[t] + atomicAdd(tri_cur + t, 1u)] = i;
+}
+
+// Corner gradients of triangle t: its records summed in ascending slot order,
+// so the sum is the same on every run.  A bucket of at most kAdjSmall
+// records is copied to this thread's LDS column and insertion-sorted there;
+// a larger one is listed for k_adj_big.  Leaves tri_off[t] and tri_cur[t]
+// zero for the next adjoint pass (k_adj_big does so for its triangles).
+constexpr uint32_t kAdjSmall = 32;
+constexpr uint32_t kAdjBigMax = 32768;  // records of one triangle in one pass: one per wave of 2^21 rays
+constexpr int kAdjBigWG = 1024;
+
+__device__ __forceinline__ void store9(float* __restrict__ out, const float (&g)[9])
+{
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+        out[c] = g[c];
+}
+
+__global__ __launch_bounds__(256) void k_adj_triangles(uint32_t nt, uint32_t* __restrict__ tri_off,
+                                                        uint32_t* __restrict__ tri_cur,
+                                                        const uint32_t* __restrict__ bucket,
+                                                        const float* __restrict__ rec_val, float* __restrict__ tgrad,
+                                                        uint32_t* __restrict__ big, uint32_t* __restrict__ n_big)
+{
+    __shared__ uint32_t col[kAdjSmall][256];
+    uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nt)
+        return;
+    uint32_t cnt = tri_cur[t];
+    if (cnt > kAdjSmall) {
+        big[atomicAdd(n_big, 1u)] = t;
+        return;
+    }
+    const uint32_t* b = bucket + tri_off[t];
+    for (uint32_t x = 0; x < cnt; ++x) {
+        uint32_t v = b[x], y = x;
+        for (; y > 0 && col[y - 1][threadIdx.x] > v; --y)
+            col[y][threadIdx.x] = col[y - 1][threadIdx.x];
+        col[y][threadIdx.x] = v;
+    }
+    float g[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+        g[c] = 0.0f;
+    for (uint32_t x = 0; x < cnt; ++x) {
+        const float* r = rec_val + 9 * (size_t)col[x][threadIdx.x];
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+            g[c] += r[c];
+    }
+    store9(tgrad + 9 * (size_t)t, g);
+    tri_off[t] = 0u;
+    tri_cur[t] = 0u;
+}
+
+// Triangles with more than kAdjSmall records (at most kAdjBigMax): one
+// workgroup per triangle, the gridded workgroups taking the listed triangles
+// in turn.  The bucket is bitonic-sorted in LDS, thread k sums sorted
+// records [k c, (k+1) c) in order (c = ceil(cnt / kAdjBigWG)), the partial
+// sums are added by a butterfly within each wave and the waves in order.
+__global__ __launch_bounds__(kAdjBigWG) void k_adj_big(const uint32_t* __restrict__ big,
+                                                      const uint32_t* __restrict__ n_big,
+                                                      uint32_t* __restrict__ tri_off, uint32_t* __restrict__ tri_cur,
+                                                      const uint32_t* __restrict__ bucket,
+                                                      const float* __restrict__ rec_val, float* __restrict__ tgrad)
+{
+    __shared__ uint32_t keys[kAdjBigMax];
+    __shared__ float wsum[kAdjBigWG / 64][9];
+    uint32_t nb = *n_big;
+    for (uint32_t q = blockIdx.x; q < nb; q += gridDim.x) {
+        uint32_t t = big[q];
+        uint32_t cnt = tri_cur[t];
+        const uint32_t* b = bucket + tri_off[t];
+        uint32_t p2 = 64;
+        while (p2 < cnt)
+            p2 <<= 1;
+        for (uint32_t x = threadIdx.x; x < p2; x += kAdjBigWG)
+            keys[x] = x < cnt ? b[x] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t k = 2; k <= p2; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t x = threadIdx.x; x < p2; x += kAdjBigWG) {
+                    uint32_t y = x ^ j;
+                    if (y > x) {
+                        uint32_t a = keys[x], c = keys[y];
+                        bool up = (x & k) == 0;
+                        if (up ? a > c : a < c) {
+                            keys[x] = c;
+                            keys[y] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        uint32_t per = (cnt + kAdjBigWG - 1) / kAdjBigWG;
+        float g[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+            g[c] = 0.0f;
+        for (uint32_t x = threadIdx.x * per; x < min(cnt, (threadIdx.x + 1) * per); ++x) {
+            const float* r = rec_val + 9 * (size_t)keys[x];
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+                g[c] += r[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+            for (int m = 32; m >= 1; m >>= 1)
+                g[c] += __shfl_xor(g[c], m, 64);
+        if ((threadIdx.x & 63u) == 0) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+                wsum[threadIdx.x >> 6][c] = g[c];
+        }
+        __syncthreads();
+        if (threadIdx.x < 9) {
+            float s = 0.0f;
+            for (int w = 0; w < kAdjBigWG / 64; ++w)
+                s += wsum[w][threadIdx.x];
+            tgrad[9 * (size_t)t + threadIdx.x] = s;
+        }
+        if (threadIdx.x == 0) {
+            tri_off[t] = 0u;
+            tri_cur[t] = 0u;
+        }
+        __syncthreads();
+    }
+}
+
+// dLoss/dvertex: the corner gradients of the vertex's incident (triangle,
+// corner) entries, in their fixed ascending order; written, or added with
+// accumulate.
+__global__ __launch_bounds__(256) void k_adj_vertices(uint32_t nv, const uint32_t* __restrict__ vt_off,
+                                                       const uint32_t* __restrict__ vt_idx,
+                                                       const float* __restrict__ tgrad, int accumulate,
+                                                       float* __restrict__ grad_v)
+{
+    uint32_t v = blockIdx.x * 256 + threadIdx.x;
+    if (v >= nv)
+        return;
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    for (uint32_t e = vt_off[v]; e < vt_off[v + 1]; ++e) {
+        const float* g = tgrad + 3 * (size_t)vt_idx[e];  // 9 t + 3 c = 3 (3 t + c)
+        gx += g[0];
+        gy += g[1];
+        gz += g[2];
+    }
+    float* out = grad_v + 3 * (size_t)v;
+    if (accumulate) {
+        out[0] = out[0] + gx;
+        out[1] = out[1] + gy;
+        out[2] = out[2] + gz;
+    } else {
+        out[0] = gx;
+        out[1] = gy;
+        out[2] = gz;
+    }
+}
+
 }  // namespace
 }  // namespace bvht
 
@@ -489,6 +692,14 @@ extern "C" int32_t bvht_mesh_destroy(bvht_mesh m)
     (void)hipFree(m->d_sched);
     (void)hipFree(m->d_cost);
     (void)hipFree(m->d_bins);
+    (void)hipFree(m->d_vt_off);
+    (void)hipFree(m->d_vt_idx);
+    (void)hipFree(m->d_tgrad);
+    (void)hipFree(m->d_poff);
+    (void)hipFree(m->d_pcur);
+    (void)hipFree(m->d_psums);
+    (void)hipFree(m->d_big);
+    (void)hipFree(m->d_nbig);
     delete m;
     return BVHT_OK;
 }
@@ -538,6 +749,17 @@ extern "C" int32_t bvht_mesh_create(bvht_ctx ctx, bvht_blas blas, const float* v
             for (int c = 0; c < 3; ++c)
                 idx[cur[faces[3 * (size_t)f + c]]++] = f;
     }
+    // vertex -> (BLAS triangle k, corner c) entries 3k + c, ascending
+    std::vector<uint32_t> vt_off(n_vertices + 1, 0), vt_idx(3 * (size_t)nt);
+    for (size_t e = 0; e < vid.size(); ++e)
+        ++vt_off[vid[e] + 1];
+    for (uint32_t i = 0; i < n_vertices; ++i)
+        vt_off[i + 1] += vt_off[i];
+    {
+        std::vector<uint32_t> cur(vt_off.begin(), vt_off.end() - 1);
+        for (size_t e = 0; e < vid.size(); ++e)
+            vt_idx[cur[vid[e]]++] = (uint32_t)e;
+    }
     bvht_mesh m = new bvht_mesh_s;
     m->ctx = ctx;
     m->blas = blas;
@@ -548,7 +770,15 @@ extern "C" int32_t bvht_mesh_create(bvht_ctx ctx, bvht_blas blas, const float* v
               hipMalloc(&m->d_faces, sizeof(uint32_t) * 3 * (size_t)n_faces) == hipSuccess &&
               hipMalloc(&m->d_vid, sizeof(uint32_t) * vid.size()) == hipSuccess &&
               hipMalloc(&m->d_vf_off, sizeof(uint32_t) * off.size()) == hipSuccess &&
-              hipMalloc(&m->d_vf_idx, sizeof(uint32_t) * idx.size()) == hipSuccess;
+              hipMalloc(&m->d_vf_idx, sizeof(uint32_t) * idx.size()) == hipSuccess &&
+              hipMalloc(&m->d_vt_off, sizeof(uint32_t) * vt_off.size()) == hipSuccess &&
+              hipMalloc(&m->d_vt_idx, sizeof(uint32_t) * vt_idx.size()) == hipSuccess &&
+              hipMalloc(&m->d_tgrad, sizeof(float) * 9 * (size_t)nt) == hipSuccess &&
+              hipMalloc(&m->d_poff, sizeof(uint32_t) * std::max<size_t>(nt, 4)) == hipSuccess &&
+              hipMalloc(&m->d_pcur, sizeof(uint32_t) * std::max<size_t>(nt, 4)) == hipSuccess &&
+              hipMalloc(&m->d_psums, sizeof(uint32_t) * scan_sums_entries(nt)) == hipSuccess &&
+              hipMalloc(&m->d_big, sizeof(uint32_t) * std::max<size_t>(nt, 4)) == hipSuccess &&
+              hipMalloc(&m->d_nbig, sizeof(uint32_t) * 4) == hipSuccess;
     if (!ok) {
         bvht_mesh_destroy(m);
         return set_error(BVHT_ENOMEM, "device allocation for a mesh of %u vertices failed", n_vertices);
@@ -565,11 +795,19 @@ extern "C" int32_t bvht_mesh_create(bvht_ctx ctx, bvht_blas blas, const float* v
             hipSuccess ||
         hipMemcpyAsync(m->d_vf_idx, idx.data(), sizeof(uint32_t) * idx.size(), hipMemcpyHostToDevice, st) !=
             hipSuccess ||
+        hipMemcpyAsync(m->d_vt_off, vt_off.data(), sizeof(uint32_t) * vt_off.size(), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(m->d_vt_idx, vt_idx.data(), sizeof(uint32_t) * vt_idx.size(), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
         hipMalloc(&d_err, sizeof(uint32_t)) != hipSuccess) {
         rc = set_error(BVHT_EDEVICE, "mesh upload failed");
     }
     if (!rc)
         rc = fill_u32(ctx, d_err, 0u, 1);
+    if (!rc)
+        rc = fill_u32(ctx, m->d_poff, 0u, nt);
+    if (!rc)
+        rc = fill_u32(ctx, m->d_pcur, 0u, nt);
     if (!rc) {
         hipLaunchKernelGGL(k_mesh_check, dim3(nblk(nt, 256)), dim3(256), 0, st, m->d_vertices, m->d_vid, nt,
                            blas->d_tris, d_err);
@@ -686,21 +924,52 @@ extern "C" int32_t bvht_trace_adjoint(bvht_ctx ctx, bvht_mesh mesh, const float*
     BVHT_REQUIRE(d_grad_vertices && d_grad_minv && d_loss, "NULL output buffer");
     BVHT_REQUIRE(n == 0 || (d_rays && d_hits && d_target), "NULL device buffer");
     BVHT_REQUIRE(n <= 0xFFFFFF00u, "too many rays (%u)", n);
-    if (!accumulate) {
-        int32_t rc = fill_u32(ctx, reinterpret_cast<uint32_t*>(d_grad_vertices), 0u, 3 * (size_t)mesh->n_vertices);
-        if (rc)
-            return rc;
-    }
+    // Passes of at most kAdjPass rays: a triangle then gets at most one record
+    // per wave of the pass (kAdjBigMax), which k_adj_big's LDS holds.  The
+    // vertex gradients of later passes add to the earlier ones in pass order.
+    constexpr uint32_t kAdjPass = kAdjBigMax * 64;
     uint32_t nb = std::max<uint32_t>(nblk(n, kAdjWG), 1);
-    int32_t rc = ensure_scratch(ctx, sizeof(double) * 16 * (size_t)nb);
+    uint32_t pass_slots = std::min<uint32_t>(nb, kAdjPass / kAdjWG) * kAdjWG;
+    // context scratch: per-block partials of the whole call, then per-slot
+    // record keys, record values (9 floats) and the triangle buckets of a pass
+    size_t keys_at = align256(sizeof(double) * 16 * (size_t)nb);
+    size_t vals_at = align256(keys_at + sizeof(uint32_t) * (size_t)pass_slots);
+    size_t bucket_at = align256(vals_at + sizeof(float) * 9 * (size_t)pass_slots);
+    int32_t rc = ensure_scratch(ctx, bucket_at + sizeof(uint32_t) * (size_t)pass_slots);
     if (rc)
         return rc;
-    double* partials = reinterpret_cast<double*>(ctx->scratch);
+    char* base = static_cast<char*>(ctx->scratch);
+    double* partials = reinterpret_cast<double*>(base);
+    uint32_t* rec_key = reinterpret_cast<uint32_t*>(base + keys_at);
+    float* rec_val = reinterpret_cast<float*>(base + vals_at);
+    uint32_t* bucket = reinterpret_cast<uint32_t*>(base + bucket_at);
     bvht_blas b = mesh->blas;
-    hipLaunchKernelGGL(k_trace_adjoint, dim3(nb), dim3(kAdjWG), 0, ctx->stream, b->d_tris, b->n_tris, mesh->d_vid,
-                       mesh->d_normals, rows_of(minv), V3{light[0], light[1], light[2]}, d_rays, n, d_hits,
-                       reinterpret_cast<const float2*>(d_target), w_depth, w_shade, d_grad_vertices, partials);
-    BVHT_HIP(hipGetLastError());
+    uint32_t nt = b->n_tris;
+    for (uint32_t first = 0; first == 0 || first < n; first += kAdjPass) {
+        uint32_t m = std::min(n - first, kAdjPass);
+        uint32_t blocks = std::max<uint32_t>(nblk(m, kAdjWG), 1);
+        uint32_t slots = blocks * kAdjWG;
+        hipLaunchKernelGGL(k_trace_adjoint, dim3(blocks), dim3(kAdjWG), 0, ctx->stream, b->d_tris, nt, mesh->d_vid,
+                           mesh->d_normals, rows_of(minv), V3{light[0], light[1], light[2]}, d_rays + first, m,
+                           d_hits + first, reinterpret_cast<const float2*>(d_target) + first, w_depth, w_shade,
+                           rec_key, rec_val, mesh->d_poff, partials + 16 * (size_t)(first / kAdjWG));
+        launch_scan(ctx, mesh->d_poff, nt, mesh->d_psums, nullptr);
+        hipLaunchKernelGGL(k_adj_place, dim3(nblk(slots, 256)), dim3(256), 0, ctx->stream, rec_key, slots,
+                           mesh->d_poff, mesh->d_pcur, bucket);
+        rc = fill_u32(ctx, mesh->d_nbig, 0u, 1);
+        if (rc)
+            return rc;
+        hipLaunchKernelGGL(k_adj_triangles, dim3(nblk(nt, 256)), dim3(256), 0, ctx->stream, nt, mesh->d_poff,
+                           mesh->d_pcur, bucket, rec_val, mesh->d_tgrad, mesh->d_big, mesh->d_nbig);
+        hipLaunchKernelGGL(k_adj_big, dim3(std::min<uint32_t>(nt, 64)), dim3(kAdjBigWG), 0, ctx->stream,
+                           mesh->d_big, mesh->d_nbig, mesh->d_poff, mesh->d_pcur, bucket, rec_val, mesh->d_tgrad);
+        hipLaunchKernelGGL(k_adj_vertices, dim3(nblk(mesh->n_vertices, 256)), dim3(256), 0, ctx->stream,
+                           mesh->n_vertices, mesh->d_vt_off, mesh->d_vt_idx, mesh->d_tgrad,
+                           (int)(accumulate || first > 0), d_grad_vertices);
+        BVHT_HIP(hipGetLastError());
+        if (n - first <= kAdjPass)
+            break;
+    }
     hipLaunchKernelGGL(k_adjoint_finish, dim3(1), dim3(kFinWG), 0, ctx->stream, partials, nb, (int)accumulate,
                        d_grad_minv, d_loss);
     BVHT_HIP(hipGetLastError());

@@ -117,6 +117,11 @@ inline uint32_t stack_cap(bvht_ctx ctx, uint32_t need)
 }
 int32_t blas_alloc(bvht_ctx ctx, uint32_t n, uint32_t node_count, bvht_blas* out);
 int32_t fill_u32(bvht_ctx ctx, uint32_t* p, uint32_t v, size_t n);
+// In-place exclusive scan of a[0, m) (m > 0, a 16-byte aligned) on ctx's
+// stream; sums holds scan_sums_entries(m) entries; the total goes to *total
+// unless it is NULL.
+void launch_scan(bvht_ctx ctx, uint32_t* a, uint32_t m, uint32_t* sums, uint32_t* total);
+uint32_t scan_sums_entries(uint32_t m);
 // Longest first: d_sched[0, n) = the units ordered by descending d_cost (a
 // permutation of 0..n-1); d_bins holds 2 * kSchedBins counters.
 constexpr uint32_t kSchedBins = 512;

@@ -479,13 +479,12 @@ int32_t bvht_trace_forward(bvht_ctx ctx, bvht_mesh mesh, const float* minv, cons
  * recomputes t,u,v of the hit triangle with the Moeller-Trumbore arithmetic
  * (in binary64, on the binary32 inputs: grazing rays make the binary32 solve
  * ill-conditioned) and propagates dLoss/d(t,u,v) in closed form to the
- * triangle's vertices
- * (scattered with atomics into d_grad_vertices, float[3 n_vertices]) and
- * through o', d' to M^-1 (d_grad_minv, float[16] column-major; the bottom
- * row is 0).  *d_loss (device double) receives the loss.  accumulate = 0
- * overwrites the three outputs, else adds to them.  The M^-1 gradient and the
- * loss are reduced in a fixed order (deterministic); the vertex scatter is
- * not.  dLoss/dM follows as -(M^-1)^T G (M^-1)^T. */
+ * triangle's vertices (d_grad_vertices, float[3 n_vertices]) and through
+ * o', d' to M^-1 (d_grad_minv, float[16] column-major; the bottom row is 0).
+ * *d_loss (device double) receives the loss.  accumulate = 0 overwrites the
+ * three outputs, else adds to them.  Every sum runs in a fixed order (no
+ * floating-point atomics): the same inputs give the same bits on every run.
+ * dLoss/dM follows as -(M^-1)^T G (M^-1)^T. */
 int32_t bvht_trace_adjoint(bvht_ctx ctx, bvht_mesh mesh, const float* minv, const float* light,
                            const bvht_ray* d_rays, uint32_t n, const bvht_hit* d_hits,
                            const float* d_target, float w_depth, float w_shade, int32_t accumulate,
