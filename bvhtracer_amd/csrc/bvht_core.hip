@@ -667,6 +667,12 @@ __global__ __launch_bounds__(256) void k_bounce_bin_write(BounceSrc src, const b
 // - not STAGED (no per-ray scratch): k_camera_bounce_regroup makes each
 //   bounce ray again from the stored hit and the regenerated camera ray, and
 //   ranks it with the same ballots.
+//
+// Wave order: the wave at grid position `slot` runs camera-ray group
+// sched[slot] (rays 64 g .. 64 g + 63), or group slot without a schedule,
+// and with `cost` its lane 0 records the wave's duration in cost[group]
+// (s_memrealtime, 10 ns ticks).  Which wave runs a group changes nothing
+// else: its rays, hits, counts and bounce rays are those of the group.
 template <int WG, int LDS_DEPTH, bool PACKED, bool STAGED>
 __global__ __launch_bounds__(WG) void k_trace_camera_bounce(const float4* __restrict__ nodes,
                                                             const float* __restrict__ tris, uint32_t ntris,
@@ -675,11 +681,16 @@ __global__ __launch_bounds__(WG) void k_trace_camera_bounce(const float4* __rest
                                                             float4* __restrict__ staged, uint32_t* __restrict__ key,
                                                             uint32_t* __restrict__ counts, uint32_t n_waves,
                                                             uint32_t* __restrict__ spill, uint32_t cap,
-                                                            uint32_t* __restrict__ status)
+                                                            uint32_t* __restrict__ status,
+                                                            const uint32_t* __restrict__ sched,
+                                                            uint32_t* __restrict__ cost)
 {
     __shared__ uint32_t lds[LDS_DEPTH * WG];
+    uint64_t t0 = (uint64_t)wall_clock64();
     uint32_t n = width * height;
-    uint32_t j = blockIdx.x * WG + threadIdx.x;
+    uint32_t slot = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+    uint32_t group = (sched && slot < n_waves) ? sched[slot] : slot;
+    uint32_t j = group * 64 + (threadIdx.x & 63u);
     bool valid = false;
     uint32_t bin = 0;
     if (j < n) {
@@ -713,11 +724,14 @@ __global__ __launch_bounds__(WG) void k_trace_camera_bounce(const float4* __rest
         if (lane == k)
             wave_count = (uint32_t)__popcll(m);
     }
-    uint32_t w = j >> 6;
-    if (w < n_waves && lane < kDirBins)
-        counts[(size_t)lane * n_waves + w] = wave_count;
+    if (group < n_waves && lane < kDirBins)
+        counts[(size_t)lane * n_waves + group] = wave_count;
     if (STAGED && j < n)
         key[j] = valid ? (bin << 8 | rank) : 0xFFFFFFFFu;
+    if (cost && lane == 0 && group < n_waves) {
+        uint64_t dt = (uint64_t)wall_clock64() - t0;
+        cost[group] = dt > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)dt;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_bounce_scatter(const float4* __restrict__ staged,
@@ -790,7 +804,7 @@ __global__ __launch_bounds__(256) void k_camera_bounce_regroup(const float* __re
 
 typedef void (*camera_bounce_fn)(const float4*, const float*, uint32_t, bvht_camera, uint32_t, uint32_t, int32_t,
                                  uint64_t, bvht_hit*, float4*, uint32_t*, uint32_t*, uint32_t, uint32_t*, uint32_t,
-                                 uint32_t*);
+                                 uint32_t*, const uint32_t*, uint32_t*);
 
 static camera_bounce_fn pick_camera_bounce(int wg, int lds, bool packed, bool staged)
 {
@@ -1169,7 +1183,7 @@ static size_t camera_pass_layout(uint32_t n, bool staged, size_t* count_at, size
 static int32_t enqueue_camera_pass(bvht_ctx ctx, bvht_blas blas, const bvht_camera* cam, uint32_t width,
                                    uint32_t height, int32_t order, uint64_t seed, bvht_hit* d_hits, bvht_ray* d_out,
                                    uint32_t* d_src, uint32_t* d_count, int wg, int lds, bool staged, char* base,
-                                   uint32_t* spill, uint32_t need)
+                                   uint32_t* spill, uint32_t need, const uint32_t* sched, uint32_t* cost)
 {
     camera_bounce_fn fn = pick_camera_bounce(wg, lds, blas->packed_stack, staged);
     if (!fn)
@@ -1185,7 +1199,7 @@ static int32_t enqueue_camera_pass(bvht_ctx ctx, bvht_blas blas, const bvht_came
     uint32_t* sums = reinterpret_cast<uint32_t*>(base + sums_at);
     hipLaunchKernelGGL(fn, dim3(blocks_for(n, wg)), dim3(wg), 0, ctx->stream,
                        reinterpret_cast<const float4*>(blas->d_nodes), blas->d_tris, blas->n_tris, *cam, width, height,
-                       order, seed, d_hits, stg, key, counts, n_waves, spill, need, ctx->d_status);
+                       order, seed, d_hits, stg, key, counts, n_waves, spill, need, ctx->d_status, sched, cost);
     launch_scan(ctx, counts, (uint32_t)entries, sums, d_count);
     if (staged)
         hipLaunchKernelGGL(k_bounce_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, stg, key, n,
@@ -1220,8 +1234,48 @@ int32_t launch_trace_camera_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_came
     int32_t rc = ensure_scratch(ctx, camera_pass_layout(n, true, &count_at, &sums_at));
     if (rc)
         return rc;
-    return enqueue_camera_pass(ctx, blas, cam, width, height, order, seed, d_hits, d_out, d_src, d_count, wg, lds,
-                               true, static_cast<char*>(ctx->scratch), spill, need);
+    // longest-first wave order from the last pass over the same tree, camera
+    // and frame; every pass records its waves' durations for the next
+    uint32_t groups = (n + 63) / 64;
+    if (groups > ctx->cam_cap) {
+        if (ctx->live_graphs > 0 || ctx->capturing)
+            return refuse_move(ctx, "the camera-pass wave schedule", sizeof(uint32_t) * 2 * (size_t)groups);
+        BVHT_HIP(hipStreamSynchronize(ctx->stream));
+        (void)hipFree(ctx->cam_cost);
+        (void)hipFree(ctx->cam_sched);
+        ctx->cam_cost = ctx->cam_sched = nullptr;
+        ctx->cam_cap = 0;
+        ctx->cam_valid = false;
+        if (hipMalloc(&ctx->cam_cost, sizeof(uint32_t) * groups) != hipSuccess ||
+            hipMalloc(&ctx->cam_sched, sizeof(uint32_t) * groups) != hipSuccess ||
+            (!ctx->cam_bins && hipMalloc(&ctx->cam_bins, sizeof(uint32_t) * 2 * kSchedBins) != hipSuccess))
+            return set_error(BVHT_ENOMEM, "camera-pass wave schedule for %u groups", groups);
+        ctx->cam_cap = groups;
+    }
+    bool same = ctx->cam_valid && ctx->cam_blas == blas && ctx->cam_w == width && ctx->cam_h == height &&
+                ctx->cam_order == order && std::memcmp(&ctx->cam_key, cam, sizeof *cam) == 0;
+    rc = enqueue_camera_pass(ctx, blas, cam, width, height, order, seed, d_hits, d_out, d_src, d_count, wg, lds,
+                             true, static_cast<char*>(ctx->scratch), spill, need, same ? ctx->cam_sched : nullptr,
+                             ctx->cam_cost);
+    if (rc)
+        return rc;
+    // the order changes little between passes over the same frame: rebuilt
+    // after the first pass and then every kCamResched passes (four small
+    // launches each time)
+    constexpr uint32_t kCamResched = 8;
+    ctx->cam_passes = same ? ctx->cam_passes + 1 : 0;
+    if (ctx->cam_passes % kCamResched == 0) {
+        rc = launch_longest_first(ctx, ctx->cam_cost, groups, ctx->cam_bins, ctx->cam_sched);
+        if (rc)
+            return rc;
+    }
+    ctx->cam_valid = true;
+    ctx->cam_blas = blas;
+    ctx->cam_w = width;
+    ctx->cam_h = height;
+    ctx->cam_order = order;
+    ctx->cam_key = *cam;
+    return BVHT_OK;
 }
 
 // Bounce rays grouped by direction bin, ray order inside a bin: counts per
@@ -1316,6 +1370,9 @@ int32_t bvht_destroy(bvht_ctx ctx)
         (void)hipFree(ctx->scratch);
     if (ctx->spill)
         (void)hipFree(ctx->spill);
+    (void)hipFree(ctx->cam_cost);
+    (void)hipFree(ctx->cam_sched);
+    (void)hipFree(ctx->cam_bins);
     if (ctx->h_status)
         (void)hipHostFree(ctx->h_status);
     (void)hipStreamDestroy(ctx->stream);
@@ -1638,7 +1695,8 @@ int32_t bvht_trace_camera_bounce_noalloc(bvht_ctx ctx, bvht_blas blas, const bvh
         lds = ctx->trace_lds;
     }
     return bvht::enqueue_camera_pass(ctx, blas, cam, width, height, order, seed, d_hits, d_out, d_src, d_count, wg,
-                                     lds, false, static_cast<char*>(ctx->noalloc_scratch), nullptr, need);
+                                     lds, false, static_cast<char*>(ctx->noalloc_scratch), nullptr, need, nullptr,
+                                     nullptr);
 }
 
 int32_t bvht_depth(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, float* d_depth)

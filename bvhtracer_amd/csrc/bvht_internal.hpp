@@ -36,6 +36,19 @@ struct bvht_ctx_s {
     // must keep their addresses (a graph holds them as kernel arguments).
     int live_graphs = 0;
     bool capturing = false;
+    // Camera-pass wave schedule (bvht_trace_camera_bounce): the durations
+    // of the last pass's 64-ray groups, and the longest-first order made
+    // from them, used by the next pass with the same tree, camera and frame.
+    uint32_t* cam_cost = nullptr;
+    uint32_t* cam_sched = nullptr;
+    uint32_t* cam_bins = nullptr;
+    uint32_t cam_cap = 0;
+    bool cam_valid = false;
+    const void* cam_blas = nullptr;
+    bvht_camera cam_key{};
+    uint32_t cam_w = 0, cam_h = 0;
+    int32_t cam_order = -1;
+    uint32_t cam_passes = 0;  // passes since the key last changed
     // Device address, on this context's device, of the fixed scratch array
     // of bvht_trace_camera_bounce_noalloc (shared by every context there).
     void* noalloc_scratch = nullptr;

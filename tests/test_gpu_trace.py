@@ -402,3 +402,31 @@ def test_two_contexts_trace_one_tree_at_once(ctx):
     finally:
         autotune.set_config(ctx, 0, 0)
         other.close()
+
+
+def test_camera_pass_wave_schedule_keeps_results(ctx):
+    """From the second pass over the same tree, camera and frame the camera
+    pass runs its 64-ray groups longest first, ordered by the durations the
+    previous pass recorded.  Every pass (other seeds, another camera and
+    back) equals the same pass on a fresh context, which runs unscheduled,
+    bit for bit."""
+    mesh = scenes.icosphere(48, radius=32.0)
+    bvh = B.BvhBuilder(B.BUILD_SAH, 2, ctx=ctx).build_for(mesh.copy())
+    W, H = 640, 360
+    cam = scenes.sphere_camera(32.0, W, H)
+    cam2 = scenes.sphere_camera(32.0, W, H, distance=2.2)
+
+    def run(c, camera, seed):
+        out = B.trace_camera_bounce(c, bvh, camera, W, H, B.ORDER_TILES, seed)
+        nb = int(out[3].download(np.uint32, 1)[0])
+        return (nb, out[0].download(np.uint8).tobytes(), out[1].download(B.RAY_DTYPE, nb).tobytes(),
+                out[2].download(np.uint32, nb).tobytes())
+
+    for camera, seed in [(cam, 1), (cam, 1), (cam, 2), (cam, 3), (cam2, 3), (cam2, 4), (cam, 5)]:
+        fresh = B.Context(0)
+        try:
+            want = run(fresh, camera, seed)
+        finally:
+            fresh.close()
+        assert run(ctx, camera, seed) == want, (seed,)
+
