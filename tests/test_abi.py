@@ -32,3 +32,27 @@ def test_library_exports_every_declared_symbol():
     lib = L.lib()  # ctypes binding of every signature resolves
     assert set(L.SIGNATURES) == set(declared_symbols())
     assert lib.bvht_last_error() is not None
+
+
+def header_arities():
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "bvht.h")).read(), flags=re.S)
+    out = {}
+    for name, args in re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(bvht_\w+)\s*\(([^;]*?)\)\s*;", text, re.M | re.S):
+        args = " ".join(args.split())
+        out[name] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_rust_sys_crate_declares_every_entry_point():
+    """No Rust toolchain exists here, so the -sys crate is checked against
+    the header as text: the same entry points with the same number of
+    arguments, and the record sizes it asserts."""
+    src = open(os.path.join(ROOT, "rust", "bvhtracer-hip-sys", "src", "lib.rs")).read()
+    rust = {}
+    for name, args in re.findall(r"pub fn (bvht_\w+)\((.*?)\)", src, re.S):
+        args = " ".join(args.split())
+        rust[name] = 0 if not args else args.count(",") + 1
+    assert rust == header_arities()
+    for rec, size in (("bvht_node", 32), ("bvht_tnode", 32), ("bvht_ray", 32), ("bvht_hit", 24),
+                      ("bvht_instance", 144), ("bvht_camera", 100), ("bvht_body", 272), ("bvht_force", 48)):
+        assert f"size_of::<{rec}>() == {size})" in src, rec
