@@ -13,13 +13,19 @@
 //     combination does not matter).  The split decision is made by one
 //     thread per node with the reference's sequential arithmetic.  The
 //     two-pointer partition permutation is reproduced exactly from prefix
-//     counts (derivation at k_part_scatter).
+//     counts (derivation at k_part_scatter).  The level loop runs on the
+//     device: the node list of the next level, the chunks and every count
+//     stay in device memory (Ctl), and the host reads the counts back only
+//     every few levels to see whether any large node is left.
 //   * Small nodes are finished by one thread each, running the reference's
 //     sequential subdivide on their range.
 //   * Node numbering: the reference allocates child pairs in depth-first
 //     pre-order (bvh.rs:440-466), so the children of the internal node with
 //     pre-order rank r are at 2+2r and 3+2r.  Ranks follow from the internal
-//     counts of the subtrees once the tree is known.
+//     counts of the subtrees once the tree is known (k_number, on the
+//     device).
+//   * Every buffer but the BLAS itself lives in a workspace the context keeps
+//     between builds.
 //
 // Per element the build keeps its centroid (Triangle::centroid, exact), the
 // min/max of its three vertices (growing a box by the three vertices equals
@@ -61,6 +67,33 @@ struct Decision {
     float pos;
     uint32_t nl;    // left count after the partition
 };
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// A node of the large-node tree: every node split level by level, their
+// children, and the small subtrees' roots.  Children are allocated in pairs
+// after their parent, so a child's index is always larger.
+struct BNode {
+    uint32_t first, count;
+    float box[6];
+    uint32_t left;   // index of the left child (right = left + 1); kNone for a leaf
+    uint32_t small;  // slot in the small-root list; kNone for a large node
+};
+
+// Build state the level kernels share on the device; the host reads it back
+// only every few levels.
+struct Ctl {
+    uint32_t L;           // large nodes of the current level
+    uint32_t C;           // chunks of the current level
+    uint32_t H;           // BNodes allocated
+    uint32_t R;           // small roots listed
+    uint32_t lvl_off;     // first entry of the current level in the level list
+    uint32_t nlevels;     // levels processed
+    uint32_t err;         // self-check flags
+    uint32_t nodes_used;  // 2 + 2 * internal nodes, after numbering
+};
+
+constexpr int kCtlWG = 1024;  // the single-workgroup level kernels
 
 constexpr uint32_t kEmptyMin = 0xFFFFFFFFu;  // key sentinels never produced by a non-NaN value
 constexpr uint32_t kEmptyMax = 0u;
@@ -140,16 +173,18 @@ __global__ void k_init_elems(const float* __restrict__ tris, uint32_t n, float4*
     ehi[i] = make_float2(hi.y, hi.z);
 }
 
-__global__ void k_init_acc(uint32_t* __restrict__ cacc, uint32_t* __restrict__ bacc, uint32_t* __restrict__ child,
-                           uint32_t L)
+// Accumulators of the level's L large nodes (L read on the device).
+__global__ void k_init_acc(const Ctl* __restrict__ ctl, uint32_t* __restrict__ cacc, uint32_t* __restrict__ bacc,
+                           uint32_t* __restrict__ child)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 6 * L)
-        cacc[i] = (i % 6) < 3 ? kEmptyMin : kEmptyMax;
-    if (i < 12 * L)
-        child[i] = (i % 12) < 6 ? kEmptyMin : kEmptyMax;
-    if (i < 3 * kBins * 7 * L) {
-        uint32_t f = i % 7;
+    size_t L = ctl->L;
+    size_t total = (size_t)3 * kBins * 7 * L;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        if (i < 6 * L)
+            cacc[i] = (i % 6) < 3 ? kEmptyMin : kEmptyMax;
+        if (i < 12 * L)
+            child[i] = (i % 12) < 6 ? kEmptyMin : kEmptyMax;
+        uint32_t f = (uint32_t)(i % 7);
         bacc[i] = f == 0 ? 0u : (f <= 3 ? kEmptyMin : kEmptyMax);
     }
 }
@@ -178,18 +213,24 @@ __global__ __launch_bounds__(kWG) void k_range_box(const float4* __restrict__ el
 }
 
 // Centroid range per node and axis (find_best_split_plane, bvh.rs:338-344).
+// The chunk kernels run a fixed grid over the level's chunks (the count is
+// read on the device), a workgroup per chunk at a time.
 __global__ __launch_bounds__(kWG) void k_centroid_range(const float4* __restrict__ ec, const Chunk* __restrict__ chunks,
+                                                        const Ctl* __restrict__ ctl,
                                                         uint32_t* __restrict__ cacc /* slot*6 */)
 {
     __shared__ uint32_t lds[6 * 4];
-    Chunk ch = chunks[blockIdx.x];
-    uint32_t v[6] = {kEmptyMin, kEmptyMin, kEmptyMin, kEmptyMax, kEmptyMax, kEmptyMax};
-    for (uint32_t e = ch.begin + threadIdx.x; e < ch.end; e += kWG) {
-        float4 c = ec[e];
-        key_min(v[0], c.x); key_min(v[1], c.y); key_min(v[2], c.z);
-        key_max(v[3], c.x); key_max(v[4], c.y); key_max(v[5], c.z);
+    uint32_t C = ctl->C;
+    for (uint32_t ci = blockIdx.x; ci < C; ci += gridDim.x) {
+        Chunk ch = chunks[ci];
+        uint32_t v[6] = {kEmptyMin, kEmptyMin, kEmptyMin, kEmptyMax, kEmptyMax, kEmptyMax};
+        for (uint32_t e = ch.begin + threadIdx.x; e < ch.end; e += kWG) {
+            float4 c = ec[e];
+            key_min(v[0], c.x); key_min(v[1], c.y); key_min(v[2], c.z);
+            key_max(v[3], c.x); key_max(v[4], c.y); key_max(v[5], c.z);
+        }
+        block_reduce_keys<6>(v, cacc + 6 * ch.slot, lds);
     }
-    block_reduce_keys<6>(v, cacc + 6 * ch.slot, lds);
 }
 
 struct AxisRange {
@@ -228,52 +269,56 @@ __device__ __forceinline__ uint32_t bin_index(float p)
 // bacc layout per slot: [axis][bin][7] = count, minx,miny,minz, maxx,maxy,maxz keys.
 __global__ __launch_bounds__(kWG) void k_bins(const float4* __restrict__ ec, const float4* __restrict__ elo,
                                               const float2* __restrict__ ehi, const Chunk* __restrict__ chunks,
-                                              const uint32_t* __restrict__ cacc, uint32_t* __restrict__ bacc,
-                                              int faithful)
+                                              const Ctl* __restrict__ ctl, const uint32_t* __restrict__ cacc,
+                                              uint32_t* __restrict__ bacc, int faithful)
 {
     __shared__ uint32_t sb[3 * kBins * 7];
-    Chunk ch = chunks[blockIdx.x];
-    for (uint32_t k = threadIdx.x; k < 3 * kBins * 7; k += kWG) {
-        uint32_t f = k % 7;
-        sb[k] = f == 0 ? 0u : (f <= 3 ? kEmptyMin : kEmptyMax);
-    }
-    AxisRange ar[3];
-    for (int a = 0; a < 3; ++a)
-        ar[a] = axis_range(cacc + 6 * ch.slot, a, faithful);
-    __syncthreads();
-    for (uint32_t e = ch.begin + threadIdx.x; e < ch.end; e += kWG) {
-        float4 c = ec[e];
-        float4 lo = elo[e];
-        float2 hi = ehi[e];
-        float cc[3] = {c.x, c.y, c.z};
-        for (int a = 0; a < 3; ++a) {
-            if (!ar[a].valid)
-                continue;
-            uint32_t b = bin_index((cc[a] - ar[a].bmin) * ar[a].scale);
-            uint32_t* s = sb + (a * kBins + b) * 7;
-            atomicAdd(s, 1u);
-            if (not_nan(lo.x)) atomicMin(s + 1, f2key(lo.x));
-            if (not_nan(lo.y)) atomicMin(s + 2, f2key(lo.y));
-            if (not_nan(lo.z)) atomicMin(s + 3, f2key(lo.z));
-            if (not_nan(lo.w)) atomicMax(s + 4, f2key(lo.w));
-            if (not_nan(hi.x)) atomicMax(s + 5, f2key(hi.x));
-            if (not_nan(hi.y)) atomicMax(s + 6, f2key(hi.y));
+    uint32_t C = ctl->C;
+    for (uint32_t ci = blockIdx.x; ci < C; ci += gridDim.x) {
+        Chunk ch = chunks[ci];
+        for (uint32_t k = threadIdx.x; k < 3 * kBins * 7; k += kWG) {
+            uint32_t f = k % 7;
+            sb[k] = f == 0 ? 0u : (f <= 3 ? kEmptyMin : kEmptyMax);
         }
-    }
-    __syncthreads();
-    uint32_t* g = bacc + (size_t)ch.slot * (3 * kBins * 7);
-    for (uint32_t k = threadIdx.x; k < 3 * kBins * 7; k += kWG) {
-        uint32_t f = k % 7, v = sb[k];
-        if (f == 0) {
-            if (v)
-                atomicAdd(g + k, v);
-        } else if (f <= 3) {
-            if (v != kEmptyMin)
-                atomicMin(g + k, v);
-        } else {
-            if (v != kEmptyMax)
-                atomicMax(g + k, v);
+        AxisRange ar[3];
+        for (int a = 0; a < 3; ++a)
+            ar[a] = axis_range(cacc + 6 * ch.slot, a, faithful);
+        __syncthreads();
+        for (uint32_t e = ch.begin + threadIdx.x; e < ch.end; e += kWG) {
+            float4 c = ec[e];
+            float4 lo = elo[e];
+            float2 hi = ehi[e];
+            float cc[3] = {c.x, c.y, c.z};
+            for (int a = 0; a < 3; ++a) {
+                if (!ar[a].valid)
+                    continue;
+                uint32_t b = bin_index((cc[a] - ar[a].bmin) * ar[a].scale);
+                uint32_t* s = sb + (a * kBins + b) * 7;
+                atomicAdd(s, 1u);
+                if (not_nan(lo.x)) atomicMin(s + 1, f2key(lo.x));
+                if (not_nan(lo.y)) atomicMin(s + 2, f2key(lo.y));
+                if (not_nan(lo.z)) atomicMin(s + 3, f2key(lo.z));
+                if (not_nan(lo.w)) atomicMax(s + 4, f2key(lo.w));
+                if (not_nan(hi.x)) atomicMax(s + 5, f2key(hi.x));
+                if (not_nan(hi.y)) atomicMax(s + 6, f2key(hi.y));
+            }
         }
+        __syncthreads();
+        uint32_t* g = bacc + (size_t)ch.slot * (3 * kBins * 7);
+        for (uint32_t k = threadIdx.x; k < 3 * kBins * 7; k += kWG) {
+            uint32_t f = k % 7, v = sb[k];
+            if (f == 0) {
+                if (v)
+                    atomicAdd(g + k, v);
+            } else if (f <= 3) {
+                if (v != kEmptyMin)
+                    atomicMin(g + k, v);
+            } else {
+                if (v != kEmptyMax)
+                    atomicMax(g + k, v);
+            }
+        }
+        __syncthreads();  // sb is read above before the next chunk resets it
     }
 }
 
@@ -340,12 +385,10 @@ __device__ __forceinline__ void sah_axis(const uint32_t counts[kBins], const FBo
 }
 
 // Decide each large node of the level (bvh.rs:405-417).
-__global__ void k_decide(const Task* __restrict__ tasks, uint32_t ntasks, const uint32_t* __restrict__ cacc,
-                         const uint32_t* __restrict__ bacc, Decision* __restrict__ dec, int faithful, uint32_t max_leaf)
+__device__ void decide_node(const Task* __restrict__ tasks, uint32_t s, const uint32_t* __restrict__ cacc,
+                            const uint32_t* __restrict__ bacc, Decision* __restrict__ dec, int faithful,
+                            uint32_t max_leaf)
 {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ntasks)
-        return;
     Task t = tasks[s];
     int best_axis = -1;
     float best_pos = 0.0f, best_cost = kFltMax;
@@ -391,47 +434,60 @@ __global__ void k_decide(const Task* __restrict__ tasks, uint32_t ntasks, const 
     dec[s] = d;
 }
 
+__global__ void k_decide(const Task* __restrict__ tasks, const Ctl* __restrict__ ctl, const uint32_t* __restrict__ cacc,
+                         const uint32_t* __restrict__ bacc, Decision* __restrict__ dec, int faithful, uint32_t max_leaf)
+{
+    uint32_t L = ctl->L;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < L; s += gridDim.x * blockDim.x)
+        decide_node(tasks, s, cacc, bacc, dec, faithful, max_leaf);
+}
+
 // Left counts per chunk for the plane splits.
 __global__ __launch_bounds__(kWG) void k_part_count(const float4* __restrict__ ec, const Chunk* __restrict__ chunks,
-                                                    const Decision* __restrict__ dec, uint32_t* __restrict__ chunk_left)
+                                                    const Ctl* __restrict__ ctl, const Decision* __restrict__ dec,
+                                                    uint32_t* __restrict__ chunk_left)
 {
     __shared__ uint32_t wc[4];
-    Chunk ch = chunks[blockIdx.x];
-    Decision d = dec[ch.slot];
-    if (d.kind != 1) {
+    uint32_t C = ctl->C;
+    for (uint32_t ci = blockIdx.x; ci < C; ci += gridDim.x) {
+        Chunk ch = chunks[ci];
+        Decision d = dec[ch.slot];
+        if (d.kind != 1) {
+            if (threadIdx.x == 0)
+                chunk_left[ci] = 0;
+            continue;
+        }
+        uint32_t cnt = 0;
+        for (uint32_t e = ch.begin + threadIdx.x; e < ch.end; e += kWG) {
+            float4 c = ec[e];
+            float v = d.axis == 0 ? c.x : (d.axis == 1 ? c.y : c.z);
+            cnt += (v < d.pos) ? 1u : 0u;
+        }
+        for (int o = 32; o > 0; o >>= 1)
+            cnt += (uint32_t)__shfl_xor((int)cnt, o);
+        if ((threadIdx.x & 63) == 0)
+            wc[threadIdx.x >> 6] = cnt;
+        __syncthreads();
         if (threadIdx.x == 0)
-            chunk_left[blockIdx.x] = 0;
-        return;
+            chunk_left[ci] = wc[0] + wc[1] + wc[2] + wc[3];
+        __syncthreads();  // wc is read above before the next chunk writes it
     }
-    uint32_t cnt = 0;
-    for (uint32_t e = ch.begin + threadIdx.x; e < ch.end; e += kWG) {
-        float4 c = ec[e];
-        float v = d.axis == 0 ? c.x : (d.axis == 1 ? c.y : c.z);
-        cnt += (v < d.pos) ? 1u : 0u;
-    }
-    for (int o = 32; o > 0; o >>= 1)
-        cnt += (uint32_t)__shfl_xor((int)cnt, o);
-    if ((threadIdx.x & 63) == 0)
-        wc[threadIdx.x >> 6] = cnt;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        chunk_left[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
 }
 
 // Exclusive prefix of chunk left counts within each node; node totals.
-__global__ void k_part_prefix(const uint32_t* __restrict__ node_chunk_begin, uint32_t ntasks,
+__global__ void k_part_prefix(const uint32_t* __restrict__ node_chunk_begin, const Ctl* __restrict__ ctl,
                               uint32_t* __restrict__ chunk_left, Decision* __restrict__ dec)
 {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ntasks)
-        return;
-    uint32_t run = 0;
-    for (uint32_t c = node_chunk_begin[s]; c < node_chunk_begin[s + 1]; ++c) {
-        uint32_t v = chunk_left[c];
-        chunk_left[c] = run;
-        run += v;
+    uint32_t L = ctl->L;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < L; s += gridDim.x * blockDim.x) {
+        uint32_t run = 0;
+        for (uint32_t c = node_chunk_begin[s]; c < node_chunk_begin[s + 1]; ++c) {
+            uint32_t v = chunk_left[c];
+            chunk_left[c] = run;
+            run += v;
+        }
+        dec[s].nl = run;
     }
-    dec[s].nl = run;
 }
 
 // Two-pointer partition (bvh.rs:420-429) as a permutation.  With n elements
@@ -448,16 +504,13 @@ __global__ void k_part_prefix(const uint32_t* __restrict__ node_chunk_begin, uin
 // Phase A records, per node, kpos[m] (position of the m-th left-stream
 // right-goer) and sel[m] (top-index of the m-th right-stream left-goer).
 template <bool PHASE_B>
-__global__ __launch_bounds__(kWG) void k_part_scatter(const float4* __restrict__ ec, const float4* __restrict__ elo,
-                                                      const float2* __restrict__ ehi, const Chunk* __restrict__ chunks,
-                                                      const Task* __restrict__ tasks, const Decision* __restrict__ dec,
-                                                      const uint32_t* __restrict__ chunk_left,
-                                                      uint32_t* __restrict__ kpos, uint32_t* __restrict__ sel,
-                                                      float4* __restrict__ oc, float4* __restrict__ olo,
-                                                      float2* __restrict__ ohi, uint32_t* __restrict__ err)
+__device__ void part_scatter_chunk(const float4* __restrict__ ec, const float4* __restrict__ elo,
+                                   const float2* __restrict__ ehi, Chunk ch, uint32_t base,
+                                   const Task* __restrict__ tasks, const Decision* __restrict__ dec,
+                                   uint32_t* __restrict__ kpos, uint32_t* __restrict__ sel, float4* __restrict__ oc,
+                                   float4* __restrict__ olo, float2* __restrict__ ohi, uint32_t* __restrict__ err,
+                                   uint32_t* wc)
 {
-    __shared__ uint32_t wc[4];
-    Chunk ch = chunks[blockIdx.x];
     Decision d = dec[ch.slot];
     if (d.kind == 0)
         return;
@@ -470,7 +523,6 @@ __global__ __launch_bounds__(kWG) void k_part_scatter(const float4* __restrict__
         float v = d.axis == 0 ? c.x : (d.axis == 1 ? c.y : c.z);
         a = (v < d.pos) ? nl : nl + 1;
     }
-    uint32_t base = chunk_left[blockIdx.x];
     for (uint32_t e0 = ch.begin; e0 < ch.end; e0 += kWG) {
         uint32_t e = e0 + threadIdx.x;
         bool in = e < ch.end;
@@ -556,20 +608,28 @@ __global__ __launch_bounds__(kWG) void k_part_scatter(const float4* __restrict__
     }
 }
 
-// Copy the partitioned ranges back and reduce the two child boxes
-// (update_node_bounds of the children, bvh.rs:462-463).
-__global__ __launch_bounds__(kWG) void k_part_commit(const float4* __restrict__ oc, const float4* __restrict__ olo,
-                                                     const float2* __restrict__ ohi, const Chunk* __restrict__ chunks,
-                                                     const Task* __restrict__ tasks, const Decision* __restrict__ dec,
-                                                     float4* __restrict__ ec, float4* __restrict__ elo,
-                                                     float2* __restrict__ ehi, uint32_t* __restrict__ child_keys)
+template <bool PHASE_B>
+__global__ __launch_bounds__(kWG) void k_part_scatter(const float4* __restrict__ ec, const float4* __restrict__ elo,
+                                                      const float2* __restrict__ ehi, const Chunk* __restrict__ chunks,
+                                                      const Ctl* __restrict__ ctl, const Task* __restrict__ tasks,
+                                                      const Decision* __restrict__ dec,
+                                                      const uint32_t* __restrict__ chunk_left,
+                                                      uint32_t* __restrict__ kpos, uint32_t* __restrict__ sel,
+                                                      float4* __restrict__ oc, float4* __restrict__ olo,
+                                                      float2* __restrict__ ohi, uint32_t* __restrict__ err)
 {
-    __shared__ uint32_t lds[12 * 4];
-    Chunk ch = chunks[blockIdx.x];
-    Decision d = dec[ch.slot];
-    if (d.kind == 0)
-        return;
-    Task t = tasks[ch.slot];
+    __shared__ uint32_t wc[4];
+    uint32_t C = ctl->C;
+    for (uint32_t ci = blockIdx.x; ci < C; ci += gridDim.x)
+        part_scatter_chunk<PHASE_B>(ec, elo, ehi, chunks[ci], chunk_left[ci], tasks, dec, kpos, sel, oc, olo, ohi,
+                                    err, wc);
+}
+
+__device__ void part_commit_chunk(const float4* __restrict__ oc, const float4* __restrict__ olo,
+                                  const float2* __restrict__ ohi, Chunk ch, Task t, Decision d,
+                                  float4* __restrict__ ec, float4* __restrict__ elo, float2* __restrict__ ehi,
+                                  uint32_t* __restrict__ child_keys, uint32_t* lds)
+{
     uint32_t nl = d.kind == 1 ? d.nl : t.count / 2;
     // a plane split that leaves one side empty is abandoned (bvh.rs:431-435),
     // or replaced by the median split when the node must be split
@@ -599,6 +659,25 @@ __global__ __launch_bounds__(kWG) void k_part_commit(const float4* __restrict__ 
     block_reduce_keys<12>(v, child_keys + 12 * ch.slot, lds);
 }
 
+// Copy the partitioned ranges back and reduce the two child boxes
+// (update_node_bounds of the children, bvh.rs:462-463).
+__global__ __launch_bounds__(kWG) void k_part_commit(const float4* __restrict__ oc, const float4* __restrict__ olo,
+                                                     const float2* __restrict__ ohi, const Chunk* __restrict__ chunks,
+                                                     const Ctl* __restrict__ ctl, const Task* __restrict__ tasks,
+                                                     const Decision* __restrict__ dec, float4* __restrict__ ec,
+                                                     float4* __restrict__ elo, float2* __restrict__ ehi,
+                                                     uint32_t* __restrict__ child_keys)
+{
+    __shared__ uint32_t lds[12 * 4];
+    uint32_t C = ctl->C;
+    for (uint32_t ci = blockIdx.x; ci < C; ci += gridDim.x) {
+        Chunk ch = chunks[ci];
+        Decision d = dec[ch.slot];
+        if (d.kind != 0)
+            part_commit_chunk(oc, olo, ohi, ch, tasks[ch.slot], d, ec, elo, ehi, child_keys, lds);
+    }
+}
+
 struct SNode {  // same layout as bvht_node; internal nodes hold a local pair index
     float bmin[3];
     uint32_t lf;
@@ -611,16 +690,23 @@ __device__ __forceinline__ float comp4(float4 c, int axis) { return axis == 0 ? 
 // Sequential subdivide (bvh.rs:396-467) of one small subtree by one thread, on
 // the element range [first, first+count); nodes go to out[0 .. 1+2*pairs),
 // child pairs allocated in depth-first pre-order (pair p -> out[1+2p], out[2+2p]).
+// A subtree of count elements has at most count - 1 pairs, so its nodes fit
+// out = snodes + 2 * first, whose 2 * count entries no other subtree uses.
 __global__ void k_small_build(float4* __restrict__ ec, float4* __restrict__ elo, float2* __restrict__ ehi,
-                              const Task* __restrict__ roots, const uint32_t* __restrict__ out_base, uint32_t nroots,
+                              const BNode* __restrict__ bn, const uint32_t* __restrict__ small_roots, uint32_t nroots,
                               SNode* __restrict__ snodes, uint32_t* __restrict__ pairs_out, int faithful,
                               uint32_t max_leaf, uint32_t* __restrict__ err)
 {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nroots)
         return;
-    Task rt = roots[r];
-    SNode* out = snodes + out_base[r];
+    BNode rn = bn[small_roots[r]];
+    Task rt;
+    rt.first = rn.first;
+    rt.count = rn.count;
+    for (int k = 0; k < 6; ++k)
+        rt.box[k] = rn.box[k];
+    SNode* out = snodes + 2 * (size_t)rn.first;
     out[0] = SNode{{rt.box[0], rt.box[1], rt.box[2]}, rt.first, {rt.box[3], rt.box[4], rt.box[5]}, rt.count};
     uint32_t npairs = 0;
     uint32_t stack[kSmall + 2];
@@ -726,15 +812,15 @@ __global__ void k_small_build(float4* __restrict__ ec, float4* __restrict__ elo,
 
 // Place the small subtrees: local node 0 at root_final, local pair p at
 // 2 + 2*(rank + p).  Internal nodes point at their pair's final index.
-__global__ void k_small_emit(const SNode* __restrict__ snodes, const uint32_t* __restrict__ out_base,
-                             const uint32_t* __restrict__ pairs, const uint32_t* __restrict__ root_final,
-                             const uint32_t* __restrict__ rank, uint32_t nroots, bvht_node* __restrict__ nodes,
-                             uint32_t node_count, uint32_t* __restrict__ err)
+__global__ void k_small_emit(const SNode* __restrict__ snodes, const BNode* __restrict__ bn,
+                             const uint32_t* __restrict__ small_roots, const uint32_t* __restrict__ pairs,
+                             const uint32_t* __restrict__ root_final, const uint32_t* __restrict__ rank, uint32_t nroots,
+                             bvht_node* __restrict__ nodes, uint32_t node_count, uint32_t* __restrict__ err)
 {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nroots)
         return;
-    const SNode* s = snodes + out_base[r];
+    const SNode* s = snodes + 2 * (size_t)bn[small_roots[r]].first;
     uint32_t np = pairs[r], rk = rank[r];
     for (uint32_t j = 0; j < 1 + 2 * np; ++j) {
         SNode sn = s[j];
@@ -755,9 +841,10 @@ __global__ void k_small_emit(const SNode* __restrict__ snodes, const uint32_t* _
 // Structural check of the finished node array (the same rules the upload
 // path checks on the host): children after their parent and inside
 // nodes_used, leaf ranges inside the triangle array.
-__global__ void k_validate_tree(const bvht_node* __restrict__ nodes, uint32_t nodes_used, uint32_t n,
+__global__ void k_validate_tree(const bvht_node* __restrict__ nodes, const Ctl* __restrict__ ctl, uint32_t n,
                                 uint32_t* __restrict__ err)
 {
+    uint32_t nodes_used = ctl->nodes_used;
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nodes_used || k == 1)
         return;
@@ -768,14 +855,6 @@ __global__ void k_validate_tree(const bvht_node* __restrict__ nodes, uint32_t no
     } else if (nd.left_first <= k || nd.left_first == 1 || nd.left_first + 1 >= nodes_used) {
         atomicOr(err, 64u);
     }
-}
-
-__global__ void k_scatter_nodes(const bvht_node* __restrict__ src, const uint32_t* __restrict__ idx, uint32_t m,
-                                bvht_node* __restrict__ nodes)
-{
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < m)
-        nodes[idx[k]] = src[k];
 }
 
 // Build order triangle array: tris_out[k] = tris_in[index of element k];
@@ -799,33 +878,340 @@ __global__ void k_gather_tris(const float* __restrict__ tin, const float4* __res
         b[q] = a[q];
 }
 
+// ------------------------------------------------- device-driven levels --
+//
+// The level loop needs no host round trip: one workgroup turns the level's
+// node list into tasks and chunks (k_level_begin), the chunk and task
+// kernels run a fixed grid over the counts in Ctl, and one workgroup turns
+// the decisions into the next level's list, the small roots and their
+// BNodes (k_level_end).  A level with no node makes every kernel return at
+// once, so the host enqueues several levels between read-backs.
+
+// Exclusive prefix of v over a kCtlWG-thread workgroup in thread order;
+// `total` gets the sum.  lds holds kCtlWG / 64 + 1 words.
+__device__ uint32_t block_scan_excl(uint32_t v, uint32_t* lds, uint32_t& total)
+{
+    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        uint32_t t = __shfl_up(incl, off, 64);
+        if (lane >= off)
+            incl += t;
+    }
+    if (lane == 63u)
+        lds[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int w = 0; w < kCtlWG / 64; ++w) {
+            uint32_t x = lds[w];
+            lds[w] = run;
+            run += x;
+        }
+        lds[kCtlWG / 64] = run;
+    }
+    __syncthreads();
+    uint32_t r = lds[wave] + incl - v;
+    total = lds[kCtlWG / 64];
+    __syncthreads();
+    return r;
+}
+
+// keys {min xyz} and {max xyz} -> box floats; empty ranges give (+MAX, -MAX)
+__device__ __forceinline__ void keys_to_box_dev(const uint32_t* kmin, const uint32_t* kmax, float* box)
+{
+    for (int a = 0; a < 3; ++a) {
+        box[a] = kmin[a] == kEmptyMin ? kFltMax : key2f(kmin[a]);
+        box[3 + a] = kmax[a] == kEmptyMax ? -kFltMax : key2f(kmax[a]);
+    }
+}
+
+// BNode 0 = the root over all n elements (update_node_bounds, bvh.rs:537);
+// a root of more than kSmall elements starts the level list, else it is the
+// only small root.
+__global__ void k_root_init(const uint32_t* __restrict__ rootk, uint32_t n, BNode* __restrict__ bn,
+                            uint32_t* __restrict__ lvl_all, uint32_t* __restrict__ small_roots,
+                            uint32_t* __restrict__ level_start, Ctl* __restrict__ ctl)
+{
+    BNode r;
+    r.first = 0;
+    r.count = n;
+    keys_to_box_dev(rootk, rootk + 3, r.box);
+    r.left = kNone;
+    Ctl c;
+    c.C = 0;
+    c.H = 1;
+    c.lvl_off = 0;
+    c.nlevels = 0;
+    c.err = 0;
+    c.nodes_used = 0;
+    if (n > kSmall) {
+        r.small = kNone;
+        lvl_all[0] = 0;
+        c.L = 1;
+        c.R = 0;
+    } else {
+        r.small = 0;
+        small_roots[0] = 0;
+        c.L = 0;
+        c.R = 1;
+    }
+    bn[0] = r;
+    level_start[0] = 0;
+    *ctl = c;
+}
+
+// Tasks of the level (node range and box) and their chunks of at most kChunk
+// elements, in node order; node_chunk[s] = the first chunk of node s.
+__global__ __launch_bounds__(kCtlWG) void k_level_begin(const BNode* __restrict__ bn,
+                                                        const uint32_t* __restrict__ lvl_all, Ctl* __restrict__ ctl,
+                                                        Task* __restrict__ tasks, uint32_t* __restrict__ node_chunk,
+                                                        Chunk* __restrict__ chunks, uint32_t lcap, uint32_t ccap)
+{
+    __shared__ uint32_t lds[kCtlWG / 64 + 1];
+    uint32_t L = ctl->L;
+    if (L == 0 || L > lcap) {  // more than lcap cannot happen: the nodes' ranges are disjoint
+        if (threadIdx.x == 0) {
+            if (L)
+                atomicOr(&ctl->err, 256u);
+            ctl->C = 0;
+            ctl->L = 0;
+        }
+        return;
+    }
+    const uint32_t* lvl = lvl_all + ctl->lvl_off;
+    uint32_t per = (L + kCtlWG - 1) / kCtlWG;
+    uint32_t s0 = min(L, threadIdx.x * per), s1 = min(L, s0 + per);
+    uint32_t local = 0;
+    for (uint32_t s = s0; s < s1; ++s) {
+        BNode nd = bn[lvl[s]];
+        Task t;
+        t.first = nd.first;
+        t.count = nd.count;
+        for (int k = 0; k < 6; ++k)
+            t.box[k] = nd.box[k];
+        tasks[s] = t;
+        local += (nd.count + kChunk - 1) / kChunk;
+    }
+    uint32_t total;
+    uint32_t run = block_scan_excl(local, lds, total);
+    for (uint32_t s = s0; s < s1; ++s) {
+        node_chunk[s] = run;
+        Task t = tasks[s];
+        uint32_t end = t.first + t.count;
+        for (uint32_t b = t.first; b < end; b += kChunk) {
+            if (run < ccap)
+                chunks[run] = Chunk{s, b, min(b + kChunk, end), 0};
+            ++run;
+        }
+    }
+    if (threadIdx.x == 0) {
+        node_chunk[L] = total;
+        if (total > ccap) {  // cannot happen: chunks <= n / kChunk + L
+            atomicOr(&ctl->err, 256u);
+            ctl->C = 0;
+            ctl->L = 0;
+        } else {
+            ctl->C = total;
+        }
+    }
+}
+
+// Whether node s of the level splits, and its left count (the host-side rule
+// of the reference's subdivide, bvh.rs:431-437, with the SAH mode's forced
+// median split).
+__device__ __forceinline__ bool node_splits(const Decision& d, uint32_t cnt, int faithful, uint32_t max_leaf,
+                                            uint32_t& nl)
+{
+    if (d.kind == 0)
+        return false;
+    nl = d.kind == 1 ? d.nl : cnt / 2;
+    if (d.kind == 1 && (nl == 0 || nl == cnt)) {
+        bool force = !faithful && max_leaf > 0 && cnt > max_leaf;
+        if (!force)
+            return false;  // abandoned split: a leaf (its range stays partitioned)
+        nl = cnt / 2;
+    }
+    return true;
+}
+
+// Children of the level's split nodes: a BNode pair each (in node order),
+// the large ones listed as the next level, the small ones as small roots.
+__global__ __launch_bounds__(kCtlWG) void k_level_end(BNode* __restrict__ bn, uint32_t* __restrict__ lvl_all,
+                                                      uint32_t* __restrict__ small_roots,
+                                                      uint32_t* __restrict__ level_start,
+                                                      const Task* __restrict__ tasks,
+                                                      const Decision* __restrict__ dec,
+                                                      const uint32_t* __restrict__ child_keys, Ctl* __restrict__ ctl,
+                                                      int faithful, uint32_t max_leaf, uint32_t hcap, uint32_t rcap)
+{
+    __shared__ uint32_t lds[kCtlWG / 64 + 1];
+    uint32_t L = ctl->L;
+    if (L == 0)
+        return;
+    uint32_t H0 = ctl->H, R0 = ctl->R, off = ctl->lvl_off, nlev = ctl->nlevels;
+    const uint32_t* lvl = lvl_all + off;
+    uint32_t per = (L + kCtlWG - 1) / kCtlWG;
+    uint32_t s0 = min(L, threadIdx.x * per), s1 = min(L, s0 + per);
+    uint32_t nsplit = 0, nlarge = 0, nsmall = 0;
+    for (uint32_t s = s0; s < s1; ++s) {
+        uint32_t cnt = tasks[s].count, nl;
+        if (node_splits(dec[s], cnt, faithful, max_leaf, nl)) {
+            uint32_t big = (nl > kSmall ? 1u : 0u) + (cnt - nl > kSmall ? 1u : 0u);
+            ++nsplit;
+            nlarge += big;
+            nsmall += 2 - big;
+        }
+    }
+    uint32_t tsplit, tlarge, tsmall;
+    uint32_t esplit = block_scan_excl(nsplit, lds, tsplit);
+    uint32_t elarge = block_scan_excl(nlarge, lds, tlarge);
+    uint32_t esmall = block_scan_excl(nsmall, lds, tsmall);
+    bool fits = (uint64_t)H0 + 2 * (uint64_t)tsplit <= hcap && (uint64_t)off + L + tlarge <= hcap &&
+                (uint64_t)R0 + tsmall <= rcap;
+    if (!fits) {  // cannot happen: the tree has fewer than 2n nodes, n elements
+        if (threadIdx.x == 0) {
+            atomicOr(&ctl->err, 128u);
+            ctl->L = 0;
+        }
+        return;
+    }
+    uint32_t child = H0 + 2 * esplit, lpos = off + L + elarge, rpos = R0 + esmall;
+    for (uint32_t s = s0; s < s1; ++s) {
+        Task t = tasks[s];
+        uint32_t nl;
+        if (!node_splits(dec[s], t.count, faithful, max_leaf, nl))
+            continue;
+        bn[lvl[s]].left = child;
+        const uint32_t* ck = child_keys + 12 * (size_t)s;  // l.min, r.min | l.max, r.max
+        for (uint32_t side = 0; side < 2; ++side) {
+            BNode c;
+            c.first = side ? t.first + nl : t.first;
+            c.count = side ? t.count - nl : nl;
+            keys_to_box_dev(ck + 3 * side, ck + 6 + 3 * side, c.box);
+            c.left = kNone;
+            if (c.count > kSmall) {
+                c.small = kNone;
+                lvl_all[lpos++] = child + side;
+            } else {
+                c.small = rpos;
+                small_roots[rpos++] = child + side;
+            }
+            bn[child + side] = c;
+        }
+        child += 2;
+    }
+    if (threadIdx.x == 0) {
+        ctl->H = H0 + 2 * tsplit;
+        ctl->R = R0 + tsmall;
+        ctl->lvl_off = off + L;
+        ctl->L = tlarge;
+        ctl->nlevels = nlev + 1;
+        level_start[nlev + 1] = off + L;
+    }
+}
+
+// Node numbering of the reference's depth-first pair allocation
+// (bvh.rs:440-466): pass 1, deepest level first, counts each large node's
+// internal nodes (a small root's are its pairs); pass 2, from the root,
+// gives each internal node its pre-order rank r (children at 2 + 2r and
+// 3 + 2r) and each child the rank its subtree starts at, writes the large
+// nodes and hands the small roots their index and starting rank.
+__global__ __launch_bounds__(kCtlWG) void k_number(const BNode* __restrict__ bn, const uint32_t* __restrict__ lvl_all,
+                                                   const uint32_t* __restrict__ level_start, Ctl* __restrict__ ctl,
+                                                   const uint32_t* __restrict__ pairs, uint32_t* __restrict__ ic,
+                                                   uint32_t* __restrict__ start, uint32_t* __restrict__ fin,
+                                                   uint32_t* __restrict__ rfinal, uint32_t* __restrict__ rrank,
+                                                   bvht_node* __restrict__ nodes, uint32_t node_count)
+{
+    uint32_t D = ctl->nlevels;
+    if (bn[0].small != kNone) {  // n <= kSmall: the root is the one small subtree
+        if (threadIdx.x == 0) {
+            rfinal[0] = 0;
+            rrank[0] = 0;
+            uint32_t total = pairs[0];
+            if ((uint64_t)2 + 2 * (uint64_t)total > node_count)
+                atomicOr(&ctl->err, 512u);
+            else
+                ctl->nodes_used = 2 + 2 * total;
+        }
+        return;
+    }
+    for (uint32_t l = D; l-- > 0;) {
+        uint32_t a = level_start[l], b = level_start[l + 1];
+        for (uint32_t s = a + threadIdx.x; s < b; s += kCtlWG) {
+            uint32_t h = lvl_all[s];
+            BNode nd = bn[h];
+            uint32_t v = 0;
+            if (nd.left != kNone) {
+                uint32_t c0 = nd.left, c1 = nd.left + 1;
+                uint32_t m0 = bn[c0].small, m1 = bn[c1].small;
+                v = 1 + (m0 != kNone ? pairs[m0] : ic[c0]) + (m1 != kNone ? pairs[m1] : ic[c1]);
+            }
+            ic[h] = v;
+        }
+        __syncthreads();
+    }
+    uint32_t total = ic[0];
+    if ((uint64_t)2 + 2 * (uint64_t)total > node_count) {
+        if (threadIdx.x == 0)
+            atomicOr(&ctl->err, 512u);
+        return;
+    }
+    if (threadIdx.x == 0) {
+        start[0] = 0;
+        fin[0] = 0;
+    }
+    __syncthreads();
+    for (uint32_t l = 0; l < D; ++l) {
+        uint32_t a = level_start[l], b = level_start[l + 1];
+        for (uint32_t s = a + threadIdx.x; s < b; s += kCtlWG) {
+            uint32_t h = lvl_all[s];
+            BNode nd = bn[h];
+            uint32_t f = fin[h], r = start[h];
+            bvht_node o;
+            o.bmin[0] = nd.box[0];
+            o.bmin[1] = nd.box[1];
+            o.bmin[2] = nd.box[2];
+            o.bmax[0] = nd.box[3];
+            o.bmax[1] = nd.box[4];
+            o.bmax[2] = nd.box[5];
+            if (nd.left != kNone) {
+                o.left_first = 2 + 2 * r;
+                o.count = 0;
+                uint32_t c0 = nd.left, c1 = nd.left + 1;
+                uint32_t m0 = bn[c0].small, m1 = bn[c1].small;
+                uint32_t r0 = r + 1, r1 = r + 1 + (m0 != kNone ? pairs[m0] : ic[c0]);
+                fin[c0] = 2 + 2 * r;
+                fin[c1] = 3 + 2 * r;
+                start[c0] = r0;
+                start[c1] = r1;
+                if (m0 != kNone) {
+                    rfinal[m0] = 2 + 2 * r;
+                    rrank[m0] = r0;
+                }
+                if (m1 != kNone) {
+                    rfinal[m1] = 3 + 2 * r;
+                    rrank[m1] = r1;
+                }
+            } else {
+                o.left_first = nd.first;
+                o.count = nd.count;
+            }
+            if (f < node_count)
+                nodes[f] = o;
+            else
+                atomicOr(&ctl->err, 512u);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        ctl->nodes_used = 2 + 2 * total;
+}
+
 // ------------------------------------------------------------ host side --
 
-struct HostNode {
-    uint32_t first, count;
-    float box[6];
-    int32_t left = -1;        // host index of the left child (right = left + 1) for large internal nodes
-    int32_t small = -1;       // small-root slot
-};
-
 inline uint32_t nblk(size_t n, uint32_t wg) { return (uint32_t)((n + wg - 1) / wg); }
-
-struct DevBuf {
-    void* p = nullptr;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    int32_t alloc(size_t bytes)
-    {
-        if (p) {
-            (void)hipFree(p);
-            p = nullptr;
-        }
-        if (hipMalloc(&p, bytes ? bytes : 4) != hipSuccess)
-            return set_error(BVHT_ENOMEM, "hipMalloc(%zu) in the BLAS build failed", bytes);
-        return BVHT_OK;
-    }
-    template <class T>
-    T* as() const { return static_cast<T*>(p); }
-};
 
 int32_t fill(hipStream_t st, uint32_t* p, uint32_t v, size_t n)
 {
@@ -836,23 +1222,6 @@ int32_t fill(hipStream_t st, uint32_t* p, uint32_t v, size_t n)
     return BVHT_OK;
 }
 
-float key2f_host(uint32_t k)
-{
-    uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
-    float f;
-    std::memcpy(&f, &u, 4);
-    return f;
-}
-
-// keys {min xyz, max xyz} -> box floats; empty ranges give (+MAX, -MAX)
-void keys_to_box(const uint32_t* k, float* box)
-{
-    for (int a = 0; a < 3; ++a) {
-        box[a] = k[a] == kEmptyMin ? 3.402823466e+38f : key2f_host(k[a]);
-        box[3 + a] = k[3 + a] == kEmptyMax ? -3.402823466e+38f : key2f_host(k[3 + a]);
-    }
-}
-
 #define RC(x)                    \
     do {                         \
         int32_t _rc = (x);       \
@@ -860,292 +1229,217 @@ void keys_to_box(const uint32_t* k, float* box)
             return _rc;          \
     } while (0)
 
+// Device workspace of the build, owned by the context and kept between
+// builds (one allocation, grown when a larger mesh comes): a build then
+// allocates nothing but the BLAS itself.  Capacities for n_cap elements:
+//   per element   input triangles, element records (two copies), partition
+//                 positions, small-subtree nodes (2 per element), small roots
+//   per BNode     2 n_cap + 1 (the tree has fewer than 2n nodes): the nodes,
+//                 the level list, internal counts, ranks, final indices,
+//                 level starts
+//   per task      n_cap / 33 + 1 (a level's large nodes have disjoint ranges
+//                 of more than 32 elements)
+//   per chunk     n_cap / kChunk + tasks + 2
+struct BuildWs {
+    void* base = nullptr;
+    size_t n_cap = 0;
+    uint32_t hcap = 0, rcap = 0, lcap = 0, ccap = 0;
+    float* tin;
+    float4 *ec, *elo, *oc, *olo;
+    float2 *ehi, *ohi;
+    uint32_t *kpos, *sel;
+    SNode* snodes;
+    uint32_t *small_roots, *pairs, *rfinal, *rrank;
+    BNode* bn;
+    uint32_t *lvl_all, *ic, *start, *fin, *level_start;
+    Task* tasks;
+    uint32_t *cacc, *bacc, *child, *node_chunk;
+    Decision* dec;
+    Chunk* chunks;
+    uint32_t* chunk_left;
+    Ctl* ctl;
+    uint32_t* rootk;
+};
+
+struct Carver {
+    char* base;
+    size_t off = 0;
+    template <class T>
+    void operator()(T*& out, size_t count)
+    {
+        if (base)
+            out = reinterpret_cast<T*>(base + off);
+        off += (sizeof(T) * count + 255) & ~(size_t)255;
+    }
+};
+
+// Lays the buffers out from `base` (or, with base NULL, only adds up the
+// size); returns the bytes used.
+size_t layout(BuildWs& w, char* base)
+{
+    Carver carve{base};
+    size_t n = w.n_cap;
+    carve(w.tin, 9 * n);
+    carve(w.ec, n);
+    carve(w.elo, n);
+    carve(w.ehi, n);
+    carve(w.oc, n);
+    carve(w.olo, n);
+    carve(w.ohi, n);
+    carve(w.kpos, n);
+    carve(w.sel, n);
+    carve(w.snodes, 2 * n);
+    carve(w.small_roots, w.rcap);
+    carve(w.pairs, w.rcap);
+    carve(w.rfinal, w.rcap);
+    carve(w.rrank, w.rcap);
+    carve(w.bn, w.hcap);
+    carve(w.lvl_all, w.hcap);
+    carve(w.ic, w.hcap);
+    carve(w.start, w.hcap);
+    carve(w.fin, w.hcap);
+    carve(w.level_start, (size_t)w.hcap + 1);
+    carve(w.tasks, w.lcap);
+    carve(w.cacc, 6 * (size_t)w.lcap);
+    carve(w.bacc, (size_t)3 * kBins * 7 * w.lcap);
+    carve(w.child, 12 * (size_t)w.lcap);
+    carve(w.node_chunk, (size_t)w.lcap + 1);
+    carve(w.dec, w.lcap);
+    carve(w.chunks, w.ccap);
+    carve(w.chunk_left, w.ccap);
+    carve(w.ctl, 1);
+    carve(w.rootk, 6);
+    return carve.off;
+}
+
+int32_t ensure_ws(bvht_ctx ctx, uint32_t n, BuildWs** out)
+{
+    BuildWs* w = static_cast<BuildWs*>(ctx->build_ws);
+    if (!w) {
+        w = new (std::nothrow) BuildWs();
+        if (!w)
+            return set_error(BVHT_ENOMEM, "host allocation failed");
+        ctx->build_ws = w;
+    }
+    if (w->n_cap < n) {
+        BuildWs g;
+        g.n_cap = std::max<size_t>(n, w->n_cap + w->n_cap / 2);
+        g.n_cap = std::min<size_t>(g.n_cap, (size_t)1 << 30);
+        BVHT_HIP(hipStreamSynchronize(ctx->stream));
+        if (w->base)
+            (void)hipFree(w->base);
+        w->base = nullptr;
+        w->n_cap = 0;
+        g.hcap = (uint32_t)(2 * g.n_cap + 1);
+        g.rcap = (uint32_t)g.n_cap;
+        g.lcap = (uint32_t)(g.n_cap / (kSmall + 1) + 1);
+        g.ccap = (uint32_t)(g.n_cap / kChunk + g.lcap + 2);
+        size_t bytes = layout(g, nullptr);
+        if (hipMalloc(&g.base, bytes) != hipSuccess)
+            return set_error(BVHT_ENOMEM, "hipMalloc(%zu) of the BLAS build workspace failed", bytes);
+        layout(g, static_cast<char*>(g.base));
+        *w = g;
+    }
+    *out = w;
+    return BVHT_OK;
+}
+
+constexpr uint32_t kChunkGrid = 2048;  // workgroups of the chunk kernels
+constexpr uint32_t kTaskGrid = 512;    // 64-thread workgroups of the per-node kernels
+
+void enqueue_level(hipStream_t st, BuildWs& w, int faithful, uint32_t max_leaf)
+{
+    Ctl* ctl = w.ctl;
+    uint32_t* err = &w.ctl->err;
+    uint32_t cg = std::min<uint32_t>(kChunkGrid, w.ccap);
+    hipLaunchKernelGGL(k_level_begin, dim3(1), dim3(kCtlWG), 0, st, w.bn, w.lvl_all, ctl, w.tasks, w.node_chunk,
+                       w.chunks, w.lcap, w.ccap);
+    hipLaunchKernelGGL(k_init_acc, dim3(1024), dim3(256), 0, st, ctl, w.cacc, w.bacc, w.child);
+    hipLaunchKernelGGL(k_centroid_range, dim3(cg), dim3(kWG), 0, st, w.ec, w.chunks, ctl, w.cacc);
+    hipLaunchKernelGGL(k_bins, dim3(cg), dim3(kWG), 0, st, w.ec, w.elo, w.ehi, w.chunks, ctl, w.cacc, w.bacc,
+                       faithful);
+    hipLaunchKernelGGL(k_decide, dim3(kTaskGrid), dim3(64), 0, st, w.tasks, ctl, w.cacc, w.bacc, w.dec, faithful,
+                       max_leaf);
+    hipLaunchKernelGGL(k_part_count, dim3(cg), dim3(kWG), 0, st, w.ec, w.chunks, ctl, w.dec, w.chunk_left);
+    hipLaunchKernelGGL(k_part_prefix, dim3(kTaskGrid), dim3(64), 0, st, w.node_chunk, ctl, w.chunk_left, w.dec);
+    hipLaunchKernelGGL((k_part_scatter<false>), dim3(cg), dim3(kWG), 0, st, w.ec, w.elo, w.ehi, w.chunks, ctl,
+                       w.tasks, w.dec, w.chunk_left, w.kpos, w.sel, w.oc, w.olo, w.ohi, err);
+    hipLaunchKernelGGL((k_part_scatter<true>), dim3(cg), dim3(kWG), 0, st, w.ec, w.elo, w.ehi, w.chunks, ctl,
+                       w.tasks, w.dec, w.chunk_left, w.kpos, w.sel, w.oc, w.olo, w.ohi, err);
+    hipLaunchKernelGGL(k_part_commit, dim3(cg), dim3(kWG), 0, st, w.oc, w.olo, w.ohi, w.chunks, ctl, w.tasks, w.dec,
+                       w.ec, w.elo, w.ehi, w.child);
+    hipLaunchKernelGGL(k_level_end, dim3(1), dim3(kCtlWG), 0, st, w.bn, w.lvl_all, w.small_roots, w.level_start,
+                       w.tasks, w.dec, w.child, ctl, faithful, max_leaf, w.hcap, w.rcap);
+}
+
 int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32_t max_leaf, bvht_blas* out)
 {
     hipStream_t st = ctx->stream;
+    BuildWs* wp = nullptr;
+    RC(ensure_ws(ctx, n, &wp));
+    BuildWs& w = *wp;
     bvht_blas b = nullptr;
     RC(bvht::blas_alloc(ctx, n, 2 * n, &b));
     struct Guard {
         bvht_blas* b;
         ~Guard() { if (*b) bvht_blas_destroy(*b); }
     } guard{&b};
-
-    DevBuf tin, ec, elo, ehi, oc, olo, ohi, kpos, sel, d_err;
-    RC(d_err.alloc(sizeof(uint32_t)));
-    RC(fill(st, d_err.as<uint32_t>(), 0u, 1));
-    RC(tin.alloc(sizeof(float) * 9 * (size_t)n));
-    RC(ec.alloc(sizeof(float4) * (size_t)n));
-    RC(elo.alloc(sizeof(float4) * (size_t)n));
-    RC(ehi.alloc(sizeof(float2) * (size_t)n));
-    RC(oc.alloc(sizeof(float4) * (size_t)n));
-    RC(olo.alloc(sizeof(float4) * (size_t)n));
-    RC(ohi.alloc(sizeof(float2) * (size_t)n));
-    RC(kpos.alloc(sizeof(uint32_t) * (size_t)n));
-    RC(sel.alloc(sizeof(uint32_t) * (size_t)n));
-    BVHT_HIP(hipMemcpyAsync(tin.p, h_tris, sizeof(float) * 9 * (size_t)n, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_init_elems, dim3(nblk(n, 256)), dim3(256), 0, st, tin.as<float>(), n, ec.as<float4>(),
-                       elo.as<float4>(), ehi.as<float2>());
-    BVHT_HIP(hipGetLastError());
-
-    // root box: update_node_bounds over everything (bvh.rs:537)
-    DevBuf rootk;
-    RC(rootk.alloc(6 * sizeof(uint32_t)));
-    RC(fill(st, rootk.as<uint32_t>(), kEmptyMin, 3));
-    RC(fill(st, rootk.as<uint32_t>() + 3, kEmptyMax, 3));
-    hipLaunchKernelGGL(k_range_box, dim3(std::min<uint32_t>(nblk(n, kWG), 1024)), dim3(kWG), 0, st, elo.as<float4>(),
-                       ehi.as<float2>(), 0u, n, rootk.as<uint32_t>());
-    BVHT_HIP(hipGetLastError());
-    uint32_t rk[6];
-    BVHT_HIP(hipMemcpyAsync(rk, rootk.p, sizeof rk, hipMemcpyDeviceToHost, st));
-    BVHT_HIP(hipStreamSynchronize(st));
-
-    std::vector<HostNode> tree;
-    tree.reserve(1024);
-    HostNode root;
-    root.first = 0;
-    root.count = n;
-    keys_to_box(rk, root.box);
-    tree.push_back(root);
-    std::vector<uint32_t> level, small_roots;
-    if (n > kSmall)
-        level.push_back(0);
-    else
-        small_roots.push_back(0);
-
-    DevBuf d_tasks, d_chunks, d_cacc, d_bacc, d_dec, d_chunk_left, d_node_chunk, d_child;
-    size_t cap_tasks = 0, cap_chunks = 0;
-    std::vector<Task> tasks;
-    std::vector<Chunk> chunks;
-    std::vector<uint32_t> node_chunk;
-    std::vector<Decision> dec;
-    std::vector<uint32_t> childk;
-    while (!level.empty()) {
-        uint32_t L = (uint32_t)level.size();
-        tasks.resize(L);
-        chunks.clear();
-        node_chunk.assign(L + 1, 0);
-        for (uint32_t s = 0; s < L; ++s) {
-            const HostNode& hn = tree[level[s]];
-            tasks[s].first = hn.first;
-            tasks[s].count = hn.count;
-            std::memcpy(tasks[s].box, hn.box, sizeof hn.box);
-            node_chunk[s] = (uint32_t)chunks.size();
-            for (uint32_t bgn = hn.first; bgn < hn.first + hn.count; bgn += kChunk)
-                chunks.push_back(Chunk{s, bgn, std::min(bgn + kChunk, hn.first + hn.count), 0});
-        }
-        node_chunk[L] = (uint32_t)chunks.size();
-        uint32_t C = (uint32_t)chunks.size();
-        if (L > cap_tasks) {
-            cap_tasks = std::max<size_t>(L, 2 * cap_tasks);
-            RC(d_tasks.alloc(sizeof(Task) * cap_tasks));
-            RC(d_cacc.alloc(sizeof(uint32_t) * 6 * cap_tasks));
-            RC(d_bacc.alloc(sizeof(uint32_t) * 3 * kBins * 7 * cap_tasks));
-            RC(d_dec.alloc(sizeof(Decision) * cap_tasks));
-            RC(d_node_chunk.alloc(sizeof(uint32_t) * (cap_tasks + 1)));
-            RC(d_child.alloc(sizeof(uint32_t) * 12 * cap_tasks));
-        }
-        if (C > cap_chunks) {
-            cap_chunks = std::max<size_t>(C, 2 * cap_chunks);
-            RC(d_chunks.alloc(sizeof(Chunk) * cap_chunks));
-            RC(d_chunk_left.alloc(sizeof(uint32_t) * cap_chunks));
-        }
-        BVHT_HIP(hipMemcpyAsync(d_tasks.p, tasks.data(), sizeof(Task) * L, hipMemcpyHostToDevice, st));
-        BVHT_HIP(hipMemcpyAsync(d_chunks.p, chunks.data(), sizeof(Chunk) * C, hipMemcpyHostToDevice, st));
-        BVHT_HIP(hipMemcpyAsync(d_node_chunk.p, node_chunk.data(), sizeof(uint32_t) * (L + 1), hipMemcpyHostToDevice,
-                                st));
-        // accumulators: centroid ranges {min xyz | max xyz}, bins {count, min xyz, max xyz},
-        // child boxes {l.min, r.min | l.max, r.max}
-        hipLaunchKernelGGL(k_init_acc, dim3(nblk((size_t)3 * kBins * 7 * L, 256)), dim3(256), 0, st,
-                           d_cacc.as<uint32_t>(), d_bacc.as<uint32_t>(), d_child.as<uint32_t>(), L);
-        hipLaunchKernelGGL(k_centroid_range, dim3(C), dim3(kWG), 0, st, ec.as<float4>(), d_chunks.as<Chunk>(),
-                           d_cacc.as<uint32_t>());
-        hipLaunchKernelGGL(k_bins, dim3(C), dim3(kWG), 0, st, ec.as<float4>(), elo.as<float4>(), ehi.as<float2>(),
-                           d_chunks.as<Chunk>(), d_cacc.as<uint32_t>(), d_bacc.as<uint32_t>(), faithful);
-        hipLaunchKernelGGL(k_decide, dim3(nblk(L, 64)), dim3(64), 0, st, d_tasks.as<Task>(), L, d_cacc.as<uint32_t>(),
-                           d_bacc.as<uint32_t>(), d_dec.as<Decision>(), faithful, max_leaf);
-        hipLaunchKernelGGL(k_part_count, dim3(C), dim3(kWG), 0, st, ec.as<float4>(), d_chunks.as<Chunk>(),
-                           d_dec.as<Decision>(), d_chunk_left.as<uint32_t>());
-        hipLaunchKernelGGL(k_part_prefix, dim3(nblk(L, 64)), dim3(64), 0, st, d_node_chunk.as<uint32_t>(), L,
-                           d_chunk_left.as<uint32_t>(), d_dec.as<Decision>());
-        hipLaunchKernelGGL((k_part_scatter<false>), dim3(C), dim3(kWG), 0, st, ec.as<float4>(), elo.as<float4>(),
-                           ehi.as<float2>(), d_chunks.as<Chunk>(), d_tasks.as<Task>(), d_dec.as<Decision>(),
-                           d_chunk_left.as<uint32_t>(), kpos.as<uint32_t>(), sel.as<uint32_t>(), oc.as<float4>(),
-                           olo.as<float4>(), ohi.as<float2>(), d_err.as<uint32_t>());
-        hipLaunchKernelGGL((k_part_scatter<true>), dim3(C), dim3(kWG), 0, st, ec.as<float4>(), elo.as<float4>(),
-                           ehi.as<float2>(), d_chunks.as<Chunk>(), d_tasks.as<Task>(), d_dec.as<Decision>(),
-                           d_chunk_left.as<uint32_t>(), kpos.as<uint32_t>(), sel.as<uint32_t>(), oc.as<float4>(),
-                           olo.as<float4>(), ohi.as<float2>(), d_err.as<uint32_t>());
-        hipLaunchKernelGGL(k_part_commit, dim3(C), dim3(kWG), 0, st, oc.as<float4>(), olo.as<float4>(),
-                           ohi.as<float2>(), d_chunks.as<Chunk>(), d_tasks.as<Task>(), d_dec.as<Decision>(),
-                           ec.as<float4>(), elo.as<float4>(), ehi.as<float2>(), d_child.as<uint32_t>());
-        BVHT_HIP(hipGetLastError());
-        dec.resize(L);
-        childk.resize((size_t)12 * L);
-        BVHT_HIP(hipMemcpyAsync(dec.data(), d_dec.p, sizeof(Decision) * L, hipMemcpyDeviceToHost, st));
-        BVHT_HIP(hipMemcpyAsync(childk.data(), d_child.p, sizeof(uint32_t) * 12 * L, hipMemcpyDeviceToHost, st));
-        BVHT_HIP(hipStreamSynchronize(st));
-
-        std::vector<uint32_t> next;
-        for (uint32_t s = 0; s < L; ++s) {
-            const Decision& d = dec[s];
-            uint32_t hid = level[s];
-            uint32_t cnt = tree[hid].count, first = tree[hid].first;
-            if (d.kind == 0)
-                continue;  // leaf
-            uint32_t nl = d.kind == 1 ? d.nl : cnt / 2;
-            if (d.kind == 1 && (nl == 0 || nl == cnt)) {
-                bool force = !faithful && max_leaf > 0 && cnt > max_leaf;
-                if (!force)
-                    continue;  // abandoned split: stays a leaf, partition already applied
-                nl = cnt / 2;
-            }
-            HostNode l, r;
-            l.first = first;
-            l.count = nl;
-            r.first = first + nl;
-            r.count = cnt - nl;
-            uint32_t kk[6];
-            const uint32_t* ck = &childk[(size_t)12 * s];
-            kk[0] = ck[0]; kk[1] = ck[1]; kk[2] = ck[2]; kk[3] = ck[6]; kk[4] = ck[7]; kk[5] = ck[8];
-            keys_to_box(kk, l.box);
-            kk[0] = ck[3]; kk[1] = ck[4]; kk[2] = ck[5]; kk[3] = ck[9]; kk[4] = ck[10]; kk[5] = ck[11];
-            keys_to_box(kk, r.box);
-            int32_t li = (int32_t)tree.size();
-            tree[hid].left = li;
-            tree.push_back(l);
-            tree.push_back(r);
-            for (int32_t c = li; c <= li + 1; ++c) {
-                if (tree[c].count > kSmall)
-                    next.push_back((uint32_t)c);
-                else
-                    small_roots.push_back((uint32_t)c);
-            }
-        }
-        level.swap(next);
-    }
-
-    // small subtrees
-    uint32_t R = (uint32_t)small_roots.size();
-    std::vector<Task> rtasks(R);
-    std::vector<uint32_t> rbase(R);
-    size_t total_local = 0;
-    for (uint32_t r = 0; r < R; ++r) {
-        const HostNode& hn = tree[small_roots[r]];
-        tree[small_roots[r]].small = (int32_t)r;
-        rtasks[r].first = hn.first;
-        rtasks[r].count = hn.count;
-        std::memcpy(rtasks[r].box, hn.box, sizeof hn.box);
-        rbase[r] = (uint32_t)total_local;
-        total_local += 2 * (size_t)hn.count;
-    }
-    DevBuf d_rt, d_rbase, d_snodes, d_pairs, d_rfinal, d_rrank;
-    std::vector<uint32_t> pairs(R), rfinal(R), rrank(R);
-    if (R) {
-        RC(d_rt.alloc(sizeof(Task) * R));
-        RC(d_rbase.alloc(sizeof(uint32_t) * R));
-        RC(d_snodes.alloc(sizeof(SNode) * total_local));
-        RC(d_pairs.alloc(sizeof(uint32_t) * R));
-        RC(d_rfinal.alloc(sizeof(uint32_t) * R));
-        RC(d_rrank.alloc(sizeof(uint32_t) * R));
-        BVHT_HIP(hipMemcpyAsync(d_rt.p, rtasks.data(), sizeof(Task) * R, hipMemcpyHostToDevice, st));
-        BVHT_HIP(hipMemcpyAsync(d_rbase.p, rbase.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_small_build, dim3(nblk(R, 64)), dim3(64), 0, st, ec.as<float4>(), elo.as<float4>(),
-                           ehi.as<float2>(), d_rt.as<Task>(), d_rbase.as<uint32_t>(), R, d_snodes.as<SNode>(),
-                           d_pairs.as<uint32_t>(), faithful, max_leaf, d_err.as<uint32_t>());
-        BVHT_HIP(hipGetLastError());
-        BVHT_HIP(hipMemcpyAsync(pairs.data(), d_pairs.p, sizeof(uint32_t) * R, hipMemcpyDeviceToHost, st));
-        BVHT_HIP(hipStreamSynchronize(st));
-    }
-
-    // internal counts (post-order) and pre-order ranks over the large tree
-    size_t H = tree.size();
-    std::vector<uint64_t> ic(H, 0);
-    for (size_t h = H; h-- > 0;) {  // children always have larger host indices
-        const HostNode& hn = tree[h];
-        if (hn.small >= 0)
-            ic[h] = pairs[hn.small];
-        else if (hn.left >= 0)
-            ic[h] = 1 + ic[hn.left] + ic[hn.left + 1];
-    }
-    uint64_t total_internal = ic[0];
-    BVHT_REQUIRE(2 + 2 * total_internal <= 2 * (uint64_t)n, "internal node count overflow");
-    std::vector<uint32_t> final_idx(H, 0);
-    std::vector<bvht_node> big_nodes;
-    std::vector<uint32_t> big_idx;
-    {
-        std::vector<uint32_t> stk;
-        stk.push_back(0);
-        final_idx[0] = 0;
-        uint64_t counter = 0;
-        while (!stk.empty()) {
-            uint32_t h = stk.back();
-            stk.pop_back();
-            const HostNode& hn = tree[h];
-            if (hn.small >= 0) {
-                rfinal[hn.small] = final_idx[h];
-                rrank[hn.small] = (uint32_t)counter;
-                counter += pairs[hn.small];
-                continue;
-            }
-            bvht_node o;
-            std::memcpy(o.bmin, hn.box, 12);
-            std::memcpy(o.bmax, hn.box + 3, 12);
-            if (hn.left >= 0) {
-                uint32_t r = (uint32_t)counter++;
-                o.left_first = 2 + 2 * r;
-                o.count = 0;
-                final_idx[hn.left] = 2 + 2 * r;
-                final_idx[hn.left + 1] = 3 + 2 * r;
-                stk.push_back((uint32_t)hn.left + 1);
-                stk.push_back((uint32_t)hn.left);
-            } else {
-                o.left_first = hn.first;
-                o.count = hn.count;
-            }
-            if (final_idx[h] >= 2 * n)
-                return set_error(BVHT_EDEVICE, "BLAS build: node index %u out of range", final_idx[h]);
-            big_nodes.push_back(o);
-            big_idx.push_back(final_idx[h]);
-        }
-    }
-    b->nodes_used = (uint32_t)(2 + 2 * total_internal);
-    BVHT_HIP(hipMemsetAsync(b->d_nodes, 0, sizeof(bvht_node) * 2 * (size_t)n, st));
-    if (!big_nodes.empty()) {
-        DevBuf d_big, d_bigi;
-        RC(d_big.alloc(sizeof(bvht_node) * big_nodes.size()));
-        RC(d_bigi.alloc(sizeof(uint32_t) * big_idx.size()));
-        BVHT_HIP(hipMemcpyAsync(d_big.p, big_nodes.data(), sizeof(bvht_node) * big_nodes.size(), hipMemcpyHostToDevice,
-                                st));
-        BVHT_HIP(hipMemcpyAsync(d_bigi.p, big_idx.data(), sizeof(uint32_t) * big_idx.size(), hipMemcpyHostToDevice,
-                                st));
-        hipLaunchKernelGGL(k_scatter_nodes, dim3(nblk(big_nodes.size(), 256)), dim3(256), 0, st,
-                           d_big.as<bvht_node>(), d_bigi.as<uint32_t>(), (uint32_t)big_nodes.size(), b->d_nodes);
-        BVHT_HIP(hipGetLastError());
-        BVHT_HIP(hipStreamSynchronize(st));
-    }
-    if (R) {
-        BVHT_HIP(hipMemcpyAsync(d_rfinal.p, rfinal.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, st));
-        BVHT_HIP(hipMemcpyAsync(d_rrank.p, rrank.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_small_emit, dim3(nblk(R, 64)), dim3(64), 0, st, d_snodes.as<SNode>(),
-                           d_rbase.as<uint32_t>(), d_pairs.as<uint32_t>(), d_rfinal.as<uint32_t>(),
-                           d_rrank.as<uint32_t>(), R, b->d_nodes, 2 * n, d_err.as<uint32_t>());
-        BVHT_HIP(hipGetLastError());
-    }
     if (hipMalloc(&b->d_perm, sizeof(uint32_t) * (size_t)n) != hipSuccess)
         return set_error(BVHT_ENOMEM, "device allocation of the %u-entry build permutation failed", n);
-    hipLaunchKernelGGL(k_gather_tris, dim3(nblk(n, 256)), dim3(256), 0, st, tin.as<float>(), ec.as<float4>(), n,
-                       b->d_tris, b->d_perm, d_err.as<uint32_t>());
-    hipLaunchKernelGGL(k_validate_tree, dim3(nblk(b->nodes_used, 256)), dim3(256), 0, st, b->d_nodes, b->nodes_used,
-                       n, d_err.as<uint32_t>());
+
+    BVHT_HIP(hipMemcpyAsync(w.tin, h_tris, sizeof(float) * 9 * (size_t)n, hipMemcpyHostToDevice, st));
+    BVHT_HIP(hipMemsetAsync(b->d_nodes, 0, sizeof(bvht_node) * 2 * (size_t)n, st));
+    hipLaunchKernelGGL(k_init_elems, dim3(nblk(n, 256)), dim3(256), 0, st, w.tin, n, w.ec, w.elo, w.ehi);
+    // root box: update_node_bounds over everything (bvh.rs:537)
+    RC(fill(st, w.rootk, kEmptyMin, 3));
+    RC(fill(st, w.rootk + 3, kEmptyMax, 3));
+    hipLaunchKernelGGL(k_range_box, dim3(std::min<uint32_t>(nblk(n, kWG), 1024)), dim3(kWG), 0, st, w.elo, w.ehi, 0u,
+                       n, w.rootk);
+    hipLaunchKernelGGL(k_root_init, dim3(1), dim3(1), 0, st, w.rootk, n, w.bn, w.lvl_all, w.small_roots,
+                       w.level_start, w.ctl);
     BVHT_HIP(hipGetLastError());
-    uint32_t errv = 0;
-    BVHT_HIP(hipMemcpyAsync(&errv, d_err.p, sizeof errv, hipMemcpyDeviceToHost, st));
-    BVHT_HIP(hipStreamSynchronize(st));
-    if (errv)
+
+    // Large nodes, level by level, read back every few levels: a level with
+    // no node costs only its launches.
+    Ctl h{};
+    for (uint32_t batch = 0;; ++batch) {
+        uint32_t levels = batch < 2 ? 4u : (batch < 4 ? 8u : (batch < 6 ? 16u : 32u));
+        for (uint32_t k = 0; k < levels; ++k)
+            enqueue_level(st, w, faithful, max_leaf);
+        BVHT_HIP(hipGetLastError());
+        BVHT_HIP(hipMemcpyAsync(&h, w.ctl, sizeof h, hipMemcpyDeviceToHost, st));
+        BVHT_HIP(hipStreamSynchronize(st));
+        if (h.err || h.L == 0)
+            break;
+    }
+    if (h.err)
         return set_error(BVHT_EDEVICE, "BLAS build self-check failed (flags 0x%x): inconsistent partition or tree",
-                         errv);
+                         h.err);
+
+    // small subtrees, numbering, placement
+    uint32_t R = h.R;
+    if (R)
+        hipLaunchKernelGGL(k_small_build, dim3(nblk(R, 64)), dim3(64), 0, st, w.ec, w.elo, w.ehi, w.bn,
+                           w.small_roots, R, w.snodes, w.pairs, faithful, max_leaf, &w.ctl->err);
+    hipLaunchKernelGGL(k_number, dim3(1), dim3(kCtlWG), 0, st, w.bn, w.lvl_all, w.level_start, w.ctl, w.pairs, w.ic,
+                       w.start, w.fin, w.rfinal, w.rrank, b->d_nodes, 2 * n);
+    if (R)
+        hipLaunchKernelGGL(k_small_emit, dim3(nblk(R, 64)), dim3(64), 0, st, w.snodes, w.bn, w.small_roots, w.pairs,
+                           w.rfinal, w.rrank, R, b->d_nodes, 2 * n, &w.ctl->err);
+    hipLaunchKernelGGL(k_gather_tris, dim3(nblk(n, 256)), dim3(256), 0, st, w.tin, w.ec, n, b->d_tris, b->d_perm,
+                       &w.ctl->err);
+    hipLaunchKernelGGL(k_validate_tree, dim3(nblk(2 * (size_t)n, 256)), dim3(256), 0, st, b->d_nodes, w.ctl, n,
+                       &w.ctl->err);
+    BVHT_HIP(hipGetLastError());
+    BVHT_HIP(hipMemcpyAsync(&h, w.ctl, sizeof h, hipMemcpyDeviceToHost, st));
     BVHT_HIP(hipMemcpyAsync(h_tris, b->d_tris, sizeof(float) * 9 * (size_t)n, hipMemcpyDeviceToHost, st));
     BVHT_HIP(hipStreamSynchronize(st));
+    if (h.err)
+        return set_error(BVHT_EDEVICE, "BLAS build self-check failed (flags 0x%x): inconsistent partition or tree",
+                         h.err);
+    if (h.nodes_used < 2 || h.nodes_used > 2 * n)
+        return set_error(BVHT_EDEVICE, "BLAS build: %u nodes used of %u", h.nodes_used, 2 * n);
+    b->nodes_used = h.nodes_used;
     RC(compute_blas_topology(b, nullptr));
     *out = b;
     b = nullptr;
@@ -1153,6 +1447,21 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
 }
 
 }  // namespace
+
+namespace bvht {
+
+void release_build_ws(bvht_ctx ctx)
+{
+    BuildWs* w = static_cast<BuildWs*>(ctx->build_ws);
+    if (!w)
+        return;
+    if (w->base)
+        (void)hipFree(w->base);
+    delete w;
+    ctx->build_ws = nullptr;
+}
+
+}  // namespace bvht
 
 extern "C" int32_t bvht_blas_build(bvht_ctx ctx, float* tris, uint32_t n, int32_t mode, uint32_t max_leaf,
                                    bvht_blas* out)

@@ -1375,6 +1375,7 @@ int32_t bvht_destroy(bvht_ctx ctx)
     (void)hipFree(ctx->cam_bins);
     if (ctx->h_status)
         (void)hipHostFree(ctx->h_status);
+    release_build_ws(ctx);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BVHT_OK;

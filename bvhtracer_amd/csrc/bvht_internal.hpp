@@ -52,6 +52,8 @@ struct bvht_ctx_s {
     // Device address, on this context's device, of the fixed scratch array
     // of bvht_trace_camera_bounce_noalloc (shared by every context there).
     void* noalloc_scratch = nullptr;
+    // BLAS build workspace (bvht_build.hip), kept between builds.
+    void* build_ws = nullptr;
 };
 
 // Bits of the context status word.
@@ -129,6 +131,8 @@ inline uint32_t stack_cap(bvht_ctx ctx, uint32_t need)
     return (ctx->stack_limit && ctx->stack_limit < need) ? ctx->stack_limit : need;
 }
 int32_t blas_alloc(bvht_ctx ctx, uint32_t n, uint32_t node_count, bvht_blas* out);
+// Frees the context's BLAS build workspace (bvht_destroy).
+void release_build_ws(bvht_ctx ctx);
 int32_t fill_u32(bvht_ctx ctx, uint32_t* p, uint32_t v, size_t n);
 // In-place exclusive scan of a[0, m) (m > 0, a 16-byte aligned) on ctx's
 // stream; sums holds scan_sums_entries(m) entries; the total goes to *total
