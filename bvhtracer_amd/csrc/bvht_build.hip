@@ -685,129 +685,340 @@ struct SNode {  // same layout as bvht_node; internal nodes hold a local pair in
     uint32_t count;
 };
 
-__device__ __forceinline__ float comp4(float4 c, int axis) { return axis == 0 ? c.x : (axis == 1 ? c.y : c.z); }
+// ------------------------------------------------ small subtrees, a wave each
+//
+// The reference's sequential subdivide (bvh.rs:396-467) of a subtree of at
+// most kSmall elements, run by one wave: lane p holds the element at
+// position first + p, lane j holds local node j, lane s holds stack entry s,
+// and lane a*8+b gathers bin b of axis a.  Every value equals the sequential
+// one: min/max are exact (the reduction order changes at most the sign of a
+// zero), counts are integers, the SAH picks the first minimal cost in the
+// sequential (axis, plane) order, and the two-pointer partition is applied
+// as the permutation it amounts to (the closed form at k_part_scatter,
+// looked up with ballots instead of the kpos/sel arrays).
 
-// Sequential subdivide (bvh.rs:396-467) of one small subtree by one thread, on
-// the element range [first, first+count); nodes go to out[0 .. 1+2*pairs),
-// child pairs allocated in depth-first pre-order (pair p -> out[1+2p], out[2+2p]).
-// A subtree of count elements has at most count - 1 pairs, so its nodes fit
-// out = snodes + 2 * first, whose 2 * count entries no other subtree uses.
-__global__ void k_small_build(float4* __restrict__ ec, float4* __restrict__ elo, float2* __restrict__ ehi,
-                              const BNode* __restrict__ bn, const uint32_t* __restrict__ small_roots, uint32_t nroots,
-                              SNode* __restrict__ snodes, uint32_t* __restrict__ pairs_out, int faithful,
-                              uint32_t max_leaf, uint32_t* __restrict__ err)
+__device__ __forceinline__ float rl(float v, uint32_t lane)
 {
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
+}
+__device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t lane)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+// DPP moves within 16-lane rows (row_shr:n = 0x110 + n, row_shl:n =
+// 0x100 + n); a lane whose source lies outside its row keeps `old`.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float old, float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t old, uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, 0xF, 0xF, false);
+}
+constexpr float kQNaN = __builtin_nanf("");
+// min / max over lanes 0..31, returned to every lane (NaN is the identity:
+// fminf / fmaxf ignore it): an inclusive scan within each 16-lane row, whose
+// lane 15 then holds the row's result, and the two rows combined.
+__device__ __forceinline__ float wmin32(float v)
+{
+    v = fminf(v, dpp_f<0x111>(kQNaN, v));
+    v = fminf(v, dpp_f<0x112>(kQNaN, v));
+    v = fminf(v, dpp_f<0x114>(kQNaN, v));
+    v = fminf(v, dpp_f<0x118>(kQNaN, v));
+    return fminf(rl(v, 15), rl(v, 31));
+}
+__device__ __forceinline__ float wmax32(float v)
+{
+    v = fmaxf(v, dpp_f<0x111>(kQNaN, v));
+    v = fmaxf(v, dpp_f<0x112>(kQNaN, v));
+    v = fmaxf(v, dpp_f<0x114>(kQNaN, v));
+    v = fmaxf(v, dpp_f<0x118>(kQNaN, v));
+    return fmaxf(rl(v, 15), rl(v, 31));
+}
+// One bin's (or a run of bins') count and its contribution to a box grown
+// by it: Aabb::grow_aabb (aabb.rs:41-46) grows by both corners, so a box
+// {lo, hi} adds min(lo, hi) to the minimum and max(lo, hi) to the maximum
+// per component; a bin whose lo.x is still f32::MAX is not grown by (the
+// sweep's check, as in sah_axis); NaN marks "nothing".
+struct BinSum {
+    uint32_t n;
+    float lo[3], hi[3];
+};
+__device__ __forceinline__ void bin_combine(BinSum& a, const BinSum& b)
+{
+    a.n += b.n;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        a.lo[q] = fminf(a.lo[q], b.lo[q]);
+        a.hi[q] = fmaxf(a.hi[q], b.hi[q]);
+    }
+}
+template <int CTRL>
+__device__ __forceinline__ BinSum dpp_bin(const BinSum& v)
+{
+    BinSum t;
+    t.n = dpp_u<CTRL>(0u, v.n);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        t.lo[q] = dpp_f<CTRL>(kQNaN, v.lo[q]);
+        t.hi[q] = dpp_f<CTRL>(kQNaN, v.hi[q]);
+    }
+    return t;
+}
+// position of the m-th set bit (m < popcount) of a 32-bit mask
+__device__ __forceinline__ uint32_t nth_set(uint32_t mask, uint32_t m)
+{
+    uint32_t pos = 0;
+    for (uint32_t w = 16; w > 0; w >>= 1) {
+        uint32_t c = __popc((mask >> pos) & ((1u << w) - 1u));
+        if (m >= c) {
+            m -= c;
+            pos += w;
+        }
+    }
+    return pos;
+}
+__device__ __forceinline__ float perm_f(uint32_t to, float v)
+{
+    return __int_as_float(__builtin_amdgcn_ds_permute((int)(to * 4), __float_as_int(v)));
+}
+
+constexpr int kSmallWavesPerWG = 4;
+
+__global__ __launch_bounds__(64 * kSmallWavesPerWG) void k_small_wave(
+    float4* __restrict__ ec, const float4* __restrict__ elo, const float2* __restrict__ ehi,
+    const BNode* __restrict__ bn, const uint32_t* __restrict__ small_roots, uint32_t nroots,
+    SNode* __restrict__ snodes, uint32_t* __restrict__ pairs_out, int faithful, uint32_t max_leaf,
+    uint32_t* __restrict__ err)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t r = blockIdx.x * kSmallWavesPerWG + (threadIdx.x >> 6);
     if (r >= nroots)
-        return;
+        return;  // wave-uniform; no workgroup barrier below
     BNode rn = bn[small_roots[r]];
-    Task rt;
-    rt.first = rn.first;
-    rt.count = rn.count;
-    for (int k = 0; k < 6; ++k)
-        rt.box[k] = rn.box[k];
-    SNode* out = snodes + 2 * (size_t)rn.first;
-    out[0] = SNode{{rt.box[0], rt.box[1], rt.box[2]}, rt.first, {rt.box[3], rt.box[4], rt.box[5]}, rt.count};
-    uint32_t npairs = 0;
-    uint32_t stack[kSmall + 2];
-    uint32_t sp = 0;
-    stack[sp++] = 0;
-    while (sp) {
-        uint32_t j = stack[--sp];
-        SNode nd = out[j];
-        uint32_t first = nd.lf, count = nd.count;
-        // find_best_split_plane (bvh.rs:333-394)
+    const uint32_t first = rn.first, count = rn.count;
+    if (count == 0 || count > kSmall) {
+        if (lane == 0)
+            atomicOr(err, 4u);
+        return;
+    }
+    float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f), lo = c;
+    float2 hi = make_float2(0.0f, 0.0f);
+    if (lane < count) {
+        c = ec[first + lane];
+        lo = elo[first + lane];
+        hi = ehi[first + lane];
+    }
+    // node j in lane j: box, first (local; the pair index once internal), count
+    float nb0 = 0.0f, nb1 = 0.0f, nb2 = 0.0f, nb3 = 0.0f, nb4 = 0.0f, nb5 = 0.0f;
+    uint32_t nlf = 0, ncnt = 0;
+    if (lane == 0) {
+        nb0 = rn.box[0]; nb1 = rn.box[1]; nb2 = rn.box[2];
+        nb3 = rn.box[3]; nb4 = rn.box[4]; nb5 = rn.box[5];
+        ncnt = count;
+    }
+    uint32_t stk = 0;  // stack entry `lane`; entry 0 = node 0
+    uint32_t sp = 1, npairs = 0;
+    const float init_min = faithful ? 1e30f : kFltMax, init_max = faithful ? 1e-30f : -kFltMax;
+    const float qnan = __int_as_float(0x7FC00000);
+    const uint32_t ba = lane >> 3, bb = lane & 7u;  // bin lanes 0..23
+    while (sp > 0) {
+        const uint32_t j = rl(stk, sp - 1);
+        --sp;
+        const uint32_t fl = rl(nlf, j), cn = rl(ncnt, j);
+        // one element: either no axis has a centroid range, or (the
+        // reference mode's 1e30 / 1e-30 range) any split leaves a side empty,
+        // and the partition of one element moves nothing: a leaf as it is
+        if (cn <= 1)
+            continue;
+        const bool act = lane >= fl && lane < fl + cn;
+        // find_best_split_plane (bvh.rs:333-394): centroid range per axis
+        float bmin[3], bmax[3], ascale[3];
+        bool valid[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float v = a == 0 ? c.x : (a == 1 ? c.y : c.z);
+            bmin[a] = fminf(init_min, wmin32(act ? v : qnan));
+            bmax[a] = fmaxf(init_max, wmax32(act ? v : qnan));
+            valid[a] = !(bmin[a] == bmax[a]);
+            ascale[a] = (float)kBins / (bmax[a] - bmin[a]);
+        }
+        // each element's bin on the valid axes, 4 bits per axis
+        uint32_t pb = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float v = a == 0 ? c.x : (a == 1 ? c.y : c.z);
+            if (valid[a])
+                pb |= bin_index((v - bmin[a]) * ascale[a]) << (4 * a);
+        }
+        // bin (ba, bb): count and box over the node's elements
+        uint32_t bcnt = 0;
+        FBox bx = box_init(faithful);
+        for (uint32_t k = fl; k < fl + cn; ++k) {
+            uint32_t pk = rl(pb, k);
+            V3 el = mk3(rl(lo.x, k), rl(lo.y, k), rl(lo.z, k));
+            V3 eh = mk3(rl(lo.w, k), rl(hi.x, k), rl(hi.y, k));
+            if (lane < 24 && ((pk >> (4 * ba)) & 15u) == bb) {
+                bcnt += 1;
+                bx.lo = min3(bx.lo, el);
+                bx.hi = max3(bx.hi, eh);
+            }
+        }
+        // SAH sweep (sah_axis): plane (ba, bb), bb < 7, has bins 0..bb on the
+        // left and bb+1..7 on the right.  Bins of one axis are 8 adjacent
+        // lanes inside a 16-lane row: an inclusive prefix and suffix scan over
+        // each 8-lane group, then each plane takes the suffix of the next lane.
+        BinSum own;
+        own.n = bcnt;
+        {
+            bool grow = bx.lo.x != kFltMax;
+            own.lo[0] = grow ? fminf(bx.lo.x, bx.hi.x) : kQNaN;
+            own.lo[1] = grow ? fminf(bx.lo.y, bx.hi.y) : kQNaN;
+            own.lo[2] = grow ? fminf(bx.lo.z, bx.hi.z) : kQNaN;
+            own.hi[0] = grow ? fmaxf(bx.lo.x, bx.hi.x) : kQNaN;
+            own.hi[1] = grow ? fmaxf(bx.lo.y, bx.hi.y) : kQNaN;
+            own.hi[2] = grow ? fmaxf(bx.lo.z, bx.hi.z) : kQNaN;
+        }
+        BinSum pre = own, suf = own;
+        {
+            BinSum t = dpp_bin<0x111>(pre);
+            if (bb >= 1)
+                bin_combine(pre, t);
+            t = dpp_bin<0x112>(pre);
+            if (bb >= 2)
+                bin_combine(pre, t);
+            t = dpp_bin<0x114>(pre);
+            if (bb >= 4)
+                bin_combine(pre, t);
+            t = dpp_bin<0x101>(suf);
+            if (bb + 1 <= 7)
+                bin_combine(suf, t);
+            t = dpp_bin<0x102>(suf);
+            if (bb + 2 <= 7)
+                bin_combine(suf, t);
+            t = dpp_bin<0x104>(suf);
+            if (bb + 4 <= 7)
+                bin_combine(suf, t);
+        }
+        BinSum rsum = dpp_bin<0x101>(suf);  // bins bb+1..7 (lanes with bb == 7 are not planes)
+        FBox lbx = box_init(faithful), rbx = box_init(faithful);
+        lbx.lo = mk3(fminf(lbx.lo.x, pre.lo[0]), fminf(lbx.lo.y, pre.lo[1]), fminf(lbx.lo.z, pre.lo[2]));
+        lbx.hi = mk3(fmaxf(lbx.hi.x, pre.hi[0]), fmaxf(lbx.hi.y, pre.hi[1]), fmaxf(lbx.hi.z, pre.hi[2]));
+        rbx.lo = mk3(fminf(rbx.lo.x, rsum.lo[0]), fminf(rbx.lo.y, rsum.lo[1]), fminf(rbx.lo.z, rsum.lo[2]));
+        rbx.hi = mk3(fmaxf(rbx.hi.x, rsum.hi[0]), fmaxf(rbx.hi.y, rsum.hi[1]), fmaxf(rbx.hi.z, rsum.hi[2]));
+        uint32_t lcnt = pre.n, rcnt = rsum.n;
+        // this lane's axis, by value (no dynamically indexed arrays)
+        bool avalid = ba == 0 ? valid[0] : (ba == 1 ? valid[1] : valid[2]);
+        float amin = ba == 0 ? bmin[0] : (ba == 1 ? bmin[1] : bmin[2]);
+        float amax = ba == 0 ? bmax[0] : (ba == 1 ? bmax[1] : bmax[2]);
+        bool cand = lane < 24 && bb < kBins - 1 && avalid;
+        if (!faithful && (lcnt == 0 || rcnt == 0))
+            cand = false;
+        float cost = (float)lcnt * box_area(lbx) + (float)rcnt * box_area(rbx);
+        bool sel = cand && cost < kFltMax;
+        float mcost = wmin32(sel ? cost : qnan);
+        uint64_t win = __ballot(sel && cost == mcost);
         int best_axis = -1;
         float best_pos = 0.0f, best_cost = kFltMax;
-        for (int a = 0; a < 3; ++a) {
-            float bmin = faithful ? 1e30f : kFltMax, bmax = faithful ? 1e-30f : -kFltMax;
-            for (uint32_t k = 0; k < count; ++k) {
-                float c = comp4(ec[first + k], a);
-                bmin = fminf(bmin, c);
-                bmax = fmaxf(bmax, c);
-            }
-            if (bmin == bmax)
-                continue;
-            AxisRange ar;
-            ar.bmin = bmin;
-            ar.bmax = bmax;
-            ar.scale = (float)kBins / (bmax - bmin);
-            ar.valid = true;
-            uint32_t counts[kBins];
-            FBox boxes[kBins];
-            for (int b = 0; b < kBins; ++b) {
-                counts[b] = 0;
-                boxes[b] = box_init(faithful);
-            }
-            for (uint32_t k = 0; k < count; ++k) {
-                uint32_t b = bin_index((comp4(ec[first + k], a) - bmin) * ar.scale);
-                float4 lo = elo[first + k];
-                float2 hi = ehi[first + k];
-                counts[b] += 1;
-                boxes[b].lo = min3(boxes[b].lo, mk3(lo.x, lo.y, lo.z));
-                boxes[b].hi = max3(boxes[b].hi, mk3(lo.w, hi.x, hi.y));
-            }
-            sah_axis(counts, boxes, ar, a, faithful, best_axis, best_pos, best_cost);
+        if (win) {
+            uint32_t wl = (uint32_t)__builtin_ctzll(win);
+            float scale = (amax - amin) / (float)kBins;
+            float pa = amin + scale * (float)(bb + 1);
+            best_axis = (int)(wl >> 3);
+            best_pos = rl(pa, wl);
+            best_cost = mcost;
         }
-        FBox nb;
-        nb.lo = mk3(nd.bmin[0], nd.bmin[1], nd.bmin[2]);
-        nb.hi = mk3(nd.bmax[0], nd.bmax[1], nd.bmax[2]);
-        float no_split_cost = (float)count * box_area(nb);
-        bool force = !faithful && max_leaf > 0 && count > max_leaf;
-        uint32_t split;
+        FBox nbx;
+        nbx.lo = mk3(rl(nb0, j), rl(nb1, j), rl(nb2, j));
+        nbx.hi = mk3(rl(nb3, j), rl(nb4, j), rl(nb5, j));
+        float no_split_cost = (float)cn * box_area(nbx);
+        bool force = !faithful && max_leaf > 0 && cn > max_leaf;
+        uint32_t lc;
         if (best_axis >= 0 && (!(best_cost >= no_split_cost) || force)) {
-            int64_t i = first, jj = (int64_t)first + count - 1;
-            while (i <= jj) {
-                if (comp4(ec[i], best_axis) < best_pos) {
-                    ++i;
+            // the two-pointer partition (bvh.rs:420-429) of positions [fl, fl + cn)
+            float v = best_axis == 0 ? c.x : (best_axis == 1 ? c.y : c.z);
+            bool f = act && v < best_pos;
+            uint32_t k = lane - fl;
+            uint32_t F = (uint32_t)(__ballot(f) >> fl);
+            uint32_t nl = __popc(F);
+            uint32_t a0 = nl == cn ? cn : (((F >> nl) & 1u) ? nl : nl + 1);
+            uint32_t RL = (uint32_t)(__ballot(act && k < a0 && !f) >> fl);
+            uint32_t LR = (uint32_t)(__ballot(act && k >= a0 && f) >> fl);
+            uint32_t to = lane;
+            if (act) {
+                uint32_t lb = __popc(F & ((1u << k) - 1u));
+                uint32_t dest;
+                if (k < a0) {
+                    uint32_t m = k - lb;
+                    dest = f ? k : (m == 0 ? cn - 1 : nth_set(LR, __popc(LR) - m) - 1);
                 } else {
-                    float4 c0 = ec[i], l0 = elo[i];
-                    float2 h0 = ehi[i];
-                    ec[i] = ec[jj];
-                    elo[i] = elo[jj];
-                    ehi[i] = ehi[jj];
-                    ec[jj] = c0;
-                    elo[jj] = l0;
-                    ehi[jj] = h0;
-                    --jj;
+                    dest = f ? nth_set(RL, nl - lb - 1) : k - 1;
                 }
+                to = fl + dest;
             }
-            split = (uint32_t)i;
+            c = make_float4(perm_f(to, c.x), perm_f(to, c.y), perm_f(to, c.z), perm_f(to, c.w));
+            lo = make_float4(perm_f(to, lo.x), perm_f(to, lo.y), perm_f(to, lo.z), perm_f(to, lo.w));
+            hi = make_float2(perm_f(to, hi.x), perm_f(to, hi.y));
+            lc = nl;
         } else if (force) {
-            split = first + count / 2;
+            lc = cn / 2;
         } else {
             continue;
         }
-        uint32_t lc = split - first;
-        if (lc == 0 || lc == count) {
+        if (lc == 0 || lc == cn) {
             if (!force)
                 continue;
-            split = first + count / 2;
-            lc = count / 2;
+            lc = cn / 2;
+        }
+        // children: boxes over each side (update_node_bounds, bvh.rs:462-463)
+        bool left = act && lane - fl < lc, right = act && lane - fl >= lc;
+        float cl[6], cr[6];
+        float ev[6] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            cl[q] = fminf(kFltMax, wmin32(left ? ev[q] : qnan));
+            cr[q] = fminf(kFltMax, wmin32(right ? ev[q] : qnan));
+            cl[3 + q] = fmaxf(-kFltMax, wmax32(left ? ev[3 + q] : qnan));
+            cr[3 + q] = fmaxf(-kFltMax, wmax32(right ? ev[3 + q] : qnan));
         }
         uint32_t p = npairs++;
         uint32_t l = 1 + 2 * p, rr = 2 + 2 * p;
-        uint32_t ranges[2][2] = {{first, lc}, {split, count - lc}};
-        for (int side = 0; side < 2; ++side) {
-            V3 lo = mk3(kFltMax, kFltMax, kFltMax), hi = mk3(-kFltMax, -kFltMax, -kFltMax);
-            for (uint32_t k = 0; k < ranges[side][1]; ++k) {
-                float4 a = elo[ranges[side][0] + k];
-                float2 b = ehi[ranges[side][0] + k];
-                lo = min3(lo, mk3(a.x, a.y, a.z));
-                hi = max3(hi, mk3(a.w, b.x, b.y));
-            }
-            out[side ? rr : l] = SNode{{lo.x, lo.y, lo.z}, ranges[side][0], {hi.x, hi.y, hi.z}, ranges[side][1]};
-        }
-        if (sp + 2 > kSmall + 2 || rr >= 2 * rt.count) {
-            atomicOr(err, 4u);
+        if (sp + 2 > kSmall + 2 || rr >= 2 * count) {
+            if (lane == 0)
+                atomicOr(err, 4u);
             break;
         }
-        out[j].lf = p;
-        out[j].count = 0;
-        stack[sp++] = rr;
-        stack[sp++] = l;
+        if (lane == l || lane == rr) {
+            bool isl = lane == l;
+            nb0 = isl ? cl[0] : cr[0]; nb1 = isl ? cl[1] : cr[1]; nb2 = isl ? cl[2] : cr[2];
+            nb3 = isl ? cl[3] : cr[3]; nb4 = isl ? cl[4] : cr[4]; nb5 = isl ? cl[5] : cr[5];
+            nlf = isl ? fl : fl + lc;
+            ncnt = isl ? lc : cn - lc;
+        }
+        if (lane == j) {
+            nlf = p;
+            ncnt = 0;
+        }
+        if (lane == sp)
+            stk = rr;
+        if (lane == sp + 1)
+            stk = l;
+        sp += 2;
     }
-    pairs_out[r] = npairs;
+    if (lane < count)
+        ec[first + lane] = c;
+    if (lane < 1 + 2 * npairs) {
+        SNode o;
+        o.bmin[0] = nb0; o.bmin[1] = nb1; o.bmin[2] = nb2;
+        o.bmax[0] = nb3; o.bmax[1] = nb4; o.bmax[2] = nb5;
+        o.count = ncnt;
+        o.lf = ncnt > 0 ? first + nlf : nlf;
+        snodes[2 * (size_t)first + lane] = o;
+    }
+    if (lane == 0)
+        pairs_out[r] = npairs;
 }
 
 // Place the small subtrees: local node 0 at root_final, local pair p at
@@ -1419,8 +1630,9 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
     // small subtrees, numbering, placement
     uint32_t R = h.R;
     if (R)
-        hipLaunchKernelGGL(k_small_build, dim3(nblk(R, 64)), dim3(64), 0, st, w.ec, w.elo, w.ehi, w.bn,
-                           w.small_roots, R, w.snodes, w.pairs, faithful, max_leaf, &w.ctl->err);
+        hipLaunchKernelGGL(k_small_wave, dim3(nblk(R, kSmallWavesPerWG)), dim3(64 * kSmallWavesPerWG), 0, st, w.ec,
+                           w.elo, w.ehi, w.bn, w.small_roots, R, w.snodes, w.pairs, faithful, max_leaf,
+                           &w.ctl->err);
     hipLaunchKernelGGL(k_number, dim3(1), dim3(kCtlWG), 0, st, w.bn, w.lvl_all, w.level_start, w.ctl, w.pairs, w.ic,
                        w.start, w.fin, w.rfinal, w.rrank, b->d_nodes, 2 * n);
     if (R)
