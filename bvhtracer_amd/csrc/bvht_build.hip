@@ -705,18 +705,6 @@ __device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t lane)
 {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }
-// DPP moves within 16-lane rows (row_shr:n = 0x110 + n, row_shl:n =
-// 0x100 + n); a lane whose source lies outside its row keeps `old`.
-template <int CTRL>
-__device__ __forceinline__ float dpp_f(float old, float v)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, 0xF, 0xF, false));
-}
-template <int CTRL>
-__device__ __forceinline__ uint32_t dpp_u(uint32_t old, uint32_t v)
-{
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, 0xF, 0xF, false);
-}
 constexpr float kQNaN = __builtin_nanf("");
 // min / max over lanes 0..31, returned to every lane (NaN is the identity:
 // fminf / fmaxf ignore it): an inclusive scan within each 16-lane row, whose

@@ -256,4 +256,51 @@ __device__ __forceinline__ float key2f(uint32_t k)
     return __uint_as_float(u);
 }
 
+// ------------------------------------------------------ lane exchanges --
+// DPP moves within 16-lane rows (quad_perm, row_shr:n = 0x110 + n, row_shl:n
+// = 0x100 + n, row_ror:n = 0x120 + n): lane i of row_shr:n reads lane i - n;
+// a lane whose source lies outside its row keeps `old`.  gfx950 has no DPP
+// wave shifts or row broadcasts; across rows the permlane swaps and readlane
+// are used.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float old, float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t old, uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, 0xF, 0xF, false);
+}
+// The value of lane `lane ^ J` (J a power of two below 64) without LDS:
+// quad permutes for 1 and 2, row shifts for 4, a row rotate for 8, and the
+// permlane swaps (v_permlane16/32_swap: odd rows of the first operand with
+// even rows of the second; the upper half of the first with the lower half
+// of the second) for 16 and 32.
+template <uint32_t J>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v, uint32_t lane)
+{
+    if constexpr (J == 1) {
+        return dpp_u<0xB1>(v, v);  // quad_perm [1, 0, 3, 2]
+    } else if constexpr (J == 2) {
+        return dpp_u<0x4E>(v, v);  // quad_perm [2, 3, 0, 1]
+    } else if constexpr (J == 4) {
+        uint32_t down = dpp_u<0x114>(v, v), up = dpp_u<0x104>(v, v);
+        return (lane & 4u) ? down : up;
+    } else if constexpr (J == 8) {
+        return dpp_u<0x128>(v, v);  // row_ror:8
+    } else if constexpr (J == 16) {
+        auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return ((lane >> 4) & 1u) ? r[0] : r[1];
+    } else {
+        static_assert(J == 32, "xor_lane: J must be a power of two below 64");
+        auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return lane >= 32u ? r[0] : r[1];
+    }
+}
+__device__ __forceinline__ float readlane_f(float v, uint32_t lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
+}
+
 }  // namespace bvht

@@ -19,6 +19,7 @@
 // The forward is binary32 (bitwise the reference arithmetic); the adjoint
 // solves each pixel in binary64 and scatters binary32.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "bvht_internal.hpp"
@@ -62,10 +63,74 @@ namespace {
 inline uint32_t nblk(size_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 inline size_t align256(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
 
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, uint32_t m)
+template <uint32_t J>
+__device__ __forceinline__ uint32_t xor_key(uint32_t v, uint32_t lane, std::integral_constant<uint32_t, J>)
 {
-    uint32_t lo = __shfl_xor((uint32_t)v, (int)m, 64), hi = __shfl_xor((uint32_t)(v >> 32), (int)m, 64);
+    return xor_lane<J>(v, lane);
+}
+template <uint32_t J>
+__device__ __forceinline__ uint64_t xor_key(uint64_t v, uint32_t lane, std::integral_constant<uint32_t, J>)
+{
+    uint32_t lo = xor_lane<J>((uint32_t)v, lane), hi = xor_lane<J>((uint32_t)(v >> 32), lane);
     return ((uint64_t)hi << 32) | lo;
+}
+template <uint32_t J>
+__device__ __forceinline__ double xor_double(double v, uint32_t lane)
+{
+    uint64_t u = (uint64_t)__double_as_longlong(v);
+    uint32_t lo = xor_lane<J>((uint32_t)u, lane), hi = xor_lane<J>((uint32_t)(u >> 32), lane);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+// Butterfly sum over the 64 lanes (partners 32, 16, .., 1), every lane
+// getting the same total.
+__device__ __forceinline__ double wave_sum_d(double v, uint32_t lane)
+{
+    v += xor_double<32>(v, lane);
+    v += xor_double<16>(v, lane);
+    v += xor_double<8>(v, lane);
+    v += xor_double<4>(v, lane);
+    v += xor_double<2>(v, lane);
+    v += xor_double<1>(v, lane);
+    return v;
+}
+
+// 64-lane bitonic sort of one key per lane, ascending in lane order:
+// stages K = 2, 4, .., 64, each merging with partner distances K/2 .. 1.
+template <class T, uint32_t K, uint32_t J>
+__device__ __forceinline__ void bitonic_merge(T& k, uint32_t lane)
+{
+    T other = xor_key(k, lane, std::integral_constant<uint32_t, J>{});
+    bool keep_min = ((lane & J) == 0) == ((lane & K) == 0);
+    k = keep_min ? (other < k ? other : k) : (other > k ? other : k);
+    if constexpr (J > 1)
+        bitonic_merge<T, K, J / 2>(k, lane);
+}
+template <class T, uint32_t K>
+__device__ __forceinline__ void bitonic_sort(T& k, uint32_t lane)
+{
+    if constexpr (K > 2)
+        bitonic_sort<T, K / 2>(k, lane);
+    bitonic_merge<T, K, K / 2>(k, lane);
+}
+
+// One Hillis-Steele step of an inclusive segmented sum within 16-lane rows
+// (f: a run starts at or before this lane within the row).
+template <uint32_t D>
+__device__ __forceinline__ void seg_step(float (&sv)[9], uint32_t& f, uint32_t lr)
+{
+    uint32_t tf = dpp_u<0x110 + D>(0u, f);
+    float t[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+        t[c] = dpp_f<0x110 + D>(0.0f, sv[c]);
+    if (lr >= D) {
+        if (!f) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+                sv[c] = sv[c] + t[c];
+        }
+        f |= tf;
+    }
 }
 
 __device__ __forceinline__ V3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
@@ -373,38 +438,46 @@ __global__ __launch_bounds__(kAdjWG) void k_trace_adjoint(
     uint32_t lane64 = threadIdx.x & 63u;
     uint32_t rkey = BVHT_MISS;
     if (__ballot(key != BVHT_MISS)) {
-        uint64_t sk = ((uint64_t)key << 6) | lane64;
-#pragma unroll
-        for (uint32_t k = 2; k <= 64; k <<= 1)
-#pragma unroll
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                uint64_t other = shfl_xor_u64(sk, j);
-                bool keep_min = ((lane64 & j) == 0) == ((lane64 & k) == 0);
-                sk = keep_min ? (other < sk ? other : sk) : (other > sk ? other : sk);
-            }
-        uint32_t src = (uint32_t)(sk & 63u);
-        uint32_t skey = (uint32_t)(sk >> 6);
+        uint32_t src, skey;
+        if (ntris < (1u << 26)) {
+            // 32-bit keys; 2^26 - 1 marks "no contribution" and sorts last
+            constexpr uint32_t kNoKey = (1u << 26) - 1u;
+            uint32_t sk = ((key == BVHT_MISS ? kNoKey : key) << 6) | lane64;
+            bitonic_sort<uint32_t, 64>(sk, lane64);
+            src = sk & 63u;
+            skey = sk >> 6;
+            if (skey == kNoKey)
+                skey = BVHT_MISS;
+        } else {
+            uint64_t sk = ((uint64_t)key << 6) | lane64;
+            bitonic_sort<uint64_t, 64>(sk, lane64);
+            src = (uint32_t)(sk & 63u);
+            skey = (uint32_t)(sk >> 6);
+        }
         float sv[9];
 #pragma unroll
         for (int c = 0; c < 9; ++c)
             sv[c] = __shfl(cv[c], (int)src, 64);
         uint32_t prev = __shfl_up(skey, 1, 64), next = __shfl_down(skey, 1, 64);
-        // first sorted position of this position's run
-        uint32_t start = (lane64 == 0 || prev != skey) ? lane64 : 0u;
+        uint32_t f = (lane64 == 0 || prev != skey) ? 1u : 0u;
+        const uint32_t lr = lane64 & 15u;
+        seg_step<1>(sv, f, lr);
+        seg_step<2>(sv, f, lr);
+        seg_step<4>(sv, f, lr);
+        seg_step<8>(sv, f, lr);
+        // carry into row r: the segmented sum through lane 16 r - 1
+        const uint32_t row = lane64 >> 4;
+        const bool f31 = __builtin_amdgcn_readlane((int)f, 31) != 0;
+        const bool f47 = __builtin_amdgcn_readlane((int)f, 47) != 0;
 #pragma unroll
-        for (uint32_t off = 1; off < 64; off <<= 1) {
-            uint32_t t = __shfl_up(start, off, 64);
-            if (lane64 >= off && t > start)
-                start = t;
-        }
-#pragma unroll
-        for (uint32_t off = 1; off < 64; off <<= 1) {
-#pragma unroll
-            for (int c = 0; c < 9; ++c) {
-                float t = __shfl_up(sv[c], off, 64);
-                if (lane64 >= off && lane64 - off >= start)
-                    sv[c] = sv[c] + t;
-            }
+        for (int c = 0; c < 9; ++c) {
+            float c1 = readlane_f(sv[c], 15);
+            float v31 = readlane_f(sv[c], 31), v47 = readlane_f(sv[c], 47);
+            float c2 = f31 ? v31 : c1 + v31;
+            float c3 = f47 ? v47 : c2 + v47;
+            float carry = row == 1 ? c1 : (row == 2 ? c2 : c3);
+            if (row > 0 && !f)
+                sv[c] = carry + sv[c];
         }
         if (skey != BVHT_MISS && (lane64 == 63u || next != skey)) {
             rkey = skey;
@@ -419,8 +492,7 @@ __global__ __launch_bounds__(kAdjWG) void k_trace_adjoint(
     // fixed-order reduction: butterfly within the wave, then the waves in order
 #pragma unroll
     for (int c = 0; c < kRed; ++c)
-        for (int m = 32; m >= 1; m >>= 1)
-            acc[c] += __shfl_xor(acc[c], m, 64);
+        acc[c] = wave_sum_d(acc[c], threadIdx.x & 63u);
     uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     if (lane == 0) {
 #pragma unroll
@@ -463,8 +535,7 @@ __global__ __launch_bounds__(kFinWG) void k_adjoint_finish(const double* __restr
     }
 #pragma unroll
     for (int c = 0; c < kRed; ++c)
-        for (int m = 32; m >= 1; m >>= 1)
-            acc[c] += __shfl_xor(acc[c], m, 64);
+        acc[c] = wave_sum_d(acc[c], threadIdx.x & 63u);
     uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     if (lane == 0) {
 #pragma unroll
