@@ -155,7 +155,7 @@ __device__ __forceinline__ int32_t block_best_match(const bvht_tnode* nodes, con
 // by one wave, without barriers.  Lane i holds list entry i in registers: the
 // node index, its box, link field, object index and subtree height (entries
 // past count keep their last values, as the reference's list does).
-// find_best_match is a shuffle minimum over the areas < f32::MAX (as
+// find_best_match is a butterfly minimum over the areas < f32::MAX (as
 // order-preserving unsigned keys) followed by a ballot for the first lane
 // holding it: the sequential scan's first minimum.  Lane 0 writes the merged
 // nodes, the root copy in node 0 and the results; the height of the root is
@@ -194,9 +194,13 @@ __device__ void wave_rebuild_loop(const float* __restrict__ bounds, bvht_tnode* 
                 key = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
             }
         }
-        uint32_t mk = key;
-        for (int m = 32; m >= 1; m >>= 1)
-            mk = min(mk, (uint32_t)__shfl_xor((int)mk, m));
+        uint32_t mk = key, ul = (uint32_t)lane;  // butterfly minimum, VALU exchanges
+        mk = min(mk, xor_lane<32>(mk, ul));
+        mk = min(mk, xor_lane<16>(mk, ul));
+        mk = min(mk, xor_lane<8>(mk, ul));
+        mk = min(mk, xor_lane<4>(mk, ul));
+        mk = min(mk, xor_lane<2>(mk, ul));
+        mk = min(mk, xor_lane<1>(mk, ul));
         mk = __builtin_amdgcn_readfirstlane(mk);
         if (mk == 0xFFFFFFFFu)
             return -1;
