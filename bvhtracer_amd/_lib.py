@@ -78,6 +78,7 @@ SIGNATURES = {
     "bvht_set_trace_config": (I32, [P, I32, I32]),
     "bvht_get_trace_config": (I32, [P, ctypes.POINTER(I32), ctypes.POINTER(I32)]),
     "bvht_set_stack_limit": (I32, [P, U32]),
+    "bvht_set_wave_schedule": (I32, [P, I32]),
     "bvht_raygen": (I32, [P, P, U32, U32, I32, P]),
     "bvht_bounce": (I32, [P, P, P, P, U32, U64, P, P]),
     "bvht_bounce_sorted": (I32, [P, P, P, P, U32, U64, P, P, P]),

@@ -115,6 +115,11 @@ class Context:
         """Cap the traversal stack of this context's traces (0: tree depth)."""
         check(L.lib().bvht_set_stack_limit(self.handle, int(max_entries)))
 
+    def set_wave_schedule(self, enable=True):
+        """Longest-first wave order for repeated TLAS traces of the same rays
+        (bvht_set_wave_schedule); hits are the same either way."""
+        check(L.lib().bvht_set_wave_schedule(self.handle, 1 if enable else 0))
+
     def event(self):
         e = ctypes.c_void_p()
         check(L.lib().bvht_event_create(self.handle, ctypes.byref(e)))

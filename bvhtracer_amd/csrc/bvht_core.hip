@@ -683,13 +683,14 @@ __global__ __launch_bounds__(WG) void k_trace_camera_bounce(const float4* __rest
                                                             uint32_t* __restrict__ spill, uint32_t cap,
                                                             uint32_t* __restrict__ status,
                                                             const uint32_t* __restrict__ sched,
+                                                            const uint32_t* __restrict__ sched_n,
                                                             uint32_t* __restrict__ cost)
 {
     __shared__ uint32_t lds[LDS_DEPTH * WG];
     uint64_t t0 = (uint64_t)wall_clock64();
     uint32_t n = width * height;
     uint32_t slot = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
-    uint32_t group = (sched && slot < n_waves) ? sched[slot] : slot;
+    uint32_t group = (sched && slot < n_waves && *sched_n == n_waves) ? sched[slot] : slot;
     uint32_t j = group * 64 + (threadIdx.x & 63u);
     bool valid = false;
     uint32_t bin = 0;
@@ -804,7 +805,7 @@ __global__ __launch_bounds__(256) void k_camera_bounce_regroup(const float* __re
 
 typedef void (*camera_bounce_fn)(const float4*, const float*, uint32_t, bvht_camera, uint32_t, uint32_t, int32_t,
                                  uint64_t, bvht_hit*, float4*, uint32_t*, uint32_t*, uint32_t, uint32_t*, uint32_t,
-                                 uint32_t*, const uint32_t*, uint32_t*);
+                                 uint32_t*, const uint32_t*, const uint32_t*, uint32_t*);
 
 static camera_bounce_fn pick_camera_bounce(int wg, int lds, bool packed, bool staged)
 {
@@ -904,6 +905,8 @@ __global__ __launch_bounds__(256) void k_sched_place(const uint32_t* __restrict_
     __syncthreads();
     if (valid)
         sched[h[b] + rank] = g;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        bins[2 * kSchedBins] = units;  // the order's unit count (sched_units)
 }
 
 static inline uint32_t blocks_for(size_t n, uint32_t wg) { return (uint32_t)((n + wg - 1) / wg); }
@@ -1183,7 +1186,8 @@ static size_t camera_pass_layout(uint32_t n, bool staged, size_t* count_at, size
 static int32_t enqueue_camera_pass(bvht_ctx ctx, bvht_blas blas, const bvht_camera* cam, uint32_t width,
                                    uint32_t height, int32_t order, uint64_t seed, bvht_hit* d_hits, bvht_ray* d_out,
                                    uint32_t* d_src, uint32_t* d_count, int wg, int lds, bool staged, char* base,
-                                   uint32_t* spill, uint32_t need, const uint32_t* sched, uint32_t* cost)
+                                   uint32_t* spill, uint32_t need, const uint32_t* sched, const uint32_t* sched_n,
+                                   uint32_t* cost)
 {
     camera_bounce_fn fn = pick_camera_bounce(wg, lds, blas->packed_stack, staged);
     if (!fn)
@@ -1199,7 +1203,8 @@ static int32_t enqueue_camera_pass(bvht_ctx ctx, bvht_blas blas, const bvht_came
     uint32_t* sums = reinterpret_cast<uint32_t*>(base + sums_at);
     hipLaunchKernelGGL(fn, dim3(blocks_for(n, wg)), dim3(wg), 0, ctx->stream,
                        reinterpret_cast<const float4*>(blas->d_nodes), blas->d_tris, blas->n_tris, *cam, width, height,
-                       order, seed, d_hits, stg, key, counts, n_waves, spill, need, ctx->d_status, sched, cost);
+                       order, seed, d_hits, stg, key, counts, n_waves, spill, need, ctx->d_status, sched, sched_n,
+                       cost);
     launch_scan(ctx, counts, (uint32_t)entries, sums, d_count);
     if (staged)
         hipLaunchKernelGGL(k_bounce_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, stg, key, n,
@@ -1248,7 +1253,7 @@ int32_t launch_trace_camera_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_came
         ctx->cam_valid = false;
         if (hipMalloc(&ctx->cam_cost, sizeof(uint32_t) * groups) != hipSuccess ||
             hipMalloc(&ctx->cam_sched, sizeof(uint32_t) * groups) != hipSuccess ||
-            (!ctx->cam_bins && hipMalloc(&ctx->cam_bins, sizeof(uint32_t) * 2 * kSchedBins) != hipSuccess))
+            (!ctx->cam_bins && hipMalloc(&ctx->cam_bins, sizeof(uint32_t) * kSchedBinsWords) != hipSuccess))
             return set_error(BVHT_ENOMEM, "camera-pass wave schedule for %u groups", groups);
         ctx->cam_cap = groups;
     }
@@ -1256,7 +1261,7 @@ int32_t launch_trace_camera_bounce(bvht_ctx ctx, bvht_blas blas, const bvht_came
                 ctx->cam_order == order && std::memcmp(&ctx->cam_key, cam, sizeof *cam) == 0;
     rc = enqueue_camera_pass(ctx, blas, cam, width, height, order, seed, d_hits, d_out, d_src, d_count, wg, lds,
                              true, static_cast<char*>(ctx->scratch), spill, need, same ? ctx->cam_sched : nullptr,
-                             ctx->cam_cost);
+                             sched_units(ctx->cam_bins), ctx->cam_cost);
     if (rc)
         return rc;
     // the order changes little between passes over the same frame: rebuilt
@@ -1373,6 +1378,9 @@ int32_t bvht_destroy(bvht_ctx ctx)
     (void)hipFree(ctx->cam_cost);
     (void)hipFree(ctx->cam_sched);
     (void)hipFree(ctx->cam_bins);
+    (void)hipFree(ctx->tls_cost);
+    (void)hipFree(ctx->tls_sched);
+    (void)hipFree(ctx->tls_bins);
     if (ctx->h_status)
         (void)hipHostFree(ctx->h_status);
     release_build_ws(ctx);
@@ -1392,6 +1400,14 @@ int32_t bvht_set_stack_limit(bvht_ctx ctx, uint32_t max_entries)
 {
     BVHT_REQUIRE(ctx, "ctx is NULL");
     ctx->stack_limit = max_entries;
+    return BVHT_OK;
+}
+
+int32_t bvht_set_wave_schedule(bvht_ctx ctx, int32_t enable)
+{
+    BVHT_REQUIRE(ctx, "ctx is NULL");
+    ctx->tls_on = enable != 0;
+    ctx->tls_valid = false;  // a new order starts from the next call's durations
     return BVHT_OK;
 }
 
@@ -1697,7 +1713,7 @@ int32_t bvht_trace_camera_bounce_noalloc(bvht_ctx ctx, bvht_blas blas, const bvh
     }
     return bvht::enqueue_camera_pass(ctx, blas, cam, width, height, order, seed, d_hits, d_out, d_src, d_count, wg,
                                      lds, false, static_cast<char*>(ctx->noalloc_scratch), nullptr, need, nullptr,
-                                     nullptr);
+                                     nullptr, nullptr);
 }
 
 int32_t bvht_depth(bvht_ctx ctx, const bvht_hit* d_hits, uint32_t n, float* d_depth)

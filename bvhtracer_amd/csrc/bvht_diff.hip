@@ -40,7 +40,7 @@ struct bvht_mesh_s {
     // with the same ray count (the mesh changes little between steps).
     uint32_t* d_sched = nullptr;   // groups: group run by wave slot k
     uint32_t* d_cost = nullptr;    // groups: duration of each group's wave (10 ns ticks)
-    uint32_t* d_bins = nullptr;    // kSchedBins counts, then kSchedBins cursors
+    uint32_t* d_bins = nullptr;    // kSchedBins counts, kSchedBins cursors, the order's unit count
     uint32_t sched_cap = 0;        // groups the buffers hold
     uint32_t sched_n = 0;          // ray count d_sched was made for (0: none)
     // Adjoint gather.  Vertex -> (BLAS triangle, corner) entries 3k + c in
@@ -266,6 +266,7 @@ __global__ __launch_bounds__(WG) void k_trace_forward(const float4* __restrict__
                                                       uint32_t* __restrict__ spill, uint32_t cap,
                                                       uint32_t* __restrict__ status,
                                                       const uint32_t* __restrict__ sched,
+                                                      const uint32_t* __restrict__ sched_n,
                                                       uint32_t* __restrict__ cost)
 {
     __shared__ uint32_t lds[LDS_DEPTH * WG];
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(WG) void k_trace_forward(const float4* __restrict__
     uint32_t slot = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
     if (slot >= groups)
         return;
-    uint32_t group = sched ? sched[slot] : slot;
+    uint32_t group = (sched && *sched_n == groups) ? sched[slot] : slot;
     uint32_t i = group * 64 + (threadIdx.x & 63);
     if (i >= n)
         return;
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(WG) void k_trace_forward(const float4* __restrict__
 
 typedef void (*forward_fn)(const float4*, const float*, const uint32_t*, const float*, Rows, V3, const bvht_ray*,
                            uint32_t, bvht_hit*, float2*, uint32_t*, uint32_t, uint32_t*, const uint32_t*,
-                           uint32_t*);
+                           const uint32_t*, uint32_t*);
 
 forward_fn pick_forward(int wg, int lds, bool packed)
 {
@@ -967,7 +968,7 @@ extern "C" int32_t bvht_trace_forward(bvht_ctx ctx, bvht_mesh mesh, const float*
         mesh->sched_cap = mesh->sched_n = 0;
         if (hipMalloc(&mesh->d_sched, sizeof(uint32_t) * groups) != hipSuccess ||
             hipMalloc(&mesh->d_cost, sizeof(uint32_t) * groups) != hipSuccess ||
-            (!mesh->d_bins && hipMalloc(&mesh->d_bins, sizeof(uint32_t) * 2 * kSchedBins) != hipSuccess))
+            (!mesh->d_bins && hipMalloc(&mesh->d_bins, sizeof(uint32_t) * kSchedBinsWords) != hipSuccess))
             return set_error(BVHT_ENOMEM, "forward schedule for %u groups", groups);
         mesh->sched_cap = groups;
     }
@@ -976,7 +977,8 @@ extern "C" int32_t bvht_trace_forward(bvht_ctx ctx, bvht_mesh mesh, const float*
     hipLaunchKernelGGL(fn, dim3(nblk(groups, wg / 64)), dim3(wg), 0, ctx->stream,
                        reinterpret_cast<const float4*>(b->d_nodes), b->d_tris, mesh->d_vid, mesh->d_normals,
                        rows_of(minv), V3{light[0], light[1], light[2]}, d_rays, n, d_hits,
-                       reinterpret_cast<float2*>(d_image), spill, need, ctx->d_status, sched, mesh->d_cost);
+                       reinterpret_cast<float2*>(d_image), spill, need, ctx->d_status, sched,
+                       sched_units(mesh->d_bins), mesh->d_cost);
     // the schedule for the next forward: longest groups first
     int32_t rc = launch_longest_first(ctx, mesh->d_cost, groups, mesh->d_bins, mesh->d_sched);
     if (rc)

@@ -54,6 +54,19 @@ struct bvht_ctx_s {
     void* noalloc_scratch = nullptr;
     // BLAS build workspace (bvht_build.hip), kept between builds.
     void* build_ws = nullptr;
+    // Longest-first wave order of bvht_trace_tlas (bvht_set_wave_schedule):
+    // the last call's group durations and the order made from them, keyed by
+    // TLAS, ray buffer and ray count.
+    bool tls_on = false;
+    uint32_t* tls_cost = nullptr;
+    uint32_t* tls_sched = nullptr;
+    uint32_t* tls_bins = nullptr;
+    uint32_t tls_cap = 0;
+    bool tls_valid = false;
+    const void* tls_tlas = nullptr;
+    const void* tls_rays = nullptr;
+    uint32_t tls_n = 0;
+    uint32_t tls_calls = 0;  // calls since the key last changed
 };
 
 // Bits of the context status word.
@@ -140,8 +153,14 @@ int32_t fill_u32(bvht_ctx ctx, uint32_t* p, uint32_t v, size_t n);
 void launch_scan(bvht_ctx ctx, uint32_t* a, uint32_t m, uint32_t* sums, uint32_t* total);
 uint32_t scan_sums_entries(uint32_t m);
 // Longest first: d_sched[0, n) = the units ordered by descending d_cost (a
-// permutation of 0..n-1); d_bins holds 2 * kSchedBins counters.
+// permutation of 0..n-1); d_bins holds kSchedBinsWords words: 2 * kSchedBins
+// counters, then the unit count the order in d_sched was made for.  A kernel
+// takes d_sched only where that count equals its own: an order made for
+// another count (an eager call of another size between the replays of a
+// graph that captured a scheduled call) is not a permutation of its groups.
 constexpr uint32_t kSchedBins = 512;
+constexpr uint32_t kSchedBinsWords = 2 * kSchedBins + 1;
+inline const uint32_t* sched_units(const uint32_t* d_bins) { return d_bins + 2 * kSchedBins; }
 int32_t launch_longest_first(bvht_ctx ctx, const uint32_t* d_cost, uint32_t n, uint32_t* d_bins, uint32_t* d_sched);
 void trace_config(bvht_ctx ctx, uint32_t need, int* wg, int* lds);
 int32_t launch_bounce_sorted(bvht_ctx ctx, const float* d_tris, uint32_t ntris, const void* rows, const void* blases,

@@ -45,10 +45,20 @@ __global__ __launch_bounds__(WG) void k_trace_tlas(const float4* __restrict__ tn
                                                    const bvht_ray* __restrict__ rays, bvht_hit* __restrict__ hits,
                                                    uint32_t n, const uint32_t* __restrict__ d_n,
                                                    bvht_stats* __restrict__ stats, uint32_t* __restrict__ spill,
-                                                   uint32_t cap, uint32_t* __restrict__ status)
+                                                   uint32_t cap, uint32_t* __restrict__ status,
+                                                   const uint32_t* __restrict__ sched,
+                                                   const uint32_t* __restrict__ sched_n, uint32_t* __restrict__ cost)
 {
     __shared__ uint32_t lds[LDS_DEPTH * WG];
-    uint32_t i = blockIdx.x * WG + threadIdx.x;
+    // 64-ray group of this wave: slot order, or the order in `sched` (a
+    // permutation of the groups); with `cost`, lane 0 records the group's
+    // duration for the next call's order
+    const uint64_t t0 = cost ? (uint64_t)wall_clock64() : 0;
+    const uint32_t n_groups = (n + 63) / 64;
+    const uint32_t slot = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+    const uint32_t group = (sched && slot < n_groups && *sched_n == n_groups) ? sched[slot] : slot;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t i = group * 64 + lane;
     uint32_t count = d_n ? min(*d_n, n) : n;
     if (i >= count)
         return;
@@ -69,10 +79,15 @@ __global__ __launch_bounds__(WG) void k_trace_tlas(const float4* __restrict__ tn
         s.inst_visits = visits;
         stats[i] = s;
     }
+    if (cost && lane == 0) {
+        uint64_t dt = (uint64_t)wall_clock64() - t0;
+        cost[group] = dt > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)dt;
+    }
 }
 
 typedef void (*trace_tlas_fn)(const float4*, const ObjRows*, const BlasRef*, const bvht_ray*, bvht_hit*, uint32_t,
-                              const uint32_t*, bvht_stats*, uint32_t*, uint32_t, uint32_t*);
+                              const uint32_t*, bvht_stats*, uint32_t*, uint32_t, uint32_t*, const uint32_t*,
+                              const uint32_t*, uint32_t*);
 
 trace_tlas_fn pick_trace_tlas(int wg, int lds, bool stats, bool packed)
 {
@@ -202,10 +217,57 @@ int32_t launch_trace_tlas(bvht_tlas t, const bvht_ray* d_rays, uint32_t n, const
     trace_tlas_fn fn = pick_trace_tlas(wg, lds, d_stats != nullptr, packed);
     if (!fn)
         return set_error(BVHT_EINVAL, "no TLAS trace kernel for workgroup %d, LDS depth %d", wg, lds);
+    // longest-first order from the last call over the same TLAS, rays and
+    // count (bvht_set_wave_schedule)
+    const uint32_t* sched = nullptr;
+    uint32_t* cost = nullptr;
+    uint32_t groups = (n + 63) / 64;
+    bool scheduled = ctx->tls_on && d_n == nullptr && d_stats == nullptr;
+    if (scheduled) {
+        if (groups > ctx->tls_cap) {
+            if (ctx->live_graphs > 0 || ctx->capturing)
+                return set_error(BVHT_EINVAL,
+                                 "the TLAS wave schedule must grow to %u groups, but a graph captured on this context "
+                                 "holds its address: make one eager call of the largest ray count first",
+                                 groups);
+            BVHT_HIP(hipStreamSynchronize(ctx->stream));
+            (void)hipFree(ctx->tls_cost);
+            (void)hipFree(ctx->tls_sched);
+            ctx->tls_cost = ctx->tls_sched = nullptr;
+            ctx->tls_cap = 0;
+            ctx->tls_valid = false;
+            if (hipMalloc(&ctx->tls_cost, sizeof(uint32_t) * groups) != hipSuccess ||
+                hipMalloc(&ctx->tls_sched, sizeof(uint32_t) * groups) != hipSuccess ||
+                (!ctx->tls_bins && hipMalloc(&ctx->tls_bins, sizeof(uint32_t) * kSchedBinsWords) != hipSuccess))
+                return set_error(BVHT_ENOMEM, "TLAS wave schedule for %u groups", groups);
+            ctx->tls_cap = groups;
+        }
+        bool same = ctx->tls_valid && ctx->tls_tlas == t && ctx->tls_rays == d_rays && ctx->tls_n == n;
+        sched = same ? ctx->tls_sched : nullptr;
+        cost = ctx->tls_cost;
+        ctx->tls_calls = same ? ctx->tls_calls + 1 : 0;
+    }
+    // the group mapping is the same with or without a schedule; a launch
+    // has one wave per 64-ray group
     hipLaunchKernelGGL(fn, dim3(nblk(n, wg)), dim3(wg), 0, ctx->stream, reinterpret_cast<const float4*>(t->d_nodes),
                        static_cast<const ObjRows*>(t->d_rows), static_cast<const BlasRef*>(t->d_blas_table), d_rays,
-                       d_hits, n, d_n, d_stats, spill, need, ctx->d_status);
+                       d_hits, n, d_n, d_stats, spill, need, ctx->d_status, sched,
+                       scheduled ? sched_units(ctx->tls_bins) : nullptr, cost);
     BVHT_HIP(hipGetLastError());
+    if (scheduled) {
+        // rebuilt after the first call and then every 8 calls, and in every
+        // captured call (a graph replays what it captured)
+        constexpr uint32_t kResched = 8;
+        if (ctx->tls_calls % kResched == 0 || ctx->capturing) {
+            int32_t rc = launch_longest_first(ctx, ctx->tls_cost, groups, ctx->tls_bins, ctx->tls_sched);
+            if (rc)
+                return rc;
+        }
+        ctx->tls_valid = true;
+        ctx->tls_tlas = t;
+        ctx->tls_rays = d_rays;
+        ctx->tls_n = n;
+    }
     return BVHT_OK;
 }
 

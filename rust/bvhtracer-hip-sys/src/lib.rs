@@ -207,6 +207,7 @@ pub fn bvht_create(device: i32, out: *mut bvht_ctx) -> i32;
     pub fn bvht_set_trace_config(ctx: bvht_ctx, wg: i32, lds_depth: i32) -> i32;
     pub fn bvht_get_trace_config(ctx: bvht_ctx, wg: *mut i32, lds_depth: *mut i32) -> i32;
     pub fn bvht_set_stack_limit(ctx: bvht_ctx, max_entries: u32) -> i32;
+    pub fn bvht_set_wave_schedule(ctx: bvht_ctx, enable: i32) -> i32;
     pub fn bvht_raygen(ctx: bvht_ctx, cam: *const bvht_camera, width: u32, height: u32, order: i32,
         d_rays: *mut bvht_ray) -> i32;
     pub fn bvht_bounce(ctx: bvht_ctx, blas: bvht_blas, d_rays: *const bvht_ray, d_hits: *const bvht_hit, n: u32,

@@ -132,6 +132,37 @@ def test_forward_wave_schedule_keeps_results(ctx, scene):
         [a.download(np.uint8).tobytes() for a in forward(ctx, scene, twin)]
 
 
+def test_forward_schedule_of_another_count_between_graph_replays(ctx, scene):
+    """A graph captures a scheduled forward of NRAY rays; between its
+    replays an eager forward of another ray count rebuilds the order for its
+    own count.  A replay then runs its groups in slot order (the order's
+    recorded unit count differs from its own), and every replay equals the
+    forward made eagerly, bit for bit."""
+    w = scene["w"]
+    mesh, _ = make_mesh(ctx, w["V"], w["F"])
+    want = [a.download(np.uint8).tobytes() for a in forward(ctx, scene, mesh)]
+    forward(ctx, scene, mesh)  # the order now exists for NRAY rays
+    d_hits, d_img = ctx.alloc(NRAY * B.HIT_DTYPE.itemsize), ctx.alloc(NRAY * 8)
+
+    def enqueue():
+        B.trace_forward(mesh, w["Minv"], w["light"], scene["d_rays"], NRAY, d_hits, d_img)
+
+    g = B.Graph.capture(ctx, enqueue, keep=(d_hits, d_img))
+    small = NRAY // 4 + 5
+    d_h, d_i = ctx.alloc(small * B.HIT_DTYPE.itemsize), ctx.alloc(small * 8)
+    try:
+        for _ in range(3):
+            B.trace_forward(mesh, w["Minv"], w["light"], scene["d_rays"], small, d_h, d_i)
+            B.trace_forward(mesh, w["Minv"], w["light"], scene["d_rays"], small, d_h, d_i)
+            d_hits.upload(np.zeros(NRAY, B.HIT_DTYPE))
+            g.launch()
+            ctx.sync()
+            assert [d_hits.download(np.uint8).tobytes(), d_img.download(np.uint8).tobytes()] == want
+            assert d_h.download(np.uint8).tobytes() == want[0][:small * B.HIT_DTYPE.itemsize]
+    finally:
+        g.close()
+
+
 def gradient_errors(gv, gm, ref_gv, ref_gm):
     """Per-vertex error on the vertex's own scale, |g - g64| / max(|g64|,
     1e-6 max |g64|) (vector norms per vertex; the floor only matters for

@@ -135,3 +135,41 @@ def test_batch_rebuild_leaves_a_sufficient_stack_bound(ctx, orc):
     assert_hits_equal(got, ref, "per-scene trace after the batched rebuild")
     assert (got["prim"] != B.MISS).sum() > 500
     batch.close()
+
+
+def test_camera_schedule_of_another_size_between_graph_replays():
+    """A graph captures a scheduled 128x128 camera pass; between its replays
+    an eager 64x48 pass rebuilds the context's wave order for its own size.
+    Each replay then runs its groups in slot order and equals the eager
+    128x128 pass bit for bit (hits, bounce rays and their sources)."""
+    ctx = B.Context(0)
+    try:
+        mesh = scenes.icosphere(16, radius=3.0, seed=5)
+        bvh = B.BvhBuilder(B.BUILD_SAH, 2, ctx=ctx).build_for(mesh.copy())
+        cam = scenes.camera(eye=(0, 0, -12), forward=(0, 0, 1), right=(1, 0, 0), up=(0, 1, 0), fovy_deg=40)
+        bufs = B.trace_camera_bounce(ctx, bvh, cam, 128, 128, B.ORDER_TILES, 3)
+        B.trace_camera_bounce(ctx, bvh, cam, 128, 128, B.ORDER_TILES, 3, *bufs)  # scheduled from now on
+
+        def snap():
+            nb = int(bufs[3].download(np.uint32, 1)[0])
+            return (bufs[0].download(B.HIT_DTYPE, 128 * 128).tobytes(), bufs[1].download(B.RAY_DTYPE, nb).tobytes(),
+                    bufs[2].download(np.uint32, nb).tobytes())
+
+        want = snap()
+
+        def enqueue():
+            B.trace_camera_bounce(ctx, bvh, cam, 128, 128, B.ORDER_TILES, 3, *bufs)
+
+        g = B.Graph.capture(ctx, enqueue)
+        small = [ctx.alloc(64 * 48 * B.HIT_DTYPE.itemsize), ctx.alloc(64 * 48 * B.RAY_DTYPE.itemsize),
+                 ctx.alloc(64 * 48 * 4), ctx.alloc(4)]
+        for _ in range(3):
+            for _ in range(2):
+                B.trace_camera_bounce(ctx, bvh, cam, 64, 48, B.ORDER_TILES, 3, *small)
+            bufs[0].upload(np.zeros(128 * 128, B.HIT_DTYPE))
+            g.launch()
+            ctx.sync()
+            assert snap() == want
+        g.close()
+    finally:
+        ctx.close()

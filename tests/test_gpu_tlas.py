@@ -287,3 +287,64 @@ def test_tlas_rebuild_paths_equal_hand_trace(ctx, orc, name):
         assert h[0]["inst"] == case["hit_inst"]
         assert st[0]["tlas_pairs"] == case["stats"]["tlas_pairs"]
         assert st[0]["inst_visits"] == case["stats"]["inst_visits"]
+
+
+def test_tlas_wave_schedule_keeps_hits(orc):
+    """With bvht_set_wave_schedule on, repeated traces of the same rays run
+    their 64-ray groups longest first from the second call (the order is
+    rebuilt every 8 calls); every call equals the unscheduled trace and the
+    oracle bit for bit, a different ray buffer in between included, and so
+    does a captured trace replayed with eager work between the replays."""
+    ctx = B.Context(0)
+    try:
+        models, blases, objs = build_scene(ctx, orc, 16, seed=7, freq=14)
+        tlas = B.TlasBuilder(ctx=ctx).build_for(objs, models)
+        cam = scenes.camera(eye=(0, 0, -45), forward=(0, 0, 1), right=(1, 0, 0), up=(0, 1, 0), fovy_deg=45,
+                            aspect=16 / 9)
+        W, H = 320, 184
+        n = W * H
+        d_rays = B.raygen(ctx, cam, W, H, B.ORDER_TILES)
+        d_hits = ctx.alloc(n * B.HIT_DTYPE.itemsize)
+        tlas.intersect_device(d_rays, n, d_hits)
+        want = d_hits.download(B.HIT_DTYPE, n)
+        rays = d_rays.download(B.RAY_DTYPE, n)
+        tnodes, _ = oracle_tlas(orc, blases, objs)
+        ref = orc.tlas_trace(tnodes, oracle_instances(orc, objs), blases, rays.view(orc.RAY_DTYPE), nthreads=8)
+        assert_hits_equal(want, ref, "unscheduled")
+        assert (want["prim"] != B.MISS).sum() > 1000
+        other = rays.copy()
+        other["o"][:, 0] += 0.25
+        d_other = ctx.to_device(other)
+        want_other = tlas.intersect(other)
+        ctx.set_wave_schedule(True)
+        for call in range(12):
+            d_hits.upload(np.zeros(n, B.HIT_DTYPE))
+            tlas.intersect_device(d_rays, n, d_hits)
+            assert d_hits.download(B.HIT_DTYPE, n).tobytes() == want.tobytes(), call
+            if call == 5:  # another buffer breaks the key; the next call starts over
+                d_o = ctx.alloc(n * B.HIT_DTYPE.itemsize)
+                tlas.intersect_device(d_other, n, d_o)
+                assert d_o.download(B.HIT_DTYPE, n).tobytes() == want_other.tobytes()
+
+        def enqueue():
+            tlas.intersect_device(d_rays, n, d_hits)
+
+        g = B.Graph.capture(ctx, enqueue, keep=(d_rays, d_hits))
+        for _ in range(4):
+            d_hits.upload(np.zeros(n, B.HIT_DTYPE))
+            g.launch()
+            ctx.sync()
+            assert d_hits.download(B.HIT_DTYPE, n).tobytes() == want.tobytes()
+            tlas.intersect_device(d_other, n, d_o)  # eager work between the replays
+            assert d_o.download(B.HIT_DTYPE, n).tobytes() == want_other.tobytes()
+            # an eager scheduled trace of another count rebuilds the order for
+            # its own groups; the next replay must not take it
+            for _ in range(2):
+                tlas.intersect_device(d_rays, n // 3 + 7, d_o)
+            assert d_o.download(B.HIT_DTYPE, n // 3 + 7).tobytes() == want[: n // 3 + 7].tobytes()
+        g.close()
+        ctx.set_wave_schedule(False)
+        tlas.intersect_device(d_rays, n, d_hits)
+        assert d_hits.download(B.HIT_DTYPE, n).tobytes() == want.tobytes()
+    finally:
+        ctx.close()
