@@ -295,10 +295,20 @@ __global__ __launch_bounds__(WG) void k_trace_blas(const float4* __restrict__ no
                                                    const bvht_ray* __restrict__ rays, bvht_hit* __restrict__ hits,
                                                    uint32_t n, const uint32_t* __restrict__ d_n,
                                                    bvht_stats* __restrict__ stats, uint32_t* __restrict__ spill,
-                                                   uint32_t cap, uint32_t* __restrict__ status)
+                                                   uint32_t cap, uint32_t* __restrict__ status,
+                                                   const uint32_t* __restrict__ sched,
+                                                   const uint32_t* __restrict__ sched_n, uint32_t* __restrict__ cost)
 {
     __shared__ uint32_t lds[LDS_DEPTH * WG];
-    uint32_t i = blockIdx.x * WG + threadIdx.x;
+    // 64-ray group of this wave: slot order, or the order in `sched` when it
+    // was made for this many groups; with `cost`, lane 0 records the group's
+    // duration (bvht_set_wave_schedule)
+    const uint64_t t0 = cost ? (uint64_t)wall_clock64() : 0;
+    const uint32_t n_groups = (n + 63) / 64;
+    const uint32_t slot = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+    const uint32_t group = (sched && slot < n_groups && *sched_n == n_groups) ? sched[slot] : slot;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t i = group * 64 + lane;
     uint32_t count = d_n ? min(*d_n, n) : n;
     if (i >= count)
         return;
@@ -318,10 +328,15 @@ __global__ __launch_bounds__(WG) void k_trace_blas(const float4* __restrict__ no
         s.inst_visits = 0;
         stats[i] = s;
     }
+    if (cost && lane == 0) {
+        uint64_t dt = (uint64_t)wall_clock64() - t0;
+        cost[group] = dt > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)dt;
+    }
 }
 
 typedef void (*trace_blas_fn)(const float4*, const float*, const bvht_ray*, bvht_hit*, uint32_t, const uint32_t*,
-                              bvht_stats*, uint32_t*, uint32_t, uint32_t*);
+                              bvht_stats*, uint32_t*, uint32_t, uint32_t*, const uint32_t*, const uint32_t*,
+                              uint32_t*);
 
 static trace_blas_fn pick_trace_blas(int wg, int lds, bool stats, bool packed)
 {
@@ -911,6 +926,59 @@ __global__ __launch_bounds__(256) void k_sched_place(const uint32_t* __restrict_
 
 static inline uint32_t blocks_for(size_t n, uint32_t wg) { return (uint32_t)((n + wg - 1) / wg); }
 
+int32_t trace_order_begin(bvht_ctx ctx, const void* tree, const void* rays, uint32_t n, bool eligible, TraceOrder* o)
+{
+    *o = TraceOrder{};
+    if (!ctx->tls_on || !eligible || n == 0)
+        return BVHT_OK;
+    uint32_t groups = (n + 63) / 64;
+    if (groups > ctx->tls_cap) {
+        if (ctx->live_graphs > 0 || ctx->capturing)
+            return set_error(BVHT_EINVAL,
+                             "the wave order must grow to %u groups, but a graph captured on this context holds its "
+                             "address: make one eager call of the largest ray count first",
+                             groups);
+        BVHT_HIP(hipStreamSynchronize(ctx->stream));
+        (void)hipFree(ctx->tls_cost);
+        (void)hipFree(ctx->tls_sched);
+        ctx->tls_cost = ctx->tls_sched = nullptr;
+        ctx->tls_cap = 0;
+        ctx->tls_valid = false;
+        if (hipMalloc(&ctx->tls_cost, sizeof(uint32_t) * groups) != hipSuccess ||
+            hipMalloc(&ctx->tls_sched, sizeof(uint32_t) * groups) != hipSuccess ||
+            (!ctx->tls_bins && hipMalloc(&ctx->tls_bins, sizeof(uint32_t) * kSchedBinsWords) != hipSuccess))
+            return set_error(BVHT_ENOMEM, "wave order for %u groups", groups);
+        ctx->tls_cap = groups;
+    }
+    bool same = ctx->tls_valid && ctx->tls_tlas == tree && ctx->tls_rays == rays && ctx->tls_n == n;
+    o->sched = same ? ctx->tls_sched : nullptr;
+    o->sched_n = sched_units(ctx->tls_bins);
+    o->cost = ctx->tls_cost;
+    o->groups = groups;
+    o->on = true;
+    ctx->tls_calls = same ? ctx->tls_calls + 1 : 0;
+    return BVHT_OK;
+}
+
+int32_t trace_order_end(bvht_ctx ctx, const void* tree, const void* rays, uint32_t n, const TraceOrder& o)
+{
+    if (!o.on)
+        return BVHT_OK;
+    // rebuilt after the first call and then every 8 calls, and in every
+    // captured call (a graph replays what it captured)
+    constexpr uint32_t kResched = 8;
+    if (ctx->tls_calls % kResched == 0 || ctx->capturing) {
+        int32_t rc = launch_longest_first(ctx, ctx->tls_cost, o.groups, ctx->tls_bins, ctx->tls_sched);
+        if (rc)
+            return rc;
+    }
+    ctx->tls_valid = true;
+    ctx->tls_tlas = tree;
+    ctx->tls_rays = rays;
+    ctx->tls_n = n;
+    return BVHT_OK;
+}
+
 int32_t launch_longest_first(bvht_ctx ctx, const uint32_t* d_cost, uint32_t n, uint32_t* d_bins, uint32_t* d_sched)
 {
     if (n == 0)
@@ -1141,11 +1209,15 @@ int32_t launch_trace_blas(bvht_ctx ctx, bvht_blas blas, const bvht_ray* d_rays, 
     trace_blas_fn fn = pick_trace_blas(wg, lds, d_stats != nullptr, blas->packed_stack);
     if (!fn)
         return set_error(BVHT_EINVAL, "no trace kernel for workgroup %d, LDS depth %d", wg, lds);
+    TraceOrder o;
+    int32_t rc = trace_order_begin(ctx, blas, d_rays, n, d_n == nullptr && d_stats == nullptr, &o);
+    if (rc)
+        return rc;
     hipLaunchKernelGGL(fn, dim3(blocks_for(n, wg)), dim3(wg), 0, ctx->stream,
                        reinterpret_cast<const float4*>(blas->d_nodes), blas->d_tris, d_rays, d_hits, n, d_n, d_stats,
-                       spill, need, ctx->d_status);
+                       spill, need, ctx->d_status, o.sched, o.sched_n, o.cost);
     BVHT_HIP(hipGetLastError());
-    return BVHT_OK;
+    return trace_order_end(ctx, blas, d_rays, n, o);
 }
 
 // Per-256-ray-block exclusive offsets of the hits (in ctx scratch) and the

@@ -162,6 +162,22 @@ constexpr uint32_t kSchedBins = 512;
 constexpr uint32_t kSchedBinsWords = 2 * kSchedBins + 1;
 inline const uint32_t* sched_units(const uint32_t* d_bins) { return d_bins + 2 * kSchedBins; }
 int32_t launch_longest_first(bvht_ctx ctx, const uint32_t* d_cost, uint32_t n, uint32_t* d_bins, uint32_t* d_sched);
+// The opt-in wave order of bvht_trace_blas / bvht_trace_tlas
+// (bvht_set_wave_schedule), keyed by tree, ray buffer and ray count.
+// trace_order_begin gives the launch its order (NULL when none applies),
+// the order's unit count and the duration buffer (both NULL when the order
+// is off for this call); trace_order_end, after the launch, rebuilds the
+// order when due.
+struct TraceOrder {
+    const uint32_t* sched = nullptr;
+    const uint32_t* sched_n = nullptr;
+    uint32_t* cost = nullptr;
+    uint32_t groups = 0;
+    bool on = false;
+};
+int32_t trace_order_begin(bvht_ctx ctx, const void* tree, const void* rays, uint32_t n, bool eligible,
+                          TraceOrder* o);
+int32_t trace_order_end(bvht_ctx ctx, const void* tree, const void* rays, uint32_t n, const TraceOrder& o);
 void trace_config(bvht_ctx ctx, uint32_t need, int* wg, int* lds);
 int32_t launch_bounce_sorted(bvht_ctx ctx, const float* d_tris, uint32_t ntris, const void* rows, const void* blases,
                              uint32_t n_obj, const bvht_ray* d_rays, const bvht_hit* d_hits, uint32_t n,

@@ -219,56 +219,15 @@ int32_t launch_trace_tlas(bvht_tlas t, const bvht_ray* d_rays, uint32_t n, const
         return set_error(BVHT_EINVAL, "no TLAS trace kernel for workgroup %d, LDS depth %d", wg, lds);
     // longest-first order from the last call over the same TLAS, rays and
     // count (bvht_set_wave_schedule)
-    const uint32_t* sched = nullptr;
-    uint32_t* cost = nullptr;
-    uint32_t groups = (n + 63) / 64;
-    bool scheduled = ctx->tls_on && d_n == nullptr && d_stats == nullptr;
-    if (scheduled) {
-        if (groups > ctx->tls_cap) {
-            if (ctx->live_graphs > 0 || ctx->capturing)
-                return set_error(BVHT_EINVAL,
-                                 "the TLAS wave schedule must grow to %u groups, but a graph captured on this context "
-                                 "holds its address: make one eager call of the largest ray count first",
-                                 groups);
-            BVHT_HIP(hipStreamSynchronize(ctx->stream));
-            (void)hipFree(ctx->tls_cost);
-            (void)hipFree(ctx->tls_sched);
-            ctx->tls_cost = ctx->tls_sched = nullptr;
-            ctx->tls_cap = 0;
-            ctx->tls_valid = false;
-            if (hipMalloc(&ctx->tls_cost, sizeof(uint32_t) * groups) != hipSuccess ||
-                hipMalloc(&ctx->tls_sched, sizeof(uint32_t) * groups) != hipSuccess ||
-                (!ctx->tls_bins && hipMalloc(&ctx->tls_bins, sizeof(uint32_t) * kSchedBinsWords) != hipSuccess))
-                return set_error(BVHT_ENOMEM, "TLAS wave schedule for %u groups", groups);
-            ctx->tls_cap = groups;
-        }
-        bool same = ctx->tls_valid && ctx->tls_tlas == t && ctx->tls_rays == d_rays && ctx->tls_n == n;
-        sched = same ? ctx->tls_sched : nullptr;
-        cost = ctx->tls_cost;
-        ctx->tls_calls = same ? ctx->tls_calls + 1 : 0;
-    }
-    // the group mapping is the same with or without a schedule; a launch
-    // has one wave per 64-ray group
+    TraceOrder o;
+    int32_t rc = trace_order_begin(ctx, t, d_rays, n, d_n == nullptr && d_stats == nullptr, &o);
+    if (rc)
+        return rc;
     hipLaunchKernelGGL(fn, dim3(nblk(n, wg)), dim3(wg), 0, ctx->stream, reinterpret_cast<const float4*>(t->d_nodes),
                        static_cast<const ObjRows*>(t->d_rows), static_cast<const BlasRef*>(t->d_blas_table), d_rays,
-                       d_hits, n, d_n, d_stats, spill, need, ctx->d_status, sched,
-                       scheduled ? sched_units(ctx->tls_bins) : nullptr, cost);
+                       d_hits, n, d_n, d_stats, spill, need, ctx->d_status, o.sched, o.sched_n, o.cost);
     BVHT_HIP(hipGetLastError());
-    if (scheduled) {
-        // rebuilt after the first call and then every 8 calls, and in every
-        // captured call (a graph replays what it captured)
-        constexpr uint32_t kResched = 8;
-        if (ctx->tls_calls % kResched == 0 || ctx->capturing) {
-            int32_t rc = launch_longest_first(ctx, ctx->tls_cost, groups, ctx->tls_bins, ctx->tls_sched);
-            if (rc)
-                return rc;
-        }
-        ctx->tls_valid = true;
-        ctx->tls_tlas = t;
-        ctx->tls_rays = d_rays;
-        ctx->tls_n = n;
-    }
-    return BVHT_OK;
+    return trace_order_end(ctx, t, d_rays, n, o);
 }
 
 }  // namespace

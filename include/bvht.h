@@ -233,17 +233,17 @@ int32_t bvht_get_trace_config(bvht_ctx ctx, int32_t* wg, int32_t* lds_depth);
  * bounds) and the next fence returns BVHT_ESTACK. */
 int32_t bvht_set_stack_limit(bvht_ctx ctx, uint32_t max_entries);
 
-/* Longest-first wave order for repeated TLAS traces (no reference
- * counterpart).  With enable != 0, each bvht_trace_tlas call on this context
- * records the duration of every 64-ray group, and a later call with the same
- * TLAS, ray buffer and ray count starts the groups longest first (the order
- * is rebuilt after the first such call and then every 8 calls).  That pays
- * off when consecutive calls trace nearly the same rays (a fixed camera over
- * a slowly changing scene); rays that change from call to call (bounce rays)
- * gain nothing.  Hits are the same either way.  Off by default;
- * bvht_trace_tlas_count and traces that collect statistics are never
- * reordered.  Turning it on while a graph captured on the context lives is
- * refused (BVHT_EINVAL) when the order buffers would have to grow. */
+/* Longest-first wave order for repeated traces (no reference counterpart).
+ * With enable != 0, each bvht_trace_blas / bvht_trace_tlas call on this
+ * context records the duration of every 64-ray group, and a later call with
+ * the same tree, ray buffer and ray count starts the groups longest first
+ * (the order is rebuilt after the first such call and then every 8 calls).
+ * That pays off when consecutive calls trace nearly the same rays (a fixed
+ * camera over a slowly changing scene); rays that change from call to call
+ * (bounce rays) gain nothing.  Hits are the same either way.  Off by
+ * default; the _count variants and traces that collect statistics are never
+ * reordered.  A call that would have to grow the order buffers while a graph
+ * captured on the context lives is refused (BVHT_EINVAL). */
 int32_t bvht_set_wave_schedule(bvht_ctx ctx, int32_t enable);
 
 /* ---- frame loop ------------------------------------------------------- */

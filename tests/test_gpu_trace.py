@@ -430,3 +430,50 @@ def test_camera_pass_wave_schedule_keeps_results(ctx):
             fresh.close()
         assert run(ctx, camera, seed) == want, (seed,)
 
+
+
+def test_blas_wave_schedule_keeps_hits(orc):
+    """bvht_set_wave_schedule on bvht_trace_blas: repeated traces of the same
+    camera rays (ordered longest first from the second call), traces of other
+    buffers and counts in between, and a captured trace replayed after eager
+    calls of another count all give the unscheduled hits bit for bit, which
+    equal the oracle's."""
+    ctx = B.Context(0)
+    try:
+        mesh = scenes.icosphere(48, radius=32.0)
+        m = mesh.copy()
+        bvh = B.BvhBuilder(B.BUILD_SAH, 2, ctx=ctx).build_for(m)
+        nodes, used, tris = bvh.download()
+        W, H = 640, 360
+        n = W * H
+        cam = scenes.sphere_camera(32.0, W, H)
+        d_rays = B.raygen(ctx, cam, W, H, B.ORDER_TILES)
+        rays = d_rays.download(B.RAY_DTYPE, n)
+        want = bvh.intersect(rays)
+        assert_hits_equal(want, orc.bvh_trace(nodes, tris, rays.view(orc.RAY_DTYPE), nthreads=8), "unscheduled")
+        d_hits = ctx.alloc(n * B.HIT_DTYPE.itemsize)
+        d_part = ctx.alloc(n * B.HIT_DTYPE.itemsize)
+        ctx.set_wave_schedule(True)
+        for call in range(10):
+            d_hits.upload(np.zeros(n, B.HIT_DTYPE))
+            bvh.intersect_device(d_rays, n, d_hits)
+            assert d_hits.download(B.HIT_DTYPE, n).tobytes() == want.tobytes(), call
+            if call in (3, 6):
+                part = n // 5 + 3
+                bvh.intersect_device(d_rays, part, d_part)
+                assert d_part.download(B.HIT_DTYPE, part).tobytes() == want[:part].tobytes()
+
+        def enqueue():
+            bvh.intersect_device(d_rays, n, d_hits)
+
+        g = B.Graph.capture(ctx, enqueue, keep=(d_rays, d_hits))
+        for _ in range(3):
+            for _ in range(2):
+                bvh.intersect_device(d_rays, n // 2 + 1, d_part)
+            d_hits.upload(np.zeros(n, B.HIT_DTYPE))
+            g.launch()
+            ctx.sync()
+            assert d_hits.download(B.HIT_DTYPE, n).tobytes() == want.tobytes()
+        g.close()
+    finally:
+        ctx.close()
