@@ -33,7 +33,35 @@ def meshes():
     same = np.tile(unit_triangle(), (40, 1))  # identical centroids: no usable axis
     flat = scenes.triangle_soup(300, seed=8)
     flat[:, 2::3] = 0.0  # every z is 0: the z axis is skipped
+    # non-finite coordinates: NaN, +-inf and f32::MAX vertices among finite ones
+    nonfinite = scenes.triangle_soup(300, seed=9)
+    nonfinite[3, 0] = np.nan
+    nonfinite[40, 4:6] = np.nan
+    nonfinite[77, 8] = np.inf
+    nonfinite[120, 1] = -np.inf
+    nonfinite[121, 3:6] = np.float32(3.4028235e38)
+    nonfinite[200] = np.nan
+    # 60 tight clusters of 17..45 triangles far apart: subtrees of either
+    # side of the 32-element boundary between the level passes and the wave
+    # builder
+    rng = np.random.default_rng(11)
+    clusters = np.concatenate([
+        (rng.random((int(k), 9), dtype=np.float32) * np.float32(0.5)
+         + np.tile(rng.random(3, dtype=np.float32) * np.float32(400), 3))
+        for k in rng.integers(17, 46, 60)]).astype(np.float32)
+    # 5000 coplanar triangles on a coarse grid: many equal centroids
+    g = np.arange(5000, dtype=np.float32)
+    plane = np.zeros((5000, 9), np.float32)
+    plane[:, 0] = (g % 50) // 5
+    plane[:, 1] = g // 500
+    plane[:, 3] = plane[:, 0] + 1
+    plane[:, 4] = plane[:, 1]
+    plane[:, 6] = plane[:, 0]
+    plane[:, 7] = plane[:, 1] + 1
     return {
+        "nonfinite": nonfinite,
+        "clusters": clusters,
+        "plane": plane,
         "one": unit_triangle().reshape(1, 9),
         "two": np.stack([(base - [5, 0, 0]).reshape(9), (base + [5, 0, 0]).reshape(9)]).astype(np.float32),
         "cube": cube_mesh(),
