@@ -77,6 +77,14 @@ public:
     Context& operator=(const Context&) = delete;
     bvht_ctx handle() const { return ctx_; }
     void sync() const { check(bvht_sync(ctx_)); }
+    /* Trace kernel configuration (workgroup x LDS stack entries; wg = 0
+     * restores the depth-based default), bvht_set_trace_config. */
+    void set_trace_config(int32_t wg, int32_t lds_depth) const { check(bvht_set_trace_config(ctx_, wg, lds_depth)); }
+    /* Bound on the traversal stack (0: the tree depth), bvht_set_stack_limit. */
+    void set_stack_limit(uint32_t max_entries) const { check(bvht_set_stack_limit(ctx_, max_entries)); }
+    /* Longest-first wave order for repeated traces of the same device rays,
+     * bvht_set_wave_schedule; hits are the same either way. */
+    void set_wave_schedule(bool enable) const { check(bvht_set_wave_schedule(ctx_, enable ? 1 : 0)); }
 
 private:
     bvht_ctx ctx_ = nullptr;

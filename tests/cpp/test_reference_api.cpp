@@ -220,6 +220,29 @@ TEST(test_intersection_all_primitives_along_ray_should_hit)  // :79-93
         CHECK(h.prim != BVHT_MISS);
 }
 
+TEST(test_context_options_keep_hits)  // no reference counterpart: trace configuration, wave order
+{
+    ModelInstance scene = stack_scene();
+    const auto& prims = scene.model()->primitives();
+    std::vector<Ray> rays;
+    for (int k = 0; k < 300; ++k)
+        rays.push_back(ray_towards({0.01f * (float)(k % 17) - 0.08f, 0.01f * (float)(k / 17) - 0.08f, 5},
+                                   unit_triangle().centroid()));
+    const auto& bvh = scene.model()->bvh();
+    auto want = bvh.intersect_batch(rays);
+    const auto& ctx = bvh.context();
+    ctx->set_trace_config(128, 8);
+    ctx->set_wave_schedule(true);
+    for (int rep = 0; rep < 3; ++rep) {
+        auto got = bvh.intersect_batch(rays);
+        for (size_t i = 0; i < rays.size(); ++i)
+            CHECK(got[i].prim == want[i].prim && got[i].t == want[i].t);
+    }
+    ctx->set_wave_schedule(false);
+    ctx->set_trace_config(0, 0);
+    CHECK(prims.size() > 0);
+}
+
 TEST(test_intersection_closest_point_should_have_lowest_t)  // :95-119
 {
     ModelInstance scene = stack_scene();
