@@ -1179,31 +1179,36 @@ __global__ __launch_bounds__(kCtlWG) void k_level_begin(const BNode* __restrict_
         return;
     }
     const uint32_t* lvl = lvl_all + ctl->lvl_off;
-    uint32_t per = (L + kCtlWG - 1) / kCtlWG;
-    uint32_t s0 = min(L, threadIdx.x * per), s1 = min(L, s0 + per);
-    uint32_t local = 0;
-    for (uint32_t s = s0; s < s1; ++s) {
-        BNode nd = bn[lvl[s]];
+    // rounds of kCtlWG consecutive nodes, one per thread (coalesced), the
+    // chunk offsets carried from round to round
+    uint32_t carry = 0;
+    for (uint32_t r0 = 0; r0 < L; r0 += kCtlWG) {
+        uint32_t s = r0 + threadIdx.x;
         Task t;
-        t.first = nd.first;
-        t.count = nd.count;
-        for (int k = 0; k < 6; ++k)
-            t.box[k] = nd.box[k];
-        tasks[s] = t;
-        local += (nd.count + kChunk - 1) / kChunk;
-    }
-    uint32_t total;
-    uint32_t run = block_scan_excl(local, lds, total);
-    for (uint32_t s = s0; s < s1; ++s) {
-        node_chunk[s] = run;
-        Task t = tasks[s];
-        uint32_t end = t.first + t.count;
-        for (uint32_t b = t.first; b < end; b += kChunk) {
-            if (run < ccap)
-                chunks[run] = Chunk{s, b, min(b + kChunk, end), 0};
-            ++run;
+        uint32_t nch = 0;
+        if (s < L) {
+            BNode nd = bn[lvl[s]];
+            t.first = nd.first;
+            t.count = nd.count;
+            for (int k = 0; k < 6; ++k)
+                t.box[k] = nd.box[k];
+            tasks[s] = t;
+            nch = (nd.count + kChunk - 1) / kChunk;
         }
+        uint32_t round_total;
+        uint32_t run = carry + block_scan_excl(nch, lds, round_total);
+        if (s < L) {
+            node_chunk[s] = run;
+            uint32_t end = t.first + t.count;
+            for (uint32_t b = t.first; b < end; b += kChunk) {
+                if (run < ccap)
+                    chunks[run] = Chunk{s, b, min(b + kChunk, end), 0};
+                ++run;
+            }
+        }
+        carry += round_total;
     }
+    const uint32_t total = carry;
     if (threadIdx.x == 0) {
         node_chunk[L] = total;
         if (total > ccap) {  // cannot happen: chunks <= n / kChunk + L
@@ -1250,22 +1255,20 @@ __global__ __launch_bounds__(kCtlWG) void k_level_end(BNode* __restrict__ bn, ui
         return;
     uint32_t H0 = ctl->H, R0 = ctl->R, off = ctl->lvl_off, nlev = ctl->nlevels;
     const uint32_t* lvl = lvl_all + off;
-    uint32_t per = (L + kCtlWG - 1) / kCtlWG;
-    uint32_t s0 = min(L, threadIdx.x * per), s1 = min(L, s0 + per);
-    uint32_t nsplit = 0, nlarge = 0, nsmall = 0;
-    for (uint32_t s = s0; s < s1; ++s) {
+    // pass 1: totals (split nodes, large children; the other children are
+    // small), to check the capacities before anything is written
+    uint32_t nsplit = 0, nlarge = 0;
+    for (uint32_t s = threadIdx.x; s < L; s += kCtlWG) {
         uint32_t cnt = tasks[s].count, nl;
         if (node_splits(dec[s], cnt, faithful, max_leaf, nl)) {
-            uint32_t big = (nl > kSmall ? 1u : 0u) + (cnt - nl > kSmall ? 1u : 0u);
             ++nsplit;
-            nlarge += big;
-            nsmall += 2 - big;
+            nlarge += (nl > kSmall ? 1u : 0u) + (cnt - nl > kSmall ? 1u : 0u);
         }
     }
-    uint32_t tsplit, tlarge, tsmall;
-    uint32_t esplit = block_scan_excl(nsplit, lds, tsplit);
-    uint32_t elarge = block_scan_excl(nlarge, lds, tlarge);
-    uint32_t esmall = block_scan_excl(nsmall, lds, tsmall);
+    uint32_t tsplit, tlarge;
+    block_scan_excl(nsplit, lds, tsplit);
+    block_scan_excl(nlarge, lds, tlarge);
+    const uint32_t tsmall = 2 * tsplit - tlarge;
     bool fits = (uint64_t)H0 + 2 * (uint64_t)tsplit <= hcap && (uint64_t)off + L + tlarge <= hcap &&
                 (uint64_t)R0 + tsmall <= rcap;
     if (!fits) {  // cannot happen: the tree has fewer than 2n nodes, n elements
@@ -1275,30 +1278,47 @@ __global__ __launch_bounds__(kCtlWG) void k_level_end(BNode* __restrict__ bn, ui
         }
         return;
     }
-    uint32_t child = H0 + 2 * esplit, lpos = off + L + elarge, rpos = R0 + esmall;
-    for (uint32_t s = s0; s < s1; ++s) {
-        Task t = tasks[s];
-        uint32_t nl;
-        if (!node_splits(dec[s], t.count, faithful, max_leaf, nl))
-            continue;
-        bn[lvl[s]].left = child;
-        const uint32_t* ck = child_keys + 12 * (size_t)s;  // l.min, r.min | l.max, r.max
-        for (uint32_t side = 0; side < 2; ++side) {
-            BNode c;
-            c.first = side ? t.first + nl : t.first;
-            c.count = side ? t.count - nl : nl;
-            keys_to_box_dev(ck + 3 * side, ck + 6 + 3 * side, c.box);
-            c.left = kNone;
-            if (c.count > kSmall) {
-                c.small = kNone;
-                lvl_all[lpos++] = child + side;
-            } else {
-                c.small = rpos;
-                small_roots[rpos++] = child + side;
-            }
-            bn[child + side] = c;
+    // pass 2: rounds of kCtlWG consecutive nodes, one per thread, placed in
+    // node order by a scan of (splits, large children) packed in one word
+    // (at most kCtlWG and 2 kCtlWG per round)
+    uint32_t c_split = 0, c_large = 0;
+    for (uint32_t r0 = 0; r0 < L; r0 += kCtlWG) {
+        uint32_t s = r0 + threadIdx.x;
+        Task t;
+        uint32_t nl = 0, big = 0;
+        bool split = false;
+        if (s < L) {
+            t = tasks[s];
+            split = node_splits(dec[s], t.count, faithful, max_leaf, nl);
+            if (split)
+                big = (nl > kSmall ? 1u : 0u) + (t.count - nl > kSmall ? 1u : 0u);
         }
-        child += 2;
+        uint32_t round_total;
+        uint32_t ex = block_scan_excl((split ? 1u : 0u) | (big << 16), lds, round_total);
+        uint32_t esplit = c_split + (ex & 0xFFFFu), elarge = c_large + (ex >> 16);
+        if (split) {
+            uint32_t child = H0 + 2 * esplit, lpos = off + L + elarge;
+            uint32_t rpos = R0 + 2 * esplit - elarge;  // small children before this node
+            bn[lvl[s]].left = child;
+            const uint32_t* ck = child_keys + 12 * (size_t)s;  // l.min, r.min | l.max, r.max
+            for (uint32_t side = 0; side < 2; ++side) {
+                BNode c;
+                c.first = side ? t.first + nl : t.first;
+                c.count = side ? t.count - nl : nl;
+                keys_to_box_dev(ck + 3 * side, ck + 6 + 3 * side, c.box);
+                c.left = kNone;
+                if (c.count > kSmall) {
+                    c.small = kNone;
+                    lvl_all[lpos++] = child + side;
+                } else {
+                    c.small = rpos;
+                    small_roots[rpos++] = child + side;
+                }
+                bn[child + side] = c;
+            }
+        }
+        c_split += round_total & 0xFFFFu;
+        c_large += round_total >> 16;
     }
     if (threadIdx.x == 0) {
         ctl->H = H0 + 2 * tsplit;
