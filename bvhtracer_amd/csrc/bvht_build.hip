@@ -91,6 +91,7 @@ struct Ctl {
     uint32_t nlevels;     // levels processed
     uint32_t err;         // self-check flags
     uint32_t nodes_used;  // 2 + 2 * internal nodes, after numbering
+    uint32_t depth;       // levels of the finished tree, after placement
 };
 
 constexpr int kCtlWG = 1024;  // the single-workgroup level kernels
@@ -1010,31 +1011,60 @@ __global__ __launch_bounds__(64 * kSmallWavesPerWG) void k_small_wave(
 }
 
 // Place the small subtrees: local node 0 at root_final, local pair p at
-// 2 + 2*(rank + p).  Internal nodes point at their pair's final index.
-__global__ void k_small_emit(const SNode* __restrict__ snodes, const BNode* __restrict__ bn,
-                             const uint32_t* __restrict__ small_roots, const uint32_t* __restrict__ pairs,
-                             const uint32_t* __restrict__ root_final, const uint32_t* __restrict__ rank, uint32_t nroots,
-                             bvht_node* __restrict__ nodes, uint32_t node_count, uint32_t* __restrict__ err)
+// 2 + 2*(rank + p).  Internal nodes point at their pair's final index.  Each
+// node's level (its subtree root's level plus its depth in the subtree; a
+// pair always comes after its parent) goes to `level`, and the deepest level
+// + 1 to *depth.
+__global__ __launch_bounds__(64) void k_small_emit(const SNode* __restrict__ snodes, const BNode* __restrict__ bn,
+                                                   const uint32_t* __restrict__ small_roots,
+                                                   const uint32_t* __restrict__ pairs,
+                                                   const uint32_t* __restrict__ root_final,
+                                                   const uint32_t* __restrict__ rank,
+                                                   const uint32_t* __restrict__ root_level, uint32_t nroots,
+                                                   bvht_node* __restrict__ nodes, uint32_t* __restrict__ level,
+                                                   uint32_t node_count, uint32_t* __restrict__ err,
+                                                   uint32_t* __restrict__ depth)
 {
+    __shared__ uint8_t ldep[64 * 2 * kSmall];  // depth of local node j of this lane's subtree
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nroots)
-        return;
-    const SNode* s = snodes + 2 * (size_t)bn[small_roots[r]].first;
-    uint32_t np = pairs[r], rk = rank[r];
-    for (uint32_t j = 0; j < 1 + 2 * np; ++j) {
-        SNode sn = s[j];
-        uint32_t fj = j == 0 ? root_final[r] : 2 + 2 * (rk + (j - 1) / 2) + (j - 1) % 2;
-        bvht_node o;
-        o.bmin[0] = sn.bmin[0]; o.bmin[1] = sn.bmin[1]; o.bmin[2] = sn.bmin[2];
-        o.bmax[0] = sn.bmax[0]; o.bmax[1] = sn.bmax[1]; o.bmax[2] = sn.bmax[2];
-        o.count = sn.count;
-        o.left_first = sn.count > 0 ? sn.lf : 2 + 2 * (rk + sn.lf);
-        if (fj >= node_count) {
-            atomicOr(err, 8u);
-            continue;
+    uint32_t deepest = 0;  // deepest level + 1 of this lane's subtree
+    if (r < nroots) {
+        uint8_t* ld = ldep + threadIdx.x * 2 * kSmall;
+        const SNode* s = snodes + 2 * (size_t)bn[small_roots[r]].first;
+        uint32_t np = pairs[r], rk = rank[r], lv0 = root_level[r];
+        ld[0] = 0;
+        for (uint32_t j = 0; j < 1 + 2 * np && j < 2 * kSmall; ++j) {
+            SNode sn = s[j];
+            uint32_t fj = j == 0 ? root_final[r] : 2 + 2 * (rk + (j - 1) / 2) + (j - 1) % 2;
+            uint32_t dj = ld[j];
+            bvht_node o;
+            o.bmin[0] = sn.bmin[0]; o.bmin[1] = sn.bmin[1]; o.bmin[2] = sn.bmin[2];
+            o.bmax[0] = sn.bmax[0]; o.bmax[1] = sn.bmax[1]; o.bmax[2] = sn.bmax[2];
+            o.count = sn.count;
+            o.left_first = sn.count > 0 ? sn.lf : 2 + 2 * (rk + sn.lf);
+            if (sn.count == 0) {
+                uint32_t c0 = 1 + 2 * sn.lf;  // local index of the left child
+                if (c0 <= j || c0 + 1 >= 2 * kSmall) {
+                    atomicOr(err, 8u);
+                    continue;
+                }
+                ld[c0] = (uint8_t)(dj + 1);
+                ld[c0 + 1] = (uint8_t)(dj + 1);
+            }
+            if (fj >= node_count) {
+                atomicOr(err, 8u);
+                continue;
+            }
+            nodes[fj] = o;
+            level[fj] = lv0 + dj;
+            deepest = max(deepest, lv0 + dj + 1);
         }
-        nodes[fj] = o;
     }
+    // one atomic per wave (every lane takes part)
+    for (int off = 32; off > 0; off >>= 1)
+        deepest = max(deepest, (uint32_t)__shfl_xor((int)deepest, off, 64));
+    if (threadIdx.x == 0 && deepest)
+        atomicMax(depth, deepest);
 }
 
 // Structural check of the finished node array (the same rules the upload
@@ -1144,6 +1174,7 @@ __global__ void k_root_init(const uint32_t* __restrict__ rootk, uint32_t n, BNod
     c.nlevels = 0;
     c.err = 0;
     c.nodes_used = 0;
+    c.depth = 0;
     if (n > kSmall) {
         r.small = kNone;
         lvl_all[0] = 0;
@@ -1341,13 +1372,16 @@ __global__ __launch_bounds__(kCtlWG) void k_number(const BNode* __restrict__ bn,
                                                    const uint32_t* __restrict__ pairs, uint32_t* __restrict__ ic,
                                                    uint32_t* __restrict__ start, uint32_t* __restrict__ fin,
                                                    uint32_t* __restrict__ rfinal, uint32_t* __restrict__ rrank,
-                                                   bvht_node* __restrict__ nodes, uint32_t node_count)
+                                                   uint32_t* __restrict__ rlevel, bvht_node* __restrict__ nodes,
+                                                   uint32_t* __restrict__ level, uint32_t node_count)
 {
     uint32_t D = ctl->nlevels;
     if (bn[0].small != kNone) {  // n <= kSmall: the root is the one small subtree
         if (threadIdx.x == 0) {
             rfinal[0] = 0;
             rrank[0] = 0;
+            rlevel[0] = 0;
+            level[1] = kNone;  // the unused node
             uint32_t total = pairs[0];
             if ((uint64_t)2 + 2 * (uint64_t)total > node_count)
                 atomicOr(&ctl->err, 512u);
@@ -1408,24 +1442,31 @@ __global__ __launch_bounds__(kCtlWG) void k_number(const BNode* __restrict__ bn,
                 if (m0 != kNone) {
                     rfinal[m0] = 2 + 2 * r;
                     rrank[m0] = r0;
+                    rlevel[m0] = l + 1;
                 }
                 if (m1 != kNone) {
                     rfinal[m1] = 3 + 2 * r;
                     rrank[m1] = r1;
+                    rlevel[m1] = l + 1;
                 }
             } else {
                 o.left_first = nd.first;
                 o.count = nd.count;
             }
-            if (f < node_count)
+            if (f < node_count) {
                 nodes[f] = o;
-            else
+                level[f] = l;
+            } else {
                 atomicOr(&ctl->err, 512u);
+            }
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         ctl->nodes_used = 2 + 2 * total;
+        level[1] = kNone;  // the unused node
+        atomicMax(&ctl->depth, D);
+    }
 }
 
 // ------------------------------------------------------------ host side --
@@ -1468,7 +1509,7 @@ struct BuildWs {
     float2 *ehi, *ohi;
     uint32_t *kpos, *sel;
     SNode* snodes;
-    uint32_t *small_roots, *pairs, *rfinal, *rrank;
+    uint32_t *small_roots, *pairs, *rfinal, *rrank, *rlevel;
     BNode* bn;
     uint32_t *lvl_all, *ic, *start, *fin, *level_start;
     Task* tasks;
@@ -1512,6 +1553,7 @@ size_t layout(BuildWs& w, char* base)
     carve(w.pairs, w.rcap);
     carve(w.rfinal, w.rcap);
     carve(w.rrank, w.rcap);
+    carve(w.rlevel, w.rcap);
     carve(w.bn, w.hcap);
     carve(w.lvl_all, w.hcap);
     carve(w.ic, w.hcap);
@@ -1642,10 +1684,11 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
                            w.elo, w.ehi, w.bn, w.small_roots, R, w.snodes, w.pairs, faithful, max_leaf,
                            &w.ctl->err);
     hipLaunchKernelGGL(k_number, dim3(1), dim3(kCtlWG), 0, st, w.bn, w.lvl_all, w.level_start, w.ctl, w.pairs, w.ic,
-                       w.start, w.fin, w.rfinal, w.rrank, b->d_nodes, 2 * n);
+                       w.start, w.fin, w.rfinal, w.rrank, w.rlevel, b->d_nodes, b->d_level, 2 * n);
     if (R)
         hipLaunchKernelGGL(k_small_emit, dim3(nblk(R, 64)), dim3(64), 0, st, w.snodes, w.bn, w.small_roots, w.pairs,
-                           w.rfinal, w.rrank, R, b->d_nodes, 2 * n, &w.ctl->err);
+                           w.rfinal, w.rrank, w.rlevel, R, b->d_nodes, b->d_level, 2 * n, &w.ctl->err,
+                           &w.ctl->depth);
     hipLaunchKernelGGL(k_gather_tris, dim3(nblk(n, 256)), dim3(256), 0, st, w.tin, w.ec, n, b->d_tris, b->d_perm,
                        &w.ctl->err);
     hipLaunchKernelGGL(k_validate_tree, dim3(nblk(2 * (size_t)n, 256)), dim3(256), 0, st, b->d_nodes, w.ctl, n,
@@ -1660,7 +1703,9 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
     if (h.nodes_used < 2 || h.nodes_used > 2 * n)
         return set_error(BVHT_EDEVICE, "BLAS build: %u nodes used of %u", h.nodes_used, 2 * n);
     b->nodes_used = h.nodes_used;
-    RC(compute_blas_topology(b, nullptr));
+    if (h.depth == 0 || h.depth > h.nodes_used)
+        return set_error(BVHT_EDEVICE, "BLAS build: depth %u out of range", h.depth);
+    RC(compute_blas_topology(b, nullptr, h.depth));  // the levels are already in place
     *out = b;
     b = nullptr;
     return BVHT_OK;

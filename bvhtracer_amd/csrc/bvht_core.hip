@@ -155,24 +155,32 @@ __global__ void k_parents(const bvht_node* __restrict__ nodes, uint32_t nodes_us
     }
 }
 
-// Level of every used node (steps to the root) and the tree depth.
-__global__ void k_levels(uint32_t nodes_used, const uint32_t* __restrict__ parent, uint32_t* __restrict__ level,
-                         uint32_t* __restrict__ depth)
+// Level of every used node (steps to the root) and the tree depth.  The
+// depth is reduced over the wave first: one atomic per wave, not per node (a
+// single counter shared by millions of nodes serialises otherwise).
+__global__ __launch_bounds__(256) void k_levels(uint32_t nodes_used, const uint32_t* __restrict__ parent,
+                                                uint32_t* __restrict__ level, uint32_t* __restrict__ depth)
 {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nodes_used)
-        return;
-    if (k == 1) {
-        level[k] = BVHT_MISS;
-        return;
+    uint32_t m = 0;  // depth below this node's level + 1, 0 for no node
+    if (k < nodes_used) {
+        if (k == 1) {
+            level[k] = BVHT_MISS;
+        } else {
+            uint32_t d = 0, p = parent[k];
+            while (p != BVHT_MISS && d < nodes_used) {
+                ++d;
+                p = parent[p];
+            }
+            level[k] = d;
+            m = d + 1;
+        }
     }
-    uint32_t d = 0, p = parent[k];
-    while (p != BVHT_MISS && d < nodes_used) {
-        ++d;
-        p = parent[p];
-    }
-    level[k] = d;
-    atomicMax(depth, d + 1);
+    // every lane of the wave takes part (no early return above)
+    for (int off = 32; off > 0; off >>= 1)
+        m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
+    if ((threadIdx.x & 63u) == 0 && m)
+        atomicMax(depth, m);
 }
 
 // Per-level node counts and the level-grouped node order.  Levels below
@@ -181,10 +189,12 @@ __global__ void k_levels(uint32_t nodes_used, const uint32_t* __restrict__ paren
 // millions of nodes serialise otherwise); deeper levels (degenerate trees)
 // go straight to the global counters.  Workgroups stride over the nodes.
 // The order within a level is arbitrary: the nodes of one level are refit
-// independently.
+// independently.  A level at or beyond `depth` (an inconsistent level array)
+// is neither counted nor placed, which the caller's count check reports.
 constexpr uint32_t kLevelsLds = 64;
 
-__global__ void k_level_hist(uint32_t nodes_used, const uint32_t* __restrict__ level, uint32_t* __restrict__ hist)
+__global__ void k_level_hist(uint32_t nodes_used, uint32_t depth, const uint32_t* __restrict__ level,
+                             uint32_t* __restrict__ hist)
 {
     __shared__ uint32_t sh[kLevelsLds];
     for (uint32_t i = threadIdx.x; i < kLevelsLds; i += blockDim.x)
@@ -194,9 +204,9 @@ __global__ void k_level_hist(uint32_t nodes_used, const uint32_t* __restrict__ l
         if (k == 1)
             continue;
         uint32_t L = level[k];
-        if (L < kLevelsLds)
+        if (L < kLevelsLds && L < depth)
             atomicAdd(&sh[L], 1u);
-        else
+        else if (L < depth)
             atomicAdd(&hist[L], 1u);
     }
     __syncthreads();
@@ -205,30 +215,36 @@ __global__ void k_level_hist(uint32_t nodes_used, const uint32_t* __restrict__ l
             atomicAdd(&hist[i], sh[i]);
 }
 
-__global__ void k_level_scatter(uint32_t nodes_used, const uint32_t* __restrict__ level, uint32_t* __restrict__ cursor,
-                                uint32_t* __restrict__ order)
+__global__ void k_level_scatter(uint32_t nodes_used, uint32_t depth, const uint32_t* __restrict__ level,
+                                uint32_t* __restrict__ cursor, uint32_t* __restrict__ order)
 {
     __shared__ uint32_t cnt[kLevelsLds], base[kLevelsLds];
-    uint32_t stride = gridDim.x * blockDim.x;
-    // one pass per block-sized slice of nodes, so the local ranks fit in registers
-    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < nodes_used; k0 += stride) {
-        for (uint32_t i = threadIdx.x; i < kLevelsLds; i += blockDim.x)
-            cnt[i] = 0;
-        __syncthreads();
-        uint32_t k = k0 + threadIdx.x;
-        bool valid = k < nodes_used && k != 1;
-        uint32_t L = valid ? level[k] : 0u;
-        uint32_t local = 0;
-        if (valid && L < kLevelsLds)
-            local = atomicAdd(&cnt[L], 1u);
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < kLevelsLds; i += blockDim.x)
-            if (cnt[i])
-                base[i] = atomicAdd(&cursor[i], cnt[i]);
-        __syncthreads();
-        if (valid)
-            order[L < kLevelsLds ? base[L] + local : atomicAdd(&cursor[L], 1u)] = k;
-        __syncthreads();
+    // each workgroup takes one contiguous range of nodes: counted first, so
+    // it claims each level's slots with one global atomic, then placed
+    uint32_t per = (nodes_used + gridDim.x - 1) / gridDim.x;
+    uint32_t k0 = min(nodes_used, blockIdx.x * per), k1 = min(nodes_used, k0 + per);
+    for (uint32_t i = threadIdx.x; i < kLevelsLds; i += blockDim.x)
+        cnt[i] = 0;
+    __syncthreads();
+    for (uint32_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+        if (k == 1)
+            continue;
+        uint32_t L = level[k];
+        if (L < kLevelsLds && L < depth)
+            atomicAdd(&cnt[L], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kLevelsLds; i += blockDim.x) {
+        base[i] = cnt[i] ? atomicAdd(&cursor[i], cnt[i]) : 0u;
+        cnt[i] = 0;
+    }
+    __syncthreads();
+    for (uint32_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+        if (k == 1)
+            continue;
+        uint32_t L = level[k];
+        if (L < depth)
+            order[L < kLevelsLds ? base[L] + atomicAdd(&cnt[L], 1u) : atomicAdd(&cursor[L], 1u)] = k;
     }
 }
 
@@ -1018,7 +1034,7 @@ int32_t fill_u32(bvht_ctx ctx, uint32_t* p, uint32_t v, size_t n)
 // Parent array, node levels, depth and the level-grouped node order on the
 // device (after a build or an upload); the per-level node counts come back to
 // the host to drive the level-by-level refit.
-int32_t compute_blas_topology(bvht_blas b, const bvht_node*)
+int32_t compute_blas_topology(bvht_blas b, const bvht_node*, uint32_t known_depth)
 {
     bvht_ctx ctx = b->ctx;
     hipStream_t st = ctx->stream;
@@ -1027,29 +1043,38 @@ int32_t compute_blas_topology(bvht_blas b, const bvht_node*)
         return rc;
     uint32_t nb = blocks_for(b->nodes_used, 256);
     hipLaunchKernelGGL(k_parents, dim3(nb), dim3(256), 0, st, b->d_nodes, b->nodes_used, b->d_parent);
-    rc = ensure_scratch(ctx, sizeof(uint32_t) * (2 * (size_t)b->nodes_used + 2));
+    // scratch: [0] depth, then the per-level counts, then the big-leaf count
+    // and the wide-leaf flag
+    rc = ensure_scratch(ctx, sizeof(uint32_t) * (2 * (size_t)b->nodes_used + 4));
     if (rc)
         return rc;
     uint32_t* d_depth = static_cast<uint32_t*>(ctx->scratch);
-    rc = fill_u32(ctx, d_depth, 0, 1);
-    if (rc)
-        return rc;
-    hipLaunchKernelGGL(k_levels, dim3(nb), dim3(256), 0, st, b->nodes_used, b->d_parent, b->d_level, d_depth);
-    BVHT_HIP(hipGetLastError());
-    uint32_t depth = 0;
-    BVHT_HIP(hipMemcpyAsync(&depth, d_depth, 4, hipMemcpyDeviceToHost, st));
-    BVHT_HIP(hipStreamSynchronize(st));
+    uint32_t depth = known_depth;
+    if (depth == 0) {
+        rc = fill_u32(ctx, d_depth, 0, 1);
+        if (rc)
+            return rc;
+        hipLaunchKernelGGL(k_levels, dim3(nb), dim3(256), 0, st, b->nodes_used, b->d_parent, b->d_level, d_depth);
+        BVHT_HIP(hipGetLastError());
+        BVHT_HIP(hipMemcpyAsync(&depth, d_depth, 4, hipMemcpyDeviceToHost, st));
+        BVHT_HIP(hipStreamSynchronize(st));
+    }
     if (depth == 0 || depth > b->nodes_used)
         return set_error(BVHT_EDEVICE, "BLAS topology: depth %u out of range", depth);
     b->depth = depth;
     uint32_t* hist = d_depth + 1;
-    rc = fill_u32(ctx, hist, 0, depth);
+    uint32_t* big = hist + depth;  // big-leaf count, wide-leaf flag
+    rc = fill_u32(ctx, hist, 0, depth + 2);
     if (rc)
         return rc;
     uint32_t nb_levels = std::min<uint32_t>(nb, 512);
-    hipLaunchKernelGGL(k_level_hist, dim3(nb_levels), dim3(256), 0, st, b->nodes_used, b->d_level, hist);
-    std::vector<uint32_t> h(depth);
-    BVHT_HIP(hipMemcpyAsync(h.data(), hist, sizeof(uint32_t) * depth, hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_level_hist, dim3(nb_levels), dim3(256), 0, st, b->nodes_used, depth, b->d_level, hist);
+    // big leaves: counted here, listed below; and whether every node fits a
+    // packed stack entry
+    hipLaunchKernelGGL(k_big_leaves, dim3(nb), dim3(256), 0, st, b->d_nodes, b->nodes_used, big, nullptr, big + 1);
+    BVHT_HIP(hipGetLastError());
+    std::vector<uint32_t> h(depth + 2);
+    BVHT_HIP(hipMemcpyAsync(h.data(), hist, sizeof(uint32_t) * (depth + 2), hipMemcpyDeviceToHost, st));
     BVHT_HIP(hipStreamSynchronize(st));
     b->level_off.assign(depth + 1, 0);
     for (uint32_t L = 0; L < depth; ++L)
@@ -1058,31 +1083,21 @@ int32_t compute_blas_topology(bvht_blas b, const bvht_node*)
         return set_error(BVHT_EDEVICE, "BLAS topology: %u nodes grouped, %u expected", b->level_off[depth],
                          b->nodes_used - 1);
     BVHT_HIP(hipMemcpyAsync(hist, b->level_off.data(), sizeof(uint32_t) * depth, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_level_scatter, dim3(nb_levels), dim3(256), 0, st, b->nodes_used, b->d_level, hist,
+    hipLaunchKernelGGL(k_level_scatter, dim3(nb_levels), dim3(256), 0, st, b->nodes_used, depth, b->d_level, hist,
                        b->d_level_order);
     BVHT_HIP(hipGetLastError());
-    // big leaves: count, allocate, list; and whether every node fits a packed
-    // stack entry
-    rc = fill_u32(ctx, d_depth, 0, 2);
-    if (rc)
-        return rc;
-    hipLaunchKernelGGL(k_big_leaves, dim3(nb), dim3(256), 0, st, b->d_nodes, b->nodes_used, d_depth, nullptr,
-                       d_depth + 1);
-    uint32_t hb[2] = {0, 0};
-    BVHT_HIP(hipMemcpyAsync(hb, d_depth, 8, hipMemcpyDeviceToHost, st));
-    BVHT_HIP(hipStreamSynchronize(st));
-    uint32_t nbig = hb[0];
-    b->packed_stack = hb[1] == 0 && b->nodes_used <= kPackMask + 1u && b->n_tris <= kPackMask + 1u;
+    uint32_t nbig = h[depth];
+    b->packed_stack = h[depth + 1] == 0 && b->nodes_used <= kPackMask + 1u && b->n_tris <= kPackMask + 1u;
     (void)hipFree(b->d_big_leaves);
     b->d_big_leaves = nullptr;
     b->n_big_leaves = nbig;
     if (nbig) {
         if (hipMalloc(&b->d_big_leaves, sizeof(uint32_t) * nbig) != hipSuccess)
             return set_error(BVHT_ENOMEM, "device allocation of the big-leaf list failed");
-        rc = fill_u32(ctx, d_depth, 0, 1);
+        rc = fill_u32(ctx, big, 0, 1);
         if (rc)
             return rc;
-        hipLaunchKernelGGL(k_big_leaves, dim3(nb), dim3(256), 0, st, b->d_nodes, b->nodes_used, d_depth,
+        hipLaunchKernelGGL(k_big_leaves, dim3(nb), dim3(256), 0, st, b->d_nodes, b->nodes_used, big,
                            b->d_big_leaves, nullptr);
         BVHT_HIP(hipGetLastError());
     }
@@ -1585,7 +1600,7 @@ int32_t bvht_blas_upload(bvht_ctx ctx, const bvht_node* nodes, uint32_t node_cou
         bvht_blas_destroy(b);
         return set_error(BVHT_EDEVICE, "upload copy failed");
     }
-    rc = compute_blas_topology(b, nodes);
+    rc = compute_blas_topology(b, nodes, 0);
     if (rc) {
         bvht_blas_destroy(b);
         return rc;

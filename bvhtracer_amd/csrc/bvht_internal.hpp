@@ -132,7 +132,10 @@ namespace bvht {
 
 int32_t set_error(int32_t code, const char* fmt, ...);
 int32_t ensure_scratch(bvht_ctx ctx, size_t bytes);
-int32_t compute_blas_topology(bvht_blas b, const bvht_node* h_nodes);
+// Parents, levels (computed here unless known_depth > 0 says the build left
+// them in d_level and the tree has that many levels), the level-grouped node
+// order and the big-leaf list of a BLAS.
+int32_t compute_blas_topology(bvht_blas b, const bvht_node* h_nodes, uint32_t known_depth);
 int32_t ensure_spill(bvht_ctx ctx, uint32_t** buf, size_t* words, size_t need);
 // After a fence on ctx's stream: BVHT_ESTACK (and the status cleared) if a
 // trace since the last check dropped a stack entry, else BVHT_OK.
