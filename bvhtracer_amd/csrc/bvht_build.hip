@@ -1648,7 +1648,7 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
     if (hipMalloc(&b->d_perm, sizeof(uint32_t) * (size_t)n) != hipSuccess)
         return set_error(BVHT_ENOMEM, "device allocation of the %u-entry build permutation failed", n);
 
-    BVHT_HIP(hipMemcpyAsync(w.tin, h_tris, sizeof(float) * 9 * (size_t)n, hipMemcpyHostToDevice, st));
+    RC(bvht::host_to_device(ctx, w.tin, h_tris, sizeof(float) * 9 * (size_t)n));
     BVHT_HIP(hipMemsetAsync(b->d_nodes, 0, sizeof(bvht_node) * 2 * (size_t)n, st));
     hipLaunchKernelGGL(k_init_elems, dim3(nblk(n, 256)), dim3(256), 0, st, w.tin, n, w.ec, w.elo, w.ehi);
     // root box: update_node_bounds over everything (bvh.rs:537)
@@ -1695,7 +1695,6 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
                        &w.ctl->err);
     BVHT_HIP(hipGetLastError());
     BVHT_HIP(hipMemcpyAsync(&h, w.ctl, sizeof h, hipMemcpyDeviceToHost, st));
-    BVHT_HIP(hipMemcpyAsync(h_tris, b->d_tris, sizeof(float) * 9 * (size_t)n, hipMemcpyDeviceToHost, st));
     BVHT_HIP(hipStreamSynchronize(st));
     if (h.err)
         return set_error(BVHT_EDEVICE, "BLAS build self-check failed (flags 0x%x): inconsistent partition or tree",
@@ -1706,6 +1705,9 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
     if (h.depth == 0 || h.depth > h.nodes_used)
         return set_error(BVHT_EDEVICE, "BLAS build: depth %u out of range", h.depth);
     RC(compute_blas_topology(b, nullptr, h.depth));  // the levels are already in place
+    // the permuted triangles back into the caller's array (build_for,
+    // bvh.rs:524, partitions the slice it is given in place)
+    RC(bvht::device_to_host(ctx, h_tris, b->d_tris, sizeof(float) * 9 * (size_t)n));
     *out = b;
     b = nullptr;
     return BVHT_OK;

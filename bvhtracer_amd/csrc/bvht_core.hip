@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "bvht_device.hpp"
@@ -1034,6 +1035,98 @@ int32_t fill_u32(bvht_ctx ctx, uint32_t* p, uint32_t v, size_t n)
 // Parent array, node levels, depth and the level-grouped node order on the
 // device (after a build or an upload); the per-level node counts come back to
 // the host to drive the level-by-level refit.
+// Host copies through the pinned staging: chunks of kStageChunk bytes
+// alternate between the two halves.  Below kStageMin bytes, or while the
+// stream is being captured, the runtime's own pageable copy is used.
+constexpr size_t kStageChunk = size_t(8) << 20;
+constexpr size_t kStageMin = size_t(4) << 20;
+constexpr unsigned kStageThreads = 4;
+
+bool ensure_stage(bvht_ctx ctx)
+{
+    if (ctx->h_stage)
+        return true;
+    for (hipEvent_t& e : ctx->stage_ev) {
+        if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return false;
+        if (hipEventRecord(e, ctx->stream) != hipSuccess)
+            return false;
+    }
+    void* p = nullptr;
+    if (hipHostMalloc(&p, 2 * kStageChunk, hipHostMallocDefault) != hipSuccess)
+        return false;
+    ctx->h_stage = static_cast<char*>(p);
+    ctx->stage_half = kStageChunk;
+    return true;
+}
+
+// memcpy split over up to kStageThreads threads (the calling one included),
+// in 4 KiB-aligned slices
+void split_memcpy(char* dst, const char* src, size_t n)
+{
+    unsigned nt = std::max(1u, std::min(kStageThreads, std::thread::hardware_concurrency()));
+    size_t per = ((n + nt - 1) / nt + 4095) & ~(size_t)4095;
+    std::thread th[kStageThreads];
+    unsigned started = 0;
+    for (unsigned t = 1; t < nt; ++t) {
+        size_t a = std::min(n, t * per), b = std::min(n, a + per);
+        if (a < b)
+            th[started++] = std::thread([=] { std::memcpy(dst + a, src + a, b - a); });
+    }
+    std::memcpy(dst, src, std::min(n, per));
+    for (unsigned t = 0; t < started; ++t)
+        th[t].join();
+}
+
+int32_t host_to_device(bvht_ctx ctx, void* dst, const void* src, size_t bytes)
+{
+    hipStream_t st = ctx->stream;
+    if (bytes < kStageMin || ctx->capturing || !ensure_stage(ctx)) {
+        BVHT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+        return BVHT_OK;
+    }
+    const char* s = static_cast<const char*>(src);
+    char* d = static_cast<char*>(dst);
+    for (size_t off = 0, i = 0; off < bytes; off += ctx->stage_half, ++i) {
+        size_t len = std::min(ctx->stage_half, bytes - off);
+        char* half = ctx->h_stage + (i % 2) * ctx->stage_half;
+        BVHT_HIP(hipEventSynchronize(ctx->stage_ev[i % 2]));  // the half's last transfer is done
+        split_memcpy(half, s + off, len);
+        BVHT_HIP(hipMemcpyAsync(d + off, half, len, hipMemcpyHostToDevice, st));
+        BVHT_HIP(hipEventRecord(ctx->stage_ev[i % 2], st));
+    }
+    return BVHT_OK;
+}
+
+int32_t device_to_host(bvht_ctx ctx, void* dst, const void* src, size_t bytes)
+{
+    hipStream_t st = ctx->stream;
+    if (bytes < kStageMin || ctx->capturing || !ensure_stage(ctx)) {
+        BVHT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+        BVHT_HIP(hipStreamSynchronize(st));
+        return BVHT_OK;
+    }
+    const char* s = static_cast<const char*>(src);
+    char* d = static_cast<char*>(dst);
+    const size_t H = ctx->stage_half;
+    size_t nch = (bytes + H - 1) / H;
+    // the DMA of chunk i + 1 runs while the host copies chunk i out (the
+    // stream orders it after any earlier transfer through the same half)
+    auto issue = [&](size_t i) -> hipError_t {
+        hipError_t e = hipMemcpyAsync(ctx->h_stage + (i % 2) * H, s + i * H, std::min(H, bytes - i * H),
+                                      hipMemcpyDeviceToHost, st);
+        return e != hipSuccess ? e : hipEventRecord(ctx->stage_ev[i % 2], st);
+    };
+    BVHT_HIP(issue(0));
+    for (size_t i = 0; i < nch; ++i) {
+        if (i + 1 < nch)
+            BVHT_HIP(issue(i + 1));
+        BVHT_HIP(hipEventSynchronize(ctx->stage_ev[i % 2]));
+        split_memcpy(d + i * H, ctx->h_stage + (i % 2) * H, std::min(H, bytes - i * H));
+    }
+    return BVHT_OK;
+}
+
 int32_t compute_blas_topology(bvht_blas b, const bvht_node*, uint32_t known_depth)
 {
     bvht_ctx ctx = b->ctx;
@@ -1470,6 +1563,11 @@ int32_t bvht_destroy(bvht_ctx ctx)
     (void)hipFree(ctx->tls_bins);
     if (ctx->h_status)
         (void)hipHostFree(ctx->h_status);
+    if (ctx->h_stage)
+        (void)hipHostFree(ctx->h_stage);
+    for (hipEvent_t e : ctx->stage_ev)
+        if (e)
+            (void)hipEventDestroy(e);
     release_build_ws(ctx);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1521,7 +1619,9 @@ int32_t bvht_device_free(bvht_ctx ctx, void* ptr)
 int32_t bvht_copy_to_device(bvht_ctx ctx, void* dst, const void* src, size_t bytes)
 {
     BVHT_REQUIRE(ctx && (bytes == 0 || (dst && src)), "NULL argument");
-    BVHT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    int32_t rc = bvht::host_to_device(ctx, dst, src, bytes);
+    if (rc)
+        return rc;
     BVHT_HIP(hipStreamSynchronize(ctx->stream));
     return BVHT_OK;
 }
@@ -1529,8 +1629,9 @@ int32_t bvht_copy_to_device(bvht_ctx ctx, void* dst, const void* src, size_t byt
 int32_t bvht_copy_to_host(bvht_ctx ctx, void* dst, const void* src, size_t bytes)
 {
     BVHT_REQUIRE(ctx && (bytes == 0 || (dst && src)), "NULL argument");
-    BVHT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    BVHT_HIP(hipStreamSynchronize(ctx->stream));
+    int32_t rc = bvht::device_to_host(ctx, dst, src, bytes);
+    if (rc)
+        return rc;
     return take_status(ctx);
 }
 
@@ -1593,12 +1694,12 @@ int32_t bvht_blas_upload(bvht_ctx ctx, const bvht_node* nodes, uint32_t node_cou
     if (rc)
         return rc;
     b->nodes_used = nodes_used;
-    if (hipMemcpyAsync(b->d_nodes, nodes, sizeof(bvht_node) * (size_t)node_count, hipMemcpyHostToDevice,
-                       ctx->stream) != hipSuccess ||
-        hipMemcpyAsync(b->d_tris, tris, sizeof(float) * 9 * (size_t)n, hipMemcpyHostToDevice, ctx->stream) !=
-            hipSuccess) {
+    rc = host_to_device(ctx, b->d_nodes, nodes, sizeof(bvht_node) * (size_t)node_count);
+    if (!rc)
+        rc = host_to_device(ctx, b->d_tris, tris, sizeof(float) * 9 * (size_t)n);
+    if (rc) {
         bvht_blas_destroy(b);
-        return set_error(BVHT_EDEVICE, "upload copy failed");
+        return rc;
     }
     rc = compute_blas_topology(b, nodes, 0);
     if (rc) {
@@ -1612,12 +1713,13 @@ int32_t bvht_blas_upload(bvht_ctx ctx, const bvht_node* nodes, uint32_t node_cou
 int32_t bvht_blas_download(bvht_ctx ctx, bvht_blas blas, bvht_node* nodes, float* tris)
 {
     BVHT_REQUIRE(ctx && blas, "NULL argument");
+    int32_t rc = BVHT_OK;
     if (nodes)
-        BVHT_HIP(hipMemcpyAsync(nodes, blas->d_nodes, sizeof(bvht_node) * (size_t)blas->node_count,
-                                hipMemcpyDeviceToHost, ctx->stream));
-    if (tris)
-        BVHT_HIP(hipMemcpyAsync(tris, blas->d_tris, sizeof(float) * 9 * (size_t)blas->n_tris,
-                                hipMemcpyDeviceToHost, ctx->stream));
+        rc = bvht::device_to_host(ctx, nodes, blas->d_nodes, sizeof(bvht_node) * (size_t)blas->node_count);
+    if (!rc && tris)
+        rc = bvht::device_to_host(ctx, tris, blas->d_tris, sizeof(float) * 9 * (size_t)blas->n_tris);
+    if (rc)
+        return rc;
     BVHT_HIP(hipStreamSynchronize(ctx->stream));
     return BVHT_OK;
 }
@@ -1637,7 +1739,9 @@ int32_t bvht_blas_set_vertices(bvht_ctx ctx, bvht_blas blas, const float* tris, 
 {
     BVHT_REQUIRE(ctx && blas && tris, "NULL argument");
     BVHT_REQUIRE(n == blas->n_tris, "triangle count %u differs from the BLAS's %u", n, blas->n_tris);
-    BVHT_HIP(hipMemcpyAsync(blas->d_tris, tris, sizeof(float) * 9 * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    int32_t rc = bvht::host_to_device(ctx, blas->d_tris, tris, sizeof(float) * 9 * (size_t)n);
+    if (rc)
+        return rc;
     BVHT_HIP(hipStreamSynchronize(ctx->stream));
     return BVHT_OK;
 }

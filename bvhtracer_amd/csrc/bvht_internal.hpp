@@ -54,6 +54,12 @@ struct bvht_ctx_s {
     void* noalloc_scratch = nullptr;
     // BLAS build workspace (bvht_build.hip), kept between builds.
     void* build_ws = nullptr;
+    // Pinned staging for large copies between pageable host memory and the
+    // device (two halves of stage_half bytes, allocated on first use), and
+    // the event of the last transfer through each half.
+    char* h_stage = nullptr;
+    size_t stage_half = 0;
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
     // Longest-first wave order of bvht_trace_tlas (bvht_set_wave_schedule):
     // the last call's group durations and the order made from them, keyed by
     // TLAS, ray buffer and ray count.
@@ -136,6 +142,14 @@ int32_t ensure_scratch(bvht_ctx ctx, size_t bytes);
 // them in d_level and the tree has that many levels), the level-grouped node
 // order and the big-leaf list of a BLAS.
 int32_t compute_blas_topology(bvht_blas b, const bvht_node* h_nodes, uint32_t known_depth);
+// Copies between pageable host memory and the device on ctx's stream.
+// host_to_device returns once src may be reused (the transfer completes in
+// stream order); device_to_host returns once dst holds the data.  Large
+// copies go through ctx's pinned staging, chunk by chunk, with the host's
+// share of each chunk split over a few threads and overlapped with the DMA
+// of the next chunk.
+int32_t host_to_device(bvht_ctx ctx, void* dst, const void* src, size_t bytes);
+int32_t device_to_host(bvht_ctx ctx, void* dst, const void* src, size_t bytes);
 int32_t ensure_spill(bvht_ctx ctx, uint32_t** buf, size_t* words, size_t need);
 // After a fence on ctx's stream: BVHT_ESTACK (and the status cleared) if a
 // trace since the last check dropped a stack entry, else BVHT_OK.
