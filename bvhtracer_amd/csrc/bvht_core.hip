@@ -1061,21 +1061,27 @@ bool ensure_stage(bvht_ctx ctx)
 }
 
 // memcpy split over up to kStageThreads threads (the calling one included),
-// in 4 KiB-aligned slices
+// in 4 KiB-aligned slices.  A slice whose thread cannot be started is copied
+// by the calling thread (no exception leaves the library).
 void split_memcpy(char* dst, const char* src, size_t n)
 {
     unsigned nt = std::max(1u, std::min(kStageThreads, std::thread::hardware_concurrency()));
     size_t per = ((n + nt - 1) / nt + 4095) & ~(size_t)4095;
     std::thread th[kStageThreads];
-    unsigned started = 0;
     for (unsigned t = 1; t < nt; ++t) {
         size_t a = std::min(n, t * per), b = std::min(n, a + per);
-        if (a < b)
-            th[started++] = std::thread([=] { std::memcpy(dst + a, src + a, b - a); });
+        if (a >= b)
+            continue;
+        try {
+            th[t] = std::thread([=] { std::memcpy(dst + a, src + a, b - a); });
+        } catch (...) {
+            std::memcpy(dst + a, src + a, b - a);
+        }
     }
     std::memcpy(dst, src, std::min(n, per));
-    for (unsigned t = 0; t < started; ++t)
-        th[t].join();
+    for (unsigned t = 1; t < nt; ++t)
+        if (th[t].joinable())
+            th[t].join();
 }
 
 int32_t host_to_device(bvht_ctx ctx, void* dst, const void* src, size_t bytes)
