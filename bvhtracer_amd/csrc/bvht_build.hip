@@ -707,23 +707,26 @@ __device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t lane)
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }
 constexpr float kQNaN = __builtin_nanf("");
-// min / max over lanes 0..31, returned to every lane (NaN is the identity:
-// fminf / fmaxf ignore it): an inclusive scan within each 16-lane row, whose
-// lane 15 then holds the row's result, and the two rows combined.
+// min / max over lanes 0..31, returned to every lane (lanes out of the
+// node pass NaN, which fminf / fmaxf ignore): an inclusive scan within each
+// 16-lane row, whose lane 15 then holds the row's result, and the two rows
+// combined.  Lane 15's chain (15 <- 14, 13, 11, 7 <- ...) reads only lanes of
+// its own row, so the zeros DPP gives lanes whose source is outside the row
+// reach only lanes that are not read.
 __device__ __forceinline__ float wmin32(float v)
 {
-    v = fminf(v, dpp_f<0x111>(kQNaN, v));
-    v = fminf(v, dpp_f<0x112>(kQNaN, v));
-    v = fminf(v, dpp_f<0x114>(kQNaN, v));
-    v = fminf(v, dpp_f<0x118>(kQNaN, v));
+    v = fminf(v, dpp_f<0x111>(v));
+    v = fminf(v, dpp_f<0x112>(v));
+    v = fminf(v, dpp_f<0x114>(v));
+    v = fminf(v, dpp_f<0x118>(v));
     return fminf(rl(v, 15), rl(v, 31));
 }
 __device__ __forceinline__ float wmax32(float v)
 {
-    v = fmaxf(v, dpp_f<0x111>(kQNaN, v));
-    v = fmaxf(v, dpp_f<0x112>(kQNaN, v));
-    v = fmaxf(v, dpp_f<0x114>(kQNaN, v));
-    v = fmaxf(v, dpp_f<0x118>(kQNaN, v));
+    v = fmaxf(v, dpp_f<0x111>(v));
+    v = fmaxf(v, dpp_f<0x112>(v));
+    v = fmaxf(v, dpp_f<0x114>(v));
+    v = fmaxf(v, dpp_f<0x118>(v));
     return fmaxf(rl(v, 15), rl(v, 31));
 }
 // One bin's (or a run of bins') count and its contribution to a box grown
@@ -744,15 +747,17 @@ __device__ __forceinline__ void bin_combine(BinSum& a, const BinSum& b)
         a.hi[q] = fmaxf(a.hi[q], b.hi[q]);
     }
 }
+// (the callers combine only lanes whose source is in the same 8-lane axis
+// group, so the zeros of out-of-row sources are never used)
 template <int CTRL>
 __device__ __forceinline__ BinSum dpp_bin(const BinSum& v)
 {
     BinSum t;
-    t.n = dpp_u<CTRL>(0u, v.n);
+    t.n = dpp_u<CTRL>(v.n);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        t.lo[q] = dpp_f<CTRL>(kQNaN, v.lo[q]);
-        t.hi[q] = dpp_f<CTRL>(kQNaN, v.hi[q]);
+        t.lo[q] = dpp_f<CTRL>(v.lo[q]);
+        t.hi[q] = dpp_f<CTRL>(v.hi[q]);
     }
     return t;
 }

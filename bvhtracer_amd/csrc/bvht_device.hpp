@@ -259,18 +259,21 @@ __device__ __forceinline__ float key2f(uint32_t k)
 // ------------------------------------------------------ lane exchanges --
 // DPP moves within 16-lane rows (quad_perm, row_shr:n = 0x110 + n, row_shl:n
 // = 0x100 + n, row_ror:n = 0x120 + n): lane i of row_shr:n reads lane i - n;
-// a lane whose source lies outside its row keeps `old`.  gfx950 has no DPP
-// wave shifts or row broadcasts; across rows the permlane swaps and readlane
-// are used.
+// a lane whose source lies outside its row reads 0 (bound_ctrl).  With no
+// "old" value to keep, the move needs no register set up first, and the
+// compiler folds it into the VALU op that consumes it (v_add_f32_dpp ...).
+// Callers use the zero only where 0 is their identity, or not at all.
+// gfx950 has no DPP wave shifts or row broadcasts; across rows the permlane
+// swaps and readlane are used.
 template <int CTRL>
-__device__ __forceinline__ float dpp_f(float old, float v)
+__device__ __forceinline__ float dpp_f(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 template <int CTRL>
-__device__ __forceinline__ uint32_t dpp_u(uint32_t old, uint32_t v)
+__device__ __forceinline__ uint32_t dpp_u(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
 }
 // The value of lane `lane ^ J` (J a power of two below 64) without LDS:
 // quad permutes for 1 and 2, row shifts for 4, a row rotate for 8, and the
@@ -281,14 +284,15 @@ template <uint32_t J>
 __device__ __forceinline__ uint32_t xor_lane(uint32_t v, uint32_t lane)
 {
     if constexpr (J == 1) {
-        return dpp_u<0xB1>(v, v);  // quad_perm [1, 0, 3, 2]
+        return dpp_u<0xB1>(v);  // quad_perm [1, 0, 3, 2]
     } else if constexpr (J == 2) {
-        return dpp_u<0x4E>(v, v);  // quad_perm [2, 3, 0, 1]
+        return dpp_u<0x4E>(v);  // quad_perm [2, 3, 0, 1]
     } else if constexpr (J == 4) {
-        uint32_t down = dpp_u<0x114>(v, v), up = dpp_u<0x104>(v, v);
+        // each lane takes the one of the two whose source is in its row
+        uint32_t down = dpp_u<0x114>(v), up = dpp_u<0x104>(v);
         return (lane & 4u) ? down : up;
     } else if constexpr (J == 8) {
-        return dpp_u<0x128>(v, v);  // row_ror:8
+        return dpp_u<0x128>(v);  // row_ror:8
     } else if constexpr (J == 16) {
         auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
         return ((lane >> 4) & 1u) ? r[0] : r[1];

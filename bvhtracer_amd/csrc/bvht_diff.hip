@@ -118,11 +118,11 @@ __device__ __forceinline__ void bitonic_sort(T& k, uint32_t lane)
 template <uint32_t D>
 __device__ __forceinline__ void seg_step(float (&sv)[9], uint32_t& f, uint32_t lr)
 {
-    uint32_t tf = dpp_u<0x110 + D>(0u, f);
+    uint32_t tf = dpp_u<0x110 + D>(f);
     float t[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c)
-        t[c] = dpp_f<0x110 + D>(0.0f, sv[c]);
+        t[c] = dpp_f<0x110 + D>(sv[c]);
     if (lr >= D) {
         if (!f) {
 #pragma unroll
