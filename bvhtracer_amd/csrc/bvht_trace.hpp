@@ -9,6 +9,17 @@ namespace bvht {
 // instead of flat accesses.
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
+// 64 bytes at a wave-uniform, 4-byte aligned address, loaded once for the
+// wave through the scalar data cache into 16 SGPRs.  The wait is inside the
+// same asm block, so the registers hold the data when the block ends.
+typedef int int16v __attribute__((ext_vector_type(16)));
+__device__ __forceinline__ int16v scalar_load64(const void* p)
+{
+    int16v r;
+    asm volatile("s_load_dwordx16 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)" : "=s"(r) : "s"(p));
+    return r;
+}
+
 // Per-lane traversal stack: the first LDS_DEPTH entries live in LDS at
 // lds[slot * WG] (lds already offset by the lane: the lanes of a wave hit
 // distinct banks), deeper entries in a per-ray column of a global buffer
@@ -116,8 +127,28 @@ __device__ __forceinline__ BlasResult traverse_blas(const float4* __restrict__ n
         while (cnt == 0) {
             if (STATS)
                 ++pairs;
-            const float4* c = nodes + 2 * (size_t)lf;
-            float4 a0 = c[0], a1 = c[1], b0 = c[2], b1 = c[3];
+            float4 a0, a1, b0, b1;
+            const uint32_t lf0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)lf);
+            if (__builtin_amdgcn_ballot_w64(lf != lf0) == 0) {
+                // every descending lane is at the same pair: one scalar load
+                // of its 64 bytes, which leaves the vector memory path (the
+                // texture address and data units) to the other waves
+                const int16v p = scalar_load64(nodes + 2 * (size_t)lf0);
+                a0 = make_float4(__int_as_float(p[0]), __int_as_float(p[1]), __int_as_float(p[2]),
+                                 __int_as_float(p[3]));
+                a1 = make_float4(__int_as_float(p[4]), __int_as_float(p[5]), __int_as_float(p[6]),
+                                 __int_as_float(p[7]));
+                b0 = make_float4(__int_as_float(p[8]), __int_as_float(p[9]), __int_as_float(p[10]),
+                                 __int_as_float(p[11]));
+                b1 = make_float4(__int_as_float(p[12]), __int_as_float(p[13]), __int_as_float(p[14]),
+                                 __int_as_float(p[15]));
+            } else {
+                const float4* c = nodes + 2 * (size_t)lf;
+                a0 = c[0];
+                a1 = c[1];
+                b0 = c[2];
+                b1 = c[3];
+            }
             bool left_near, near_hit, far_hit;
             if (FINITE) {
                 float kl = slab_key(a0, a1, ray);
