@@ -5,7 +5,7 @@ import pytest
 
 import bvhtracer_amd as B
 from bvhtracer_amd import scenes
-from helpers import assert_hits_equal, comb_tree, oracle_rays_to_product, upload
+from helpers import assert_hits_equal, assert_nodes_equal, comb_tree, oracle_rays_to_product, upload
 
 pytestmark = pytest.mark.gpu
 f32 = np.float32
@@ -477,3 +477,32 @@ def test_blas_wave_schedule_keeps_hits(orc):
         g.close()
     finally:
         ctx.close()
+
+
+def test_trace_after_refit_sees_the_new_boxes(ctx, orc):
+    """Trace, deform the vertices and refit on the device, trace again: the
+    second trace (whose uniform node pairs go through the scalar data cache)
+    must use the refitted boxes, giving the oracle's hits on the oracle's
+    refit of the same tree; then the same again in the other direction."""
+    mesh = scenes.icosphere(48, radius=8.0)
+    nodes, used, perm = orc.bvh_build(mesh, False, 2)
+    bvh = upload(ctx, nodes, used, perm)
+    W, H = 256, 192
+    cam = scenes.sphere_camera(8.0, W, H)
+    rays = B.raygen(ctx, cam, W, H, B.ORDER_TILES).download(B.RAY_DTYPE, W * H)
+    first = bvh.intersect(rays)
+    assert_hits_equal(first, orc.bvh_trace(nodes, perm, rays.view(orc.RAY_DTYPE), nthreads=8), "before refit")
+    rng = np.random.default_rng(5)
+    shifted = perm.copy()
+    for step in range(2):
+        # push every triangle along its own random offset (boxes move by up to 0.4)
+        shifted = (shifted + rng.uniform(-0.2, 0.2, size=(shifted.shape[0], 1)).astype(f32)).astype(f32)
+        bvh.refit(shifted)
+        ref_nodes = nodes.copy()
+        orc.bvh_refit(ref_nodes, used, shifted)
+        g_nodes, _, _ = bvh.download()
+        assert_nodes_equal(g_nodes, ref_nodes, used)
+        got = bvh.intersect(rays)
+        ref = orc.bvh_trace(ref_nodes, shifted, rays.view(orc.RAY_DTYPE), nthreads=8)
+        assert_hits_equal(got, ref, f"after refit {step}")
+        assert got.tobytes() != first.tobytes()
