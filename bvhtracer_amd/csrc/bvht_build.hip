@@ -1650,7 +1650,8 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
         bvht_blas* b;
         ~Guard() { if (*b) bvht_blas_destroy(*b); }
     } guard{&b};
-    if (hipMalloc(&b->d_perm, sizeof(uint32_t) * (size_t)n) != hipSuccess)
+    b->d_perm = static_cast<uint32_t*>(bvht::blas_buffer(ctx, sizeof(uint32_t) * (size_t)n));
+    if (!b->d_perm)
         return set_error(BVHT_ENOMEM, "device allocation of the %u-entry build permutation failed", n);
 
     RC(bvht::host_to_device(ctx, w.tin, h_tris, sizeof(float) * 9 * (size_t)n));

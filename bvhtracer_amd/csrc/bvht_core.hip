@@ -1266,6 +1266,39 @@ static int32_t validate_nodes(const bvht_node* nodes, uint32_t node_count, uint3
     return BVHT_OK;
 }
 
+constexpr size_t kBlasPoolBytes = size_t(1) << 30;
+constexpr size_t kBlasPoolEntries = 32;
+
+void* blas_buffer(bvht_ctx ctx, size_t bytes)
+{
+    size_t best = ctx->blas_pool.size();
+    for (size_t i = 0; i < ctx->blas_pool.size(); ++i) {
+        size_t s = ctx->blas_pool[i].second;
+        if (s >= bytes && s <= bytes + bytes / 4 && (best == ctx->blas_pool.size() || s < ctx->blas_pool[best].second))
+            best = i;
+    }
+    if (best < ctx->blas_pool.size()) {
+        void* p = ctx->blas_pool[best].first;
+        ctx->blas_pool_bytes -= ctx->blas_pool[best].second;
+        ctx->blas_pool.erase(ctx->blas_pool.begin() + best);
+        return p;
+    }
+    void* p = nullptr;
+    return hipMalloc(&p, bytes) == hipSuccess ? p : nullptr;
+}
+
+void blas_buffer_release(bvht_ctx ctx, void* p, size_t bytes)
+{
+    if (!p)
+        return;
+    if (ctx && ctx->blas_pool.size() < kBlasPoolEntries && ctx->blas_pool_bytes + bytes <= kBlasPoolBytes) {
+        ctx->blas_pool.emplace_back(p, bytes);
+        ctx->blas_pool_bytes += bytes;
+        return;
+    }
+    (void)hipFree(p);
+}
+
 int32_t blas_alloc(bvht_ctx ctx, uint32_t n, uint32_t node_count, bvht_blas* out)
 {
     bvht_blas b = new (std::nothrow) bvht_blas_s();
@@ -1274,11 +1307,12 @@ int32_t blas_alloc(bvht_ctx ctx, uint32_t n, uint32_t node_count, bvht_blas* out
     b->ctx = ctx;
     b->n_tris = n;
     b->node_count = node_count;
-    bool ok = hipMalloc(&b->d_nodes, sizeof(bvht_node) * (size_t)node_count) == hipSuccess &&
-              hipMalloc(&b->d_tris, sizeof(float) * 9 * (size_t)n) == hipSuccess &&
-              hipMalloc(&b->d_parent, sizeof(uint32_t) * (size_t)node_count) == hipSuccess &&
-              hipMalloc(&b->d_level, sizeof(uint32_t) * (size_t)node_count) == hipSuccess &&
-              hipMalloc(&b->d_level_order, sizeof(uint32_t) * (size_t)node_count) == hipSuccess;
+    b->d_nodes = static_cast<bvht_node*>(blas_buffer(ctx, sizeof(bvht_node) * (size_t)node_count));
+    b->d_tris = static_cast<float*>(blas_buffer(ctx, sizeof(float) * 9 * (size_t)n));
+    b->d_parent = static_cast<uint32_t*>(blas_buffer(ctx, sizeof(uint32_t) * (size_t)node_count));
+    b->d_level = static_cast<uint32_t*>(blas_buffer(ctx, sizeof(uint32_t) * (size_t)node_count));
+    b->d_level_order = static_cast<uint32_t*>(blas_buffer(ctx, sizeof(uint32_t) * (size_t)node_count));
+    bool ok = b->d_nodes && b->d_tris && b->d_parent && b->d_level && b->d_level_order;
     if (!ok) {
         bvht_blas_destroy(b);
         return set_error(BVHT_ENOMEM, "device allocation for a BLAS of %u triangles failed", n);
@@ -1571,6 +1605,9 @@ int32_t bvht_destroy(bvht_ctx ctx)
         (void)hipHostFree(ctx->h_status);
     if (ctx->h_stage)
         (void)hipHostFree(ctx->h_stage);
+    for (auto& e : ctx->blas_pool)
+        (void)hipFree(e.first);
+    ctx->blas_pool.clear();
     for (hipEvent_t e : ctx->stage_ev)
         if (e)
             (void)hipEventDestroy(e);
@@ -1782,14 +1819,18 @@ int32_t bvht_blas_destroy(bvht_blas b)
 {
     if (!b)
         return BVHT_OK;
-    if (b->ctx)
-        (void)hipStreamSynchronize(b->ctx->stream);
-    (void)hipFree(b->d_nodes);
-    (void)hipFree(b->d_tris);
-    (void)hipFree(b->d_perm);
-    (void)hipFree(b->d_parent);
-    (void)hipFree(b->d_level);
-    (void)hipFree(b->d_level_order);
+    // traces of this tree may be queued on any context's stream; a buffer
+    // goes back to the pool only once the device is idle (hipFree waits the
+    // same way)
+    (void)hipDeviceSynchronize();
+    bvht_ctx c = b->ctx;
+    size_t nc = b->node_count, nt = b->n_tris;
+    blas_buffer_release(c, b->d_nodes, sizeof(bvht_node) * nc);
+    blas_buffer_release(c, b->d_tris, sizeof(float) * 9 * nt);
+    blas_buffer_release(c, b->d_perm, sizeof(uint32_t) * nt);
+    blas_buffer_release(c, b->d_parent, sizeof(uint32_t) * nc);
+    blas_buffer_release(c, b->d_level, sizeof(uint32_t) * nc);
+    blas_buffer_release(c, b->d_level_order, sizeof(uint32_t) * nc);
     (void)hipFree(b->d_big_leaves);
     (void)hipFree(b->d_attr_normals);
     (void)hipFree(b->d_attr_tex);

@@ -60,6 +60,11 @@ struct bvht_ctx_s {
     char* h_stage = nullptr;
     size_t stage_half = 0;
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    // Device buffers of destroyed BLASes built or uploaded on this context,
+    // kept for the next BLAS (a rebuild every frame allocates nothing), up
+    // to kBlasPoolBytes in all; freed by bvht_destroy.
+    std::vector<std::pair<void*, size_t>> blas_pool;
+    size_t blas_pool_bytes = 0;
     // Longest-first wave order of bvht_trace_tlas (bvht_set_wave_schedule):
     // the last call's group durations and the order made from them, keyed by
     // TLAS, ray buffer and ray count.
@@ -149,6 +154,12 @@ int32_t compute_blas_topology(bvht_blas b, const bvht_node* h_nodes, uint32_t kn
 // share of each chunk split over a few threads and overlapped with the DMA
 // of the next chunk.
 int32_t host_to_device(bvht_ctx ctx, void* dst, const void* src, size_t bytes);
+// A device buffer of at least `bytes` for a BLAS: a pooled one of this
+// context (at most a quarter larger) or a new hipMalloc; nullptr if neither.
+void* blas_buffer(bvht_ctx ctx, size_t bytes);
+// Back to the context's pool (or freed when the pool is full).  The caller
+// has made sure no queued work on any stream still uses it.
+void blas_buffer_release(bvht_ctx ctx, void* p, size_t bytes);
 int32_t device_to_host(bvht_ctx ctx, void* dst, const void* src, size_t bytes);
 int32_t ensure_spill(bvht_ctx ctx, uint32_t** buf, size_t* words, size_t need);
 // After a fence on ctx's stream: BVHT_ESTACK (and the status cleared) if a
