@@ -1123,26 +1123,37 @@ __global__ void k_gather_tris(const float* __restrict__ tin, const float4* __res
 
 // Exclusive prefix of v over a kCtlWG-thread workgroup in thread order;
 // `total` gets the sum.  lds holds kCtlWG / 64 + 1 words.
+// Inclusive prefix over each 16-lane row (DPP row shifts; a lane whose
+// source is outside its row adds 0).
+__device__ __forceinline__ uint32_t row_scan_incl(uint32_t x)
+{
+    x += dpp_u<0x111>(x);
+    x += dpp_u<0x112>(x);
+    x += dpp_u<0x114>(x);
+    x += dpp_u<0x118>(x);
+    return x;
+}
+
 __device__ uint32_t block_scan_excl(uint32_t v, uint32_t* lds, uint32_t& total)
 {
+    static_assert(kCtlWG / 64 <= 16, "one 16-lane row scans the wave totals");
     uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    uint32_t incl = v;
-    for (uint32_t off = 1; off < 64; off <<= 1) {
-        uint32_t t = __shfl_up(incl, off, 64);
-        if (lane >= off)
-            incl += t;
-    }
+    // the wave's inclusive prefix: rows, then the carries of the rows before
+    uint32_t incl = row_scan_incl(v);
+    uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)incl, 15);
+    uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)incl, 31);
+    uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)incl, 47);
+    incl += lane < 16u ? 0u : (lane < 32u ? r0 : (lane < 48u ? r0 + r1 : r0 + r1 + r2));
     if (lane == 63u)
         lds[wave] = incl;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (int w = 0; w < kCtlWG / 64; ++w) {
-            uint32_t x = lds[w];
-            lds[w] = run;
-            run += x;
-        }
-        lds[kCtlWG / 64] = run;
+    if (wave == 0) {  // exclusive prefix of the wave totals in lanes 0..15
+        uint32_t x = lane < (uint32_t)(kCtlWG / 64) ? lds[lane] : 0u;
+        uint32_t xi = row_scan_incl(x);
+        if (lane < (uint32_t)(kCtlWG / 64))
+            lds[lane] = xi - x;
+        if (lane == 15u)
+            lds[kCtlWG / 64] = xi;
     }
     __syncthreads();
     uint32_t r = lds[wave] + incl - v;
