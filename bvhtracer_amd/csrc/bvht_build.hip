@@ -92,6 +92,7 @@ struct Ctl {
     uint32_t err;         // self-check flags
     uint32_t nodes_used;  // 2 + 2 * internal nodes, after numbering
     uint32_t depth;       // levels of the finished tree, after placement
+    uint32_t done;        // workgroups of k_level_end finished (the last resets it)
 };
 
 constexpr int kCtlWG = 1024;  // the single-workgroup level kernels
@@ -476,10 +477,50 @@ __global__ __launch_bounds__(kWG) void k_part_count(const float4* __restrict__ e
 }
 
 // Exclusive prefix of chunk left counts within each node; node totals.
+// Whether node s of the level splits, and its left count (the host-side rule
+// of the reference's subdivide, bvh.rs:431-437, with the SAH mode's forced
+// median split).
+__device__ __forceinline__ bool node_splits(const Decision& d, uint32_t cnt, int faithful, uint32_t max_leaf,
+                                            uint32_t& nl)
+{
+    if (d.kind == 0)
+        return false;
+    nl = d.kind == 1 ? d.nl : cnt / 2;
+    if (d.kind == 1 && (nl == 0 || nl == cnt)) {
+        bool force = !faithful && max_leaf > 0 && cnt > max_leaf;
+        if (!force)
+            return false;  // abandoned split: a leaf (its range stays partitioned)
+        nl = cnt / 2;
+    }
+    return true;
+}
+
+// The level's nodes in kLvlWGs contiguous ranges, a multiple of kCtlWG
+// nodes each (k_part_prefix counts and k_level_end places them the same way).
+constexpr uint32_t kLvlWGs = 64;
+__device__ __forceinline__ uint32_t level_range_len(uint32_t L)
+{
+    return ((L + kLvlWGs - 1) / kLvlWGs + kCtlWG - 1) / kCtlWG * kCtlWG;
+}
+__device__ __forceinline__ void level_range(uint32_t L, uint32_t& s0, uint32_t& s1)
+{
+    uint32_t per = level_range_len(L);
+    s0 = min(L, blockIdx.x * per);
+    s1 = min(L, s0 + per);
+}
+
+// Per node: the chunks' left counts to offsets and the node's left count;
+// then whether it splits, and how many of its children are large, summed
+// into the counts of its range of the level (wgsum, cleared by
+// k_level_begin).  A wave's 64 consecutive nodes lie in one range (ranges
+// are multiples of kCtlWG nodes), so one lane adds the wave's counts.
 __global__ void k_part_prefix(const uint32_t* __restrict__ node_chunk_begin, const Ctl* __restrict__ ctl,
-                              uint32_t* __restrict__ chunk_left, Decision* __restrict__ dec)
+                              uint32_t* __restrict__ chunk_left, Decision* __restrict__ dec,
+                              const Task* __restrict__ tasks, int faithful, uint32_t max_leaf,
+                              uint32_t* __restrict__ wgsum)
 {
     uint32_t L = ctl->L;
+    uint32_t per = level_range_len(L);
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < L; s += gridDim.x * blockDim.x) {
         uint32_t run = 0;
         for (uint32_t c = node_chunk_begin[s]; c < node_chunk_begin[s + 1]; ++c) {
@@ -487,7 +528,22 @@ __global__ void k_part_prefix(const uint32_t* __restrict__ node_chunk_begin, con
             chunk_left[c] = run;
             run += v;
         }
+        Decision d = dec[s];
+        d.nl = run;
         dec[s].nl = run;
+        uint32_t cnt = tasks[s].count, nl = 0;
+        bool split = node_splits(d, cnt, faithful, max_leaf, nl);
+        uint32_t big = split ? (nl > kSmall ? 1u : 0u) + (cnt - nl > kSmall ? 1u : 0u) : 0u;
+        uint64_t ms = __builtin_amdgcn_ballot_w64(split);
+        uint64_t m1 = __builtin_amdgcn_ballot_w64(big >= 1u), m2 = __builtin_amdgcn_ballot_w64(big == 2u);
+        uint64_t act = __builtin_amdgcn_ballot_w64(true);
+        if ((threadIdx.x & 63u) == (uint32_t)__builtin_ctzll(act)) {
+            uint32_t r = s / per;
+            if (ms)
+                atomicAdd(&wgsum[2 * r], (uint32_t)__popcll(ms));
+            if (m1)
+                atomicAdd(&wgsum[2 * r + 1], (uint32_t)(__popcll(m1) + __popcll(m2)));
+        }
     }
 }
 
@@ -1191,6 +1247,7 @@ __global__ void k_root_init(const uint32_t* __restrict__ rootk, uint32_t n, BNod
     c.err = 0;
     c.nodes_used = 0;
     c.depth = 0;
+    c.done = 0;
     if (n > kSmall) {
         r.small = kNone;
         lvl_all[0] = 0;
@@ -1212,7 +1269,8 @@ __global__ void k_root_init(const uint32_t* __restrict__ rootk, uint32_t n, BNod
 __global__ __launch_bounds__(kCtlWG) void k_level_begin(const BNode* __restrict__ bn,
                                                         const uint32_t* __restrict__ lvl_all, Ctl* __restrict__ ctl,
                                                         Task* __restrict__ tasks, uint32_t* __restrict__ node_chunk,
-                                                        Chunk* __restrict__ chunks, uint32_t lcap, uint32_t ccap)
+                                                        Chunk* __restrict__ chunks, uint32_t lcap, uint32_t ccap,
+                                                        uint32_t* __restrict__ wgsum)
 {
     __shared__ uint32_t lds[kCtlWG / 64 + 1];
     uint32_t L = ctl->L;
@@ -1225,6 +1283,8 @@ __global__ __launch_bounds__(kCtlWG) void k_level_begin(const BNode* __restrict_
         }
         return;
     }
+    if (threadIdx.x < 2 * kLvlWGs)  // the level's range counts (k_part_prefix adds to them)
+        wgsum[threadIdx.x] = 0;
     const uint32_t* lvl = lvl_all + ctl->lvl_off;
     // rounds of kCtlWG consecutive nodes, one per thread (coalesced), the
     // chunk offsets carried from round to round
@@ -1268,112 +1328,115 @@ __global__ __launch_bounds__(kCtlWG) void k_level_begin(const BNode* __restrict_
     }
 }
 
-// Whether node s of the level splits, and its left count (the host-side rule
-// of the reference's subdivide, bvh.rs:431-437, with the SAH mode's forced
-// median split).
-__device__ __forceinline__ bool node_splits(const Decision& d, uint32_t cnt, int faithful, uint32_t max_leaf,
-                                            uint32_t& nl)
-{
-    if (d.kind == 0)
-        return false;
-    nl = d.kind == 1 ? d.nl : cnt / 2;
-    if (d.kind == 1 && (nl == 0 || nl == cnt)) {
-        bool force = !faithful && max_leaf > 0 && cnt > max_leaf;
-        if (!force)
-            return false;  // abandoned split: a leaf (its range stays partitioned)
-        nl = cnt / 2;
-    }
-    return true;
-}
-
 // Children of the level's split nodes: a BNode pair each (in node order),
 // the large ones listed as the next level, the small ones as small roots.
+// Each workgroup places its range from the counts of the ranges before it;
+// the last workgroup to finish (every other one has read ctl by then)
+// advances ctl to the next level.
 __global__ __launch_bounds__(kCtlWG) void k_level_end(BNode* __restrict__ bn, uint32_t* __restrict__ lvl_all,
                                                       uint32_t* __restrict__ small_roots,
                                                       uint32_t* __restrict__ level_start,
                                                       const Task* __restrict__ tasks,
                                                       const Decision* __restrict__ dec,
                                                       const uint32_t* __restrict__ child_keys, Ctl* __restrict__ ctl,
-                                                      int faithful, uint32_t max_leaf, uint32_t hcap, uint32_t rcap)
+                                                      const uint32_t* __restrict__ wgsum, int faithful,
+                                                      uint32_t max_leaf, uint32_t hcap, uint32_t rcap)
 {
     __shared__ uint32_t lds[kCtlWG / 64 + 1];
+    __shared__ uint32_t base[4];  // splits and large children before this range; level totals
     uint32_t L = ctl->L;
     if (L == 0)
         return;
     uint32_t H0 = ctl->H, R0 = ctl->R, off = ctl->lvl_off, nlev = ctl->nlevels;
     const uint32_t* lvl = lvl_all + off;
-    // pass 1: totals (split nodes, large children; the other children are
-    // small), to check the capacities before anything is written
-    uint32_t nsplit = 0, nlarge = 0;
-    for (uint32_t s = threadIdx.x; s < L; s += kCtlWG) {
-        uint32_t cnt = tasks[s].count, nl;
-        if (node_splits(dec[s], cnt, faithful, max_leaf, nl)) {
-            ++nsplit;
-            nlarge += (nl > kSmall ? 1u : 0u) + (cnt - nl > kSmall ? 1u : 0u);
+    if (threadIdx.x < 64) {  // one wave scans the kLvlWGs range counts
+        uint32_t lane = threadIdx.x;
+        uint32_t xs = lane < kLvlWGs ? wgsum[2 * lane] : 0u, xl = lane < kLvlWGs ? wgsum[2 * lane + 1] : 0u;
+        uint32_t is = row_scan_incl(xs), il = row_scan_incl(xl);
+        uint32_t cs0 = (uint32_t)__builtin_amdgcn_readlane((int)is, 15), cs1 = (uint32_t)__builtin_amdgcn_readlane((int)is, 31),
+                 cs2 = (uint32_t)__builtin_amdgcn_readlane((int)is, 47);
+        uint32_t cl0 = (uint32_t)__builtin_amdgcn_readlane((int)il, 15), cl1 = (uint32_t)__builtin_amdgcn_readlane((int)il, 31),
+                 cl2 = (uint32_t)__builtin_amdgcn_readlane((int)il, 47);
+        is += lane < 16u ? 0u : (lane < 32u ? cs0 : (lane < 48u ? cs0 + cs1 : cs0 + cs1 + cs2));
+        il += lane < 16u ? 0u : (lane < 32u ? cl0 : (lane < 48u ? cl0 + cl1 : cl0 + cl1 + cl2));
+        if (lane == blockIdx.x) {
+            base[0] = is - xs;
+            base[1] = il - xl;
+        }
+        if (lane == 63u) {
+            base[2] = is;
+            base[3] = il;
         }
     }
-    uint32_t tsplit, tlarge;
-    block_scan_excl(nsplit, lds, tsplit);
-    block_scan_excl(nlarge, lds, tlarge);
+    __syncthreads();
+    const uint32_t tsplit = base[2], tlarge = base[3];
     const uint32_t tsmall = 2 * tsplit - tlarge;
     bool fits = (uint64_t)H0 + 2 * (uint64_t)tsplit <= hcap && (uint64_t)off + L + tlarge <= hcap &&
                 (uint64_t)R0 + tsmall <= rcap;
-    if (!fits) {  // cannot happen: the tree has fewer than 2n nodes, n elements
-        if (threadIdx.x == 0) {
-            atomicOr(&ctl->err, 128u);
-            ctl->L = 0;
-        }
-        return;
-    }
-    // pass 2: rounds of kCtlWG consecutive nodes, one per thread, placed in
-    // node order by a scan of (splits, large children) packed in one word
-    // (at most kCtlWG and 2 kCtlWG per round)
-    uint32_t c_split = 0, c_large = 0;
-    for (uint32_t r0 = 0; r0 < L; r0 += kCtlWG) {
-        uint32_t s = r0 + threadIdx.x;
-        Task t;
-        uint32_t nl = 0, big = 0;
-        bool split = false;
-        if (s < L) {
-            t = tasks[s];
-            split = node_splits(dec[s], t.count, faithful, max_leaf, nl);
-            if (split)
-                big = (nl > kSmall ? 1u : 0u) + (t.count - nl > kSmall ? 1u : 0u);
-        }
-        uint32_t round_total;
-        uint32_t ex = block_scan_excl((split ? 1u : 0u) | (big << 16), lds, round_total);
-        uint32_t esplit = c_split + (ex & 0xFFFFu), elarge = c_large + (ex >> 16);
-        if (split) {
-            uint32_t child = H0 + 2 * esplit, lpos = off + L + elarge;
-            uint32_t rpos = R0 + 2 * esplit - elarge;  // small children before this node
-            bn[lvl[s]].left = child;
-            const uint32_t* ck = child_keys + 12 * (size_t)s;  // l.min, r.min | l.max, r.max
-            for (uint32_t side = 0; side < 2; ++side) {
-                BNode c;
-                c.first = side ? t.first + nl : t.first;
-                c.count = side ? t.count - nl : nl;
-                keys_to_box_dev(ck + 3 * side, ck + 6 + 3 * side, c.box);
-                c.left = kNone;
-                if (c.count > kSmall) {
-                    c.small = kNone;
-                    lvl_all[lpos++] = child + side;
-                } else {
-                    c.small = rpos;
-                    small_roots[rpos++] = child + side;
-                }
-                bn[child + side] = c;
+    if (fits) {  // cannot fail: the tree has fewer than 2n nodes, n elements
+        uint32_t s0, s1;
+        level_range(L, s0, s1);
+        // rounds of kCtlWG consecutive nodes, one per thread, placed in node
+        // order by a scan of (splits, large children) packed in one word (at
+        // most kCtlWG and 2 kCtlWG per round)
+        uint32_t c_split = base[0], c_large = base[1];
+        for (uint32_t r0 = s0; r0 < s1; r0 += kCtlWG) {
+            uint32_t s = r0 + threadIdx.x;
+            Task t;
+            uint32_t nl = 0, big = 0;
+            bool split = false;
+            if (s < s1) {
+                t = tasks[s];
+                split = node_splits(dec[s], t.count, faithful, max_leaf, nl);
+                if (split)
+                    big = (nl > kSmall ? 1u : 0u) + (t.count - nl > kSmall ? 1u : 0u);
             }
+            uint32_t round_total;
+            uint32_t ex = block_scan_excl((split ? 1u : 0u) | (big << 16), lds, round_total);
+            uint32_t esplit = c_split + (ex & 0xFFFFu), elarge = c_large + (ex >> 16);
+            if (split) {
+                uint32_t child = H0 + 2 * esplit, lpos = off + L + elarge;
+                uint32_t rpos = R0 + 2 * esplit - elarge;  // small children before this node
+                bn[lvl[s]].left = child;
+                const uint32_t* ck = child_keys + 12 * (size_t)s;  // l.min, r.min | l.max, r.max
+                for (uint32_t side = 0; side < 2; ++side) {
+                    BNode c;
+                    c.first = side ? t.first + nl : t.first;
+                    c.count = side ? t.count - nl : nl;
+                    keys_to_box_dev(ck + 3 * side, ck + 6 + 3 * side, c.box);
+                    c.left = kNone;
+                    if (c.count > kSmall) {
+                        c.small = kNone;
+                        lvl_all[lpos++] = child + side;
+                    } else {
+                        c.small = rpos;
+                        small_roots[rpos++] = child + side;
+                    }
+                    bn[child + side] = c;
+                }
+            }
+            c_split += round_total & 0xFFFFu;
+            c_large += round_total >> 16;
         }
-        c_split += round_total & 0xFFFFu;
-        c_large += round_total >> 16;
     }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        ctl->H = H0 + 2 * tsplit;
-        ctl->R = R0 + tsmall;
-        ctl->lvl_off = off + L;
-        ctl->L = tlarge;
-        ctl->nlevels = nlev + 1;
-        level_start[nlev + 1] = off + L;
+        __threadfence();
+        if (atomicAdd(&ctl->done, 1u) == gridDim.x - 1) {  // the last workgroup
+            __threadfence();
+            if (fits) {
+                ctl->H = H0 + 2 * tsplit;
+                ctl->R = R0 + tsmall;
+                ctl->lvl_off = off + L;
+                ctl->L = tlarge;
+                ctl->nlevels = nlev + 1;
+                level_start[nlev + 1] = off + L;
+            } else {
+                atomicOr(&ctl->err, 128u);
+                ctl->L = 0;
+            }
+            ctl->done = 0;
+        }
     }
 }
 
@@ -1535,6 +1598,7 @@ struct BuildWs {
     uint32_t* chunk_left;
     Ctl* ctl;
     uint32_t* rootk;
+    uint32_t* wgsum;  // per-range (splits, large children) of the level
 };
 
 struct Carver {
@@ -1585,6 +1649,7 @@ size_t layout(BuildWs& w, char* base)
     carve(w.chunks, w.ccap);
     carve(w.chunk_left, w.ccap);
     carve(w.ctl, 1);
+    carve(w.wgsum, 2 * (size_t)kLvlWGs);
     carve(w.rootk, 6);
     return carve.off;
 }
@@ -1630,7 +1695,7 @@ void enqueue_level(hipStream_t st, BuildWs& w, int faithful, uint32_t max_leaf)
     uint32_t* err = &w.ctl->err;
     uint32_t cg = std::min<uint32_t>(kChunkGrid, w.ccap);
     hipLaunchKernelGGL(k_level_begin, dim3(1), dim3(kCtlWG), 0, st, w.bn, w.lvl_all, ctl, w.tasks, w.node_chunk,
-                       w.chunks, w.lcap, w.ccap);
+                       w.chunks, w.lcap, w.ccap, w.wgsum);
     hipLaunchKernelGGL(k_init_acc, dim3(1024), dim3(256), 0, st, ctl, w.cacc, w.bacc, w.child);
     hipLaunchKernelGGL(k_centroid_range, dim3(cg), dim3(kWG), 0, st, w.ec, w.chunks, ctl, w.cacc);
     hipLaunchKernelGGL(k_bins, dim3(cg), dim3(kWG), 0, st, w.ec, w.elo, w.ehi, w.chunks, ctl, w.cacc, w.bacc,
@@ -1638,15 +1703,16 @@ void enqueue_level(hipStream_t st, BuildWs& w, int faithful, uint32_t max_leaf)
     hipLaunchKernelGGL(k_decide, dim3(kTaskGrid), dim3(64), 0, st, w.tasks, ctl, w.cacc, w.bacc, w.dec, faithful,
                        max_leaf);
     hipLaunchKernelGGL(k_part_count, dim3(cg), dim3(kWG), 0, st, w.ec, w.chunks, ctl, w.dec, w.chunk_left);
-    hipLaunchKernelGGL(k_part_prefix, dim3(kTaskGrid), dim3(64), 0, st, w.node_chunk, ctl, w.chunk_left, w.dec);
+    hipLaunchKernelGGL(k_part_prefix, dim3(kTaskGrid), dim3(64), 0, st, w.node_chunk, ctl, w.chunk_left, w.dec,
+                       w.tasks, faithful, max_leaf, w.wgsum);
     hipLaunchKernelGGL((k_part_scatter<false>), dim3(cg), dim3(kWG), 0, st, w.ec, w.elo, w.ehi, w.chunks, ctl,
                        w.tasks, w.dec, w.chunk_left, w.kpos, w.sel, w.oc, w.olo, w.ohi, err);
     hipLaunchKernelGGL((k_part_scatter<true>), dim3(cg), dim3(kWG), 0, st, w.ec, w.elo, w.ehi, w.chunks, ctl,
                        w.tasks, w.dec, w.chunk_left, w.kpos, w.sel, w.oc, w.olo, w.ohi, err);
     hipLaunchKernelGGL(k_part_commit, dim3(cg), dim3(kWG), 0, st, w.oc, w.olo, w.ohi, w.chunks, ctl, w.tasks, w.dec,
                        w.ec, w.elo, w.ehi, w.child);
-    hipLaunchKernelGGL(k_level_end, dim3(1), dim3(kCtlWG), 0, st, w.bn, w.lvl_all, w.small_roots, w.level_start,
-                       w.tasks, w.dec, w.child, ctl, faithful, max_leaf, w.hcap, w.rcap);
+    hipLaunchKernelGGL(k_level_end, dim3(kLvlWGs), dim3(kCtlWG), 0, st, w.bn, w.lvl_all, w.small_roots,
+                       w.level_start, w.tasks, w.dec, w.child, ctl, w.wgsum, faithful, max_leaf, w.hcap, w.rcap);
 }
 
 int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32_t max_leaf, bvht_blas* out)
