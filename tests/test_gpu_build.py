@@ -8,7 +8,7 @@ import pytest
 
 import bvhtracer_amd as B
 from bvhtracer_amd import scenes
-from helpers import assert_hits_equal, assert_nodes_equal
+from helpers import assert_hits_equal, assert_nodes_equal, upload
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -153,3 +153,32 @@ def test_build_then_trace(ctx, orc):
 def test_build_rejects_empty(ctx):
     with pytest.raises(B.BvhtError):
         B.BvhBuilder(ctx=ctx).build_for(np.zeros((0, 9), np.float32))
+
+
+def test_builds_on_reused_buffers_match_oracle(orc):
+    """A destroyed BLAS's device buffers go back to its context and serve the
+    next BLAS: a large build, a smaller one (reusing the large buffers'
+    pool only where sizes are close), the large one again and an upload
+    after a destroy all give the oracle's node arrays and triangles."""
+    ctx = B.Context(0)
+    try:
+        big = scenes.icosphere(40, radius=10.0)
+        small = scenes.icosphere(12, radius=3.0)
+        ref = {k: orc.bvh_build(m, False, 2) for k, m in (("big", big), ("small", small))}
+        for name, mesh in (("big", big), ("small", small), ("big", big), ("big", big)):
+            m = np.ascontiguousarray(mesh.copy())
+            bvh = B.BvhBuilder(B.BUILD_SAH, 2, ctx=ctx).build_for(m)
+            nodes, used, perm = ref[name]
+            g_nodes, g_used, g_tris = bvh.download()
+            assert g_used == used
+            assert_nodes_equal(g_nodes, nodes, used)
+            assert g_tris.tobytes() == perm.tobytes()
+            assert bvh.depth() == orc.bvh_depth(nodes)
+            bvh.close()
+        nodes, used, perm = ref["big"]
+        up = upload(ctx, nodes, used, perm)
+        g_nodes, g_used, g_tris = up.download()
+        assert g_nodes.tobytes() == nodes.tobytes() and g_tris.tobytes() == perm.tobytes()
+        up.close()
+    finally:
+        ctx.close()
