@@ -269,6 +269,13 @@ __device__ __forceinline__ uint32_t bin_index(float p)
 
 // Bin counts and boxes per node, axis and bin (bvh.rs:349-358).
 // bacc layout per slot: [axis][bin][7] = count, minx,miny,minz, maxx,maxy,maxz keys.
+// The current value of an LDS word that other lanes may be changing with
+// atomics (a relaxed atomic load: no ordering, no tearing).
+__device__ __forceinline__ uint32_t lds_peek(uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __global__ __launch_bounds__(kWG) void k_bins(const float4* __restrict__ ec, const float4* __restrict__ elo,
                                               const float2* __restrict__ ehi, const Chunk* __restrict__ chunks,
                                               const Ctl* __restrict__ ctl, const uint32_t* __restrict__ cacc,
@@ -297,12 +304,16 @@ __global__ __launch_bounds__(kWG) void k_bins(const float4* __restrict__ ec, con
                 uint32_t b = bin_index((cc[a] - ar[a].bmin) * ar[a].scale);
                 uint32_t* s = sb + (a * kBins + b) * 7;
                 atomicAdd(s, 1u);
-                if (not_nan(lo.x)) atomicMin(s + 1, f2key(lo.x));
-                if (not_nan(lo.y)) atomicMin(s + 2, f2key(lo.y));
-                if (not_nan(lo.z)) atomicMin(s + 3, f2key(lo.z));
-                if (not_nan(lo.w)) atomicMax(s + 4, f2key(lo.w));
-                if (not_nan(hi.x)) atomicMax(s + 5, f2key(hi.x));
-                if (not_nan(hi.y)) atomicMax(s + 6, f2key(hi.y));
+                // a bound is only lowered (raised); a key that does not pass
+                // the value read now cannot pass the final one, so its atomic
+                // is skipped (a stale read only costs an extra atomic)
+                uint32_t k;
+                if (not_nan(lo.x) && (k = f2key(lo.x)) < lds_peek(s + 1)) atomicMin(s + 1, k);
+                if (not_nan(lo.y) && (k = f2key(lo.y)) < lds_peek(s + 2)) atomicMin(s + 2, k);
+                if (not_nan(lo.z) && (k = f2key(lo.z)) < lds_peek(s + 3)) atomicMin(s + 3, k);
+                if (not_nan(lo.w) && (k = f2key(lo.w)) > lds_peek(s + 4)) atomicMax(s + 4, k);
+                if (not_nan(hi.x) && (k = f2key(hi.x)) > lds_peek(s + 5)) atomicMax(s + 5, k);
+                if (not_nan(hi.y) && (k = f2key(hi.y)) > lds_peek(s + 6)) atomicMax(s + 6, k);
             }
         }
         __syncthreads();
