@@ -182,3 +182,39 @@ def test_builds_on_reused_buffers_match_oracle(orc):
         up.close()
     finally:
         ctx.close()
+
+
+def _large_nodes_per_level(nodes, used):
+    """Nodes of more than 32 triangles at each depth of a node array (the
+    levels the device build processes in its level loop)."""
+    lf = nodes["left_first"][:used].astype(np.int64)
+    cnt = nodes["count"][:used].astype(np.int64)
+    levels, frontier = [], np.array([0])
+    while frontier.size:
+        levels.append(frontier)
+        inner = frontier[cnt[frontier] == 0]
+        frontier = np.concatenate([lf[inner], lf[inner] + 1])
+    size = np.zeros(used, np.int64)
+    for lv in reversed(levels):
+        leaf = cnt[lv] > 0
+        size[lv[leaf]] = cnt[lv[leaf]]
+        inner = lv[~leaf]
+        size[inner] = size[lf[inner]] + size[lf[inner] + 1]
+    return [int((size[lv] > 32).sum()) for lv in levels]
+
+
+def test_build_with_wide_levels_matches_oracle(ctx, orc):
+    """4.8M triangles: the widest level holds more than 64 x 1024 large nodes,
+    so each of the 64 workgroups that place a level takes several 1024-node
+    rounds, and the level's task list takes over 64 rounds."""
+    mesh = scenes.icosphere(490, radius=32.0)
+    nodes, used, perm = orc.bvh_build(mesh, False, 2)
+    assert max(_large_nodes_per_level(nodes, used)) > 64 * 1024
+    m = mesh.copy()
+    bvh = B.BvhBuilder(B.BUILD_SAH, 2, ctx=ctx).build_for(m)
+    g_nodes, g_used, g_tris = bvh.download()
+    assert g_used == used
+    assert_nodes_equal(g_nodes, nodes, used)
+    assert g_tris.tobytes() == perm.tobytes()
+    assert bvh.depth() == orc.bvh_depth(nodes)
+    bvh.close()
