@@ -774,6 +774,84 @@ __device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t lane)
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }
 constexpr float kQNaN = __builtin_nanf("");
+// fminf / fmaxf as one v_min_f32 / v_max_f32, for operands that are never a
+// signalling NaN.  The compiler's fminf adds a canonicalize (v_max_f32 x, x)
+// of each operand it cannot prove canonical; none of these is needed here:
+// every value the small-subtree pass compares is a centroid or box bound made
+// by arithmetic or by fminf / fmaxf (canonical), moved by DPP, readlane or
+// ds_permute, or a constant (finite, or the quiet NaN that marks "nothing"),
+// and on such operands v_min_f32 / v_max_f32 give fminf / fmaxf bit for bit.
+// The _s forms take the first operand from an SGPR (a readlane result).
+__device__ __forceinline__ float cmin(float a, float b)
+{
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float cmax(float a, float b)
+{
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float cmin_s(float s, float v)
+{
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "s"(s), "v"(v));
+    return r;
+}
+__device__ __forceinline__ float cmax_s(float s, float v)
+{
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "s"(s), "v"(v));
+    return r;
+}
+// cmin(a, dpp_f<CTRL>(b)) / cmax in one DPP instruction (row_shr:n is
+// 0x110 + n, row_shl:n 0x100 + n; a source lane outside the row reads 0, as
+// dpp_f gives).  The s_nop provides the two wait states between a VALU write
+// of b and its DPP read.
+#define BVHT_DPP_MINMAX(OP, CTRL)                                                                     \
+    do {                                                                                              \
+        if constexpr (CTRL == 0x111)                                                                  \
+            asm("s_nop 1\n\t" OP "_dpp %0, %2, %1 row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1"  \
+                : "=v"(r) : "v"(a), "v"(b));                                                          \
+        else if constexpr (CTRL == 0x112)                                                             \
+            asm("s_nop 1\n\t" OP "_dpp %0, %2, %1 row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1"  \
+                : "=v"(r) : "v"(a), "v"(b));                                                          \
+        else if constexpr (CTRL == 0x114)                                                             \
+            asm("s_nop 1\n\t" OP "_dpp %0, %2, %1 row_shr:4 row_mask:0xf bank_mask:0xf bound_ctrl:1"  \
+                : "=v"(r) : "v"(a), "v"(b));                                                          \
+        else if constexpr (CTRL == 0x118)                                                             \
+            asm("s_nop 1\n\t" OP "_dpp %0, %2, %1 row_shr:8 row_mask:0xf bank_mask:0xf bound_ctrl:1"  \
+                : "=v"(r) : "v"(a), "v"(b));                                                          \
+        else if constexpr (CTRL == 0x101)                                                             \
+            asm("s_nop 1\n\t" OP "_dpp %0, %2, %1 row_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1"  \
+                : "=v"(r) : "v"(a), "v"(b));                                                          \
+        else if constexpr (CTRL == 0x102)                                                             \
+            asm("s_nop 1\n\t" OP "_dpp %0, %2, %1 row_shl:2 row_mask:0xf bank_mask:0xf bound_ctrl:1"  \
+                : "=v"(r) : "v"(a), "v"(b));                                                          \
+        else                                                                                          \
+        {                                                                                             \
+            static_assert(CTRL == 0x104, "unsupported DPP control");                                  \
+            asm("s_nop 1\n\t" OP "_dpp %0, %2, %1 row_shl:4 row_mask:0xf bank_mask:0xf bound_ctrl:1"  \
+                : "=v"(r) : "v"(a), "v"(b));                                                          \
+        }                                                                                             \
+    } while (0)
+template <int CTRL>
+__device__ __forceinline__ float cmin_dpp(float a, float b)
+{
+    float r;
+    BVHT_DPP_MINMAX("v_min_f32", CTRL);
+    return r;
+}
+template <int CTRL>
+__device__ __forceinline__ float cmax_dpp(float a, float b)
+{
+    float r;
+    BVHT_DPP_MINMAX("v_max_f32", CTRL);
+    return r;
+}
+#undef BVHT_DPP_MINMAX
 // min / max over lanes 0..31, returned to every lane (lanes out of the
 // node pass NaN, which fminf / fmaxf ignore): an inclusive scan within each
 // 16-lane row, whose lane 15 then holds the row's result, and the two rows
@@ -782,18 +860,18 @@ constexpr float kQNaN = __builtin_nanf("");
 // reach only lanes that are not read.
 __device__ __forceinline__ float wmin32(float v)
 {
-    v = fminf(v, dpp_f<0x111>(v));
-    v = fminf(v, dpp_f<0x112>(v));
-    v = fminf(v, dpp_f<0x114>(v));
-    v = fminf(v, dpp_f<0x118>(v));
+    v = cmin_dpp<0x111>(v, v);
+    v = cmin_dpp<0x112>(v, v);
+    v = cmin_dpp<0x114>(v, v);
+    v = cmin_dpp<0x118>(v, v);
     return fminf(rl(v, 15), rl(v, 31));
 }
 __device__ __forceinline__ float wmax32(float v)
 {
-    v = fmaxf(v, dpp_f<0x111>(v));
-    v = fmaxf(v, dpp_f<0x112>(v));
-    v = fmaxf(v, dpp_f<0x114>(v));
-    v = fmaxf(v, dpp_f<0x118>(v));
+    v = cmax_dpp<0x111>(v, v);
+    v = cmax_dpp<0x112>(v, v);
+    v = cmax_dpp<0x114>(v, v);
+    v = cmax_dpp<0x118>(v, v);
     return fmaxf(rl(v, 15), rl(v, 31));
 }
 // One bin's (or a run of bins') count and its contribution to a box grown
@@ -805,17 +883,28 @@ struct BinSum {
     uint32_t n;
     float lo[3], hi[3];
 };
-__device__ __forceinline__ void bin_combine(BinSum& a, const BinSum& b)
+// One scan step over the bins of an axis: where `take`, v combines with the
+// value DPP control CTRL brings (the callers take only sources in the same
+// 8-lane axis group, so the zeros of out-of-row sources are never used).
+template <int CTRL>
+__device__ __forceinline__ void bin_scan_step(BinSum& v, bool take)
 {
-    a.n += b.n;
+    const uint32_t n = v.n + dpp_u<CTRL>(v.n);
+    float lo[3], hi[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        a.lo[q] = fminf(a.lo[q], b.lo[q]);
-        a.hi[q] = fmaxf(a.hi[q], b.hi[q]);
+        lo[q] = cmin_dpp<CTRL>(v.lo[q], v.lo[q]);
+        hi[q] = cmax_dpp<CTRL>(v.hi[q], v.hi[q]);
+    }
+    if (take) {
+        v.n = n;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            v.lo[q] = lo[q];
+            v.hi[q] = hi[q];
+        }
     }
 }
-// (the callers combine only lanes whose source is in the same 8-lane axis
-// group, so the zeros of out-of-row sources are never used)
 template <int CTRL>
 __device__ __forceinline__ BinSum dpp_bin(const BinSum& v)
 {
@@ -923,8 +1012,8 @@ __global__ __launch_bounds__(64 * kSmallWavesPerWG) void k_small_wave(
             V3 eh = mk3(rl(lo.w, k), rl(hi.x, k), rl(hi.y, k));
             if (lane < 24 && ((pk >> (4 * ba)) & 15u) == bb) {
                 bcnt += 1;
-                bx.lo = min3(bx.lo, el);
-                bx.hi = max3(bx.hi, eh);
+                bx.lo = mk3(cmin_s(el.x, bx.lo.x), cmin_s(el.y, bx.lo.y), cmin_s(el.z, bx.lo.z));
+                bx.hi = mk3(cmax_s(eh.x, bx.hi.x), cmax_s(eh.y, bx.hi.y), cmax_s(eh.z, bx.hi.z));
             }
         }
         // SAH sweep (sah_axis): plane (ba, bb), bb < 7, has bins 0..bb on the
@@ -935,34 +1024,20 @@ __global__ __launch_bounds__(64 * kSmallWavesPerWG) void k_small_wave(
         own.n = bcnt;
         {
             bool grow = bx.lo.x != kFltMax;
-            own.lo[0] = grow ? fminf(bx.lo.x, bx.hi.x) : kQNaN;
-            own.lo[1] = grow ? fminf(bx.lo.y, bx.hi.y) : kQNaN;
-            own.lo[2] = grow ? fminf(bx.lo.z, bx.hi.z) : kQNaN;
-            own.hi[0] = grow ? fmaxf(bx.lo.x, bx.hi.x) : kQNaN;
-            own.hi[1] = grow ? fmaxf(bx.lo.y, bx.hi.y) : kQNaN;
-            own.hi[2] = grow ? fmaxf(bx.lo.z, bx.hi.z) : kQNaN;
+            own.lo[0] = grow ? cmin(bx.lo.x, bx.hi.x) : kQNaN;
+            own.lo[1] = grow ? cmin(bx.lo.y, bx.hi.y) : kQNaN;
+            own.lo[2] = grow ? cmin(bx.lo.z, bx.hi.z) : kQNaN;
+            own.hi[0] = grow ? cmax(bx.lo.x, bx.hi.x) : kQNaN;
+            own.hi[1] = grow ? cmax(bx.lo.y, bx.hi.y) : kQNaN;
+            own.hi[2] = grow ? cmax(bx.lo.z, bx.hi.z) : kQNaN;
         }
         BinSum pre = own, suf = own;
-        {
-            BinSum t = dpp_bin<0x111>(pre);
-            if (bb >= 1)
-                bin_combine(pre, t);
-            t = dpp_bin<0x112>(pre);
-            if (bb >= 2)
-                bin_combine(pre, t);
-            t = dpp_bin<0x114>(pre);
-            if (bb >= 4)
-                bin_combine(pre, t);
-            t = dpp_bin<0x101>(suf);
-            if (bb + 1 <= 7)
-                bin_combine(suf, t);
-            t = dpp_bin<0x102>(suf);
-            if (bb + 2 <= 7)
-                bin_combine(suf, t);
-            t = dpp_bin<0x104>(suf);
-            if (bb + 4 <= 7)
-                bin_combine(suf, t);
-        }
+        bin_scan_step<0x111>(pre, bb >= 1);
+        bin_scan_step<0x112>(pre, bb >= 2);
+        bin_scan_step<0x114>(pre, bb >= 4);
+        bin_scan_step<0x101>(suf, bb + 1 <= 7);
+        bin_scan_step<0x102>(suf, bb + 2 <= 7);
+        bin_scan_step<0x104>(suf, bb + 4 <= 7);
         BinSum rsum = dpp_bin<0x101>(suf);  // bins bb+1..7 (lanes with bb == 7 are not planes)
         FBox lbx = box_init(faithful), rbx = box_init(faithful);
         lbx.lo = mk3(fminf(lbx.lo.x, pre.lo[0]), fminf(lbx.lo.y, pre.lo[1]), fminf(lbx.lo.z, pre.lo[2]));
