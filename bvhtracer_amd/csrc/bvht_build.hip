@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -1830,10 +1831,19 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
     BVHT_HIP(hipGetLastError());
 
     // Large nodes, level by level, read back every few levels: a level with
-    // no node costs only its launches.
+    // no node costs only its launches (about 25 us for its 11), a read-back
+    // a host round trip.  With forced splits (SAH mode, max_leaf > 0) the
+    // large nodes span about 1.15-1.25 log2(n / kSmall) levels (19 for the
+    // 1.3M-triangle sphere, 14 for 82k triangles, 10 for 12.6k), so the first
+    // read-back comes after 1.15 log2(n / kSmall) levels; without them the
+    // tree stops early (6 levels on the same spheres in the reference mode)
+    // and the first read-back comes after 4.  Then 2, 4, 8, ... levels.
     Ctl h{};
+    const bool forced = !faithful && max_leaf > 0;
+    const uint32_t first_levels =
+        forced ? std::max<uint32_t>(1u, (uint32_t)(1.15 * std::log2(std::max(1.0, (double)n / kSmall)))) : 4u;
     for (uint32_t batch = 0;; ++batch) {
-        uint32_t levels = batch < 2 ? 4u : (batch < 4 ? 8u : (batch < 6 ? 16u : 32u));
+        uint32_t levels = batch == 0 ? first_levels : std::min<uint32_t>(32u, 1u << batch);
         for (uint32_t k = 0; k < levels; ++k)
             enqueue_level(st, w, faithful, max_leaf);
         BVHT_HIP(hipGetLastError());
