@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "bvht_device.hpp"
@@ -1856,37 +1857,57 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
         return set_error(BVHT_EDEVICE, "BLAS build self-check failed (flags 0x%x): inconsistent partition or tree",
                          h.err);
 
-    // small subtrees, numbering, placement
+    // small subtrees; then the permuted triangles, which depend on nothing
+    // else, go back to the caller's array (build_for, bvh.rs:524, partitions
+    // the slice it is given in place) on the copy stream while numbering,
+    // placement and validation run
     uint32_t R = h.R;
     if (R)
         hipLaunchKernelGGL(k_small_wave, dim3(nblk(R, kSmallWavesPerWG)), dim3(64 * kSmallWavesPerWG), 0, st, w.ec,
                            w.elo, w.ehi, w.bn, w.small_roots, R, w.snodes, w.pairs, faithful, max_leaf,
                            &w.ctl->err);
+    hipLaunchKernelGGL(k_gather_tris, dim3(nblk(n, 256)), dim3(256), 0, st, w.tin, w.ec, n, b->d_tris, b->d_perm,
+                       &w.ctl->err);
+    BVHT_HIP(hipGetLastError());
+    RC(bvht::mark_for_copy(ctx));
     hipLaunchKernelGGL(k_number, dim3(1), dim3(kCtlWG), 0, st, w.bn, w.lvl_all, w.level_start, w.ctl, w.pairs, w.ic,
                        w.start, w.fin, w.rfinal, w.rrank, w.rlevel, b->d_nodes, b->d_level, 2 * n);
     if (R)
         hipLaunchKernelGGL(k_small_emit, dim3(nblk(R, 64)), dim3(64), 0, st, w.snodes, w.bn, w.small_roots, w.pairs,
                            w.rfinal, w.rrank, w.rlevel, R, b->d_nodes, b->d_level, 2 * n, &w.ctl->err,
                            &w.ctl->depth);
-    hipLaunchKernelGGL(k_gather_tris, dim3(nblk(n, 256)), dim3(256), 0, st, w.tin, w.ec, n, b->d_tris, b->d_perm,
-                       &w.ctl->err);
     hipLaunchKernelGGL(k_validate_tree, dim3(nblk(2 * (size_t)n, 256)), dim3(256), 0, st, b->d_nodes, w.ctl, n,
                        &w.ctl->err);
     BVHT_HIP(hipGetLastError());
+    // (the control block's read-back comes after the copy: a copy into
+    // pageable memory waits for the work queued before it)
+    const size_t tri_bytes = sizeof(float) * 9 * (size_t)n;
+    int32_t rc = bvht::device_to_host_beside(ctx, h_tris, b->d_tris, tri_bytes);
     BVHT_HIP(hipMemcpyAsync(&h, w.ctl, sizeof h, hipMemcpyDeviceToHost, st));
     BVHT_HIP(hipStreamSynchronize(st));
-    if (h.err)
-        return set_error(BVHT_EDEVICE, "BLAS build self-check failed (flags 0x%x): inconsistent partition or tree",
-                         h.err);
-    if (h.nodes_used < 2 || h.nodes_used > 2 * n)
-        return set_error(BVHT_EDEVICE, "BLAS build: %u nodes used of %u", h.nodes_used, 2 * n);
-    b->nodes_used = h.nodes_used;
-    if (h.depth == 0 || h.depth > h.nodes_used)
-        return set_error(BVHT_EDEVICE, "BLAS build: depth %u out of range", h.depth);
-    RC(compute_blas_topology(b, nullptr, h.depth));  // the levels are already in place
-    // the permuted triangles back into the caller's array (build_for,
-    // bvh.rs:524, partitions the slice it is given in place)
-    RC(bvht::device_to_host(ctx, h_tris, b->d_tris, sizeof(float) * 9 * (size_t)n));
+    if (rc == BVHT_OK) {
+        if (h.err)
+            rc = set_error(BVHT_EDEVICE, "BLAS build self-check failed (flags 0x%x): inconsistent partition or tree",
+                           h.err);
+        else if (h.nodes_used < 2 || h.nodes_used > 2 * n)
+            rc = set_error(BVHT_EDEVICE, "BLAS build: %u nodes used of %u", h.nodes_used, 2 * n);
+        else if (h.depth == 0 || h.depth > h.nodes_used)
+            rc = set_error(BVHT_EDEVICE, "BLAS build: depth %u out of range", h.depth);
+        if (rc == BVHT_OK) {
+            b->nodes_used = h.nodes_used;
+            rc = compute_blas_topology(b, nullptr, h.depth);  // the levels are already in place
+        }
+        if (rc != BVHT_OK) {
+            // the build failed after the copy: the caller's array gets its
+            // triangles back in their original order (still in the upload
+            // buffer); the first error stays the one reported
+            const std::string msg = bvht_last_error();
+            (void)bvht::device_to_host(ctx, h_tris, w.tin, tri_bytes);
+            set_error(rc, "%s", msg.c_str());
+        }
+    }
+    if (rc != BVHT_OK)
+        return rc;
     *out = b;
     b = nullptr;
     return BVHT_OK;
