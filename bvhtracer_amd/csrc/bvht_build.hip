@@ -1836,15 +1836,18 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
     // a host round trip.  With forced splits (SAH mode, max_leaf > 0) the
     // large nodes span about 1.15-1.25 log2(n / kSmall) levels (19 for the
     // 1.3M-triangle sphere, 14 for 82k triangles, 10 for 12.6k), so the first
-    // read-back comes after 1.15 log2(n / kSmall) levels; without them the
-    // tree stops early (6 levels on the same spheres in the reference mode)
-    // and the first read-back comes after 4.  Then 2, 4, 8, ... levels.
+    // read-back comes after 1.15 log2(n / kSmall) levels, then after 2, 4,
+    // 8, ... more.  Without them the tree stops early (6 levels on those
+    // spheres and on bigben.tri, 7 on unity.tri, 10 on the 12.6k soup in the
+    // reference mode): the first read-back comes after at most 7, then after
+    // 4, 8, ... more.
     Ctl h{};
     const bool forced = !faithful && max_leaf > 0;
-    const uint32_t first_levels =
-        forced ? std::max<uint32_t>(1u, (uint32_t)(1.15 * std::log2(std::max(1.0, (double)n / kSmall)))) : 4u;
+    const uint32_t predicted =
+        std::max<uint32_t>(1u, (uint32_t)(1.15 * std::log2(std::max(1.0, (double)n / kSmall))));
+    const uint32_t first_levels = forced ? predicted : std::min<uint32_t>(7u, predicted);
     for (uint32_t batch = 0;; ++batch) {
-        uint32_t levels = batch == 0 ? first_levels : std::min<uint32_t>(32u, 1u << batch);
+        uint32_t levels = batch == 0 ? first_levels : std::min<uint32_t>(32u, (forced ? 1u : 2u) << batch);
         for (uint32_t k = 0; k < levels; ++k)
             enqueue_level(st, w, faithful, max_leaf);
         BVHT_HIP(hipGetLastError());
