@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <condition_variable>
 #include <cstring>
 #include <mutex>
 #include <thread>
@@ -1042,6 +1043,73 @@ constexpr size_t kStageChunk = size_t(8) << 20;
 constexpr size_t kStageMin = size_t(4) << 20;
 constexpr unsigned kStageThreads = 4;
 
+// Helper threads of a context's staged copies: kStageThreads - 1 threads,
+// each copying its slice of every chunk; the calling thread copies the first
+// slice.  They start with the staging and stay until bvht_destroy (starting
+// threads per chunk had cost about 85 us for three).
+struct CopyPool {
+    struct Slice {
+        char* dst;
+        const char* src;
+        size_t n;
+    };
+    std::mutex m;
+    std::condition_variable go, done;
+    Slice slice[kStageThreads];
+    uint64_t round = 0;  // rounds posted
+    unsigned busy = 0;   // helpers still copying the current round
+    bool stop = false;
+    std::vector<std::thread> th;
+};
+
+static void copy_helper(CopyPool* p, unsigned id)
+{
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> lk(p->m);
+    for (;;) {
+        p->go.wait(lk, [&] { return p->stop || p->round != seen; });
+        if (p->stop)
+            return;
+        seen = p->round;
+        const CopyPool::Slice sl = p->slice[id];
+        lk.unlock();
+        if (sl.n)
+            std::memcpy(sl.dst, sl.src, sl.n);
+        lk.lock();
+        if (--p->busy == 0)
+            p->done.notify_one();
+    }
+}
+
+static CopyPool* start_copy_pool()
+{
+    CopyPool* p = new (std::nothrow) CopyPool;
+    if (!p)
+        return nullptr;
+    unsigned want = std::min(kStageThreads, std::max(1u, std::thread::hardware_concurrency())) - 1;
+    try {
+        for (unsigned t = 0; t < want; ++t)
+            p->th.emplace_back(copy_helper, p, t);
+    } catch (...) {  // fewer helpers: the rounds use those that started
+    }
+    return p;
+}
+
+void stop_copy_pool(CopyPool* p)
+{
+    if (!p)
+        return;
+    {
+        std::lock_guard<std::mutex> lk(p->m);
+        p->stop = true;
+    }
+    p->go.notify_all();
+    for (std::thread& t : p->th)
+        if (t.joinable())
+            t.join();
+    delete p;
+}
+
 bool ensure_stage(bvht_ctx ctx)
 {
     if (ctx->h_stage)
@@ -1057,31 +1125,38 @@ bool ensure_stage(bvht_ctx ctx)
         return false;
     ctx->h_stage = static_cast<char*>(p);
     ctx->stage_half = kStageChunk;
+    if (!ctx->copy_pool)
+        ctx->copy_pool = start_copy_pool();
     return true;
 }
 
-// memcpy split over up to kStageThreads threads (the calling one included),
-// in 4 KiB-aligned slices.  A slice whose thread cannot be started is copied
-// by the calling thread (no exception leaves the library).
-void split_memcpy(char* dst, const char* src, size_t n)
+// memcpy split over the context's helper threads and the calling one, in
+// 4 KiB-aligned slices (one plain memcpy without helpers).
+void split_memcpy(bvht_ctx ctx, char* dst, const char* src, size_t n)
 {
-    unsigned nt = std::max(1u, std::min(kStageThreads, std::thread::hardware_concurrency()));
-    size_t per = ((n + nt - 1) / nt + 4095) & ~(size_t)4095;
-    std::thread th[kStageThreads];
-    for (unsigned t = 1; t < nt; ++t) {
-        size_t a = std::min(n, t * per), b = std::min(n, a + per);
-        if (a >= b)
-            continue;
-        try {
-            th[t] = std::thread([=] { std::memcpy(dst + a, src + a, b - a); });
-        } catch (...) {
-            std::memcpy(dst + a, src + a, b - a);
-        }
+    CopyPool* p = ctx->copy_pool;
+    if (!p || p->th.empty()) {
+        std::memcpy(dst, src, n);
+        return;
     }
-    std::memcpy(dst, src, std::min(n, per));
-    for (unsigned t = 1; t < nt; ++t)
-        if (th[t].joinable())
-            th[t].join();
+    const unsigned nt = 1 + (unsigned)p->th.size();
+    const size_t per = ((n + nt - 1) / nt + 4095) & ~(size_t)4095;
+    try {
+        std::unique_lock<std::mutex> lk(p->m);
+        for (unsigned t = 1; t < nt; ++t) {
+            size_t a = std::min(n, t * per), b = std::min(n, a + per);
+            p->slice[t - 1] = CopyPool::Slice{dst + a, src + a, b - a};
+        }
+        p->busy = nt - 1;
+        ++p->round;
+        lk.unlock();
+        p->go.notify_all();
+        std::memcpy(dst, src, std::min(n, per));
+        lk.lock();
+        p->done.wait(lk, [&] { return p->busy == 0; });
+    } catch (...) {  // a lock that cannot be taken: the whole copy here
+        std::memcpy(dst, src, n);
+    }
 }
 
 int32_t host_to_device(bvht_ctx ctx, void* dst, const void* src, size_t bytes)
@@ -1097,7 +1172,7 @@ int32_t host_to_device(bvht_ctx ctx, void* dst, const void* src, size_t bytes)
         size_t len = std::min(ctx->stage_half, bytes - off);
         char* half = ctx->h_stage + (i % 2) * ctx->stage_half;
         BVHT_HIP(hipEventSynchronize(ctx->stage_ev[i % 2]));  // the half's last transfer is done
-        split_memcpy(half, s + off, len);
+        split_memcpy(ctx, half, s + off, len);
         BVHT_HIP(hipMemcpyAsync(d + off, half, len, hipMemcpyHostToDevice, st));
         BVHT_HIP(hipEventRecord(ctx->stage_ev[i % 2], st));
     }
@@ -1127,7 +1202,7 @@ static int32_t device_to_host_on(bvht_ctx ctx, hipStream_t st, void* dst, const 
         if (i + 1 < nch)
             BVHT_HIP(issue(i + 1));
         BVHT_HIP(hipEventSynchronize(ctx->stage_ev[i % 2]));
-        split_memcpy(d + i * H, ctx->h_stage + (i % 2) * H, std::min(H, bytes - i * H));
+        split_memcpy(ctx, d + i * H, ctx->h_stage + (i % 2) * H, std::min(H, bytes - i * H));
     }
     return BVHT_OK;
 }
@@ -1647,6 +1722,7 @@ int32_t bvht_destroy(bvht_ctx ctx)
     }
     if (ctx->copy_ev)
         (void)hipEventDestroy(ctx->copy_ev);
+    bvht::stop_copy_pool(ctx->copy_pool);
     release_build_ws(ctx);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

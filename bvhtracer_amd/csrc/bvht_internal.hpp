@@ -10,6 +10,10 @@
 
 #include "../../include/bvht.h"
 
+namespace bvht {
+struct CopyPool;
+}
+
 struct bvht_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -64,6 +68,9 @@ struct bvht_ctx_s {
     // the host that runs beside kernels still queued on `stream`.
     hipStream_t copy_stream = nullptr;
     hipEvent_t copy_ev = nullptr;
+    // Helper threads of the staged copies (started with the staging, joined
+    // by bvht_destroy).
+    bvht::CopyPool* copy_pool = nullptr;
     // Device buffers of destroyed BLASes built or uploaded on this context,
     // kept for the next BLAS (a rebuild every frame allocates nothing), up
     // to kBlasPoolBytes in all; freed by bvht_destroy.
