@@ -129,12 +129,15 @@ typedef struct bvht_graph_s* bvht_graph;
 
 /* ---- context ---------------------------------------------------------- */
 
-/* Create a context on HIP device `device` with its own stream. */
+/* Create a context on HIP device `device` with its own stream (a second
+ * stream, for a build's copy back to the host, is made on first use).  One
+ * host thread at a time may use a context. */
 int32_t bvht_create(int32_t device, bvht_ctx* out);
 /* Destroy the BLASes, TLASes, batches and graphs made on the context first.
  * The context keeps, until this call, its build workspace, its pinned copy
- * staging and the device buffers of destroyed BLASes (up to 1 GiB, reused
- * by the next BLAS built or uploaded on it). */
+ * staging with three helper threads for the host side of large copies, and
+ * the device buffers of destroyed BLASes (up to 1 GiB, reused by the next
+ * BLAS built or uploaded on it). */
 int32_t bvht_destroy(bvht_ctx ctx);
 /* Message for the last failing call on this thread ("" if none). */
 const char* bvht_last_error(void);
