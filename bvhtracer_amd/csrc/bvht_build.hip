@@ -6,8 +6,8 @@
 // produced breadth first:
 //
 //   * Large nodes (more than kSmall primitives) are processed a level at a
-//     time.  Their primitive ranges are cut into chunks of kChunk elements,
-//     one workgroup per chunk; per-node reductions (centroid range, bin
+//     time.  Their primitive ranges are cut into chunks of 512 to 4096
+//     elements, one workgroup per chunk; per-node reductions (centroid range, bin
 //     counts and boxes, child boxes) combine chunk partials with atomics on
 //     order-preserving float keys (min/max are exact, so the order of
 //     combination does not matter).  The split decision is made by one
@@ -17,7 +17,7 @@
 //     device: the node list of the next level, the chunks and every count
 //     stay in device memory (Ctl), and the host reads the counts back only
 //     every few levels to see whether any large node is left.
-//   * Small nodes are finished by one thread each, running the reference's
+//   * Small nodes are finished by one wave each, running the reference's
 //     sequential subdivide on their range.
 //   * Node numbering: the reference allocates child pairs in depth-first
 //     pre-order (bvh.rs:440-466), so the children of the internal node with
@@ -46,7 +46,10 @@ using namespace bvht;
 namespace {
 
 constexpr uint32_t kSmall = 32;     // nodes with <= kSmall primitives are finished by one thread
-constexpr uint32_t kChunk = 4096;   // elements per workgroup in the level kernels
+// Elements per chunk (a workgroup's share) in the level kernels: a power of
+// two from kChunkMin to kChunkMax, picked per build by chunk_elems.
+constexpr uint32_t kChunkMin = 512;
+constexpr uint32_t kChunkMax = 4096;
 constexpr uint32_t kWG = 256;
 constexpr int kBins = 8;
 
@@ -1352,13 +1355,13 @@ __global__ void k_root_init(const uint32_t* __restrict__ rootk, uint32_t n, BNod
     *ctl = c;
 }
 
-// Tasks of the level (node range and box) and their chunks of at most kChunk
-// elements, in node order; node_chunk[s] = the first chunk of node s.
+// Tasks of the level (node range and box) and their chunks of at most
+// `chunk` elements, in node order; node_chunk[s] = the first chunk of node s.
 __global__ __launch_bounds__(kCtlWG) void k_level_begin(const BNode* __restrict__ bn,
                                                         const uint32_t* __restrict__ lvl_all, Ctl* __restrict__ ctl,
                                                         Task* __restrict__ tasks, uint32_t* __restrict__ node_chunk,
                                                         Chunk* __restrict__ chunks, uint32_t lcap, uint32_t ccap,
-                                                        uint32_t* __restrict__ wgsum)
+                                                        uint32_t* __restrict__ wgsum, uint32_t chunk)
 {
     __shared__ uint32_t lds[kCtlWG / 64 + 1];
     uint32_t L = ctl->L;
@@ -1388,16 +1391,16 @@ __global__ __launch_bounds__(kCtlWG) void k_level_begin(const BNode* __restrict_
             for (int k = 0; k < 6; ++k)
                 t.box[k] = nd.box[k];
             tasks[s] = t;
-            nch = (nd.count + kChunk - 1) / kChunk;
+            nch = (nd.count + chunk - 1) / chunk;
         }
         uint32_t round_total;
         uint32_t run = carry + block_scan_excl(nch, lds, round_total);
         if (s < L) {
             node_chunk[s] = run;
             uint32_t end = t.first + t.count;
-            for (uint32_t b = t.first; b < end; b += kChunk) {
+            for (uint32_t b = t.first; b < end; b += chunk) {
                 if (run < ccap)
-                    chunks[run] = Chunk{s, b, min(b + kChunk, end), 0};
+                    chunks[run] = Chunk{s, b, min(b + chunk, end), 0};
                 ++run;
             }
         }
@@ -1406,7 +1409,7 @@ __global__ __launch_bounds__(kCtlWG) void k_level_begin(const BNode* __restrict_
     const uint32_t total = carry;
     if (threadIdx.x == 0) {
         node_chunk[L] = total;
-        if (total > ccap) {  // cannot happen: chunks <= n / kChunk + L
+        if (total > ccap) {  // cannot happen: chunks <= n / chunk + L
             atomicOr(&ctl->err, 256u);
             ctl->C = 0;
             ctl->L = 0;
@@ -1666,7 +1669,7 @@ int32_t fill(hipStream_t st, uint32_t* p, uint32_t v, size_t n)
 //                 level starts
 //   per task      n_cap / 33 + 1 (a level's large nodes have disjoint ranges
 //                 of more than 32 elements)
-//   per chunk     n_cap / kChunk + tasks + 2
+//   per chunk     n_cap / kChunkMin + tasks + 2
 struct BuildWs {
     void* base = nullptr;
     size_t n_cap = 0;
@@ -1763,7 +1766,7 @@ int32_t ensure_ws(bvht_ctx ctx, uint32_t n, BuildWs** out)
         g.hcap = (uint32_t)(2 * g.n_cap + 1);
         g.rcap = (uint32_t)g.n_cap;
         g.lcap = (uint32_t)(g.n_cap / (kSmall + 1) + 1);
-        g.ccap = (uint32_t)(g.n_cap / kChunk + g.lcap + 2);
+        g.ccap = (uint32_t)(g.n_cap / kChunkMin + g.lcap + 2);
         size_t bytes = layout(g, nullptr);
         if (hipMalloc(&g.base, bytes) != hipSuccess)
             return set_error(BVHT_ENOMEM, "hipMalloc(%zu) of the BLAS build workspace failed", bytes);
@@ -1777,13 +1780,26 @@ int32_t ensure_ws(bvht_ctx ctx, uint32_t n, BuildWs** out)
 constexpr uint32_t kChunkGrid = 2048;  // workgroups of the chunk kernels
 constexpr uint32_t kTaskGrid = 512;    // 64-thread workgroups of the per-node kernels
 
-void enqueue_level(hipStream_t st, BuildWs& w, int faithful, uint32_t max_leaf)
+// Chunk size for n elements: about 640 chunks over the root level (the power
+// of two at or below n / 640) within [kChunkMin, kChunkMax].  Larger chunks
+// mean fewer per-chunk flushes of partial sums (global atomics), smaller
+// ones more workgroups to hide latency.  Measured: 2048 for 1.3M elements
+// (1024 and 4096 slower), 512 for 21k-82k (1024 and 4096 slower).
+uint32_t chunk_elems(uint32_t n)
+{
+    uint32_t c = kChunkMin;
+    while (c < kChunkMax && 2 * (uint64_t)c * 640 <= n)
+        c *= 2;
+    return c;
+}
+
+void enqueue_level(hipStream_t st, BuildWs& w, int faithful, uint32_t max_leaf, uint32_t chunk)
 {
     Ctl* ctl = w.ctl;
     uint32_t* err = &w.ctl->err;
     uint32_t cg = std::min<uint32_t>(kChunkGrid, w.ccap);
     hipLaunchKernelGGL(k_level_begin, dim3(1), dim3(kCtlWG), 0, st, w.bn, w.lvl_all, ctl, w.tasks, w.node_chunk,
-                       w.chunks, w.lcap, w.ccap, w.wgsum);
+                       w.chunks, w.lcap, w.ccap, w.wgsum, chunk);
     hipLaunchKernelGGL(k_init_acc, dim3(1024), dim3(256), 0, st, ctl, w.cacc, w.bacc, w.child);
     hipLaunchKernelGGL(k_centroid_range, dim3(cg), dim3(kWG), 0, st, w.ec, w.chunks, ctl, w.cacc);
     hipLaunchKernelGGL(k_bins, dim3(cg), dim3(kWG), 0, st, w.ec, w.elo, w.ehi, w.chunks, ctl, w.cacc, w.bacc,
@@ -1846,10 +1862,11 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
     const uint32_t predicted =
         std::max<uint32_t>(1u, (uint32_t)(1.15 * std::log2(std::max(1.0, (double)n / kSmall))));
     const uint32_t first_levels = forced ? predicted : std::min<uint32_t>(7u, predicted);
+    const uint32_t chunk = chunk_elems(n);
     for (uint32_t batch = 0;; ++batch) {
         uint32_t levels = batch == 0 ? first_levels : std::min<uint32_t>(32u, (forced ? 1u : 2u) << batch);
         for (uint32_t k = 0; k < levels; ++k)
-            enqueue_level(st, w, faithful, max_leaf);
+            enqueue_level(st, w, faithful, max_leaf, chunk);
         BVHT_HIP(hipGetLastError());
         BVHT_HIP(hipMemcpyAsync(&h, w.ctl, sizeof h, hipMemcpyDeviceToHost, st));
         BVHT_HIP(hipStreamSynchronize(st));
