@@ -6,7 +6,7 @@
 // produced breadth first:
 //
 //   * Large nodes (more than kSmall primitives) are processed a level at a
-//     time.  Their primitive ranges are cut into chunks of 512 to 4096
+//     time.  Their primitive ranges are cut into chunks of 256 to 4096
 //     elements, one workgroup per chunk; per-node reductions (centroid range, bin
 //     counts and boxes, child boxes) combine chunk partials with atomics on
 //     order-preserving float keys (min/max are exact, so the order of
@@ -48,7 +48,7 @@ namespace {
 constexpr uint32_t kSmall = 32;     // nodes with <= kSmall primitives are finished by one thread
 // Elements per chunk (a workgroup's share) in the level kernels: a power of
 // two from kChunkMin to kChunkMax, picked per build by chunk_elems.
-constexpr uint32_t kChunkMin = 512;
+constexpr uint32_t kChunkMin = 256;
 constexpr uint32_t kChunkMax = 4096;
 constexpr uint32_t kWG = 256;
 constexpr int kBins = 8;
@@ -1780,15 +1780,17 @@ int32_t ensure_ws(bvht_ctx ctx, uint32_t n, BuildWs** out)
 constexpr uint32_t kChunkGrid = 2048;  // workgroups of the chunk kernels
 constexpr uint32_t kTaskGrid = 512;    // 64-thread workgroups of the per-node kernels
 
-// Chunk size for n elements: about 640 chunks over the root level (the power
-// of two at or below n / 640) within [kChunkMin, kChunkMax].  Larger chunks
-// mean fewer per-chunk flushes of partial sums (global atomics), smaller
-// ones more workgroups to hide latency.  Measured: 2048 for 1.3M elements
-// (1024 and 4096 slower), 512 for 21k-82k (1024 and 4096 slower).
+// Chunk size for n elements: the power of two at or below 2 sqrt(n), within
+// [kChunkMin, kChunkMax] -- so about sqrt(n) / 2 chunks over the root level.
+// Larger chunks mean fewer per-chunk flushes of partial sums (global
+// atomics), smaller ones more workgroups to hide latency.  Measured best of
+// 256 / 512 / 1024 / 2048 / 4096: 2048 for 1.3M elements, 512 for 82k, 256
+// for 12.6k.
 uint32_t chunk_elems(uint32_t n)
 {
+    const double target = 2.0 * std::sqrt((double)n);
     uint32_t c = kChunkMin;
-    while (c < kChunkMax && 2 * (uint64_t)c * 640 <= n)
+    while (c < kChunkMax && 2.0 * c <= target)
         c *= 2;
     return c;
 }
