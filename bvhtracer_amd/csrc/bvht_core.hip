@@ -1216,26 +1216,41 @@ int32_t mark_for_copy(bvht_ctx ctx)
 {
     if (!ctx->copy_ev)
         BVHT_HIP(hipEventCreateWithFlags(&ctx->copy_ev, hipEventDisableTiming));
-    if (!ctx->copy_stream)
-        BVHT_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
     BVHT_HIP(hipEventRecord(ctx->copy_ev, ctx->stream));
     return BVHT_OK;
 }
 
+// One copy stream per device for the process (default priority), made on
+// first use and kept.  Streams map onto a few hardware queues
+// (GPU_MAX_HW_QUEUES, 4 by default): a copy stream per context would make
+// the streams of several contexts share queues, so that their work no
+// longer overlaps, and a stream made per copy cost about 0.2 ms per build.
+static hipStream_t process_copy_stream(int device)
+{
+    static std::mutex m;
+    static hipStream_t streams[64] = {};
+    if (device < 0 || device >= 64)
+        return nullptr;
+    std::lock_guard<std::mutex> lk(m);
+    if (!streams[device] && hipStreamCreateWithFlags(&streams[device], hipStreamNonBlocking) != hipSuccess)
+        streams[device] = nullptr;
+    return streams[device];
+}
+
 int32_t device_to_host_beside(bvht_ctx ctx, void* dst, const void* src, size_t bytes)
 {
-    if (!ctx->copy_stream || !ctx->copy_ev)
-        return set_error(BVHT_EINVAL, "device_to_host_beside without mark_for_copy");
     // a copy below the staging threshold goes through the runtime's pageable
-    // path, which measured slower from the second stream (82k-triangle
-    // build 1.92 -> 2.06 ms): it runs on ctx's stream after all queued work
-    if (bytes < kStageMin || ctx->capturing)
+    // path, which measured slower from a second stream (82k-triangle build
+    // 1.92 -> 2.06 ms): it runs on ctx's stream after all queued work
+    hipStream_t cs = nullptr;
+    if (bytes < kStageMin || ctx->capturing || !ctx->copy_ev || !ensure_stage(ctx) ||
+        !(cs = process_copy_stream(ctx->device)))
         return device_to_host_on(ctx, ctx->stream, dst, src, bytes);
-    // every earlier transfer through the staging was queued on ctx's stream
-    // before the mark, and device_to_host_on returns only once its own
-    // transfers are done
-    BVHT_HIP(hipStreamWaitEvent(ctx->copy_stream, ctx->copy_ev, 0));
-    return device_to_host_on(ctx, ctx->copy_stream, dst, src, bytes);
+    // every earlier transfer through this context's staging was queued on
+    // ctx's stream before the mark, and device_to_host_on returns only once
+    // its own transfers are done
+    BVHT_HIP(hipStreamWaitEvent(cs, ctx->copy_ev, 0));
+    return device_to_host_on(ctx, cs, dst, src, bytes);
 }
 
 int32_t compute_blas_topology(bvht_blas b, const bvht_node*, uint32_t known_depth)
@@ -1716,10 +1731,6 @@ int32_t bvht_destroy(bvht_ctx ctx)
     for (hipEvent_t e : ctx->stage_ev)
         if (e)
             (void)hipEventDestroy(e);
-    if (ctx->copy_stream) {
-        (void)hipStreamSynchronize(ctx->copy_stream);
-        (void)hipStreamDestroy(ctx->copy_stream);
-    }
     if (ctx->copy_ev)
         (void)hipEventDestroy(ctx->copy_ev);
     bvht::stop_copy_pool(ctx->copy_pool);

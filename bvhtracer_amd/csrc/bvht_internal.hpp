@@ -64,9 +64,8 @@ struct bvht_ctx_s {
     char* h_stage = nullptr;
     size_t stage_half = 0;
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
-    // Second stream (default priority, created on first use) for a copy to
-    // the host that runs beside kernels still queued on `stream`.
-    hipStream_t copy_stream = nullptr;
+    // Mark of the work a copy to the host beside later kernels waits for
+    // (mark_for_copy / device_to_host_beside).
     hipEvent_t copy_ev = nullptr;
     // Helper threads of the staged copies (started with the staging, joined
     // by bvht_destroy).
@@ -173,10 +172,11 @@ void* blas_buffer(bvht_ctx ctx, size_t bytes);
 void blas_buffer_release(bvht_ctx ctx, void* p, size_t bytes);
 int32_t device_to_host(bvht_ctx ctx, void* dst, const void* src, size_t bytes);
 // mark_for_copy marks the work queued on ctx's stream so far;
-// device_to_host_beside then copies on ctx's copy stream after that work
-// only, so kernels queued on ctx's stream after the mark run during the
-// copy (copies below the staging threshold run on ctx's stream after all
-// queued work instead).  It returns once dst holds the data.
+// device_to_host_beside then copies on the device's copy stream (one per
+// device for the process) after that work only, so kernels queued on ctx's
+// stream after the mark run during the copy (copies below the staging
+// threshold run on ctx's stream after all queued work instead).  It returns
+// once dst holds the data.
 int32_t mark_for_copy(bvht_ctx ctx);
 int32_t device_to_host_beside(bvht_ctx ctx, void* dst, const void* src, size_t bytes);
 int32_t ensure_spill(bvht_ctx ctx, uint32_t** buf, size_t* words, size_t need);

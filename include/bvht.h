@@ -129,9 +129,10 @@ typedef struct bvht_graph_s* bvht_graph;
 
 /* ---- context ---------------------------------------------------------- */
 
-/* Create a context on HIP device `device` with its own stream (a second
- * stream, for a build's copy back to the host, is made on first use).  One
- * host thread at a time may use a context. */
+/* Create a context on HIP device `device` with its own stream.  (A build's
+ * copy back to the host runs on one more stream per device, shared by all
+ * contexts and made on first use.)  One host thread at a time may use a
+ * context. */
 int32_t bvht_create(int32_t device, bvht_ctx* out);
 /* Destroy the BLASes, TLASes, batches and graphs made on the context first.
  * The context keeps, until this call, its build workspace, its pinned copy
