@@ -218,3 +218,25 @@ def test_build_with_wide_levels_matches_oracle(ctx, orc):
     assert g_tris.tobytes() == perm.tobytes()
     assert bvh.depth() == orc.bvh_depth(nodes)
     bvh.close()
+
+
+@pytest.mark.parametrize("freq,mode,max_leaf", [(128, B.BUILD_SAH, 2), (128, B.BUILD_REFERENCE, 0),
+                                                 (181, B.BUILD_SAH, 2), (181, B.BUILD_SAH, 0)])
+def test_build_mid_sizes_match_oracle(ctx, orc, freq, mode, max_leaf):
+    """327,680 and 655,220 triangles: the level kernels cut node ranges into
+    chunks of 1024 elements here (256 at the 12.6k soup, 512 at 82k, 2048 at
+    1.3M and 4096 at 4.8M, all covered above), the copy back runs beside the
+    numbering kernels with forced splits, and SAH mode without a leaf bound
+    stops splitting on its own."""
+    mesh = scenes.icosphere(freq, radius=16.0)
+    faithful = mode == B.BUILD_REFERENCE
+    nodes, used, perm = orc.bvh_build(mesh, faithful, max_leaf)
+    m = mesh.copy()
+    bvh = B.BvhBuilder(mode, max_leaf, ctx=ctx).build_for(m)
+    g_nodes, g_used, g_tris = bvh.download()
+    assert g_used == used
+    assert_nodes_equal(g_nodes, nodes, used)
+    assert g_tris.tobytes() == perm.tobytes()
+    assert m.tobytes() == perm.tobytes()  # the caller's array holds the permuted triangles
+    assert bvh.depth() == orc.bvh_depth(nodes)
+    bvh.close()
