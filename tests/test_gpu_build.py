@@ -225,9 +225,9 @@ def test_build_with_wide_levels_matches_oracle(ctx, orc):
 def test_build_mid_sizes_match_oracle(ctx, orc, freq, mode, max_leaf):
     """327,680 and 655,220 triangles: the level kernels cut node ranges into
     chunks of 1024 elements here (256 at the 12.6k soup, 512 at 82k, 2048 at
-    1.3M and 4096 at 4.8M, all covered above), the copy back runs beside the
-    numbering kernels with forced splits, and SAH mode without a leaf bound
-    stops splitting on its own."""
+    1.3M and 4096 at 4.8M, all covered above), the copy back (11 and 22 MB)
+    runs on the second stream beside the numbering kernels, and SAH mode
+    without a leaf bound stops splitting on its own."""
     mesh = scenes.icosphere(freq, radius=16.0)
     faithful = mode == B.BUILD_REFERENCE
     nodes, used, perm = orc.bvh_build(mesh, faithful, max_leaf)
