@@ -10,8 +10,9 @@
 //     elements, one workgroup per chunk; per-node reductions (centroid range, bin
 //     counts and boxes, child boxes) combine chunk partials with atomics on
 //     order-preserving float keys (min/max are exact, so the order of
-//     combination does not matter).  The split decision is made by one
-//     thread per node with the reference's sequential arithmetic.  The
+//     combination does not matter).  The split decision is made by four
+//     lanes per node (one per axis, combined in axis order) with the
+//     reference's sequential arithmetic.  The
 //     two-pointer partition permutation is reproduced exactly from prefix
 //     counts (derivation at k_part_scatter).  The level loop runs on the
 //     device: the node list of the next level, the chunks and every count

@@ -1088,6 +1088,7 @@ static CopyPool* start_copy_pool()
         return nullptr;
     unsigned want = std::min(kStageThreads, std::max(1u, std::thread::hardware_concurrency())) - 1;
     try {
+        p->th.reserve(want);
         for (unsigned t = 0; t < want; ++t)
             p->th.emplace_back(copy_helper, p, t);
     } catch (...) {  // fewer helpers: the rounds use those that started
