@@ -75,6 +75,10 @@ struct Decision {
 };
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+// Per split node, the keys part_commit_chunk reduces for its two children:
+// mins (box lo left, right; centroid min left, right) then maxes (box hi
+// left, right; centroid max left, right), three each.
+constexpr uint32_t kChildKeys = 24;
 
 // A node of the large-node tree: every node split level by level, their
 // children, and the small subtrees' roots.  Children are allocated in pairs
@@ -182,16 +186,13 @@ __global__ void k_init_elems(const float* __restrict__ tris, uint32_t n, float4*
 }
 
 // Accumulators of the level's L large nodes (L read on the device).
-__global__ void k_init_acc(const Ctl* __restrict__ ctl, uint32_t* __restrict__ cacc, uint32_t* __restrict__ bacc,
-                           uint32_t* __restrict__ child)
+__global__ void k_init_acc(const Ctl* __restrict__ ctl, uint32_t* __restrict__ bacc, uint32_t* __restrict__ child)
 {
     size_t L = ctl->L;
     size_t total = (size_t)3 * kBins * 7 * L;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-        if (i < 6 * L)
-            cacc[i] = (i % 6) < 3 ? kEmptyMin : kEmptyMax;
-        if (i < 12 * L)
-            child[i] = (i % 12) < 6 ? kEmptyMin : kEmptyMax;
+        if (i < kChildKeys * L)
+            child[i] = (i % kChildKeys) < kChildKeys / 2 ? kEmptyMin : kEmptyMax;
         uint32_t f = (uint32_t)(i % 7);
         bacc[i] = f == 0 ? 0u : (f <= 3 ? kEmptyMin : kEmptyMax);
     }
@@ -220,25 +221,20 @@ __global__ __launch_bounds__(kWG) void k_range_box(const float4* __restrict__ el
     block_reduce_keys<6>(v, keys, lds);
 }
 
-// Centroid range per node and axis (find_best_split_plane, bvh.rs:338-344).
-// The chunk kernels run a fixed grid over the level's chunks (the count is
-// read on the device), a workgroup per chunk at a time.
-__global__ __launch_bounds__(kWG) void k_centroid_range(const float4* __restrict__ ec, const Chunk* __restrict__ chunks,
-                                                        const Ctl* __restrict__ ctl,
-                                                        uint32_t* __restrict__ cacc /* slot*6 */)
+// Centroid range of elements [0, n) into keys[6] = {min xyz, max xyz}: the
+// root's (find_best_split_plane, bvh.rs:338-344).  The children's come out of
+// part_commit_chunk, which reads every element of a split node anyway.
+__global__ __launch_bounds__(kWG) void k_centroid_root(const float4* __restrict__ ec, uint32_t n,
+                                                       uint32_t* __restrict__ keys)
 {
     __shared__ uint32_t lds[6 * 4];
-    uint32_t C = ctl->C;
-    for (uint32_t ci = blockIdx.x; ci < C; ci += gridDim.x) {
-        Chunk ch = chunks[ci];
-        uint32_t v[6] = {kEmptyMin, kEmptyMin, kEmptyMin, kEmptyMax, kEmptyMax, kEmptyMax};
-        for (uint32_t e = ch.begin + threadIdx.x; e < ch.end; e += kWG) {
-            float4 c = ec[e];
-            key_min(v[0], c.x); key_min(v[1], c.y); key_min(v[2], c.z);
-            key_max(v[3], c.x); key_max(v[4], c.y); key_max(v[5], c.z);
-        }
-        block_reduce_keys<6>(v, cacc + 6 * ch.slot, lds);
+    uint32_t v[6] = {kEmptyMin, kEmptyMin, kEmptyMin, kEmptyMax, kEmptyMax, kEmptyMax};
+    for (uint32_t e = blockIdx.x * kWG + threadIdx.x; e < n; e += gridDim.x * kWG) {
+        float4 c = ec[e];
+        key_min(v[0], c.x); key_min(v[1], c.y); key_min(v[2], c.z);
+        key_max(v[3], c.x); key_max(v[4], c.y); key_max(v[5], c.z);
     }
+    block_reduce_keys<6>(v, keys, lds);
 }
 
 struct AxisRange {
@@ -739,12 +735,9 @@ __device__ void part_commit_chunk(const float4* __restrict__ oc, const float4* _
     // or replaced by the median split when the node must be split
     if (d.kind == 1 && (nl == 0 || nl == t.count))
         nl = t.count / 2;
-    uint32_t v[12];
-    for (int k = 0; k < 12; ++k)
-        v[k] = kEmptyMin;
-    // layout: l.min xyz, r.min xyz (min) | l.max xyz, r.max xyz (max)
-    for (int k = 6; k < 12; ++k)
-        v[k] = kEmptyMax;
+    uint32_t v[kChildKeys];
+    for (uint32_t k = 0; k < kChildKeys; ++k)
+        v[k] = k < kChildKeys / 2 ? kEmptyMin : kEmptyMax;
     for (uint32_t e = ch.begin + threadIdx.x; e < ch.end; e += kWG) {
         float4 c = oc[e];
         float4 lo = olo[e];
@@ -756,15 +749,22 @@ __device__ void part_commit_chunk(const float4* __restrict__ oc, const float4* _
         key_min(v[side + 0], lo.x);
         key_min(v[side + 1], lo.y);
         key_min(v[side + 2], lo.z);
-        key_max(v[6 + side + 0], lo.w);
-        key_max(v[6 + side + 1], hi.x);
-        key_max(v[6 + side + 2], hi.y);
+        key_min(v[6 + side + 0], c.x);
+        key_min(v[6 + side + 1], c.y);
+        key_min(v[6 + side + 2], c.z);
+        key_max(v[12 + side + 0], lo.w);
+        key_max(v[12 + side + 1], hi.x);
+        key_max(v[12 + side + 2], hi.y);
+        key_max(v[18 + side + 0], c.x);
+        key_max(v[18 + side + 1], c.y);
+        key_max(v[18 + side + 2], c.z);
     }
-    block_reduce_keys<12>(v, child_keys + 12 * ch.slot, lds);
+    block_reduce_keys<kChildKeys>(v, child_keys + kChildKeys * ch.slot, lds);
 }
 
 // Copy the partitioned ranges back and reduce the two child boxes
-// (update_node_bounds of the children, bvh.rs:462-463).
+// (update_node_bounds of the children, bvh.rs:462-463) and the children's
+// centroid ranges (the next level's find_best_split_plane, bvh.rs:338-344).
 __global__ __launch_bounds__(kWG) void k_part_commit(const float4* __restrict__ oc, const float4* __restrict__ olo,
                                                      const float2* __restrict__ ohi, const Chunk* __restrict__ chunks,
                                                      const Ctl* __restrict__ ctl, const Task* __restrict__ tasks,
@@ -772,7 +772,7 @@ __global__ __launch_bounds__(kWG) void k_part_commit(const float4* __restrict__ 
                                                      float4* __restrict__ elo, float2* __restrict__ ehi,
                                                      uint32_t* __restrict__ child_keys)
 {
-    __shared__ uint32_t lds[12 * 4];
+    __shared__ uint32_t lds[kChildKeys * 4];
     uint32_t C = ctl->C;
     for (uint32_t ci = blockIdx.x; ci < C; ci += gridDim.x) {
         Chunk ch = chunks[ci];
@@ -1464,7 +1464,8 @@ __global__ __launch_bounds__(kCtlWG) void k_level_end(BNode* __restrict__ bn, ui
                                                       const Decision* __restrict__ dec,
                                                       const uint32_t* __restrict__ child_keys, Ctl* __restrict__ ctl,
                                                       const uint32_t* __restrict__ wgsum, int faithful,
-                                                      uint32_t max_leaf, uint32_t hcap, uint32_t rcap)
+                                                      uint32_t max_leaf, uint32_t hcap, uint32_t rcap,
+                                                      uint32_t* __restrict__ cacc, uint32_t lcap)
 {
     __shared__ uint32_t lds[kCtlWG / 64 + 1];
     __shared__ uint32_t base[4];  // splits and large children before this range; level totals
@@ -1522,15 +1523,23 @@ __global__ __launch_bounds__(kCtlWG) void k_level_end(BNode* __restrict__ bn, ui
                 uint32_t child = H0 + 2 * esplit, lpos = off + L + elarge;
                 uint32_t rpos = R0 + 2 * esplit - elarge;  // small children before this node
                 bn[lvl[s]].left = child;
-                const uint32_t* ck = child_keys + 12 * (size_t)s;  // l.min, r.min | l.max, r.max
+                const uint32_t* ck = child_keys + kChildKeys * (size_t)s;
                 for (uint32_t side = 0; side < 2; ++side) {
                     BNode c;
                     c.first = side ? t.first + nl : t.first;
                     c.count = side ? t.count - nl : nl;
-                    keys_to_box_dev(ck + 3 * side, ck + 6 + 3 * side, c.box);
+                    keys_to_box_dev(ck + 3 * side, ck + 12 + 3 * side, c.box);
                     c.left = kNone;
                     if (c.count > kSmall) {
                         c.small = kNone;
+                        // its centroid range, as the next level's slot
+                        // lpos - (off + L) reads it
+                        const uint32_t ns = lpos - (off + L);
+                        if (ns < lcap)
+                            for (uint32_t k = 0; k < 3; ++k) {
+                                cacc[6 * (size_t)ns + k] = ck[6 + 3 * side + k];
+                                cacc[6 * (size_t)ns + 3 + k] = ck[18 + 3 * side + k];
+                            }
                         lvl_all[lpos++] = child + side;
                     } else {
                         c.small = rpos;
@@ -1767,7 +1776,7 @@ size_t layout(BuildWs& w, char* base)
     carve(w.tasks, w.lcap);
     carve(w.cacc, 6 * (size_t)w.lcap);
     carve(w.bacc, (size_t)3 * kBins * 7 * w.lcap);
-    carve(w.child, 12 * (size_t)w.lcap);
+    carve(w.child, kChildKeys * (size_t)w.lcap);
     carve(w.node_chunk, (size_t)w.lcap + 1);
     carve(w.dec, w.lcap);
     carve(w.chunks, w.ccap);
@@ -1835,8 +1844,7 @@ void enqueue_level(hipStream_t st, BuildWs& w, int faithful, uint32_t max_leaf, 
     uint32_t cg = std::min<uint32_t>(kChunkGrid, w.ccap);
     hipLaunchKernelGGL(k_level_begin, dim3(1), dim3(kCtlWG), 0, st, w.bn, w.lvl_all, ctl, w.tasks, w.node_chunk,
                        w.chunks, w.lcap, w.ccap, w.wgsum, chunk);
-    hipLaunchKernelGGL(k_init_acc, dim3(1024), dim3(256), 0, st, ctl, w.cacc, w.bacc, w.child);
-    hipLaunchKernelGGL(k_centroid_range, dim3(cg), dim3(kWG), 0, st, w.ec, w.chunks, ctl, w.cacc);
+    hipLaunchKernelGGL(k_init_acc, dim3(1024), dim3(256), 0, st, ctl, w.bacc, w.child);
     hipLaunchKernelGGL(k_bins, dim3(cg), dim3(kWG), 0, st, w.ec, w.elo, w.ehi, w.chunks, ctl, w.cacc, w.bacc,
                        faithful);
     hipLaunchKernelGGL(k_decide, dim3(kTaskGrid), dim3(64), 0, st, w.tasks, ctl, w.cacc, w.bacc, w.dec, faithful,
@@ -1851,7 +1859,8 @@ void enqueue_level(hipStream_t st, BuildWs& w, int faithful, uint32_t max_leaf, 
     hipLaunchKernelGGL(k_part_commit, dim3(cg), dim3(kWG), 0, st, w.oc, w.olo, w.ohi, w.chunks, ctl, w.tasks, w.dec,
                        w.ec, w.elo, w.ehi, w.child);
     hipLaunchKernelGGL(k_level_end, dim3(kLvlWGs), dim3(kCtlWG), 0, st, w.bn, w.lvl_all, w.small_roots,
-                       w.level_start, w.tasks, w.dec, w.child, ctl, w.wgsum, faithful, max_leaf, w.hcap, w.rcap);
+                       w.level_start, w.tasks, w.dec, w.child, ctl, w.wgsum, faithful, max_leaf, w.hcap, w.rcap,
+                       w.cacc, w.lcap);
 }
 
 int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32_t max_leaf, bvht_blas* out)
@@ -1880,6 +1889,12 @@ int32_t build_impl(bvht_ctx ctx, float* h_tris, uint32_t n, int faithful, uint32
                        n, w.rootk);
     hipLaunchKernelGGL(k_root_init, dim3(1), dim3(1), 0, st, w.rootk, n, w.bn, w.lvl_all, w.small_roots,
                        w.level_start, w.ctl);
+    // the root's centroid range (slot 0 of the first level; k_level_end
+    // writes those of the later levels' nodes)
+    RC(fill(st, w.cacc, kEmptyMin, 3));
+    RC(fill(st, w.cacc + 3, kEmptyMax, 3));
+    hipLaunchKernelGGL(k_centroid_root, dim3(std::min<uint32_t>(nblk(n, kWG), 1024)), dim3(kWG), 0, st, w.ec, n,
+                       w.cacc);
     BVHT_HIP(hipGetLastError());
 
     // Large nodes, level by level, read back every few levels: a level with
